@@ -1,0 +1,245 @@
+"""ctypes wrapper of the CPU restatement in ``wql_oracle.c``.
+
+TEST INFRASTRUCTURE ONLY.  This module is the checker that the HIP product
+path is compared against; it is imported by ``tests/``,
+``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` and by
+nothing in the product package.
+
+Arrays use the layout documented in ``wql_oracle.h``: numpy arrays of shape
+``(ncomp, nz, ny, nx)`` (x fastest), float64, C-contiguous.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libwql_oracle.so")
+
+SRC_GAUSSIAN, SRC_COSBELL, SRC_RICKER = 0, 1, 2
+TAU0 = 1.0 / 0.315949074074074  # block.f90:242
+
+
+def build(force: bool = False) -> str:
+    """Compile the restatement with ``make`` (gcc, -ffp-contract=off)."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-s", "-C", _HERE, "-B" if force else "libwql_oracle.so"],
+                       check=True)
+    return _LIB_PATH
+
+
+class _Source(C.Structure):
+    _fields_ = [("type", C.c_int), ("order", C.c_int), ("alpha", C.c_int),
+                ("near", C.c_int * 3),
+                ("duration", C.c_double), ("t_init", C.c_double),
+                ("mXX", C.c_double), ("mYY", C.c_double), ("mZZ", C.c_double),
+                ("mXY", C.c_double), ("mXZ", C.c_double), ("mYZ", C.c_double),
+                ("loc", C.c_double * 3), ("near_phys", C.c_double * 3)]
+
+
+_P = C.POINTER(C.c_double)
+
+
+class _Block(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("curvilinear", C.c_int), ("nmech", C.c_int), ("nsrc", C.c_int),
+                ("bc", C.c_int * 6), ("pml", C.c_int * 6), ("npml", C.c_int * 6),
+                ("hq", C.c_double), ("hr", C.c_double), ("hs", C.c_double),
+                ("tau0", C.c_double),
+                ("tau", C.c_double * 8), ("weight", C.c_double * 8),
+                ("F", _P), ("DF", _P),
+                ("metx", _P), ("mety", _P), ("metz", _P), ("J", _P),
+                ("M", _P), ("eta", _P), ("Deta", _P),
+                ("Qs_inv", _P), ("Qp_inv", _P),
+                ("pQ", _P * 6), ("pDQ", _P * 6),
+                ("nl", _P * 6), ("nm", _P * 6), ("nn", _P * 6),
+                ("src", C.POINTER(_Source))]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build()
+        L = C.CDLL(_LIB_PATH)
+        L.orc_rk_coefficients.argtypes = [_P, _P, _P]
+        L.orc_sbp_closure.argtypes = [_P]
+        L.orc_metrics.argtypes = [C.c_int, C.c_int, C.c_int, _P, _P, _P, _P, _P]
+        L.orc_normals.argtypes = [C.POINTER(_Block)]
+        L.orc_q_coefficients.argtypes = [C.c_int, C.c_double, C.c_double, _P, _P]
+        L.orc_q_modulus_correction.argtypes = [C.POINTER(_Block), C.c_double]
+        L.orc_q_modulus_correction.restype = C.c_int
+        for name in ("orc_set_rates", "orc_impose_boundary_condition"):
+            getattr(L, name).argtypes = [C.POINTER(_Block)]
+        L.orc_scale_rates.argtypes = [C.POINTER(_Block), C.c_double]
+        L.orc_update_fields.argtypes = [C.POINTER(_Block), C.c_double]
+        L.orc_set_moment_tensor.argtypes = [C.POINTER(_Block), C.c_double]
+        L.orc_time_step.argtypes = [C.POINTER(_Block), C.c_double, C.c_int, _P,
+                                    C.c_int, C.POINTER(C.c_long), _P]
+        L.orc_source_time.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double]
+        L.orc_source_time.restype = C.c_double
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_P)
+
+
+def rk_coefficients():
+    A = np.zeros(5); B = np.zeros(5); Cc = np.zeros(6)
+    lib().orc_rk_coefficients(_ptr(A), _ptr(B), _ptr(Cc))
+    return A, B, Cc
+
+
+def sbp_closure():
+    t = np.zeros(54)
+    lib().orc_sbp_closure(_ptr(t))
+    return t.reshape(6, 9)
+
+
+def metrics(X):
+    """derivatives() restatement. X: (3, nz, ny, nx). Returns metx, mety, metz, J."""
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    _, nz, ny, nx = X.shape
+    metx = np.zeros((3, nz, ny, nx)); mety = np.zeros_like(metx); metz = np.zeros_like(metx)
+    J = np.zeros((nz, ny, nx))
+    lib().orc_metrics(nx, ny, nz, _ptr(X), _ptr(metx), _ptr(mety), _ptr(metz), _ptr(J))
+    return metx, mety, metz, J
+
+
+def q_coefficients(nmech, fmin=0.05, fmax=20.0):
+    tau = np.zeros(8); w = np.zeros(8)
+    lib().orc_q_coefficients(nmech, fmin, fmax, _ptr(tau), _ptr(w))
+    return tau[:nmech].copy(), w[:nmech].copy()
+
+
+def source_time(kind, t, t0, w):
+    return lib().orc_source_time(kind, t, t0, w)
+
+
+class OracleBlock:
+    """One block of the restatement.  Fill the public numpy attributes, then
+    call the stage routines or :meth:`time_step`."""
+
+    def __init__(self, nx, ny, nz, *, curvilinear, hq=None, hr=None, hs=None):
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.curvilinear = int(curvilinear)
+        self.hq = 1.0 / (nx - 1) if hq is None else hq
+        self.hr = 1.0 / (ny - 1) if hr is None else hr
+        self.hs = 1.0 / (nz - 1) if hs is None else hs
+        self.tau0 = TAU0
+        shp = (nz, ny, nx)
+        self.F = np.zeros((9,) + shp)
+        self.DF = np.full((9,) + shp, 1.0e40)          # fields.f90:39
+        self.metx = np.zeros((3,) + shp); self.mety = np.zeros((3,) + shp)
+        self.metz = np.zeros((3,) + shp); self.J = np.zeros(shp)
+        self.M = np.zeros((3,) + shp)
+        self.bc = [-1] * 6
+        self.pml = [0] * 6
+        self.npml = [0] * 6
+        self.nmech = 0
+        self.tau = np.zeros(8); self.weight = np.zeros(8)
+        self.eta = None; self.Deta = None; self.Qs_inv = None; self.Qp_inv = None
+        self.pQ = [None] * 6; self.pDQ = [None] * 6
+        fshape = [(3, nz, ny), (3, nz, ny), (3, nz, nx), (3, nz, nx), (3, ny, nx), (3, ny, nx)]
+        self.nl = [np.zeros(s) for s in fshape]
+        self.nm = [np.zeros(s) for s in fshape]
+        self.nn = [np.zeros(s) for s in fshape]
+        self.sources = []
+        self._struct = None
+
+    # -- setup helpers --------------------------------------------------
+    def set_anelastic(self, nmech, Qs_inv, Qp_inv, fref=1.0, fmin=0.05, fmax=20.0,
+                      correct_modulus=True):
+        tau, w = q_coefficients(nmech, fmin, fmax)
+        self.nmech = nmech
+        self.tau[:] = 0; self.weight[:] = 0
+        self.tau[:nmech] = tau; self.weight[:nmech] = w
+        shp = (self.nz, self.ny, self.nx)
+        self.Qs_inv = np.ascontiguousarray(np.broadcast_to(Qs_inv, shp), dtype=np.float64)
+        self.Qp_inv = np.ascontiguousarray(np.broadcast_to(Qp_inv, shp), dtype=np.float64)
+        self.eta = np.zeros((6, nmech) + shp)
+        self.Deta = np.zeros((6, nmech) + shp)   # material.f90:389-404
+        if correct_modulus:
+            rc = lib().orc_q_modulus_correction(C.byref(self._sync()), fref)
+            if rc != 0:
+                raise ValueError("invalid Q modulus correction: weight/Q sum >= 1")
+
+    def set_pml(self, face, npml):
+        shp = (9, self.nz, self.ny, self.nx)
+        self.pml[face] = 1
+        self.npml[face] = npml
+        self.pQ[face] = np.zeros(shp)
+        self.pDQ[face] = np.full(shp, 1.0e40)     # pml.f90:46
+
+    def add_source(self, *, kind, t_init, duration, mxx=0., myy=0., mzz=0., mxy=0., mxz=0.,
+                   myz=0., loc, near, near_phys, order=4, alpha=0):
+        s = _Source()
+        s.type = kind; s.order = order; s.alpha = alpha
+        s.near[:] = [int(v) for v in near]
+        s.duration = duration; s.t_init = t_init
+        s.mXX, s.mYY, s.mZZ, s.mXY, s.mXZ, s.mYZ = mxx, myy, mzz, mxy, mxz, myz
+        s.loc[:] = list(loc); s.near_phys[:] = list(near_phys)
+        self.sources.append(s)
+
+    def _sync(self):
+        b = _Block()
+        b.nx, b.ny, b.nz = self.nx, self.ny, self.nz
+        b.curvilinear = self.curvilinear
+        b.nmech = self.nmech
+        b.nsrc = len(self.sources)
+        b.bc[:] = list(self.bc); b.pml[:] = list(self.pml); b.npml[:] = list(self.npml)
+        b.hq, b.hr, b.hs, b.tau0 = self.hq, self.hr, self.hs, self.tau0
+        b.tau[:] = list(self.tau); b.weight[:] = list(self.weight)
+        for name in ("F", "DF", "metx", "mety", "metz", "J", "M", "eta", "Deta",
+                     "Qs_inv", "Qp_inv"):
+            setattr(b, name, _ptr(getattr(self, name)))
+        for f in range(6):
+            b.pQ[f] = _ptr(self.pQ[f]); b.pDQ[f] = _ptr(self.pDQ[f])
+            b.nl[f] = _ptr(self.nl[f]); b.nm[f] = _ptr(self.nm[f]); b.nn[f] = _ptr(self.nn[f])
+        if self.sources:
+            self._src_arr = (_Source * len(self.sources))(*self.sources)
+            b.src = C.cast(self._src_arr, C.POINTER(_Source))
+        self._struct = b
+        return b
+
+    # -- stage routines ---------------------------------------------------
+    def normals(self):
+        lib().orc_normals(C.byref(self._sync()))
+
+    def scale_rates(self, A):
+        lib().orc_scale_rates(C.byref(self._sync()), A)
+
+    def set_rates(self):
+        lib().orc_set_rates(C.byref(self._sync()))
+
+    def set_moment_tensor(self, t):
+        lib().orc_set_moment_tensor(C.byref(self._sync()), t)
+
+    def impose_boundary_condition(self):
+        lib().orc_impose_boundary_condition(C.byref(self._sync()))
+
+    def update_fields(self, inc):
+        lib().orc_update_fields(C.byref(self._sync()), inc)
+
+    def time_step(self, dt, nsteps, t=0.0, receivers=None):
+        """Advance nsteps; receivers: flat 0-based node indices.
+        Returns (t, seismograms[nsteps, nrec, 9])."""
+        tt = np.array([t], dtype=np.float64)
+        if receivers is None:
+            rec = np.zeros(1, dtype=np.int64); nrec = 0
+        else:
+            rec = np.ascontiguousarray(receivers, dtype=np.int64); nrec = rec.size
+        seis = np.zeros((max(nsteps, 1), max(nrec, 1), 9))
+        lib().orc_time_step(C.byref(self._sync()), dt, nsteps, _ptr(tt), nrec,
+                            rec.ctypes.data_as(C.POINTER(C.c_long)), _ptr(seis))
+        return float(tt[0]), seis[:, :nrec] if nrec else None

@@ -1,0 +1,98 @@
+/*
+ * wql_oracle.h -- CPU restatement (FP64) of the WaveQLab3D-Q per-block time
+ * step.  TEST INFRASTRUCTURE ONLY: this library is the checker that the HIP
+ * product path is compared against.  Only tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg may load it.  Nothing in the product links it.
+ *
+ * Array layout (all arrays are plain C arrays, no ghost nodes):
+ *   node (x,y,z), 1-based as in the reference, x = q fastest:
+ *       idx = (x-1) + nx*((y-1) + ny*(z-1)),   N = nx*ny*nz
+ *   component-major: F[c*N + idx], c = 0..8 (vx,vy,vz,sxx,syy,szz,sxy,sxz,syz)
+ *   metrics: metx[m*N + idx] = (q_x, r_x, s_x)[m], mety = (q_y, r_y, s_y),
+ *            metz = (q_z, r_z, s_z)     (reference grid.f90:398-471)
+ *   material M[k*N + idx] = (lambda, mu, rho)[k]   (material.f90:696-702)
+ *   memory variables eta[(c6*nmech + i)*N + idx], c6 = 0..5 -> components 4..9
+ *   PML auxiliaries: full-block [9][N] arrays per face (only slab nodes used)
+ *   face frames: nl/nm/nn[f][k*NF + fidx], f = 0..5 = Lx,Rx,Ly,Ry,Lz,Rz,
+ *     q faces fidx = (y-1)+ny*(z-1); r faces (x-1)+nx*(z-1); s faces (x-1)+nx*(y-1)
+ */
+#ifndef WQL_ORACLE_H
+#define WQL_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORC_SRC_GAUSSIAN = 0, ORC_SRC_COSBELL = 1, ORC_SRC_RICKER = 2 };
+
+typedef struct orc_source {
+  int type;
+  int order;          /* 2 or 4 (singular_source) */
+  int alpha;          /* 0, 25, 50, 100 for order 4 */
+  int near[3];        /* nearest node, 1-based */
+  double duration, t_init;
+  double mXX, mYY, mZZ, mXY, mXZ, mYZ;
+  double loc[3];
+  double near_phys[3];
+} orc_source;
+
+typedef struct orc_block {
+  int nx, ny, nz;
+  int curvilinear;    /* type_of_mesh: 0 cartesian, 1 curvilinear */
+  int nmech;          /* 0 elastic, 4 = anelastic-Q4, 8 = anelastic-Q8 */
+  int nsrc;
+  int bc[6];          /* Lx,Rx,Ly,Ry,Lz,Rz; 1 char., 2 free surface, 3 clamped, <=0 none */
+  int pml[6];
+  int npml[6];
+  double hq, hr, hs, tau0;
+  double tau[8], weight[8];
+  double *F, *DF;
+  double *metx, *mety, *metz, *J;
+  double *M;
+  double *eta, *Deta;
+  double *Qs_inv, *Qp_inv;
+  double *pQ[6], *pDQ[6];
+  double *nl[6], *nm[6], *nn[6];
+  orc_source *src;
+} orc_block;
+
+/* KC(5,4) low-storage coefficients (time_step.f90:53-62): A[5], B[5], C[6] */
+void orc_rk_coefficients(double *A, double *B, double *C);
+
+/* SBP-6 first-derivative closure table (JU_xJU_yJU_z6.f90:74-182), 6x9 */
+void orc_sbp_closure(double *table54);
+
+/* derivatives() (grid.f90:398-471): metrics and J from coordinates X[3][N] */
+void orc_metrics(int nx, int ny, int nz, const double *X,
+                 double *metx, double *mety, double *metz, double *J);
+
+/* local_orth_vectors_{q,r,s} (unit_normals.f90:8-168) for all six faces */
+void orc_normals(orc_block *b);
+
+/* fixed-coefficient Q4/Q8 tau/weights (anelastic_q{4,8}_model.f90) */
+void orc_q_coefficients(int nmech, double fmin, double fmax, double *tau, double *weight);
+
+/* per-node unrelaxed-modulus correction (material.f90:425-453 / 258-280);
+ * returns 0 or -1 if weight/Q sum >= 1 at some node */
+int orc_q_modulus_correction(orc_block *b, double fref);
+
+/* stage pieces, in reference order */
+void orc_scale_rates(orc_block *b, double A);               /* fields.f90:51-155 */
+void orc_set_rates(orc_block *b);                           /* elastic.f90:13-29 */
+void orc_set_moment_tensor(orc_block *b, double t);         /* moment_tensor.f90:875-935 */
+void orc_impose_boundary_condition(orc_block *b);           /* RHS_Interior.f90:1584-1768 */
+void orc_update_fields(orc_block *b, double inc);           /* fields.f90:158-263 */
+
+/* time_step_RK (time_step.f90:68-150) for nsteps steps; t is in/out.
+ * If nrec > 0, rec_idx[nrec] (0-based flat node index) are sampled at stage 1
+ * of each step (time t0) into seis[(step*nrec + r)*9 + c]. */
+void orc_time_step(orc_block *b, double dt, int nsteps, double *t,
+                   int nrec, const long *rec_idx, double *seis);
+
+/* source_time (moment_tensor.f90:813-872) + Ricker (new) */
+double orc_source_time(int type, double t, double t0, double w);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
