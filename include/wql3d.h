@@ -1,0 +1,188 @@
+/*
+ * wql3d.h -- C ABI of libwql3d_hip.so, the MI355X (gfx950) implementation of
+ * the WaveQLab3D-Q per-block RK time step: 6th-order SBP traditional RHS
+ * (Cartesian and curvilinear), anelastic Q4/Q8 memory variables, PML
+ * auxiliary fields, SAT boundary penalties, moment-tensor source injection
+ * and the Kennedy-Carpenter (5,4) low-storage RK update.
+ *
+ * One block of one rank lives on one GPU.  There is no halo exchange: on one
+ * rank the reference's exchanges are sends to MPI_PROC_NULL
+ * (mpi3dcomm.f90:478-527), so ghost nodes never carry data.
+ *
+ * Conventions
+ *  - Every entry point returns 0 on success or a WQL_E* code; the message of
+ *    the last failure on the calling thread is wql_last_error().  A Fortran
+ *    shim maps a non-zero code to fatal_local('RUN-HIP-<code>', message)
+ *    (diagnostics.f90:175-204).
+ *  - Host arrays are Fortran arrays passed by c_loc of their first element
+ *    together with their bounds (wql_box), so arrays with ghost nodes and
+ *    non-unit lower bounds are read and written in place.  If
+ *    desc->arrays_on_device is 1 the same pointers are device pointers.
+ *  - Calls are stream-ordered on the block's own HIP stream; download and
+ *    sample synchronise.  No entry point allocates device memory except
+ *    wql_block_create / upload / download / set_* (never inside a step).
+ *  - Calls on one block come from one host thread.
+ */
+#ifndef WQL3D_H
+#define WQL3D_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* error codes */
+enum {
+  WQL_OK = 0,
+  WQL_EINVAL = 1,   /* bad argument / descriptor */
+  WQL_EHIP = 2,     /* HIP runtime error */
+  WQL_ENOMEM = 3,   /* device memory capacity exceeded (RUN-GPU-MEM-002) */
+  WQL_ESTATE = 4,   /* call not valid in the block's current state */
+  WQL_ENODEV = 5    /* no GPU visible */
+};
+
+enum { WQL_CARTESIAN = 0, WQL_CURVILINEAR = 1 };       /* type_of_mesh */
+enum { WQL_FP64 = 0, WQL_FP32_MIXED = 1 };             /* arithmetic mode */
+enum { WQL_BC_NONE = 0, WQL_BC_CHARACTERISTIC = 1,     /* BoundaryConditions.f90:28-30 */
+       WQL_BC_FREE_SURFACE = 2, WQL_BC_CLAMPED = 3 };
+enum { WQL_SRC_GAUSSIAN = 0, WQL_SRC_COSBELL = 1, WQL_SRC_RICKER = 2 };
+
+/* upload/download masks */
+enum {
+  WQL_F = 1u, WQL_DF = 2u, WQL_GEOMETRY = 4u, WQL_MATERIAL = 8u,
+  WQL_MEMORY = 16u,   /* eta, Deta */
+  WQL_QINV = 32u,     /* Qs_inv, Qp_inv */
+  WQL_PML = 64u,      /* PML Q, DQ */
+  WQL_FACES = 128u,   /* n_l, n_m, n_n */
+  WQL_ALL = 255u
+};
+
+/* Fortran bounds of the first three dimensions of an array
+ * (for face arrays only lo[0..1], hi[0..1] are used) */
+typedef struct wql_box {
+  int32_t lo[3];
+  int32_t hi[3];
+} wql_box;
+
+/* Block descriptor: the fields of block_type / block_fields / block_grid_t /
+ * block_material / block_pml / block_boundary (datatypes.f90:24-247) that the
+ * time step reads or writes. */
+typedef struct wql_block_desc {
+  int32_t nq, nr, ns;        /* nodes of the block, nodes are 1..n per axis */
+  int32_t mesh;              /* WQL_CARTESIAN / WQL_CURVILINEAR */
+  int32_t precision;         /* WQL_FP64 / WQL_FP32_MIXED */
+  int32_t nmech;             /* 0 elastic, 4 anelastic-Q4, 8 anelastic-Q8 */
+  int32_t bc[6];             /* Lx,Rx,Ly,Ry,Lz,Rz (boundary_type Lx..Rz) */
+  int32_t pml[6];            /* PMLB(1..6)%pml */
+  int32_t npml[6];           /* PMLB(1..6)%N_pml */
+  int32_t arrays_on_device;  /* 0: host pointers, 1: device pointers */
+  double hq, hr, hs;         /* G%hq, G%hr, G%hs = 1/(n-1) */
+  double tau0;               /* B%tau0 (block.f90:242) */
+  double tau[8], weight[8];  /* tau_Q / weight_Q (Q4) or tau_Q8 / weight_Q8 (Q8) */
+  wql_box body;              /* bounds of F, DF, metric*, J, M, eta*, Q*_inv */
+  const double *metricx, *metricy, *metricz;  /* body x 3 */
+  const double *J;                            /* body */
+  const double *M;                            /* body x 3: lambda, mu, rho */
+  double *F, *DF;                             /* body x 9 */
+  double *eta[6], *Deta[6];                   /* eta4..eta9 (+Q/Q8 suffix): body x nmech */
+  const double *Qs_inv, *Qp_inv;              /* body */
+  wql_box pml_box[6];                         /* bounds of PMLB(f)%Q / %DQ */
+  double *pml_Q[6], *pml_DQ[6];               /* pml_box x 9 */
+  wql_box face_box[6];                        /* bounds of B(f)%n_l/n_m/n_n (2-D) */
+  const double *n_l[6], *n_m[6], *n_n[6];     /* face_box x 3 */
+} wql_block_desc;
+
+/* moment tensor point source (moment_tensor.f90: moment_tensor type) */
+typedef struct wql_source {
+  int32_t type;              /* WQL_SRC_* ; for RICKER duration = peak frequency */
+  int32_t order;             /* 2 or 4 (singular_source) */
+  int32_t alpha;             /* 0, 25, 50, 100 (order 4 stencil) */
+  int32_t near[3];           /* nearest node, 1-based */
+  double duration, t_init;
+  double mXX, mYY, mZZ, mXY, mXZ, mYZ;
+  double location[3];
+  double near_phys[3];
+} wql_source;
+
+typedef struct wql_block wql_block;
+
+/* ---- accelerator_runtime seam (accelerator_runtime_cpu.f90:19-64) ------- */
+/* initialize_accelerator_runtime: bind rank to device local_rank % ndev */
+int wql_runtime_init(int32_t local_rank, int32_t *device);
+/* finalize_accelerator_runtime */
+int wql_runtime_finalize(void);
+/* enforce_device_memory_capacity (domain.f90:405): WQL_ENOMEM if the bytes
+ * exceed free device memory; env WQL3D_GPU_MEMORY_BYTES overrides capacity */
+int wql_capacity_check(int64_t predicted_bytes);
+
+/* ---- device_data seam (device_data_cpu.f90:8-26) ------------------------- */
+/* enter_domain_device_data: create device mirrors (hipMalloc, once) */
+int wql_block_create(const wql_block_desc *desc, wql_block **out);
+/* device bytes held by the block (persistent_memory.f90:47-105 analogue) */
+int64_t wql_block_device_bytes(const wql_block *b);
+/* copy host arrays selected by mask to the device mirrors */
+int wql_block_upload(wql_block *b, uint32_t mask);
+/* update_domain_host_data: copy device mirrors back to the host arrays */
+int wql_block_download(wql_block *b, uint32_t mask);
+/* set F=0, DF=1e40, eta=Deta=0, PML Q=0, DQ=1e40 on the device, as
+ * init_fields / init_anelastic_Q*_properties / init_pml do */
+int wql_block_init_state(wql_block *b);
+/* exit_domain_device_data */
+void wql_block_destroy(wql_block *b);
+
+/* ---- sources and receivers ---------------------------------------------- */
+/* init_moment_tensor result (moment_tensor.f90:8-242); n <= 8 */
+int wql_block_set_sources(wql_block *b, int32_t n, const wql_source *src);
+/* receivers at nodes ijk[3*n] (1-based), sampled at stage 1 of each step
+ * (time_step.f90:137) into a device buffer of nslots steps */
+int wql_block_set_receivers(wql_block *b, int32_t n, const int32_t *ijk, int32_t nslots);
+/* copy recorded samples [slot][receiver][9] for slots 0..nslots-1 to host */
+int wql_block_get_seismograms(wql_block *b, int32_t nslots, double *out);
+
+/* ---- rk_vector_kernels seam (rk_vector_cpu.f90:7-19) -------------------- */
+/* scale_rate_array over DF, Deta*, PML DQ of the block (fields.f90:51-155) */
+int wql_rk_scale(wql_block *b, double A);
+/* update_state_array over F, eta*, PML Q (fields.f90:158-263) */
+int wql_rk_update(wql_block *b, double Bdt);
+/* the same two kernels on any device array of n doubles (bit-exact with the
+ * CPU expressions rate = c*rate and state = state + inc*rate) */
+int wql_scale_rate_array(double *rate, int64_t n, double coefficient);
+int wql_update_state_array(double *state, const double *rate, int64_t n, double increment);
+
+/* ---- set_rates_elastic seam (elastic.f90:13-29): RHS_Center +
+ *      RHS_near_boundaries incl. Q4/Q8 and PML: DF += L(F) ------------------ */
+int wql_rhs(wql_block *b);
+/* try_traditional_cartesian_rhs seam (traditional_cartesian_rhs_openacc.f90:10-123):
+ * strict-interior only; *handled = 0 when not eligible (then nothing ran) */
+int wql_rhs_interior(wql_block *b, int32_t *handled);
+/* set_moment_tensor (moment_tensor.f90:875-935) at time t */
+int wql_set_moment_tensor(wql_block *b, double t);
+/* impose_boundary_condition (RHS_Interior.f90:1584-1768) */
+int wql_sat(wql_block *b);
+
+/* ---- fused paths --------------------------------------------------------- */
+/* one RK stage: DF = A*DF + L(F) + source(t) - SAT; F += Bdt*DF (F ping-pong),
+ * memory variables and PML auxiliaries likewise */
+int wql_stage_fused(wql_block *b, double A, double Bdt, double t);
+/* time_step_RK (time_step.f90:68-150): nsteps steps of 5 fused stages;
+ * *t advanced as the reference advances D%t; receivers recorded */
+int wql_time_step(wql_block *b, double dt, int32_t nsteps, double *t);
+
+/* ---- host metric pipeline (grid.f90:398-471 derivatives(), metrics.f90) on
+ * the device: X[3] coordinates (node-major, x fastest, no ghosts, device
+ * pointer) -> metricx/y/z[3] and J (same layout, device pointers) -------- */
+int wql_grid_metrics(int32_t nq, int32_t nr, int32_t ns, const double *X,
+                     double *metricx, double *metricy, double *metricz, double *J);
+
+/* ---- diagnostics --------------------------------------------------------- */
+const char *wql_last_error(void);
+/* synchronise the block's stream */
+int wql_block_sync(wql_block *b);
+/* the block's HIP stream (hipStream_t) for event timing by the caller */
+void *wql_block_stream(wql_block *b);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

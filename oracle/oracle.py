@@ -69,6 +69,7 @@ def lib():
         L = C.CDLL(_LIB_PATH)
         L.orc_rk_coefficients.argtypes = [_P, _P, _P]
         L.orc_sbp_closure.argtypes = [_P]
+        L.orc_apply_d1.argtypes = [C.c_int, C.c_double, _P, _P]
         L.orc_metrics.argtypes = [C.c_int, C.c_int, C.c_int, _P, _P, _P, _P, _P]
         L.orc_normals.argtypes = [C.POINTER(_Block)]
         L.orc_q_coefficients.argtypes = [C.c_int, C.c_double, C.c_double, _P, _P]
@@ -104,6 +105,14 @@ def sbp_closure():
     t = np.zeros(54)
     lib().orc_sbp_closure(_ptr(t))
     return t.reshape(6, 9)
+
+
+def apply_d1(u, h):
+    """SBP-6 D1 of the restatement applied to a 1-D array."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    du = np.zeros_like(u)
+    lib().orc_apply_d1(u.size, h, _ptr(u), _ptr(du))
+    return du
 
 
 def metrics(X):

@@ -83,6 +83,17 @@ static int d1_terms(int i, int n, double *c, int *col) {
   }
 }
 
+/* D1 (row terms of JJU_x6 / D6x) applied to a 1-D array u[n]: du = (1/h) D u */
+void orc_apply_d1(int n, double h, const double *u, double *du) {
+  double c[8]; int col[8];
+  for (int i = 1; i <= n; ++i) {
+    int nt = d1_terms(i, n, c, col);
+    double s = 0.0;
+    for (int k = 0; k < nt; ++k) s = (k == 0) ? c[k] * u[col[k] - 1] : s + c[k] * u[col[k] - 1];
+    du[i - 1] = (1.0 / h) * s;
+  }
+}
+
 /* ---------------------------------------------------------------------------
  * RK coefficients, Kennedy-Carpenter (5,4) 2N-storage (time_step.f90:53-62)
  * ------------------------------------------------------------------------- */

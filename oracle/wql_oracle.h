@@ -62,6 +62,9 @@ void orc_rk_coefficients(double *A, double *B, double *C);
 /* SBP-6 first-derivative closure table (JU_xJU_yJU_z6.f90:74-182), 6x9 */
 void orc_sbp_closure(double *table54);
 
+/* the SBP-6 first-derivative rows of JJU_x6 / D6x applied to u[n] */
+void orc_apply_d1(int n, double h, const double *u, double *du);
+
 /* derivatives() (grid.f90:398-471): metrics and J from coordinates X[3][N] */
 void orc_metrics(int nx, int ny, int nz, const double *X,
                  double *metx, double *mety, double *metz, double *J);

@@ -1,0 +1,198 @@
+"""Parity of the HIP path (libwql3d_hip.so through its C ABI) with the oracle.
+
+Tolerances: FP64 paths must match the oracle to relative L2 <= 1e-12 per
+field after N steps (the bar the reference's own cpu-release oracle sets);
+receiver seismograms sample by sample to 1e-12 relative to the trace peak.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from waveqlab3d_q_gpu_amd import _lib as L
+from waveqlab3d_q_gpu_amd import problems as PB
+from waveqlab3d_q_gpu_amd.block import time_step_RK
+
+from helpers import (field_errors, gpu_block, oracle_block, receiver_flat, rel_l2,
+                     small_curvilinear)
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-12
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _runtime():
+    import ctypes
+    dev = ctypes.c_int32()
+    L.check(L.lib().wql_runtime_init(0, ctypes.byref(dev)))
+    yield
+
+
+def _download_all(blk):
+    blk.download(L.WQL_F | L.WQL_DF | L.WQL_MEMORY | L.WQL_PML)
+
+
+@pytest.mark.parametrize("fused", [True, False])
+def test_reference_kat_41(oracle, fused):
+    prob = PB.kat_cartesian_41()
+    met = oracle.metrics(prob.X)
+    orc = oracle_block(oracle, prob, met)
+    orc.time_step(prob.dt, prob.nsteps)
+    blk = gpu_block(prob, met)
+    time_step_RK(blk, prob.dt, prob.nsteps, fused=fused)
+    blk.download(L.WQL_F)
+    got = np.abs(blk.F).max()
+    ref = 1.6623128367649574e+02
+    print(f"KAT max|F| gpu={got!r} ref={ref!r}", flush=True)
+    assert abs(got - ref) / ref < 1e-13
+    errs = field_errors(blk.F, orc.F)
+    assert max(errs) <= TOL64, errs
+
+
+def test_cartesian_bump_fields_and_seismograms(oracle):
+    prob = PB.cartesian_bump(n=33, nsteps=40, nrec=8)
+    met = oracle.metrics(prob.X)
+    orc = oracle_block(oracle, prob, met)
+    _, seis_o = orc.time_step(prob.dt, prob.nsteps, receivers=receiver_flat(prob))
+    blk = gpu_block(prob, met)
+    blk.set_receivers(prob.receivers, prob.nsteps)
+    time_step_RK(blk, prob.dt, prob.nsteps)
+    _download_all(blk)
+    errs = field_errors(blk.F, orc.F)
+    print("field rel L2:", ["%.2e" % e for e in errs], flush=True)
+    assert max(errs) <= TOL64
+    assert rel_l2(blk.DF, orc.DF) <= TOL64
+    seis_g = blk.seismograms()
+    for r in range(prob.receivers.shape[0]):
+        for c in range(9):
+            ref = seis_o[:, r, c]
+            peak = np.abs(ref).max()
+            assert np.abs(seis_g[:, r, c] - ref).max() <= TOL64 * max(peak, 1e-30), (r, c)
+
+
+@pytest.mark.parametrize("nmech,fused", [(8, True), (8, False), (4, True), (0, True)])
+def test_curvilinear_q_pml(oracle, nmech, fused):
+    prob = small_curvilinear(nmech=nmech)
+    met = oracle.metrics(prob.X)
+    assert met[3].min() > 0
+    orc = oracle_block(oracle, prob, met)
+    _, seis_o = orc.time_step(prob.dt, prob.nsteps, receivers=receiver_flat(prob))
+    blk = gpu_block(prob, met)
+    blk.set_receivers(prob.receivers, prob.nsteps)
+    time_step_RK(blk, prob.dt, prob.nsteps, fused=fused)
+    _download_all(blk)
+    errs = field_errors(blk.F, orc.F)
+    print(f"nmech={nmech} fused={fused} field rel L2:", ["%.2e" % e for e in errs], flush=True)
+    assert max(errs) <= TOL64
+    if nmech:
+        assert rel_l2(blk.eta, orc.eta) <= TOL64
+        assert rel_l2(blk.Deta, orc.Deta) <= TOL64
+    for f in range(6):
+        if prob.pml[f]:
+            lo, hi = PB.pml_slab_box(f, prob.n, prob.npml[f])
+            sl = (slice(None), slice(lo[2] - 1, hi[2]), slice(lo[1] - 1, hi[1]),
+                  slice(lo[0] - 1, hi[0]))
+            assert rel_l2(blk.pml_Q[f], orc.pQ[f][sl]) <= TOL64, f
+    if fused:
+        seis_g = blk.seismograms()
+        assert rel_l2(seis_g, seis_o) <= TOL64
+
+
+def test_stage_level_rates(oracle):
+    """One stage of the reference order: DF after scale+rhs+source+sat."""
+    prob = small_curvilinear(nmech=8, nsteps=1)
+    met = oracle.metrics(prob.X)
+    rng = np.random.default_rng(1)
+    F0 = rng.uniform(-1, 1, (9,) + prob.X.shape[1:])
+    F0[3:] *= 1e6
+    prob.F0 = F0
+    orc = oracle_block(oracle, prob, met)
+    orc.eta[...] = rng.uniform(-1, 1, orc.eta.shape)
+    for f in range(6):
+        if orc.pQ[f] is not None:
+            orc.pQ[f][...] = rng.uniform(-1, 1, orc.pQ[f].shape)
+    blk = gpu_block(prob, met)
+    blk.eta[...] = orc.eta
+    for f in range(6):
+        if prob.pml[f]:
+            lo, hi = PB.pml_slab_box(f, prob.n, prob.npml[f])
+            blk.pml_Q[f][...] = orc.pQ[f][:, lo[2] - 1:hi[2], lo[1] - 1:hi[1], lo[0] - 1:hi[0]]
+    blk.upload(L.WQL_MEMORY | L.WQL_PML)
+    t = 0.13
+    orc.scale_rates(0.0); orc.set_rates(); orc.set_moment_tensor(t); orc.impose_boundary_condition()
+    blk.scale_rates(0.0); blk.set_rates(); blk.set_moment_tensor(t); blk.impose_boundary_condition()
+    _download_all(blk)
+    assert rel_l2(blk.DF, orc.DF) <= TOL64
+    assert rel_l2(blk.Deta, orc.Deta) <= TOL64
+
+
+def test_interior_rhs_seam(oracle):
+    """try_traditional_cartesian_rhs seam: strict interior only."""
+    prob = PB.cartesian_bump(n=28, nsteps=1)
+    met = oracle.metrics(prob.X)
+    orc = oracle_block(oracle, prob, met)
+    orc.DF[...] = 0.0
+    orc.set_rates()
+    blk = gpu_block(prob, met)
+    blk.scale_rates(0.0)
+    assert blk.try_interior_rhs()
+    blk.download(L.WQL_DF)
+    core = (slice(None), slice(6, -6), slice(6, -6), slice(6, -6))
+    assert rel_l2(blk.DF[core], orc.DF[core]) <= TOL64
+    shell = np.ones(blk.DF.shape[1:], bool)
+    shell[6:-6, 6:-6, 6:-6] = False
+    assert np.all(blk.DF[:, shell] == 0.0)
+
+
+def test_rk_vector_kernels_bit_exact():
+    """tests/rk_vector_kernel_test.f90:6-35: asymmetric bounds
+    (-2:4, 3:7, -1:1, 4); the GPU result must equal the CPU expression."""
+    import torch
+    i = np.arange(-2, 5)[:, None, None, None]
+    j = np.arange(3, 8)[None, :, None, None]
+    k = np.arange(-1, 2)[None, None, :, None]
+    c = np.arange(1, 5)[None, None, None, :]
+    state = ((11 * i - 7 * j + 5 * k + 3 * c) / 13.0).astype(np.float64)
+    rate = ((-2 * i + 9 * j - 4 * k + 7 * c) / 17.0).astype(np.float64)
+    state, rate = np.broadcast_arrays(state, rate)
+    state = np.asfortranarray(state); rate = np.asfortranarray(rate)
+    exp_rate = -0.375 * rate
+    exp_state = state + 0.125 * exp_rate
+    from waveqlab3d_q_gpu_amd import scale_rate_array, update_state_array
+    r = torch.from_numpy(rate.ravel(order="F").copy()).cuda()
+    s = torch.from_numpy(state.ravel(order="F").copy()).cuda()
+    scale_rate_array(r, -0.375)
+    update_state_array(s, r, 0.125)
+    assert np.array_equal(r.cpu().numpy(), exp_rate.ravel(order="F"))
+    assert np.array_equal(s.cpu().numpy(), exp_state.ravel(order="F"))
+
+
+def test_grid_metrics_kernel(oracle):
+    import torch
+    from waveqlab3d_q_gpu_amd import grid_metrics
+    prob = small_curvilinear(n=26)
+    ref = oracle.metrics(prob.X)
+    got = grid_metrics(torch.from_numpy(prob.X).cuda())
+    for g, r in zip(got, ref):
+        assert rel_l2(g.cpu().numpy(), r) < 1e-14
+
+
+def test_fp32_mixed_error_budget(oracle):
+    """FP32-mixed (FP64 RK accumulation of F/DF): stated error against the
+    FP64 oracle, per field and per receiver."""
+    prob = small_curvilinear(nmech=8, nsteps=30)
+    met = oracle.metrics(prob.X)
+    orc = oracle_block(oracle, prob, met)
+    _, seis_o = orc.time_step(prob.dt, prob.nsteps, receivers=receiver_flat(prob))
+    blk = gpu_block(prob, met, precision=L.WQL_FP32_MIXED)
+    blk.set_receivers(prob.receivers, prob.nsteps)
+    time_step_RK(blk, prob.dt, prob.nsteps)
+    _download_all(blk)
+    errs = field_errors(blk.F, orc.F)
+    seis_g = blk.seismograms()
+    serr = [rel_l2(seis_g[:, r], seis_o[:, r]) for r in range(seis_o.shape[1])]
+    print("fp32-mixed field rel L2:", ["%.2e" % e for e in errs], flush=True)
+    print("fp32-mixed seismogram rel L2:", ["%.2e" % e for e in serr], flush=True)
+    assert max(errs) <= 1e-4
+    assert max(serr) <= 1e-5

@@ -1,0 +1,320 @@
+"""Pins of the oracle's time step against the reference's known-answer value
+and against closed forms / invariants of the discretisation.
+
+  * KAT: inputfile/test_elastic_traditional_o6.in (41^3 Cartesian, traditional
+    order 6, characteristic SAT on all faces, Cos-Bell moment source order 4,
+    CFL 0.2, t_final 0.02): the reference prints max|field| =
+    1.6623128367649574E+02 (PHASE5_TRADITIONAL_CARTESIAN_RHS.md:35).
+  * set_rates on cubic fields is exact (Cartesian with a different spacing
+    per axis; curvilinear with a non-diagonal affine map): pins component
+    wiring, Hooke's law, divergence, 1/(rho J) and the closure rows.
+  * SAT: the penalty vanishes when the boundary condition holds (traction
+    free for r=1, zero incoming characteristic for r=0), and the discrete
+    energy never grows with free-surface + characteristic SATs.
+  * Q4/Q8: uniform strain rate has the closed-form relaxation solution.
+  * PML: the damping profile including the z-low |z - npml| quirk.
+  * RK: 4th-order convergence of the KC(5,4) scheme on the relaxation ODE.
+  * singular_source: the discrete delta has unit mass and zero first moment.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from waveqlab3d_q_gpu_amd import problems as PB
+
+
+def oracle_block(O, prob, metrics=None):
+    nq, nr, ns = prob.n
+    b = O.OracleBlock(nq, nr, ns, curvilinear=prob.mesh)
+    b.metx, b.mety, b.metz, b.J = metrics if metrics is not None else O.metrics(prob.X)
+    b.M[...] = prob.M
+    b.bc = list(prob.bc)
+    for f in range(6):
+        if prob.pml[f]:
+            b.set_pml(f, prob.npml[f])
+    if prob.nmech:
+        b.set_anelastic(prob.nmech, prob.Qs_inv, prob.Qp_inv, correct_modulus=False)
+    b.normals()
+    for s in prob.sources:
+        b.add_source(kind=s.kind, t_init=s.t_init, duration=s.duration, mxx=s.mxx, myy=s.myy,
+                     mzz=s.mzz, mxy=s.mxy, mxz=s.mxz, myz=s.myz, loc=s.location, near=s.near,
+                     near_phys=s.near_phys, order=s.order, alpha=s.alpha)
+    if prob.F0 is not None:
+        b.F[...] = prob.F0
+    return b
+
+
+def test_reference_kat_41(oracle):
+    prob = PB.kat_cartesian_41()
+    assert prob.nsteps == 6
+    b = oracle_block(oracle, prob)
+    b.time_step(prob.dt, prob.nsteps)
+    got = np.abs(b.F).max()
+    ref = 1.6623128367649574e+02
+    assert abs(got - ref) / ref < 1e-14, (repr(got), ref)
+
+
+def _cubic_fields(X, rng):
+    """Random cubic polynomials in physical coordinates for 9 components and
+    their exact gradients."""
+    F = np.zeros((9,) + X.shape[1:])
+    G = np.zeros((9, 3) + X.shape[1:])
+    x, y, z = X[0], X[1], X[2]
+    for c in range(9):
+        a = rng.uniform(-1, 1, 10)
+        F[c] = (a[0] + a[1] * x + a[2] * y + a[3] * z + a[4] * x * y + a[5] * y * z
+                + a[6] * x ** 2 + a[7] * z ** 3 + a[8] * x * y * z + a[9] * y ** 2 * x)
+        G[c, 0] = a[1] + a[4] * y + 2 * a[6] * x + a[8] * y * z + a[9] * y ** 2
+        G[c, 1] = a[2] + a[4] * x + a[5] * z + a[8] * x * z + 2 * a[9] * y * x
+        G[c, 2] = a[3] + a[5] * y + 3 * a[7] * z ** 2 + a[8] * x * y
+    return F, G
+
+
+def _exact_rates(F, G, M):
+    lam, mu, rho = M
+    dv = G[:3]          # dv[c, d] = d v_c / d x_d
+    ds = G[3:]
+    R = np.zeros_like(F)
+    R[0] = (ds[0, 0] + ds[3, 1] + ds[4, 2]) / rho
+    R[1] = (ds[3, 0] + ds[1, 1] + ds[5, 2]) / rho
+    R[2] = (ds[4, 0] + ds[5, 1] + ds[2, 2]) / rho
+    tr = dv[0, 0] + dv[1, 1] + dv[2, 2]
+    R[3] = lam * tr + 2 * mu * dv[0, 0]
+    R[4] = lam * tr + 2 * mu * dv[1, 1]
+    R[5] = lam * tr + 2 * mu * dv[2, 2]
+    R[6] = mu * (dv[0, 1] + dv[1, 0])
+    R[7] = mu * (dv[0, 2] + dv[2, 0])
+    R[8] = mu * (dv[1, 2] + dv[2, 1])
+    return R
+
+
+@pytest.mark.parametrize("mesh", ["cartesian", "curvilinear"])
+def test_set_rates_exact_on_cubics(oracle, mesh):
+    rng = np.random.default_rng(11)
+    n = (16, 14, 15)
+    if mesh == "cartesian":
+        X = PB.cartesian_grid(n, (-1.0, 0.0, 2.0), (1.5, 1.3, 3.4))
+    else:
+        A = np.array([[1.2, 0.3, -0.1], [0.2, 0.9, 0.25], [-0.15, 0.1, 1.1]])
+        Xc = PB.cartesian_grid(n, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
+        X = np.einsum("ij,jzyx->izyx", A, Xc) + np.array([0.5, -0.3, 0.2])[:, None, None, None]
+    M = np.empty((3,) + X.shape[1:])
+    M[0] = 2.0 + 0.1 * X[0]
+    M[1] = 1.5 + 0.05 * X[1]
+    M[2] = 2.5 - 0.1 * X[2]
+    prob = PB.Problem(n=n, mesh=0 if mesh == "cartesian" else 1, X=X, M=M, bc=(0,) * 6,
+                      dt=0.0, nsteps=0)
+    b = oracle_block(oracle, prob)
+    F, G = _cubic_fields(X, rng)
+    b.F[...] = F
+    b.DF[...] = 0.0
+    b.set_rates()
+    R = _exact_rates(F, G, M)
+    err = np.abs(b.DF - R).max() / np.abs(R).max()
+    assert err < 1e-11, err
+
+
+def _cartesian_problem(n=18, bc=(1, 1, 2, 1, 1, 1)):
+    nn = (n, n, n)
+    X = PB.cartesian_grid(nn, (0.0,) * 3, (1.0,) * 3)
+    M = PB.material_from_rho_s_p(nn, 2.0, 1.0, 2.0)
+    return PB.Problem(n=nn, mesh=0, X=X, M=M, bc=bc, dt=0.0, nsteps=0)
+
+
+def test_sat_vanishes_when_bc_holds(oracle):
+    rng = np.random.default_rng(5)
+    # free surface on every face: traction-free boundary -> zero penalty
+    prob = _cartesian_problem(bc=(2,) * 6)
+    b = oracle_block(oracle, prob)
+    F = rng.uniform(-1, 1, b.F.shape)
+    b.F[...] = F
+    b.DF[...] = 0.0
+    b.impose_boundary_condition()
+    scale = np.abs(b.DF).max()          # penalty of an unconstrained state
+    assert scale > 1.0
+    # zero normal tractions on each face: x-faces sxx=sxy=sxz=0, etc.
+    F[3][:, :, [0, -1]] = 0; F[6][:, :, [0, -1]] = 0; F[7][:, :, [0, -1]] = 0
+    F[4][:, [0, -1], :] = 0; F[6][:, [0, -1], :] = 0; F[8][:, [0, -1], :] = 0
+    F[5][[0, -1], :, :] = 0; F[7][[0, -1], :, :] = 0; F[8][[0, -1], :, :] = 0
+    b.F[...] = F
+    b.DF[...] = 0.0
+    b.impose_boundary_condition()
+    assert np.abs(b.DF).max() < 1e-14 * scale
+    # characteristic on x-low: incoming characteristic Z v - T = 0 (BCm sign)
+    prob = _cartesian_problem(bc=(1, 0, 0, 0, 0, 0))
+    b = oracle_block(oracle, prob)
+    lam, mu, rho = prob.M[:, 0, 0, 0]
+    Zp, Zs = rho * math.sqrt((lam + 2 * mu) / rho), rho * math.sqrt(mu / rho)
+    F = rng.uniform(-1, 1, b.F.shape)
+    # n = (1,0,0): T = (sxx, sxy, sxz); BCm: p = 0.5(Z v - T) = 0  <=> T = Z v
+    F[3][:, :, 0] = Zp * F[0][:, :, 0]
+    F[6][:, :, 0] = Zs * F[1][:, :, 0]
+    F[7][:, :, 0] = Zs * F[2][:, :, 0]
+    b.F[...] = F
+    b.DF[...] = 0.0
+    b.impose_boundary_condition()
+    assert np.abs(b.DF).max() < 1e-14 * scale
+    # and a state that violates it is penalised
+    b.F[3][:, :, 0] += 1.0
+    b.impose_boundary_condition()
+    assert np.abs(b.DF[:, :, :, 0]).max() > 1e-3
+
+
+def _energy(O, b, H):
+    lam, mu, rho = b.M
+    v2 = b.F[0] ** 2 + b.F[1] ** 2 + b.F[2] ** 2
+    sxx, syy, szz, sxy, sxz, syz = b.F[3:]
+    # complementary energy sigma : C^-1 : sigma for isotropic C
+    tr = sxx + syy + szz
+    comp = (sxx ** 2 + syy ** 2 + szz ** 2 + 2 * (sxy ** 2 + sxz ** 2 + syz ** 2)
+            - lam / (3 * lam + 2 * mu) * tr ** 2) / (2 * mu)
+    w = H[2][:, None, None] * H[1][None, :, None] * H[0][None, None, :]
+    return 0.5 * np.sum(w * b.J * (rho * v2 + comp))
+
+
+def _norm_weights(n):
+    h = np.ones(n)
+    w = [0.315949074074074, 1.390393518518519, 0.627546296296296, 1.240509259259259,
+         0.911689814814815, 1.013912037037037]
+    # exact values 13649/43200, 12013/8640, 2711/4320, 5359/4320, 7877/8640, 43801/43200
+    w = [13649 / 43200, 12013 / 8640, 2711 / 4320, 5359 / 4320, 7877 / 8640, 43801 / 43200]
+    h[:6] = w
+    h[-6:] = w[::-1]
+    return h / (n - 1)
+
+
+def test_energy_never_grows(oracle):
+    rng = np.random.default_rng(9)
+    prob = _cartesian_problem(n=16, bc=(1, 1, 2, 1, 2, 1))
+    b = oracle_block(oracle, prob)
+    b.F[...] = rng.uniform(-1, 1, b.F.shape) * 1e-2
+    H = [_norm_weights(k) for k in prob.n]
+    dt = 0.1 / 15 / math.sqrt(1 + 4)
+    e0 = _energy(oracle, b, H)
+    es = [e0]
+    t = 0.0
+    for _ in range(20):
+        t, _ = b.time_step(dt, 1, t)
+        es.append(_energy(oracle, b, H))
+    es = np.array(es)
+    assert np.all(np.diff(es) <= 1e-12 * e0), np.diff(es).max()
+    assert es[-1] < 0.99 * es[0]     # characteristic faces absorb
+
+
+def _uniform_strain_problem(nmech, n=14):
+    nn = (n, n, n)
+    X = PB.cartesian_grid(nn, (0.0,) * 3, (1.0,) * 3)
+    M = PB.material_from_rho_s_p(nn, 2.0, 1.0, 2.0)
+    tau, w = PB.q_coefficients(nmech)
+    Qs = np.full(X.shape[1:], 1 / 30.0)
+    Qp = np.full(X.shape[1:], 1 / 60.0)
+    return PB.Problem(n=nn, mesh=0, X=X, M=M, bc=(0,) * 6, dt=0.0, nsteps=0, nmech=nmech,
+                      tau=tau, weight=w, Qs_inv=Qs, Qp_inv=Qp)
+
+
+def _relaxation_exact(prob, grad, t):
+    """Closed form for spatially uniform, constant velocity gradient."""
+    lam, mu = prob.M[0].flat[0], prob.M[1].flat[0]
+    Qs, Qp = prob.Qs_inv.flat[0], prob.Qp_inv.flat[0]
+    e = np.array([grad[0, 0], grad[1, 1], grad[2, 2], grad[0, 1] + grad[1, 0],
+                  grad[0, 2] + grad[2, 0], grad[1, 2] + grad[2, 1]])
+    tr = e[:3].sum()
+    eta = np.zeros((6, prob.nmech))
+    sig = np.zeros(6)
+    hooke = np.array([lam * tr + 2 * mu * e[0], lam * tr + 2 * mu * e[1],
+                      lam * tr + 2 * mu * e[2], mu * e[3], mu * e[4], mu * e[5]])
+    sig[:] = hooke * t
+    for i in range(prob.nmech):
+        w, ta = prob.weight[i], prob.tau[i]
+        inf = np.array([w * 2 * mu * Qs * e[c] + w * ((lam + 2 * mu) * Qp - 2 * mu * Qs) * tr
+                        for c in range(3)] + [w * mu * Qs * e[c] for c in range(3, 6)])
+        eta[:, i] = inf * (1 - math.exp(-t / ta))
+        sig -= inf * (t - ta * (1 - math.exp(-t / ta)))
+    return eta, sig
+
+
+@pytest.mark.parametrize("nmech", [4, 8])
+def test_q_relaxation_closed_form(oracle, nmech):
+    prob = _uniform_strain_problem(nmech)
+    b = oracle_block(oracle, prob)
+    grad = np.array([[0.3, -0.2, 0.1], [0.05, -0.4, 0.2], [0.15, 0.07, 0.25]])
+    X = prob.X
+    for c in range(3):
+        b.F[c] = grad[c, 0] * X[0] + grad[c, 1] * X[1] + grad[c, 2] * X[2]
+    dt, nsteps = 2e-4, 200
+    t, _ = b.time_step(dt, nsteps)
+    eta_ex, sig_ex = _relaxation_exact(prob, grad, t)
+    for c6 in range(6):
+        for i in range(nmech):
+            got = b.eta[c6, i]
+            assert np.allclose(got, eta_ex[c6, i], rtol=1e-9, atol=1e-13), (c6, i)
+        assert np.allclose(b.F[3 + c6], sig_ex[c6], rtol=1e-9, atol=1e-13), c6
+
+
+def test_rk_fourth_order(oracle):
+    prob = _uniform_strain_problem(8, n=12)
+    grad = np.array([[0.3, 0.0, 0.0], [0.0, -0.4, 0.0], [0.0, 0.0, 0.25]])
+    errs = []
+    T = 0.05
+    for nsteps in (10, 20, 40):
+        b = oracle_block(oracle, prob)
+        for c in range(3):
+            b.F[c] = grad[c, 0] * prob.X[0] + grad[c, 1] * prob.X[1] + grad[c, 2] * prob.X[2]
+        t, _ = b.time_step(T / nsteps, nsteps)
+        eta_ex, _ = _relaxation_exact(prob, grad, t)
+        errs.append(abs(b.eta[0, 0].flat[0] - eta_ex[0, 0]))
+    rates = [math.log2(errs[k] / errs[k + 1]) for k in range(2)]
+    assert all(3.7 < r < 4.5 for r in rates), rates
+
+
+def test_pml_damping_profile(oracle):
+    n = (20, 21, 22)
+    X = PB.cartesian_grid(n, (0.0,) * 3, (1.0,) * 3)
+    M = PB.material_from_rho_s_p(n, 2.0, 1.0, 2.0)
+    npml = 7
+    prob = PB.Problem(n=n, mesh=0, X=X, M=M, bc=(0,) * 6, dt=0.0, nsteps=0,
+                      pml=(1,) * 6, npml=(npml,) * 6)
+    b = oracle_block(oracle, prob)
+    d0 = 3.0 * math.log(1000.0)
+    for f in range(6):
+        b.pQ[f][...] = 1.0
+        b.pDQ[f][...] = 0.0
+    b.DF[...] = 0.0
+    b.set_rates()
+    idx = [np.arange(1, k + 1) for k in n]
+    expect = [
+        d0 * np.abs(idx[0] - (1 + npml)) / npml, d0 * np.abs(idx[0] - (n[0] - npml)) / npml,
+        d0 * np.abs(idx[1] - (1 + npml)) / npml, d0 * np.abs(idx[1] - (n[1] - npml)) / npml,
+        d0 * np.abs(idx[2] - npml) / npml,        # z-low quirk (RHS_Interior.f90:1919)
+        d0 * np.abs(idx[2] - (n[2] - npml)) / npml]
+    for f in range(6):
+        ax = f // 2
+        sl = [0, 0, 0]
+        sl[2 - ax] = slice(None)
+        # mid-slab line through the block centre of the other two axes
+        line = [n[2] // 2, n[1] // 2, n[0] // 2]
+        line[2 - ax] = slice(None)
+        got = -b.pDQ[f][(8,) + tuple(line)] - 1.0
+        inslab = (idx[ax] <= npml) if f % 2 == 0 else (idx[ax] >= n[ax] - npml + 1)
+        assert np.allclose(got[inslab], expect[f][inslab], rtol=1e-14, atol=1e-14), f
+
+
+def test_singular_source_moments(oracle):
+    n = (21, 21, 21)
+    X = PB.cartesian_grid(n, (0.0,) * 3, (2.0,) * 3)
+    M = PB.material_from_rho_s_p(n, 1.0, 1.0, 2.0)
+    h = 0.1
+    for order, loc in ((4, (1.0, 1.0, 1.0)), (2, (1.03, 0.98, 1.01))):
+        src = PB.point_source(X, loc, oracle.SRC_GAUSSIAN, t_init=0.0, duration=1.0,
+                              order=order, mxz=2.0)
+        prob = PB.Problem(n=n, mesh=0, X=X, M=M, bc=(0,) * 6, dt=0, nsteps=0, sources=[src])
+        b = oracle_block(oracle, prob)
+        b.DF[...] = 0.0
+        b.set_moment_tensor(0.0)
+        A = 1.0 / math.sqrt(2 * math.pi)
+        g = b.DF[7] / (-A * 2.0)
+        assert abs(g.sum() * h ** 3 - 1.0) < 1e-12
+        for ax in range(3):
+            first = (g * (X[ax] - loc[ax])).sum() * h ** 3
+            assert abs(first) < 1e-12, (order, ax, first)

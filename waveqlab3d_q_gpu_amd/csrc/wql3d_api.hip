@@ -1,0 +1,758 @@
+// C-ABI layer of libwql3d_hip.so: runtime, block lifecycle (device mirrors),
+// upload/download, and the stage orchestration of time_step_RK
+// (time_step.f90:68-150).
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <vector>
+
+#include "wql3d_internal.h"
+
+using namespace wql;
+
+namespace {
+
+thread_local char g_err[512] = "";
+
+int fail(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                        \
+  do {                                                                                       \
+    hipError_t e_ = (expr);                                                                  \
+    if (e_ != hipSuccess) return fail(WQL_EHIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                                      __FILE__, __LINE__);                                   \
+  } while (0)
+
+// Kennedy-Carpenter (5,4) 2N-storage coefficients (time_step.f90:53-62)
+const double kRkA[5] = {0.0, -567301805773.0 / 1357537059087.0,
+                        -2404267990393.0 / 2016746695238.0, -3550918686646.0 / 2091501179385.0,
+                        -1275806237668.0 / 842570457699.0};
+const double kRkB[5] = {1432997174477.0 / 9575080441755.0, 5161836677717.0 / 13612068292357.0,
+                        1720146321549.0 / 2090206949498.0, 3134564353537.0 / 4481467310338.0,
+                        2277821191437.0 / 14882151754819.0};
+const double kRkC[6] = {0.0,
+                        1432997174477.0 / 9575080441755.0,
+                        2526269341429.0 / 6820363962896.0,
+                        2006345519317.0 / 3224310063776.0,
+                        2802321613138.0 / 2924317926251.0,
+                        1.0};
+
+long long box_extent(const wql_box& b, int d) { return (long long)b.hi[d] - b.lo[d] + 1; }
+
+}  // namespace
+
+struct wql_block {
+  wql_block_desc d;
+  int nx, ny, nz;
+  long long P;
+  size_t esz;                // sizeof(T) of geometry/material/memory arrays
+  int cur;                   // current F buffer
+  double* F[2];
+  double* DF;
+  void* G;
+  void* M;
+  void* eta;
+  void* Deta;
+  void* Qs;
+  void* Qp;
+  void* pQ[6];
+  void* pDQ[6];
+  long long Ps[6];           // PML slab plane sizes
+  int slab_lo[6][3];         // slab node origin (0-based)
+  int slab_n[6][3];          // slab extents
+  void* face[6];
+  long long NF[6];
+  hipStream_t stream;
+  int nsrc;
+  wql_source src[kMaxSources];
+  int nrec;
+  int nslots;
+  long long* rec_idx;
+  double* seis;
+  int slot;
+  long long bytes;
+};
+
+static int g_device = -1;
+
+extern "C" const char* wql_last_error(void) { return g_err; }
+
+extern "C" int wql_runtime_init(int32_t local_rank, int32_t* device) {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n <= 0) return fail(WQL_ENODEV, "no GPU visible (hipGetDeviceCount: %s)",
+                                              hipGetErrorString(e));
+  const int dev = (local_rank < 0 ? 0 : local_rank) % n;
+  HIP_TRY(hipSetDevice(dev));
+  g_device = dev;
+  if (device) *device = dev;
+  return WQL_OK;
+}
+
+extern "C" int wql_runtime_finalize(void) {
+  if (g_device >= 0) HIP_TRY(hipDeviceSynchronize());
+  g_device = -1;
+  return WQL_OK;
+}
+
+extern "C" int wql_capacity_check(int64_t predicted) {
+  size_t freeb = 0, total = 0;
+  const char* env = getenv("WQL3D_GPU_MEMORY_BYTES");
+  if (env && *env) {
+    freeb = (size_t)strtoull(env, nullptr, 10);
+  } else {
+    HIP_TRY(hipMemGetInfo(&freeb, &total));
+  }
+  if (predicted < 0 || (uint64_t)predicted > (uint64_t)freeb)
+    return fail(WQL_ENOMEM, "RUN-GPU-MEM-002: predicted %lld bytes exceed device capacity %llu",
+                (long long)predicted, (unsigned long long)freeb);
+  return WQL_OK;
+}
+
+// ---------------------------------------------------------------------------
+static long long face_size(const wql_block* b, int f) {
+  const int axis = f >> 1;
+  return axis == 0 ? (long long)b->ny * b->nz : (axis == 1 ? (long long)b->nx * b->nz
+                                                           : (long long)b->nx * b->ny);
+}
+
+static int validate(const wql_block_desc* d) {
+  if (!d) return fail(WQL_EINVAL, "null descriptor");
+  const int n[3] = {d->nq, d->nr, d->ns};
+  for (int k = 0; k < 3; ++k)
+    if (n[k] < 12) return fail(WQL_EINVAL, "axis %d has %d nodes; SBP-6 needs >= 12", k, n[k]);
+  if (d->mesh != WQL_CARTESIAN && d->mesh != WQL_CURVILINEAR)
+    return fail(WQL_EINVAL, "invalid mesh %d", d->mesh);
+  if (d->precision != WQL_FP64 && d->precision != WQL_FP32_MIXED)
+    return fail(WQL_EINVAL, "invalid precision %d", d->precision);
+  if (d->nmech != 0 && d->nmech != 4 && d->nmech != 8)
+    return fail(WQL_EINVAL, "nmech must be 0, 4 or 8 (got %d)", d->nmech);
+  for (int f = 0; f < 6; ++f) {
+    if (d->bc[f] > 3) return fail(WQL_EINVAL, "invalid bc code %d on face %d", d->bc[f], f);
+    if (d->pml[f]) {
+      if (d->npml[f] < 1 || d->npml[f] > n[f >> 1])
+        return fail(WQL_EINVAL, "invalid npml %d on face %d", d->npml[f], f);
+    }
+  }
+  for (int a = 0; a < 3; ++a) {
+    const int lo = d->pml[2 * a] ? d->npml[2 * a] : 0, hi = d->pml[2 * a + 1] ? d->npml[2 * a + 1] : 0;
+    if (lo + hi > n[a]) return fail(WQL_EINVAL, "PML slabs overlap on axis %d", a);
+  }
+  if (d->nmech)
+    for (int i = 0; i < d->nmech; ++i)
+      if (!(d->tau[i] > 0.0)) return fail(WQL_EINVAL, "tau[%d] must be positive", i);
+  if (d->hq <= 0 || d->hr <= 0 || d->hs <= 0) return fail(WQL_EINVAL, "h must be positive");
+  return WQL_OK;
+}
+
+static int dalloc(wql_block* b, void** p, size_t bytes) {
+  if (bytes == 0) { *p = nullptr; return WQL_OK; }
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess)
+    return fail(WQL_ENOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+  b->bytes += (long long)bytes;
+  return WQL_OK;
+}
+
+extern "C" void wql_block_destroy(wql_block* b) {
+  if (!b) return;
+  (void)hipStreamSynchronize(b->stream);
+  void* ptrs[] = {b->F[0], b->F[1], b->DF, b->G, b->M, b->eta, b->Deta, b->Qs, b->Qp,
+                  b->rec_idx, b->seis};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  for (int f = 0; f < 6; ++f) {
+    if (b->pQ[f]) (void)hipFree(b->pQ[f]);
+    if (b->pDQ[f]) (void)hipFree(b->pDQ[f]);
+    if (b->face[f]) (void)hipFree(b->face[f]);
+  }
+  (void)hipStreamDestroy(b->stream);
+  free(b);
+}
+
+extern "C" int wql_block_create(const wql_block_desc* d, wql_block** out) {
+  if (!out) return fail(WQL_EINVAL, "null output pointer");
+  *out = nullptr;
+  int rc = validate(d);
+  if (rc) return rc;
+  wql_block* b = (wql_block*)calloc(1, sizeof(wql_block));
+  if (!b) return fail(WQL_ENOMEM, "host allocation failed");
+  b->d = *d;
+  b->nx = d->nq; b->ny = d->nr; b->nz = d->ns;
+  b->P = (long long)b->nx * b->ny * b->nz;
+  b->esz = d->precision == WQL_FP64 ? sizeof(double) : sizeof(float);
+  hipError_t e = hipStreamCreate(&b->stream);  // default priority
+  if (e != hipSuccess) { free(b); return fail(WQL_EHIP, "hipStreamCreate: %s", hipGetErrorString(e)); }
+  const long long P = b->P;
+  const int ngeo = d->mesh == WQL_CURVILINEAR ? 10 : 3;
+#define ALLOC(ptr, bytes)                                  \
+  do {                                                     \
+    int r_ = dalloc(b, (void**)&(ptr), (size_t)(bytes));   \
+    if (r_) { wql_block_destroy(b); return r_; }           \
+  } while (0)
+  ALLOC(b->F[0], 9 * P * sizeof(double));
+  ALLOC(b->F[1], 9 * P * sizeof(double));
+  ALLOC(b->DF, 9 * P * sizeof(double));
+  ALLOC(b->G, ngeo * P * b->esz);
+  ALLOC(b->M, 3 * P * b->esz);
+  if (d->nmech) {
+    ALLOC(b->eta, 6LL * d->nmech * P * b->esz);
+    ALLOC(b->Deta, 6LL * d->nmech * P * b->esz);
+    ALLOC(b->Qs, P * b->esz);
+    ALLOC(b->Qp, P * b->esz);
+  }
+  const int n[3] = {b->nx, b->ny, b->nz};
+  for (int f = 0; f < 6; ++f) {
+    if (d->pml[f]) {
+      const int ax = f >> 1;
+      for (int k = 0; k < 3; ++k) { b->slab_lo[f][k] = 0; b->slab_n[f][k] = n[k]; }
+      b->slab_n[f][ax] = d->npml[f];
+      b->slab_lo[f][ax] = (f & 1) ? n[ax] - d->npml[f] : 0;
+      b->Ps[f] = (long long)b->slab_n[f][0] * b->slab_n[f][1] * b->slab_n[f][2];
+      ALLOC(b->pQ[f], 9 * b->Ps[f] * b->esz);
+      ALLOC(b->pDQ[f], 9 * b->Ps[f] * b->esz);
+    }
+    if (d->bc[f] > 0) {
+      b->NF[f] = face_size(b, f);
+      ALLOC(b->face[f], 10 * b->NF[f] * b->esz);
+    }
+  }
+#undef ALLOC
+  *out = b;
+  return wql_block_init_state(b);
+}
+
+extern "C" int64_t wql_block_device_bytes(const wql_block* b) { return b ? b->bytes : 0; }
+
+extern "C" int wql_block_sync(wql_block* b) {
+  if (!b) return fail(WQL_EINVAL, "null block");
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  return WQL_OK;
+}
+
+extern "C" void* wql_block_stream(wql_block* b) { return b ? (void*)b->stream : nullptr; }
+
+// ---------------------------------------------------------------------------
+// upload / download of one 3-D component between a Fortran-bounded double
+// array and a dense device plane of T.  (x0,y0,z0): 0-based origin of the
+// node range inside the block, (mx,my,mz): its extents.
+// ---------------------------------------------------------------------------
+struct View {
+  const double* base;  // element (lo0, lo1, lo2) of component 0
+  wql_box box;
+  long long sx, sy, sz, sc;
+};
+
+static View make_view(const double* base, const wql_box& box) {
+  View v;
+  v.base = base;
+  v.box = box;
+  v.sx = 1;
+  v.sy = box_extent(box, 0);
+  v.sz = v.sy * box_extent(box, 1);
+  v.sc = v.sz * box_extent(box, 2);
+  return v;
+}
+
+// offset of 1-based node (x1,y1,z1) component c in the view
+static long long view_off(const View& v, long long x1, long long y1, long long z1, int c) {
+  return (x1 - v.box.lo[0]) * v.sx + (y1 - v.box.lo[1]) * v.sy + (z1 - v.box.lo[2]) * v.sz +
+         (long long)c * v.sc;
+}
+
+static int check_view(const View& v, const int lo1[3], const int n[3], const char* name) {
+  for (int k = 0; k < 3; ++k)
+    if (lo1[k] < v.box.lo[k] || lo1[k] + n[k] - 1 > v.box.hi[k])
+      return fail(WQL_EINVAL, "array %s bounds [%d,%d] on dim %d do not cover nodes [%d,%d]", name,
+                  v.box.lo[k], v.box.hi[k], k + 1, lo1[k], lo1[k] + n[k] - 1);
+  return WQL_OK;
+}
+
+template <typename T>
+static int put_comp(wql_block* b, T* dst, const View& v, int c, const int lo1[3], const int n[3],
+                    const char* name) {
+  if (!v.base) return fail(WQL_EINVAL, "array %s is null", name);
+  int rc = check_view(v, lo1, n, name);
+  if (rc) return rc;
+  const long long off0 = view_off(v, lo1[0], lo1[1], lo1[2], c);
+  const long long cnt = (long long)n[0] * n[1] * n[2];
+  if (b->d.arrays_on_device) {
+    HIP_TRY(launch_gather<T>(dst, v.base + off0, n[0], n[1], n[2], 1, v.sx, v.sy, v.sz, 0, 0,
+                             b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    return WQL_OK;
+  }
+  std::vector<T> h((size_t)cnt);
+  for (long long z = 0; z < n[2]; ++z)
+    for (long long y = 0; y < n[1]; ++y) {
+      const double* src = v.base + off0 + y * v.sy + z * v.sz;
+      T* o = h.data() + (size_t)(n[0] * (y + (long long)n[1] * z));
+      for (long long x = 0; x < n[0]; ++x) o[x] = (T)src[x];
+    }
+  HIP_TRY(hipMemcpyAsync(dst, h.data(), cnt * sizeof(T), hipMemcpyHostToDevice, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  return WQL_OK;
+}
+
+template <typename T>
+static int get_comp(wql_block* b, const T* src, const View& v, int c, const int lo1[3],
+                    const int n[3], const char* name) {
+  if (!v.base) return fail(WQL_EINVAL, "array %s is null", name);
+  int rc = check_view(v, lo1, n, name);
+  if (rc) return rc;
+  const long long off0 = view_off(v, lo1[0], lo1[1], lo1[2], c);
+  const long long cnt = (long long)n[0] * n[1] * n[2];
+  double* base = const_cast<double*>(v.base);
+  if (b->d.arrays_on_device) {
+    HIP_TRY(launch_scatter<T>(base + off0, src, n[0], n[1], n[2], 1, v.sx, v.sy, v.sz, 0, 0,
+                              b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    return WQL_OK;
+  }
+  std::vector<T> h((size_t)cnt);
+  HIP_TRY(hipMemcpyAsync(h.data(), src, cnt * sizeof(T), hipMemcpyDeviceToHost, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  for (long long z = 0; z < n[2]; ++z)
+    for (long long y = 0; y < n[1]; ++y) {
+      double* o = base + off0 + y * v.sy + z * v.sz;
+      const T* s = h.data() + (size_t)(n[0] * (y + (long long)n[1] * z));
+      for (long long x = 0; x < n[0]; ++x) o[x] = (double)s[x];
+    }
+  return WQL_OK;
+}
+
+// face frame + |grad xi| planes of face f (unit_normals.f90:8-168 results,
+// norm as in BoundaryConditions.f90:114-118)
+template <typename T>
+static int put_face(wql_block* b, int f) {
+  const wql_block_desc& d = b->d;
+  const int axis = f >> 1;
+  const int n1 = axis == 0 ? b->ny : b->nx, n2 = axis == 2 ? b->ny : b->nz;
+  const long long NF = (long long)n1 * n2;
+  const wql_box& fb = d.face_box[f];
+  if (!d.n_l[f] || !d.n_m[f] || !d.n_n[f]) return fail(WQL_EINVAL, "face %d frame arrays null", f);
+  if (fb.lo[0] > 1 || fb.hi[0] < n1 || fb.lo[1] > 1 || fb.hi[1] < n2)
+    return fail(WQL_EINVAL, "face %d frame bounds do not cover the face", f);
+  const long long e0 = box_extent(fb, 0), e1 = box_extent(fb, 1);
+  const int fixed1 = (f & 1) ? (axis == 0 ? b->nx : (axis == 1 ? b->ny : b->nz)) : 1;
+  std::vector<double> frame(9 * NF), met(3 * NF);
+  // frames
+  const double* fr[3] = {d.n_l[f], d.n_m[f], d.n_n[f]};
+  std::vector<double> tmp;
+  for (int k = 0; k < 3; ++k) {
+    const double* src = fr[k];
+    if (d.arrays_on_device) {
+      tmp.resize(3 * e0 * e1);
+      HIP_TRY(hipMemcpy(tmp.data(), src, tmp.size() * sizeof(double), hipMemcpyDeviceToHost));
+      src = tmp.data();
+    }
+    for (int c = 0; c < 3; ++c)
+      for (long long j = 0; j < n2; ++j)
+        for (long long i = 0; i < n1; ++i)
+          frame[(3 * k + c) * NF + i + n1 * j] =
+              src[(i + 1 - fb.lo[0]) + e0 * ((j + 1 - fb.lo[1]) + e1 * c)];
+  }
+  // metric column `axis` of metricx/y/z at the face nodes
+  const double* mets[3] = {d.metricx, d.metricy, d.metricz};
+  const View body0 = make_view(nullptr, d.body);
+  const long long fx1 = axis == 0 ? fixed1 : 1, fy1 = axis == 1 ? fixed1 : 1, fz1 = axis == 2 ? fixed1 : 1;
+  const int dims[3] = {axis == 0 ? 1 : b->nx, axis == 1 ? 1 : b->ny, axis == 2 ? 1 : b->nz};
+  for (int m = 0; m < 3; ++m) {
+    if (!mets[m]) return fail(WQL_EINVAL, "metric array %d null", m);
+    const long long off0 = view_off(body0, fx1, fy1, fz1, axis);
+    if (d.arrays_on_device) {
+      double* tmpd = nullptr;
+      HIP_TRY(hipMalloc((void**)&tmpd, NF * sizeof(double)));
+      hipError_t e = launch_gather<double>(tmpd, mets[m] + off0, dims[0], dims[1], dims[2], 1,
+                                           body0.sx, body0.sy, body0.sz, 0, 0, b->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+      if (e == hipSuccess)
+        e = hipMemcpy(met.data() + m * NF, tmpd, NF * sizeof(double), hipMemcpyDeviceToHost);
+      (void)hipFree(tmpd);
+      if (e != hipSuccess) return fail(WQL_EHIP, "face metric gather: %s", hipGetErrorString(e));
+    } else {
+      for (long long z = 0; z < dims[2]; ++z)
+        for (long long y = 0; y < dims[1]; ++y)
+          for (long long x = 0; x < dims[0]; ++x)
+            met[m * NF + x + dims[0] * (y + (long long)dims[1] * z)] =
+                mets[m][off0 + x * body0.sx + y * body0.sy + z * body0.sz];
+    }
+  }
+  std::vector<T> out(10 * NF);
+  for (long long k = 0; k < 9 * NF; ++k) out[k] = (T)frame[k];
+  for (long long k = 0; k < NF; ++k) {
+    const double qx = met[k], qy = met[NF + k], qz = met[2 * NF + k];
+    out[9 * NF + k] = (T)sqrt(qx * qx + qy * qy + qz * qz);
+  }
+  HIP_TRY(hipMemcpy(b->face[f], out.data(), out.size() * sizeof(T), hipMemcpyHostToDevice));
+  return WQL_OK;
+}
+
+template <typename T>
+static int transfer(wql_block* b, uint32_t mask, bool up) {
+  const wql_block_desc& d = b->d;
+  const long long P = b->P;
+  const int lo1[3] = {1, 1, 1};
+  const int n[3] = {b->nx, b->ny, b->nz};
+  const View body = make_view(nullptr, d.body);
+  int rc = 0;
+  auto xfer = [&](void* dev, const double* host, int c, const char* nm, const int* l,
+                  const int* nn, const View& vtemplate) -> int {
+    View v = vtemplate;
+    v.base = host;
+    return up ? put_comp<T>(b, (T*)dev, v, c, l, nn, nm) : get_comp<T>(b, (const T*)dev, v, c, l, nn, nm);
+  };
+  auto xfer64 = [&](double* dev, const double* host, int c, const char* nm) -> int {
+    View v = body;
+    v.base = host;
+    return up ? put_comp<double>(b, dev, v, c, lo1, n, nm)
+              : get_comp<double>(b, dev, v, c, lo1, n, nm);
+  };
+  if (mask & WQL_F)
+    for (int c = 0; c < 9 && !rc; ++c) rc = xfer64(b->F[b->cur] + c * P, d.F, c, "F");
+  if (mask & WQL_DF)
+    for (int c = 0; c < 9 && !rc; ++c) rc = xfer64(b->DF + c * P, d.DF, c, "DF");
+  if (rc) return rc;
+  if (mask & WQL_GEOMETRY) {
+    T* G = (T*)b->G;
+    if (d.mesh == WQL_CURVILINEAR) {
+      const double* m3[3] = {d.metricx, d.metricy, d.metricz};
+      for (int m = 0; m < 3 && !rc; ++m)
+        for (int c = 0; c < 3 && !rc; ++c)
+          rc = xfer(G + (3 * m + c) * P, m3[m], c, "metric", lo1, n, body);
+      if (!rc) rc = xfer(G + 9 * P, d.J, 0, "J", lo1, n, body);
+    } else {
+      rc = xfer(G, d.metricx, 0, "metricx", lo1, n, body);
+      if (!rc) rc = xfer(G + P, d.metricy, 1, "metricy", lo1, n, body);
+      if (!rc) rc = xfer(G + 2 * P, d.metricz, 2, "metricz", lo1, n, body);
+    }
+    if (rc) return rc;
+  }
+  if (mask & WQL_MATERIAL)
+    for (int c = 0; c < 3 && !rc; ++c) rc = xfer((T*)b->M + c * P, d.M, c, "M", lo1, n, body);
+  if (rc) return rc;
+  if (d.nmech && (mask & WQL_MEMORY)) {
+    for (int c6 = 0; c6 < 6 && !rc; ++c6)
+      for (int i = 0; i < d.nmech && !rc; ++i) {
+        rc = xfer((T*)b->eta + (c6 * d.nmech + i) * P, d.eta[c6], i, "eta", lo1, n, body);
+        if (!rc) rc = xfer((T*)b->Deta + (c6 * d.nmech + i) * P, d.Deta[c6], i, "Deta", lo1, n, body);
+      }
+    if (rc) return rc;
+  }
+  if (d.nmech && (mask & WQL_QINV)) {
+    rc = xfer(b->Qs, d.Qs_inv, 0, "Qs_inv", lo1, n, body);
+    if (!rc) rc = xfer(b->Qp, d.Qp_inv, 0, "Qp_inv", lo1, n, body);
+    if (rc) return rc;
+  }
+  if (mask & WQL_PML) {
+    for (int f = 0; f < 6 && !rc; ++f) {
+      if (!d.pml[f]) continue;
+      const View pv = make_view(nullptr, d.pml_box[f]);
+      int sl[3], sn[3];
+      for (int k = 0; k < 3; ++k) { sl[k] = b->slab_lo[f][k] + 1; sn[k] = b->slab_n[f][k]; }
+      for (int c = 0; c < 9 && !rc; ++c) {
+        rc = xfer((T*)b->pQ[f] + c * b->Ps[f], d.pml_Q[f], c, "pml_Q", sl, sn, pv);
+        if (!rc) rc = xfer((T*)b->pDQ[f] + c * b->Ps[f], d.pml_DQ[f], c, "pml_DQ", sl, sn, pv);
+      }
+    }
+    if (rc) return rc;
+  }
+  if (up && (mask & WQL_FACES))
+    for (int f = 0; f < 6 && !rc; ++f)
+      if (d.bc[f] > 0) rc = put_face<T>(b, f);
+  return rc;
+}
+
+extern "C" int wql_block_upload(wql_block* b, uint32_t mask) {
+  if (!b) return fail(WQL_EINVAL, "null block");
+  return b->d.precision == WQL_FP64 ? transfer<double>(b, mask, true) : transfer<float>(b, mask, true);
+}
+
+extern "C" int wql_block_download(wql_block* b, uint32_t mask) {
+  if (!b) return fail(WQL_EINVAL, "null block");
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  return b->d.precision == WQL_FP64 ? transfer<double>(b, mask & ~WQL_FACES, false)
+                                    : transfer<float>(b, mask & ~WQL_FACES, false);
+}
+
+template <typename T>
+static int init_state_t(wql_block* b) {
+  const long long P = b->P;
+  const int nm = b->d.nmech;
+  HIP_TRY(launch_fill<double>(b->F[0], 9 * P, 0.0, b->stream));
+  HIP_TRY(launch_fill<double>(b->F[1], 9 * P, 0.0, b->stream));
+  HIP_TRY(launch_fill<double>(b->DF, 9 * P, 1.0e40, b->stream));   // fields.f90:39
+  if (nm) {
+    HIP_TRY(launch_fill<T>((T*)b->eta, 6LL * nm * P, 0.0, b->stream));  // material.f90:389-404
+    HIP_TRY(launch_fill<T>((T*)b->Deta, 6LL * nm * P, 0.0, b->stream));
+  }
+  for (int f = 0; f < 6; ++f)
+    if (b->pQ[f]) {
+      HIP_TRY(launch_fill<T>((T*)b->pQ[f], 9 * b->Ps[f], 0.0, b->stream));  // pml.f90:45-46
+      // 1e40 in double; the fp32 mirror holds +inf, which the A(1)=0 scaling maps to 0
+      HIP_TRY(launch_fill<T>((T*)b->pDQ[f], 9 * b->Ps[f], 1.0e40, b->stream));
+    }
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  return WQL_OK;
+}
+
+extern "C" int wql_block_init_state(wql_block* b) {
+  if (!b) return fail(WQL_EINVAL, "null block");
+  b->cur = 0;
+  b->slot = 0;
+  return b->d.precision == WQL_FP64 ? init_state_t<double>(b) : init_state_t<float>(b);
+}
+
+// ---------------------------------------------------------------------------
+extern "C" int wql_block_set_sources(wql_block* b, int32_t n, const wql_source* src) {
+  if (!b) return fail(WQL_EINVAL, "null block");
+  if (n < 0 || n > kMaxSources) return fail(WQL_EINVAL, "at most %d sources", kMaxSources);
+  for (int i = 0; i < n; ++i) {
+    const wql_source& s = src[i];
+    if (s.order != 2 && s.order != 4) return fail(WQL_EINVAL, "source order must be 2 or 4");
+    if (s.type < 0 || s.type > 2) return fail(WQL_EINVAL, "invalid source type %d", s.type);
+    b->src[i] = s;
+  }
+  b->nsrc = n;
+  return WQL_OK;
+}
+
+extern "C" int wql_block_set_receivers(wql_block* b, int32_t n, const int32_t* ijk, int32_t nslots) {
+  if (!b) return fail(WQL_EINVAL, "null block");
+  if (n < 0 || nslots < 0) return fail(WQL_EINVAL, "negative receiver count");
+  if (b->rec_idx) { (void)hipFree(b->rec_idx); b->bytes -= b->nrec * (long long)sizeof(long long); b->rec_idx = nullptr; }
+  if (b->seis) { (void)hipFree(b->seis); b->bytes -= 9LL * b->nrec * b->nslots * (long long)sizeof(double); b->seis = nullptr; }
+  b->nrec = 0;
+  b->nslots = 0;
+  b->slot = 0;
+  if (n == 0) return WQL_OK;
+  std::vector<long long> idx(n);
+  for (int r = 0; r < n; ++r) {
+    const int x = ijk[3 * r] - 1, y = ijk[3 * r + 1] - 1, z = ijk[3 * r + 2] - 1;
+    if (x < 0 || x >= b->nx || y < 0 || y >= b->ny || z < 0 || z >= b->nz)
+      return fail(WQL_EINVAL, "receiver %d outside the block", r);
+    idx[r] = x + (long long)b->nx * (y + (long long)b->ny * z);
+  }
+  int rc = dalloc(b, (void**)&b->rec_idx, n * sizeof(long long));
+  if (rc) return rc;
+  rc = dalloc(b, (void**)&b->seis, (size_t)9 * n * (nslots > 0 ? nslots : 1) * sizeof(double));
+  if (rc) return rc;
+  HIP_TRY(hipMemcpy(b->rec_idx, idx.data(), n * sizeof(long long), hipMemcpyHostToDevice));
+  b->nrec = n;
+  b->nslots = nslots;
+  return WQL_OK;
+}
+
+extern "C" int wql_block_get_seismograms(wql_block* b, int32_t nslots, double* out) {
+  if (!b) return fail(WQL_EINVAL, "null block");
+  if (nslots > b->nslots) return fail(WQL_EINVAL, "only %d slots allocated", b->nslots);
+  if (nslots <= 0 || b->nrec == 0) return WQL_OK;
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  HIP_TRY(hipMemcpy(out, b->seis, (size_t)9 * b->nrec * nslots * sizeof(double),
+                    hipMemcpyDeviceToHost));
+  return WQL_OK;
+}
+
+// ---------------------------------------------------------------------------
+static double source_time(int type, double t, double t0, double w) {
+  const double pi = 3.141592653589793;
+  switch (type) {
+    case WQL_SRC_GAUSSIAN:  // moment_tensor.f90:829-833
+      return (w / (sqrt(2.0 * pi))) * exp((-(w * w) * ((t - t0) * (t - t0))) / 2.0);
+    case WQL_SRC_COSBELL:   // moment_tensor.f90:843-849
+      return (t < t0) ? (1.0 - cos(2.0 * pi * t / t0)) / t0 : 0.0;
+    case WQL_SRC_RICKER: {  // Ricker wavelet (no reference counterpart): w = peak frequency
+      const double a = pi * w * (t - t0);
+      return (1.0 - 2.0 * a * a) * exp(-a * a);
+    }
+  }
+  return 0.0;
+}
+
+template <typename T>
+static StepArgs<T> make_args(wql_block* b, double t) {
+  const wql_block_desc& d = b->d;
+  StepArgs<T> a;
+  memset(&a, 0, sizeof a);
+  a.nx = b->nx; a.ny = b->ny; a.nz = b->nz;
+  a.P = b->P;
+  a.nmech = d.nmech;
+  a.h[0] = d.hq; a.h[1] = d.hr; a.h[2] = d.hs;
+  for (int k = 0; k < 3; ++k) { a.ih[k] = 1.0 / a.h[k]; a.pen[k] = d.tau0 / a.h[k]; }
+  a.Fin = b->F[b->cur];
+  a.Fout = b->F[b->cur ^ 1];
+  a.DF = b->DF;
+  a.G = (const T*)b->G;
+  a.M = (const T*)b->M;
+  a.eta = (T*)b->eta;
+  a.Deta = (T*)b->Deta;
+  a.Qs = (const T*)b->Qs;
+  a.Qp = (const T*)b->Qp;
+  for (int i = 0; i < d.nmech; ++i) { a.tau_inv[i] = 1.0 / d.tau[i]; a.wgt[i] = d.weight[i]; }
+  for (int f = 0; f < 6; ++f) {
+    a.pml[f] = d.pml[f];
+    a.npml[f] = d.pml[f] ? d.npml[f] : 0;
+    a.pQ[f] = (T*)b->pQ[f];
+    a.pDQ[f] = (T*)b->pDQ[f];
+    a.bc[f] = d.bc[f] > 0 ? d.bc[f] : 0;
+    a.face[f] = (const T*)b->face[f];
+  }
+  a.nsrc = b->nsrc;
+  for (int s = 0; s < b->nsrc; ++s) {
+    const wql_source& S = b->src[s];
+    DevSource& D = a.src[s];
+    for (int k = 0; k < 3; ++k) { D.near[k] = S.near[k] - 1; D.off[k] = S.location[k] - S.near_phys[k]; }
+    D.order = S.order;
+    D.alpha = S.alpha;
+    const double A = source_time(S.type, t, S.t_init, S.duration);
+    const double m[6] = {S.mXX, S.mYY, S.mZZ, S.mXY, S.mXZ, S.mYZ};
+    for (int c = 0; c < 6; ++c) D.am[c] = A * m[c];
+  }
+  // strict interior (JU_xJU_yJU_z6.f90:16158-16173), 0-based inclusive
+  auto lo = [&](int f) { return d.pml[f] ? (d.npml[f] > 6 ? d.npml[f] : 6) : 6; };
+  a.ix0 = lo(0); a.ix1 = b->nx - 1 - lo(1);
+  a.iy0 = lo(2); a.iy1 = b->ny - 1 - lo(3);
+  a.iz0 = lo(4); a.iz1 = b->nz - 1 - lo(5);
+  return a;
+}
+
+template <typename T>
+static int scale_t(wql_block* b, double A) {
+  const long long P = b->P;
+  HIP_TRY(launch_scale<double>(b->DF, 9 * P, A, b->stream));
+  if (b->d.nmech) HIP_TRY(launch_scale<T>((T*)b->Deta, 6LL * b->d.nmech * P, A, b->stream));
+  for (int f = 0; f < 6; ++f)
+    if (b->pDQ[f]) HIP_TRY(launch_scale<T>((T*)b->pDQ[f], 9 * b->Ps[f], A, b->stream));
+  return WQL_OK;
+}
+
+template <typename T>
+static int update_t(wql_block* b, double inc) {
+  const long long P = b->P;
+  HIP_TRY(launch_update<double>(b->F[b->cur], b->DF, 9 * P, inc, b->stream));
+  if (b->d.nmech)
+    HIP_TRY(launch_update<T>((T*)b->eta, (const T*)b->Deta, 6LL * b->d.nmech * P, inc, b->stream));
+  for (int f = 0; f < 6; ++f)
+    if (b->pQ[f]) HIP_TRY(launch_update<T>((T*)b->pQ[f], (const T*)b->pDQ[f], 9 * b->Ps[f], inc, b->stream));
+  return WQL_OK;
+}
+
+extern "C" int wql_rk_scale(wql_block* b, double A) {
+  if (!b) return fail(WQL_EINVAL, "null block");
+  return b->d.precision == WQL_FP64 ? scale_t<double>(b, A) : scale_t<float>(b, A);
+}
+
+extern "C" int wql_rk_update(wql_block* b, double inc) {
+  if (!b) return fail(WQL_EINVAL, "null block");
+  return b->d.precision == WQL_FP64 ? update_t<double>(b, inc) : update_t<float>(b, inc);
+}
+
+extern "C" int wql_scale_rate_array(double* rate, int64_t n, double c) {
+  if (n < 0 || (n > 0 && !rate)) return fail(WQL_EINVAL, "bad array");
+  HIP_TRY(launch_scale<double>(rate, n, c, 0));
+  HIP_TRY(hipStreamSynchronize(0));
+  return WQL_OK;
+}
+
+extern "C" int wql_update_state_array(double* state, const double* rate, int64_t n, double inc) {
+  if (n < 0 || (n > 0 && (!state || !rate))) return fail(WQL_EINVAL, "bad array");
+  HIP_TRY(launch_update<double>(state, rate, n, inc, 0));
+  HIP_TRY(hipStreamSynchronize(0));
+  return WQL_OK;
+}
+
+static int need_fp64(wql_block* b, const char* what) {
+  if (!b) return fail(WQL_EINVAL, "null block");
+  if (b->d.precision != WQL_FP64)
+    return fail(WQL_ESTATE, "%s is the FP64 reference-order path; use wql_stage_fused for FP32-mixed", what);
+  return WQL_OK;
+}
+
+extern "C" int wql_rhs(wql_block* b) {
+  int rc = need_fp64(b, "wql_rhs");
+  if (rc) return rc;
+  HIP_TRY(launch_rhs(make_args<double>(b, 0.0), b->d.mesh, RhsPart::kAll, b->stream));
+  return WQL_OK;
+}
+
+extern "C" int wql_rhs_interior(wql_block* b, int32_t* handled) {
+  if (handled) *handled = 0;
+  int rc = need_fp64(b, "wql_rhs_interior");
+  if (rc) return rc;
+  HIP_TRY(launch_rhs(make_args<double>(b, 0.0), b->d.mesh, RhsPart::kInterior, b->stream));
+  if (handled) *handled = 1;
+  return WQL_OK;
+}
+
+extern "C" int wql_set_moment_tensor(wql_block* b, double t) {
+  int rc = need_fp64(b, "wql_set_moment_tensor");
+  if (rc) return rc;
+  if (b->nsrc) HIP_TRY(launch_source(make_args<double>(b, t), b->d.mesh, b->stream));
+  return WQL_OK;
+}
+
+extern "C" int wql_sat(wql_block* b) {
+  int rc = need_fp64(b, "wql_sat");
+  if (rc) return rc;
+  HIP_TRY(launch_sat(make_args<double>(b, 0.0), b->stream));
+  return WQL_OK;
+}
+
+static int record(wql_block* b) {
+  if (b->nrec && b->slot < b->nslots) {
+    HIP_TRY(launch_sample(b->F[b->cur], b->P, b->rec_idx, b->nrec,
+                          b->seis + (size_t)9 * b->nrec * b->slot, b->stream));
+    ++b->slot;
+  }
+  return WQL_OK;
+}
+
+extern "C" int wql_stage_fused(wql_block* b, double A, double Bdt, double t) {
+  if (!b) return fail(WQL_EINVAL, "null block");
+  if (b->d.precision == WQL_FP64) {
+    StepArgs<double> a = make_args<double>(b, t);
+    a.A = A; a.Bdt = Bdt;
+    HIP_TRY(launch_stage_fused<double>(a, b->d.mesh, b->stream));
+  } else {
+    StepArgs<float> a = make_args<float>(b, t);
+    a.A = A; a.Bdt = Bdt;
+    HIP_TRY(launch_stage_fused<float>(a, b->d.mesh, b->stream));
+  }
+  b->cur ^= 1;
+  return WQL_OK;
+}
+
+extern "C" int wql_time_step(wql_block* b, double dt, int32_t nsteps, double* t) {
+  if (!b || !t) return fail(WQL_EINVAL, "null argument");
+  for (int n = 0; n < nsteps; ++n) {
+    const double t0 = *t;
+    for (int s = 0; s < 5; ++s) {
+      if (s == 0) {
+        int rc = record(b);  // receivers sample F at t0 (time_step.f90:137)
+        if (rc) return rc;
+      }
+      int rc = wql_stage_fused(b, kRkA[s], kRkB[s] * dt, *t);
+      if (rc) return rc;
+      *t = t0 + kRkC[s + 1] * dt;
+    }
+  }
+  return WQL_OK;
+}
+
+extern "C" int wql_grid_metrics(int32_t nq, int32_t nr, int32_t ns, const double* X, double* mx,
+                                double* my, double* mz, double* J) {
+  if (nq < 12 || nr < 12 || ns < 12) return fail(WQL_EINVAL, "SBP-6 needs >= 12 nodes per axis");
+  if (!X || !mx || !my || !mz || !J) return fail(WQL_EINVAL, "null array");
+  HIP_TRY(launch_metrics(nq, nr, ns, X, mx, my, mz, J, 0));
+  HIP_TRY(hipStreamSynchronize(0));
+  return WQL_OK;
+}

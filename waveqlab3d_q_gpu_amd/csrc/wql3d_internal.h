@@ -1,0 +1,93 @@
+// Internal types shared by the C-ABI layer (wql3d_api.hip) and the kernels
+// (wql3d_kernels.hip).
+//
+// Device layout of one block (no ghost nodes; node (x,y,z) 0-based, x fastest):
+//   idx = x + nx*(y + ny*z),  P = nx*ny*nz
+//   F[2]  : 9 planes of double each (ping-pong for the fused stage)
+//   DF    : 9 planes of double
+//   G     : geometry planes of type T: cartesian 3 (q_x, r_y, s_z);
+//           curvilinear 10 (q_x,r_x,s_x, q_y,r_y,s_y, q_z,r_z,s_z, J)
+//   M     : 3 planes (lambda, mu, rho) of type T
+//   eta   : 6*nmech planes of T, plane (c6*nmech + i); Deta likewise
+//   Qs,Qp : 1 plane each of T
+//   pQ/pDQ[f] : 9 planes over the PML slab of face f (slab-local index)
+//   face[f]   : 10 planes over the face (l[3], m[3], n[3], |grad xi|)
+// T = double (WQL_FP64) or float (WQL_FP32_MIXED).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/wql3d.h"
+
+namespace wql {
+
+constexpr int kMaxSources = 8;
+
+struct DevSource {
+  int near[3];      // 0-based nearest node
+  int order, alpha;
+  double off[3];    // location - near_phys, per axis
+  double am[6];     // A(t) * (mXX, mYY, mZZ, mXY, mXZ, mYZ) for the stage
+};
+
+template <typename T>
+struct StepArgs {
+  int nx, ny, nz;
+  long long P;
+  int nmech;
+  double ih[3];     // 1/hq, 1/hr, 1/hs
+  double h[3];      // hq, hr, hs
+  double pen[3];    // tau0/h per axis (SAT)
+  const double* Fin;
+  double* Fout;
+  double* DF;
+  const T* G;
+  const T* M;
+  T* eta;
+  T* Deta;
+  const T* Qs;
+  const T* Qp;
+  double tau_inv[8];
+  double wgt[8];
+  int pml[6];
+  int npml[6];
+  T* pQ[6];
+  T* pDQ[6];
+  int bc[6];
+  const T* face[6];
+  int nsrc;
+  DevSource src[kMaxSources];
+  double A, Bdt;    // fused stage coefficients
+  int ix0, ix1, iy0, iy1, iz0, iz1;  // strict interior [lo, hi] 0-based (RHS_Center)
+};
+
+// Launchers (wql3d_kernels.hip).  All return hipError_t of the launch.
+enum class RhsPart { kAll = 0, kInterior = 1 };
+
+template <typename T>
+hipError_t launch_stage_fused(const StepArgs<T>& a, int mesh, hipStream_t s);
+hipError_t launch_rhs(const StepArgs<double>& a, int mesh, RhsPart part, hipStream_t s);
+hipError_t launch_sat(const StepArgs<double>& a, hipStream_t s);
+hipError_t launch_source(const StepArgs<double>& a, int mesh, hipStream_t s);
+template <typename T>
+hipError_t launch_scale(T* x, long long n, double A, hipStream_t s);
+template <typename T>
+hipError_t launch_update(T* x, const T* r, long long n, double inc, hipStream_t s);
+template <typename T>
+hipError_t launch_fill(T* x, long long n, double v, hipStream_t s);
+// copy a strided (Fortran-bounded) double array into a dense T plane set and back
+template <typename T>
+hipError_t launch_gather(T* dst, const double* src, int nx, int ny, int nz, int ncomp,
+                         long long sx, long long sy, long long sz, long long sc,
+                         long long off0, hipStream_t s);
+template <typename T>
+hipError_t launch_scatter(double* dst, const T* src, int nx, int ny, int nz, int ncomp,
+                          long long sx, long long sy, long long sz, long long sc,
+                          long long off0, hipStream_t s);
+hipError_t launch_sample(const double* F, long long P, const long long* idx, int n,
+                         double* out, hipStream_t s);
+hipError_t launch_metrics(int nx, int ny, int nz, const double* X, double* mx, double* my,
+                          double* mz, double* J, hipStream_t s);
+
+}  // namespace wql

@@ -1,0 +1,860 @@
+// HIP kernels for gfx950 (MI355X) of the per-block WaveQLab3D-Q time step.
+//
+// One thread per node.  The workgroup (256 threads = 4 wavefronts) covers a
+// contiguous run of the (x,y) plane of one z level; x is the fastest index,
+// so every plane load of a wavefront is one unit-stride 512 B (fp64) or
+// 256 B (fp32) transaction.  Stencil neighbours are re-read through the L1 /
+// L2 caches; the per-node streams (DF, memory variables, PML auxiliaries)
+// are read and written exactly once per stage.
+//
+// The SBP operator is table driven: rows 0..5 and n-6..n-1 of every axis use
+// the 6x9 closure table (reference JU_xJU_yJU_z6.f90:74-299), other rows the
+// 6-tap central stencil (JU_xJU_yJU_z6.f90:53-59).  Closure rows differ only
+// per (y,z) line for the y/z axes, so wavefronts stay convergent there.
+#include "wql3d_internal.h"
+
+namespace wql {
+
+// SBP-6 left closure rows (columns 0..8), reference JU_xJU_yJU_z6.f90:74-182.
+// The literal 0.013435342414630 (row 3, column 6) is the double value, as in
+// the reference's cpu-release build with -fdefault-real-8.
+__constant__ double kSbpL[6][9] = {
+    {-1.582533518939116, 2.033378678700676, -0.141512858744873, -0.450398306578272,
+     0.104488069284042, 0.036577936277544, 0.0, 0.0, 0.0},
+    {-0.462059195631158, 0.0, 0.287258622978251, 0.258816087376832, -0.069112065532624,
+     -0.014903449191300, 0.0, 0.0, 0.0},
+    {0.071247104721830, -0.636451095137907, 0.0, 0.606235523609147, -0.022902190275815,
+     -0.018129342917256, 0.0, 0.0, 0.0},
+    {0.114713313798970, -0.290087484386815, -0.306681191361148, 0.0, 0.520262285050482,
+     -0.051642265516119, 0.013435342414630, 0.0, 0.0},
+    {-0.036210680656541, 0.105400944933782, 0.015764336127392, -0.707905442575989, 0.0,
+     0.769199413962647, -0.164529643265203, 0.018281071473911, 0.0},
+    {-0.011398193015050, 0.020437334208704, 0.011220896474665, 0.063183694641876,
+     -0.691649024426814, 0.0, 0.739709139060752, -0.147941827812150, 0.016437980868017},
+};
+
+// singular_source order-4 stencils (moment_tensor.f90:980-1121), alpha 0,25,50,100
+__constant__ double kSrc4[4][9] = {
+    {-0.01171875, -0.03125, 0.046875, 0.28125, 0.4296875, 0.28125, 0.046875, -0.03125,
+     -0.01171875},
+    {-0.00246875, -0.02275, -0.014125, 0.16575, 0.4171875, 0.38675, 0.117875, -0.02975,
+     -0.01846875},
+    {-0.00728125, -0.028, 0.014875, 0.224, 0.4265625, 0.336, 0.081375, -0.032, -0.01553125},
+    {-0.0095625, -0.02990625, 0.03053125, 0.25284375, 0.42890625, 0.30903125, 0.06384375,
+     -0.03196875, -0.01371875},
+};
+
+// Apply f(offset, coefficient) for every tap of SBP D1 row i of an n-point axis.
+template <typename Fn>
+__device__ __forceinline__ void for_taps(int i, int n, Fn&& f) {
+  if (i >= 6 && i < n - 6) {
+    f(-3, -1.0 / 60.0);
+    f(-2, 0.15);
+    f(-1, -0.75);
+    f(1, 0.75);
+    f(2, -0.15);
+    f(3, 1.0 / 60.0);
+  } else if (i < 6) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) f(k - i, kSbpL[i][k]);
+  } else {
+    const int r = n - 1 - i;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) f(n - 9 + k - i, -kSbpL[r][8 - k]);
+  }
+}
+
+__device__ __forceinline__ double scale_rate(double A, double x) { return A == 0.0 ? 0.0 : A * x; }
+__device__ __forceinline__ float scale_rate(double A, float x) {
+  return A == 0.0 ? 0.0f : (float)A * x;
+}
+
+// ---------------------------------------------------------------------------
+// U[m][c] = JJU_x6 (JU_xJU_yJU_z6.f90:12-809) evaluated with metric array m
+// (0 metricx, 1 metricy, 2 metricz): c<3 -> sum_d X_m(pt,d)/h_d D_d(v_c);
+// c>=3 -> sum_d 1/h_d D_d(J X_m(.,d) sigma_c).
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ void derivs_curv(const StepArgs<T>& a, const int pos[3],
+                                            long long id, T U[3][9]) {
+  const long long P = a.P;
+  const long long st[3] = {1, a.nx, (long long)a.nx * a.ny};
+  const int n[3] = {a.nx, a.ny, a.nz};
+  T dv[3][3];
+  T S[3][6];
+#pragma unroll
+  for (int m = 0; m < 3; ++m)
+#pragma unroll
+    for (int c = 0; c < 6; ++c) S[m][c] = T(0);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const T ih = (T)a.ih[d];
+    T v0 = T(0), v1 = T(0), v2 = T(0);
+    for_taps(pos[d], n[d], [&](int off, double cd) {
+      const long long p = id + (long long)off * st[d];
+      const T ck = ih * (T)cd;
+      v0 += ck * (T)a.Fin[p];
+      v1 += ck * (T)a.Fin[P + p];
+      v2 += ck * (T)a.Fin[2 * P + p];
+      const T j = a.G[9 * P + p];
+      const T jm0 = j * a.G[(0 + d) * P + p];
+      const T jm1 = j * a.G[(3 + d) * P + p];
+      const T jm2 = j * a.G[(6 + d) * P + p];
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        const T t = ck * (T)a.Fin[(3 + c) * P + p];
+        S[0][c] += jm0 * t;
+        S[1][c] += jm1 * t;
+        S[2][c] += jm2 * t;
+      }
+    });
+    dv[d][0] = v0;
+    dv[d][1] = v1;
+    dv[d][2] = v2;
+  }
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    const T X0 = a.G[(3 * m + 0) * P + id], X1 = a.G[(3 * m + 1) * P + id],
+            X2 = a.G[(3 * m + 2) * P + id];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) U[m][c] = X0 * dv[0][c] + X1 * dv[1][c] + X2 * dv[2][c];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) U[m][3 + c] = S[m][c];
+  }
+}
+
+// Cartesian mesh: diagonal metrics only, J-free form of the interior branch
+// (JU_xJU_yJU_z6.f90:16426-16470): U[d][c] = X_d(pt) / h_d * D_d(F_c).
+template <typename T>
+__device__ __forceinline__ void derivs_cart(const StepArgs<T>& a, const int pos[3],
+                                            long long id, T U[3][9]) {
+  const long long P = a.P;
+  const long long st[3] = {1, a.nx, (long long)a.nx * a.ny};
+  const int n[3] = {a.nx, a.ny, a.nz};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    T acc[9];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) acc[c] = T(0);
+    for_taps(pos[d], n[d], [&](int off, double cd) {
+      const long long p = id + (long long)off * st[d];
+      const T ck = (T)cd;
+#pragma unroll
+      for (int c = 0; c < 9; ++c) acc[c] += ck * (T)a.Fin[c * P + p];
+    });
+    const T s = a.G[d * P + id] * (T)a.ih[d];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) U[d][c] = s * acc[c];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// PML geometry (pml.f90:11-168; RHS_Interior.f90:1846-2451)
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ bool pml_slab(const StepArgs<T>& a, int f, int x, int y, int z,
+                                         long long* sidx, double* d) {
+  const int np = a.npml[f];
+  const double d0 = 3.0 * 6.907755278982137;  // 6/(2*1)*log(1/Rf), Rf = 1e-3
+  double r;
+  switch (f) {
+    case 0:
+      if (x >= np) return false;
+      *sidx = x + (long long)np * (y + (long long)a.ny * z);
+      r = (double)(x - np);              // x1 - (1+np), x1 = x+1
+      break;
+    case 1:
+      if (x < a.nx - np) return false;
+      *sidx = (x - (a.nx - np)) + (long long)np * (y + (long long)a.ny * z);
+      r = (double)(x + 1 - (a.nx - np));
+      break;
+    case 2:
+      if (y >= np) return false;
+      *sidx = x + (long long)a.nx * (y + (long long)np * z);
+      r = (double)(y - np);
+      break;
+    case 3:
+      if (y < a.ny - np) return false;
+      *sidx = x + (long long)a.nx * ((y - (a.ny - np)) + (long long)np * z);
+      r = (double)(y + 1 - (a.ny - np));
+      break;
+    case 4:
+      if (z >= np) return false;
+      *sidx = x + (long long)a.nx * (y + (long long)a.ny * z);
+      r = (double)(z + 1 - np);          // z-low profile |z - npml| (RHS_Interior.f90:1919)
+      break;
+    default:
+      if (z < a.nz - np) return false;
+      *sidx = x + (long long)a.nx * (y + (long long)a.ny * (z - (a.nz - np)));
+      r = (double)(z + 1 - (a.nz - np));
+      break;
+  }
+  *d = d0 * fabs(r / (double)np);
+  return true;
+}
+
+template <typename T>
+__device__ __forceinline__ long long slab_plane(const StepArgs<T>& a, int f) {
+  const long long np = a.npml[f];
+  return f < 2 ? np * a.ny * a.nz : (f < 4 ? (long long)a.nx * np * a.nz : (long long)a.nx * a.ny * np);
+}
+
+// ---------------------------------------------------------------------------
+// SAT boundary data ubc at a face node: BCm/BCp (BoundaryConditions.f90:783-924)
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ void sat_ubc(const StepArgs<T>& a, int f, long long fidx, long long NF,
+                                        const T u[9], T lam, T mu, T rho, T ubc[9]) {
+  const T* fd = a.face[f];
+  const T l0 = fd[fidx], l1 = fd[NF + fidx], l2 = fd[2 * NF + fidx];
+  const T m0 = fd[3 * NF + fidx], m1 = fd[4 * NF + fidx], m2 = fd[5 * NF + fidx];
+  const T n0 = fd[6 * NF + fidx], n1 = fd[7 * NF + fidx], n2 = fd[8 * NF + fidx];
+  const T norm = fd[9 * NF + fidx];
+  const int code = a.bc[f];
+  const T r = code == 1 ? T(0) : (code == 2 ? T(1) : T(-1));
+  const T cp = sqrt((T(2) * mu + lam) / rho), cs = sqrt(mu / rho);
+  const T Zp = rho * cp, Zs = rho * cs;
+  const T T0 = n0 * u[3] + n1 * u[6] + n2 * u[7];
+  const T T1 = n0 * u[6] + n1 * u[4] + n2 * u[8];
+  const T T2 = n0 * u[7] + n1 * u[8] + n2 * u[5];
+  const T vl = l0 * u[0] + l1 * u[1] + l2 * u[2];
+  const T vm = m0 * u[0] + m1 * u[1] + m2 * u[2];
+  const T vn = n0 * u[0] + n1 * u[1] + n2 * u[2];
+  const T Tl = l0 * T0 + l1 * T1 + l2 * T2;
+  const T Tm = m0 * T0 + m1 * T1 + m2 * T2;
+  const T Tn = n0 * T0 + n1 * T1 + n2 * T2;
+  const T sg = (f & 1) ? T(1) : T(-1);   // BCp on high faces, BCm on low faces
+  const T a1 = T(0.5) * (T(1) - r), a2 = T(0.5) * (T(1) + r) * sg;
+  const T pl = a1 * Zs * vl + a2 * Tl;
+  const T pm = a1 * Zs * vm + a2 * Tm;
+  const T pn = a1 * Zp * vn + a2 * Tn;
+  const T ql = a1 * vl + a2 / Zs * Tl;
+  const T qm = a1 * vm + a2 / Zs * Tm;
+  const T qn = a1 * vn + a2 / Zp * Tn;
+  const T Fx = l0 * pl + m0 * pm + n0 * pn;
+  const T Fy = l1 * pl + m1 * pm + n1 * pn;
+  const T Fz = l2 * pl + m2 * pm + n2 * pn;
+  const T Gx = l0 * ql + m0 * qm + n0 * qn;
+  const T Gy = l1 * ql + m1 * qm + n1 * qn;
+  const T Gz = l2 * ql + m2 * qm + n2 * qn;
+  const T sn = sg * norm;
+  ubc[0] = norm * Fx;
+  ubc[1] = norm * Fy;
+  ubc[2] = norm * Fz;
+  ubc[3] = sn * n0 * Gx;
+  ubc[4] = sn * n1 * Gy;
+  ubc[5] = sn * n2 * Gz;
+  ubc[6] = sn * (n1 * Gx + n0 * Gy);
+  ubc[7] = sn * (n2 * Gx + n0 * Gz);
+  ubc[8] = sn * (n2 * Gy + n1 * Gz);
+}
+
+// face index of node (x,y,z) on face f; returns false if not on the face
+template <typename T>
+__device__ __forceinline__ bool on_face(const StepArgs<T>& a, int f, int x, int y, int z,
+                                        long long* fidx, long long* NF) {
+  switch (f) {
+    case 0: *fidx = y + (long long)a.ny * z; *NF = (long long)a.ny * a.nz; return x == 0;
+    case 1: *fidx = y + (long long)a.ny * z; *NF = (long long)a.ny * a.nz; return x == a.nx - 1;
+    case 2: *fidx = x + (long long)a.nx * z; *NF = (long long)a.nx * a.nz; return y == 0;
+    case 3: *fidx = x + (long long)a.nx * z; *NF = (long long)a.nx * a.nz; return y == a.ny - 1;
+    case 4: *fidx = x + (long long)a.nx * y; *NF = (long long)a.nx * a.ny; return z == 0;
+    default: *fidx = x + (long long)a.nx * y; *NF = (long long)a.nx * a.ny; return z == a.nz - 1;
+  }
+}
+
+// singular_source (moment_tensor.f90:938-1124)
+__device__ __forceinline__ double singular_source(int i, int i0, double h, double aa, int order,
+                                                  int alpha) {
+  if (order == 2) {
+    const double beta = (1.0 - 2.0 * aa) / 4.0 + 0.25;
+    const double gam = (2.0 * aa - 1.0) / 16.0;
+    const double gap = (2.0 * aa + 1.0) / 16.0;
+    if (i == i0 - 2) return (-gam) / h;
+    if (i == i0 - 1) return (beta + 3.0 * gam - gap) / h;
+    if (i == i0) return (1.0 - aa - 2.0 * beta - 3.0 * gam + 3.0 * gap) / h;
+    if (i == i0 + 1) return (aa + beta + gam - 3.0 * gap) / h;
+    if (i == i0 + 2) return gap / h;
+    return 0.0;
+  }
+  const int k = i - i0 + 4;
+  if (k < 0 || k > 8) return 0.0;
+  const int row = alpha == 25 ? 1 : (alpha == 50 ? 2 : (alpha == 100 ? 3 : 0));
+  return kSrc4[row][k] / h;
+}
+
+// source contribution (to subtract from DF(4..9)) at node (x,y,z)
+template <typename T, int MESH>
+__device__ __forceinline__ bool source_at(const StepArgs<T>& a, int x, int y, int z, long long id,
+                                          double out[6]) {
+  bool any = false;
+  for (int s = 0; s < a.nsrc; ++s) {
+    const DevSource& S = a.src[s];
+    const int r = S.order == 2 ? 2 : 4;
+    if (x < S.near[0] - r || x > S.near[0] + r || y < S.near[1] - r || y > S.near[1] + r ||
+        z < S.near[2] - r || z > S.near[2] + r)
+      continue;
+    const long long P = a.P;
+    const double qx = (double)a.G[0 * P + id];
+    const double ry = (double)a.G[(MESH ? 4 : 1) * P + id];
+    const double sz = (double)a.G[(MESH ? 8 : 2) * P + id];
+    const double hx = a.h[0] / qx, hy = a.h[1] / ry, hz = a.h[2] / sz;
+    const double dx = singular_source(x, S.near[0], hx, S.off[0] / hx, S.order, S.alpha);
+    const double dy = singular_source(y, S.near[1], hy, S.off[1] / hy, S.order, S.alpha);
+    const double dz = singular_source(z, S.near[2], hz, S.off[2] / hz, S.order, S.alpha);
+    const double delta = dx * dy * dz;
+    if (!any) {
+#pragma unroll
+      for (int c = 0; c < 6; ++c) out[c] = 0.0;
+      any = true;
+    }
+#pragma unroll
+    for (int c = 0; c < 6; ++c) out[c] += S.am[c] * delta;
+  }
+  return any;
+}
+
+// ---------------------------------------------------------------------------
+// The per-node kernel.
+//   MODE 0: fused RK stage   DF = A*DF + rates;  F_out = F_in + Bdt*DF
+//   MODE 1: rates only       DF += rates  (set_rates_elastic, all nodes)
+//   MODE 2: rates only, strict interior nodes (RHS_Center)
+// In MODE 0 rates include the moment-tensor source and the SAT penalty.
+// ---------------------------------------------------------------------------
+template <typename T, int MESH, int NMECH, int MODE>
+__global__ __launch_bounds__(256) void k_node(const StepArgs<T> a) {
+  const long long plane = (long long)a.nx * a.ny;
+  const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= plane) return;
+  const int z = blockIdx.y;
+  const int y = (int)(t / a.nx);
+  const int x = (int)(t - (long long)y * a.nx);
+  if (MODE == 2) {
+    if (x < a.ix0 || x > a.ix1 || y < a.iy0 || y > a.iy1 || z < a.iz0 || z > a.iz1) return;
+  }
+  const long long P = a.P;
+  const long long id = t + (long long)z * plane;
+  const int pos[3] = {x, y, z};
+
+  T U[3][9];
+  if (MESH) derivs_curv<T>(a, pos, id, U);
+  else derivs_cart<T>(a, pos, id, U);
+
+  const T lam = a.M[id], mu = a.M[P + id], rho = a.M[2 * P + id];
+  const T Jp = MESH ? a.G[9 * P + id] : T(1);
+  const T rhoJ_inv = T(1) / (rho * Jp);
+  const T l2m = lam + T(2) * mu;
+
+  // elastic rates (RHS_Interior.f90:238-254)
+  T DU[9];
+  DU[0] = (U[0][3] + U[1][6] + U[2][7]) * rhoJ_inv;
+  DU[1] = (U[0][6] + U[1][4] + U[2][8]) * rhoJ_inv;
+  DU[2] = (U[0][7] + U[1][8] + U[2][5]) * rhoJ_inv;
+  DU[3] = l2m * U[0][0] + lam * (U[1][1] + U[2][2]);
+  DU[4] = l2m * U[1][1] + lam * (U[0][0] + U[2][2]);
+  DU[5] = l2m * U[2][2] + lam * (U[0][0] + U[1][1]);
+  DU[6] = mu * (U[1][0] + U[0][1]);
+  DU[7] = mu * (U[2][0] + U[0][2]);
+  DU[8] = mu * (U[2][1] + U[1][2]);
+
+  // PML slab membership, one face per axis (pml_damping_and_q)
+  int fa[3] = {-1, -1, -1};
+  long long sidx[3] = {0, 0, 0};
+  T dd[3] = {T(0), T(0), T(0)};
+  if (MODE != 2) {
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+      if (!a.pml[f]) continue;
+      long long si;
+      double d;
+      if (pml_slab(a, f, x, y, z, &si, &d)) {
+        fa[f >> 1] = f;
+        sidx[f >> 1] = si;
+        dd[f >> 1] = (T)d;
+      }
+    }
+  }
+  const bool inpml = fa[0] >= 0 || fa[1] >= 0 || fa[2] >= 0;
+
+  // anelastic Q4/Q8 (RHS_Interior.f90:2599-2906; interior JU_xJU_yJU_z6.f90:16299-16376)
+  if (NMECH > 0) {
+    T e[6];
+    e[0] = U[0][0];
+    e[1] = U[1][1];
+    e[2] = U[2][2];
+    e[3] = U[1][0] + U[0][1];
+    e[4] = U[2][0] + U[0][2];
+    e[5] = U[2][1] + U[1][2];
+    if (inpml) {
+      T qx[3] = {T(0), T(0), T(0)}, qy[3] = {T(0), T(0), T(0)}, qz[3] = {T(0), T(0), T(0)};
+      if (fa[0] >= 0) {
+        const T* Q = a.pQ[fa[0]];
+        const long long Ps = slab_plane(a, fa[0]);
+        qx[0] = Q[3 * Ps + sidx[0]]; qx[1] = Q[6 * Ps + sidx[0]]; qx[2] = Q[7 * Ps + sidx[0]];
+      }
+      if (fa[1] >= 0) {
+        const T* Q = a.pQ[fa[1]];
+        const long long Ps = slab_plane(a, fa[1]);
+        qy[0] = Q[4 * Ps + sidx[1]]; qy[1] = Q[6 * Ps + sidx[1]]; qy[2] = Q[8 * Ps + sidx[1]];
+      }
+      if (fa[2] >= 0) {
+        const T* Q = a.pQ[fa[2]];
+        const long long Ps = slab_plane(a, fa[2]);
+        qz[0] = Q[5 * Ps + sidx[2]]; qz[1] = Q[7 * Ps + sidx[2]]; qz[2] = Q[8 * Ps + sidx[2]];
+      }
+      e[0] = e[0] - dd[0] * qx[0];
+      e[1] = e[1] - dd[1] * qy[0];
+      e[2] = e[2] - dd[2] * qz[0];
+      e[3] = e[3] - dd[0] * qx[1] - dd[1] * qy[1];
+      e[4] = e[4] - dd[0] * qx[2] - dd[2] * qz[1];
+      e[5] = e[5] - dd[1] * qy[2] - dd[2] * qz[2];
+    }
+    const T tr = e[0] + e[1] + e[2];
+    const T Qs = a.Qs[id], Qp = a.Qp[id];
+    const T twomuQs = T(2) * mu * Qs;
+    const T pfac = l2m * Qp - twomuQs;
+    const T muQs = mu * Qs;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      T ev[NMECH > 0 ? NMECH : 1];
+      T sum = T(0);
+#pragma unroll
+      for (int i = 0; i < NMECH; ++i) {
+        ev[i] = a.eta[((long long)(c * NMECH + i)) * P + id];
+        sum += ev[i];
+      }
+      DU[3 + c] -= sum;
+#pragma unroll
+      for (int i = 0; i < NMECH; ++i) {
+        const T w = (T)a.wgt[i];
+        const T ti = (T)a.tau_inv[i];
+        const T src = c < 3 ? (w * twomuQs) * e[c] + (w * pfac) * tr : (w * muQs) * e[c];
+        const T r = (src - ev[i]) * ti;
+        const long long k = ((long long)(c * NMECH + i)) * P + id;
+        if (MODE == 0) {
+          const T dn = scale_rate(a.A, a.Deta[k]) + r;
+          a.Deta[k] = dn;
+          a.eta[k] = ev[i] + (T)a.Bdt * dn;
+        } else {
+          a.Deta[k] = a.Deta[k] + r;
+        }
+      }
+    }
+  }
+
+  // SAT penalty terms at face nodes (impose_boundary_condition), fused mode
+  double satDF[9];
+  bool has_sat = false;
+  if (MODE == 0) {
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+      if (a.bc[f] <= 0) continue;
+      long long fidx, NF;
+      if (!on_face(a, f, x, y, z, &fidx, &NF)) continue;
+      T u[9], ubc[9];
+#pragma unroll
+      for (int c = 0; c < 9; ++c) u[c] = (T)a.Fin[c * P + id];
+      sat_ubc(a, f, fidx, NF, u, lam, mu, rho, ubc);
+      const T pen = (T)a.pen[f >> 1];
+      T Ux[9];
+      Ux[0] = T(1) / rho * ubc[0];
+      Ux[1] = T(1) / rho * ubc[1];
+      Ux[2] = T(1) / rho * ubc[2];
+      Ux[3] = (T(2) * mu + lam) * ubc[3] + lam * (ubc[4] + ubc[5]);
+      Ux[4] = (T(2) * mu + lam) * ubc[4] + lam * (ubc[3] + ubc[5]);
+      Ux[5] = (T(2) * mu + lam) * ubc[5] + lam * (ubc[3] + ubc[4]);
+      Ux[6] = mu * ubc[6];
+      Ux[7] = mu * ubc[7];
+      Ux[8] = mu * ubc[8];
+      if (!has_sat) {
+#pragma unroll
+        for (int c = 0; c < 9; ++c) satDF[c] = 0.0;
+        has_sat = true;
+      }
+#pragma unroll
+      for (int c = 0; c < 9; ++c) satDF[c] += (double)(pen * Ux[c]);
+    }
+  }
+
+  // PML feedback and auxiliary rates (RHS_Interior.f90:1846-2265)
+  if (inpml) {
+    const T cfs = T(1);
+#pragma unroll
+    for (int ax = 2; ax >= 0; --ax) {   // reference order: z, y, x slabs
+      const int f = fa[ax];
+      if (f < 0) continue;
+      const T d = dd[ax];
+      T* Q = a.pQ[f];
+      T* DQ = a.pDQ[f];
+      const long long Ps = slab_plane(a, f);
+      const long long si = sidx[ax];
+      T q[9];
+#pragma unroll
+      for (int c = 0; c < 9; ++c) q[c] = Q[c * Ps + si];
+      DU[0] -= d * Jp * q[0] * rhoJ_inv;
+      DU[1] -= d * Jp * q[1] * rhoJ_inv;
+      DU[2] -= d * Jp * q[2] * rhoJ_inv;
+      DU[3] -= d * (l2m * q[3] + lam * q[4] + lam * q[5]);
+      DU[4] -= d * (lam * q[3] + l2m * q[4] + lam * q[5]);
+      DU[5] -= d * (lam * q[3] + lam * q[4] + l2m * q[5]);
+      DU[6] -= mu * d * q[6];
+      DU[7] -= mu * d * q[7];
+      DU[8] -= mu * d * q[8];
+      T s[9];
+      const T iJ = T(1) / Jp;
+      if (ax == 0) {
+        s[0] = U[0][3] * iJ; s[1] = U[0][6] * iJ; s[2] = U[0][7] * iJ;
+        s[3] = U[0][0]; s[4] = T(0); s[5] = T(0); s[6] = U[0][1]; s[7] = U[0][2]; s[8] = T(0);
+      } else if (ax == 1) {
+        s[0] = U[1][6] * iJ; s[1] = U[1][4] * iJ; s[2] = U[1][8] * iJ;
+        s[3] = T(0); s[4] = U[1][1]; s[5] = T(0); s[6] = U[1][0]; s[7] = T(0); s[8] = U[1][2];
+      } else {
+        s[0] = U[2][7] * iJ; s[1] = U[2][8] * iJ; s[2] = U[2][5] * iJ;
+        s[3] = T(0); s[4] = T(0); s[5] = U[2][2]; s[6] = T(0); s[7] = U[2][0]; s[8] = U[2][1];
+      }
+      // SAT on the slab's own face adds to this slab's DQ (RHS_Interior.f90:1640-1643)
+      T sat[9];
+      bool fsat = false;
+      if (MODE == 0 && a.bc[f] > 0) {
+        long long fidx, NF;
+        if (on_face(a, f, x, y, z, &fidx, &NF)) {
+          T u[9];
+#pragma unroll
+          for (int c = 0; c < 9; ++c) u[c] = (T)a.Fin[c * P + id];
+          sat_ubc(a, f, fidx, NF, u, lam, mu, rho, sat);
+          fsat = true;
+        }
+      }
+      const T pen = (T)a.pen[ax];
+#pragma unroll
+      for (int c = 0; c < 9; ++c) {
+        T r = s[c] - (d + cfs) * q[c];
+        if (fsat) r -= pen * sat[c];
+        if (MODE == 0) {
+          const T dn = scale_rate(a.A, DQ[c * Ps + si]) + r;
+          DQ[c * Ps + si] = dn;
+          Q[c * Ps + si] = q[c] + (T)a.Bdt * dn;
+        } else {
+          DQ[c * Ps + si] = DQ[c * Ps + si] + r;
+        }
+      }
+    }
+  }
+
+  if (MODE == 0) {
+    double srcv[6];
+    const bool has_src = a.nsrc > 0 && source_at<T, MESH>(a, x, y, z, id, srcv);
+#pragma unroll
+    for (int c = 0; c < 9; ++c) {
+      double df = scale_rate(a.A, a.DF[c * P + id]) + (double)DU[c];
+      if (has_src && c >= 3) df -= srcv[c - 3];
+      if (has_sat) df -= satDF[c];
+      a.DF[c * P + id] = df;
+      a.Fout[c * P + id] = a.Fin[c * P + id] + a.Bdt * df;
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < 9; ++c) a.DF[c * P + id] = a.DF[c * P + id] + (double)DU[c];
+  }
+}
+
+// SAT on one face (impose_boundary_condition, unfused): DF -= tau0/h*structure(ubc),
+// PML DQ -= tau0/h*ubc
+__global__ __launch_bounds__(256) void k_sat_face(const StepArgs<double> a, int f) {
+  const int axis = f >> 1;
+  const int n1 = axis == 0 ? a.ny : a.nx;
+  const int n2 = axis == 2 ? a.ny : a.nz;
+  const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= (long long)n1 * n2) return;
+  const int j = (int)(t / n1), i = (int)(t - (long long)j * n1);
+  const int fixed = (f & 1) ? (axis == 0 ? a.nx - 1 : (axis == 1 ? a.ny - 1 : a.nz - 1)) : 0;
+  int x, y, z;
+  if (axis == 0) { x = fixed; y = i; z = j; }
+  else if (axis == 1) { x = i; y = fixed; z = j; }
+  else { x = i; y = j; z = fixed; }
+  const long long P = a.P;
+  const long long id = x + (long long)a.nx * (y + (long long)a.ny * z);
+  const double lam = a.M[id], mu = a.M[P + id], rho = a.M[2 * P + id];
+  double u[9], ubc[9];
+  for (int c = 0; c < 9; ++c) u[c] = a.Fin[c * P + id];
+  sat_ubc(a, f, t, (long long)n1 * n2, u, lam, mu, rho, ubc);
+  const double pen = a.pen[axis];
+  double Ux[9];
+  Ux[0] = 1.0 / rho * ubc[0];
+  Ux[1] = 1.0 / rho * ubc[1];
+  Ux[2] = 1.0 / rho * ubc[2];
+  Ux[3] = (2.0 * mu + lam) * ubc[3] + lam * (ubc[4] + ubc[5]);
+  Ux[4] = (2.0 * mu + lam) * ubc[4] + lam * (ubc[3] + ubc[5]);
+  Ux[5] = (2.0 * mu + lam) * ubc[5] + lam * (ubc[3] + ubc[4]);
+  Ux[6] = mu * ubc[6];
+  Ux[7] = mu * ubc[7];
+  Ux[8] = mu * ubc[8];
+  for (int c = 0; c < 9; ++c) a.DF[c * P + id] = a.DF[c * P + id] - pen * Ux[c];
+  if (a.pml[f]) {
+    long long si;
+    double d;
+    if (pml_slab(a, f, x, y, z, &si, &d)) {
+      const long long Ps = slab_plane(a, f);
+      for (int c = 0; c < 9; ++c) a.pDQ[f][c * Ps + si] = a.pDQ[f][c * Ps + si] - pen * ubc[c];
+    }
+  }
+}
+
+// set_moment_tensor over the +-4 (order 4) or +-2 (order 2) node box
+__global__ __launch_bounds__(256) void k_source(const StepArgs<double> a, int s, int mesh) {
+  const DevSource& S = a.src[s];
+  const int r = S.order == 2 ? 2 : 4;
+  const int w = 2 * r + 1;
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= w * w * w) return;
+  const int x = S.near[0] - r + t % w;
+  const int y = S.near[1] - r + (t / w) % w;
+  const int z = S.near[2] - r + t / (w * w);
+  if (x < 0 || x >= a.nx || y < 0 || y >= a.ny || z < 0 || z >= a.nz) return;
+  const long long P = a.P;
+  const long long id = x + (long long)a.nx * (y + (long long)a.ny * z);
+  const double qx = a.G[id];
+  const double ry = a.G[(mesh ? 4 : 1) * P + id];
+  const double sz = a.G[(mesh ? 8 : 2) * P + id];
+  const double hx = a.h[0] / qx, hy = a.h[1] / ry, hz = a.h[2] / sz;
+  const double dx = singular_source(x, S.near[0], hx, S.off[0] / hx, S.order, S.alpha);
+  const double dy = singular_source(y, S.near[1], hy, S.off[1] / hy, S.order, S.alpha);
+  const double dz = singular_source(z, S.near[2], hz, S.off[2] / hz, S.order, S.alpha);
+  const double delta = dx * dy * dz;
+  for (int c = 0; c < 6; ++c) a.DF[(3 + c) * P + id] = a.DF[(3 + c) * P + id] - S.am[c] * delta;
+}
+
+// rk_vector kernels (rk_vector_openacc.f90:22-75): no FMA contraction, so the
+// result is bit-identical to the CPU expressions.
+template <typename T>
+__global__ __launch_bounds__(256) void k_scale(T* x, long long n, double A) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  x[i] = scale_rate(A, x[i]);
+}
+__device__ __forceinline__ double mul_add_rn(double s, double inc, double r) {
+  return __dadd_rn(s, __dmul_rn(inc, r));
+}
+__device__ __forceinline__ float mul_add_rn(float s, double inc, float r) {
+  return __fadd_rn(s, __fmul_rn((float)inc, r));
+}
+template <typename T>
+__global__ __launch_bounds__(256) void k_update(T* x, const T* r, long long n, double inc) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  x[i] = mul_add_rn(x[i], inc, r[i]);
+}
+template <typename T>
+__global__ __launch_bounds__(256) void k_fill(T* x, long long n, double v) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  x[i] = (T)v;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_gather(T* dst, const double* src, int nx, int ny, int nz,
+                                                long long sx, long long sy, long long sz,
+                                                long long sc, long long off0, int c) {
+  const long long plane = (long long)nx * ny;
+  const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= plane) return;
+  const int z = blockIdx.y;
+  const int y = (int)(t / nx), x = (int)(t - (long long)y * nx);
+  const double v = src[off0 + x * sx + y * sy + z * sz + c * sc];
+  dst[(long long)c * plane * nz + t + z * plane] = (T)v;
+}
+template <typename T>
+__global__ __launch_bounds__(256) void k_scatter(double* dst, const T* src, int nx, int ny, int nz,
+                                                 long long sx, long long sy, long long sz,
+                                                 long long sc, long long off0, int c) {
+  const long long plane = (long long)nx * ny;
+  const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= plane) return;
+  const int z = blockIdx.y;
+  const int y = (int)(t / nx), x = (int)(t - (long long)y * nx);
+  dst[off0 + x * sx + y * sy + z * sz + c * sc] =
+      (double)src[(long long)c * plane * nz + t + z * plane];
+}
+
+__global__ void k_sample(const double* F, long long P, const long long* idx, int n, double* out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 9 * n) return;
+  const int r = t / 9, c = t - 9 * (t / 9);
+  out[t] = F[c * P + idx[r]];
+}
+
+// derivatives() on the device (grid.f90:398-471 with metrics.f90 D6x/D6y/D6z)
+__global__ __launch_bounds__(256) void k_metrics(int nx, int ny, int nz, const double* X,
+                                                 double* mx, double* my, double* mz, double* J) {
+  const long long plane = (long long)nx * ny;
+  const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= plane) return;
+  const int z = blockIdx.y;
+  const int y = (int)(t / nx), x = (int)(t - (long long)y * nx);
+  const long long P = plane * nz;
+  const long long id = t + z * plane;
+  const long long st[3] = {1, nx, plane};
+  const int pos[3] = {x, y, z}, n[3] = {nx, ny, nz};
+  double D[3][3];
+  for (int d = 0; d < 3; ++d) {
+    const double ih = 1.0 / (1.0 / (double)(n[d] - 1));
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for_taps(pos[d], n[d], [&](int off, double c) {
+      const long long p = id + (long long)off * st[d];
+      s0 += c * X[p];
+      s1 += c * X[P + p];
+      s2 += c * X[2 * P + p];
+    });
+    D[d][0] = ih * s0;
+    D[d][1] = ih * s1;
+    D[d][2] = ih * s2;
+  }
+  const double* Xq = D[0];
+  const double* Xr = D[1];
+  const double* Xs = D[2];
+  const double j = Xq[0] * (Xr[1] * Xs[2] - Xr[2] * Xs[1]) - Xq[1] * (Xr[0] * Xs[2] - Xr[2] * Xs[0]) +
+                   Xq[2] * (Xr[0] * Xs[1] - Xr[1] * Xs[0]);
+  const double ij = 1.0 / j;
+  J[id] = j;
+  mx[id] = ij * (Xr[1] * Xs[2] - Xr[2] * Xs[1]);
+  mx[P + id] = ij * (Xq[2] * Xs[1] - Xq[1] * Xs[2]);
+  mx[2 * P + id] = ij * (Xq[1] * Xr[2] - Xq[2] * Xr[1]);
+  my[id] = ij * (Xr[2] * Xs[0] - Xr[0] * Xs[2]);
+  my[P + id] = ij * (Xq[0] * Xs[2] - Xq[2] * Xs[0]);
+  my[2 * P + id] = ij * (Xq[2] * Xr[0] - Xq[0] * Xr[2]);
+  mz[id] = ij * (Xr[0] * Xs[1] - Xr[1] * Xs[0]);
+  mz[P + id] = ij * (Xq[1] * Xs[0] - Xq[0] * Xs[1]);
+  mz[2 * P + id] = ij * (Xq[0] * Xr[1] - Xq[1] * Xr[0]);
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static inline dim3 plane_grid(int nx, int ny, int nz) {
+  const long long plane = (long long)nx * ny;
+  return dim3((unsigned)((plane + 255) / 256), (unsigned)nz, 1);
+}
+
+template <typename T, int MODE>
+static hipError_t launch_node(const StepArgs<T>& a, int mesh, hipStream_t s) {
+  const dim3 g = plane_grid(a.nx, a.ny, a.nz);
+  const dim3 b(256);
+#define WQL_CASE(MS, NM)                                                   \
+  if (mesh == MS && a.nmech == NM) {                                       \
+    hipLaunchKernelGGL((k_node<T, MS, NM, MODE>), g, b, 0, s, a);          \
+    return hipGetLastError();                                              \
+  }
+  WQL_CASE(0, 0) WQL_CASE(0, 4) WQL_CASE(0, 8)
+  WQL_CASE(1, 0) WQL_CASE(1, 4) WQL_CASE(1, 8)
+#undef WQL_CASE
+  return hipErrorInvalidValue;
+}
+
+template <typename T>
+hipError_t launch_stage_fused(const StepArgs<T>& a, int mesh, hipStream_t s) {
+  return launch_node<T, 0>(a, mesh, s);
+}
+template hipError_t launch_stage_fused<double>(const StepArgs<double>&, int, hipStream_t);
+template hipError_t launch_stage_fused<float>(const StepArgs<float>&, int, hipStream_t);
+
+hipError_t launch_rhs(const StepArgs<double>& a, int mesh, RhsPart part, hipStream_t s) {
+  return part == RhsPart::kAll ? launch_node<double, 1>(a, mesh, s)
+                               : launch_node<double, 2>(a, mesh, s);
+}
+
+hipError_t launch_sat(const StepArgs<double>& a, hipStream_t s) {
+  for (int f = 0; f < 6; ++f) {
+    if (a.bc[f] <= 0) continue;
+    const int axis = f >> 1;
+    const long long n1 = axis == 0 ? a.ny : a.nx;
+    const long long n2 = axis == 2 ? a.ny : a.nz;
+    hipLaunchKernelGGL(k_sat_face, dim3((unsigned)((n1 * n2 + 255) / 256)), dim3(256), 0, s, a, f);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+hipError_t launch_source(const StepArgs<double>& a, int mesh, hipStream_t s) {
+  for (int k = 0; k < a.nsrc; ++k) {
+    const int r = a.src[k].order == 2 ? 2 : 4;
+    const int w = 2 * r + 1;
+    hipLaunchKernelGGL(k_source, dim3((w * w * w + 255) / 256), dim3(256), 0, s, a, k, mesh);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+template <typename T>
+hipError_t launch_scale(T* x, long long n, double A, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_scale<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, n, A);
+  return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_update(T* x, const T* r, long long n, double inc, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_update<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, r, n, inc);
+  return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_fill(T* x, long long n, double v, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, n, v);
+  return hipGetLastError();
+}
+template hipError_t launch_scale<double>(double*, long long, double, hipStream_t);
+template hipError_t launch_scale<float>(float*, long long, double, hipStream_t);
+template hipError_t launch_update<double>(double*, const double*, long long, double, hipStream_t);
+template hipError_t launch_update<float>(float*, const float*, long long, double, hipStream_t);
+template hipError_t launch_fill<double>(double*, long long, double, hipStream_t);
+template hipError_t launch_fill<float>(float*, long long, double, hipStream_t);
+
+template <typename T>
+hipError_t launch_gather(T* dst, const double* src, int nx, int ny, int nz, int ncomp,
+                         long long sx, long long sy, long long sz, long long sc, long long off0,
+                         hipStream_t s) {
+  for (int c = 0; c < ncomp; ++c) {
+    hipLaunchKernelGGL(k_gather<T>, plane_grid(nx, ny, nz), dim3(256), 0, s, dst, src, nx, ny, nz,
+                       sx, sy, sz, sc, off0, c);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+template <typename T>
+hipError_t launch_scatter(double* dst, const T* src, int nx, int ny, int nz, int ncomp,
+                          long long sx, long long sy, long long sz, long long sc, long long off0,
+                          hipStream_t s) {
+  for (int c = 0; c < ncomp; ++c) {
+    hipLaunchKernelGGL(k_scatter<T>, plane_grid(nx, ny, nz), dim3(256), 0, s, dst, src, nx, ny, nz,
+                       sx, sy, sz, sc, off0, c);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+template hipError_t launch_gather<double>(double*, const double*, int, int, int, int, long long,
+                                          long long, long long, long long, long long, hipStream_t);
+template hipError_t launch_gather<float>(float*, const double*, int, int, int, int, long long,
+                                         long long, long long, long long, long long, hipStream_t);
+template hipError_t launch_scatter<double>(double*, const double*, int, int, int, int, long long,
+                                           long long, long long, long long, long long, hipStream_t);
+template hipError_t launch_scatter<float>(double*, const float*, int, int, int, int, long long,
+                                          long long, long long, long long, long long, hipStream_t);
+
+hipError_t launch_sample(const double* F, long long P, const long long* idx, int n, double* out,
+                         hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sample, dim3((9 * n + 255) / 256), dim3(256), 0, s, F, P, idx, n, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_metrics(int nx, int ny, int nz, const double* X, double* mx, double* my,
+                          double* mz, double* J, hipStream_t s) {
+  hipLaunchKernelGGL(k_metrics, plane_grid(nx, ny, nz), dim3(256), 0, s, nx, ny, nz, X, mx, my, mz, J);
+  return hipGetLastError();
+}
+
+}  // namespace wql

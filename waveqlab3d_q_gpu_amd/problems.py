@@ -1,0 +1,435 @@
+"""Synthetic block set-ups (host-side initialisation, numpy).
+
+These play the role of the reference's init_domain (domain.f90:30-315):
+grid coordinates, material, anelastic coefficients, PML flags, boundary
+codes, face frames, sources, receivers and the time step.  Metric terms are
+NOT computed here: pass ``metrics_fn`` (the device pipeline
+:func:`waveqlab3d_q_gpu_amd.block.grid_metrics`, or any function with the
+signature ``X -> (metricx, metricy, metricz, J)``).
+
+Array layout: ``(ncomp, nz, ny, nx)`` C order == Fortran ``A(x, y, z, c)``.
+Axis convention (grid.f90:1350-1357): r (y, 2nd index) is depth, the r-min
+face (face 2, "Ly") is the free surface, the r-max face (face 3) the bottom.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib as L
+from .block import SourceSpec
+
+FACE_NAMES = ("Lx", "Rx", "Ly", "Ry", "Lz", "Rz")
+
+
+@dataclass
+class Problem:
+    n: tuple                       # (nq, nr, ns)
+    mesh: int
+    X: np.ndarray                  # (3, nz, ny, nx)
+    M: np.ndarray                  # (3, nz, ny, nx) lambda, mu, rho
+    bc: tuple
+    dt: float
+    nsteps: int
+    pml: tuple = (0,) * 6
+    npml: tuple = (0,) * 6
+    nmech: int = 0
+    tau: np.ndarray = None
+    weight: np.ndarray = None
+    Qs_inv: np.ndarray = None
+    Qp_inv: np.ndarray = None
+    F0: np.ndarray = None          # initial fields (None = zero)
+    sources: list = field(default_factory=list)
+    receivers: np.ndarray = None   # (nrec, 3) 1-based node indices
+    name: str = ""
+
+
+# ---------------------------------------------------------------------------
+# grids
+# ---------------------------------------------------------------------------
+def cartesian_grid(n, a, b):
+    """cartesian_grid_3d (grid.f90:214-241): x_i = a + (i-1)*(b-a)/(n-1)."""
+    nq, nr, ns = n
+    bh = [(b[k] - a[k]) / float(n[k] - 1) for k in range(3)]
+    x = a[0] + np.arange(nq) * bh[0]
+    y = a[1] + np.arange(nr) * bh[1]
+    z = a[2] + np.arange(ns) * bh[2]
+    X = np.empty((3, ns, nr, nq))
+    X[0] = x[None, None, :]
+    X[1] = y[None, :, None]
+    X[2] = z[:, None, None]
+    return X
+
+
+def gaussian_hills(xs, zs, seed, nhills=12, width=(500.0, 2000.0), height=(-300.0, 300.0)):
+    """Surface elevation e(x, z) = sum of seeded Gaussian hills."""
+    rng = np.random.default_rng(seed)
+    Lx0, Lx1 = xs.min(), xs.max()
+    Lz0, Lz1 = zs.min(), zs.max()
+    cx = rng.uniform(Lx0, Lx1, nhills)
+    cz = rng.uniform(Lz0, Lz1, nhills)
+    w = rng.uniform(width[0], width[1], nhills)
+    hgt = rng.uniform(height[0], height[1], nhills)
+    e = np.zeros((zs.size, xs.size))
+    for k in range(nhills):
+        e += hgt[k] * np.exp(-(((xs[None, :] - cx[k]) ** 2 + (zs[:, None] - cz[k]) ** 2)
+                               / (2.0 * w[k] ** 2)))
+    return e
+
+
+def topography_grid(n, h, seed=7, **hill_kw):
+    """Curvilinear block: x, z uniform with spacing h; depth coordinate y runs
+    from the surface y_top(x, z) = -e(x, z) (r = 1) down to the flat bottom
+    y = (nr-1)*h (r = nr), linearly in r (transfinite blend between the two
+    bounding surfaces)."""
+    nq, nr, ns = n
+    xs = np.arange(nq) * h
+    zs = np.arange(ns) * h
+    e = gaussian_hills(xs, zs, seed, **hill_kw)       # (ns, nq)
+    ytop = -e
+    ybot = (nr - 1) * h
+    r = np.arange(nr) / (nr - 1)
+    X = np.empty((3, ns, nr, nq))
+    X[0] = xs[None, None, :]
+    X[2] = zs[:, None, None]
+    X[1] = ytop[:, None, :] + r[None, :, None] * (ybot - ytop[:, None, :])
+    return X
+
+
+def von_karman(shape, h, seed, std=0.05, corr=1000.0, hurst=0.3):
+    """Seeded von Karman random field (spectrum (1 + k^2 a^2)^-(H + 3/2)),
+    zero mean, standard deviation ``std``; shape (nz, ny, nx)."""
+    rng = np.random.default_rng(seed)
+    noise = rng.standard_normal(shape).astype(np.float32)
+    spec = np.fft.rfftn(noise)
+    kz = np.fft.fftfreq(shape[0], d=h) * 2 * np.pi
+    ky = np.fft.fftfreq(shape[1], d=h) * 2 * np.pi
+    kx = np.fft.rfftfreq(shape[2], d=h) * 2 * np.pi
+    k2 = (kz[:, None, None] ** 2 + ky[None, :, None] ** 2 + kx[None, None, :] ** 2)
+    amp = (1.0 + k2 * corr * corr) ** (-(hurst + 1.5) / 2.0)
+    field_ = np.fft.irfftn(spec * amp.astype(np.float32), s=shape).astype(np.float64)
+    field_ -= field_.mean()
+    sd = field_.std()
+    return field_ * (std / sd) if sd > 0 else field_
+
+
+# ---------------------------------------------------------------------------
+# material and anelastic coefficients
+# ---------------------------------------------------------------------------
+def material_from_rho_s_p(n, rho, cs, cp):
+    """init_material default problem (material.f90:696-702)."""
+    nq, nr, ns = n
+    M = np.empty((3, ns, nr, nq))
+    M[0] = rho * (cp * cp - 2.0 * cs * cs)
+    M[1] = rho * cs * cs
+    M[2] = rho
+    return M
+
+
+def material_from_speeds(rho, cs, cp):
+    M = np.empty((3,) + cs.shape)
+    M[1] = rho * cs * cs
+    M[0] = rho * cp * cp - 2.0 * M[1]
+    M[2] = rho
+    return M
+
+
+def q_coefficients(nmech, fmin=0.05, fmax=20.0):
+    """build_q{4,8}_fixed_coefficients (anelastic_q8_model.f90:100-116,
+    anelastic_q4_model.f90:97-111)."""
+    pi = 3.141592653589793
+    taumin = 1.0 / (2.0 * pi * fmax)
+    taumax = 1.0 / (2.0 * pi * fmin)
+    tau = np.array([math.exp(math.log(taumin) + (2.0 * k - 1.0) / (2.0 * nmech)
+                             * math.log(taumax / taumin)) for k in range(1, nmech + 1)])
+    if nmech == 8:
+        w = np.array([1.50589707, 0.0, 0.52793567, 0.53065494,
+                      0.32862132, 0.64375916, 0.0, 1.32751442])
+    elif nmech == 4:
+        w = np.array([1.549360, 0.804277, 0.887718, 1.464160])
+    else:
+        raise ValueError("nmech must be 4 or 8")
+    return tau, w
+
+
+def modulus_correction(M, Qs_inv, Qp_inv, tau, weight, fref=1.0):
+    """Unrelaxed-modulus correction at fref (material.f90:425-453 for Q8,
+    258-280 for Q4), per node; returns the corrected M."""
+    pi = 3.141592653589793
+    wref = 2.0 * pi * fref
+    vS = np.zeros_like(Qs_inv)
+    vP = np.zeros_like(Qp_inv)
+    for l in range(len(tau)):
+        dS = (wref * wref * (tau[l] * tau[l]) + 1.0) * (1.0 / Qs_inv)
+        dP = (wref * wref * (tau[l] * tau[l]) + 1.0) * (1.0 / Qp_inv)
+        vS = vS + weight[l] / dS
+        vP = vP + weight[l] / dP
+    if np.any(vS >= 1.0) or np.any(vP >= 1.0):
+        raise ValueError("invalid anelastic modulus correction: weight/Q sum >= 1")
+    lam, mu, rho = M[0], M[1], M[2]
+    vs = np.sqrt(mu / rho)
+    vp = np.sqrt((lam + 2.0 * mu) / rho)
+    muS = rho * (vs * vs) / (1.0 - vS)
+    muP = rho * (vp * vp) / (1.0 - vP)
+    vs = np.sqrt(muS / rho)
+    vp = np.sqrt(muP / rho)
+    out = np.empty_like(M)
+    out[1] = (vs * vs) * rho
+    out[0] = (vp * vp) * rho - 2.0 * out[1]
+    out[2] = rho
+    return out
+
+
+def relaxation_dt_limit(tau):
+    """q{4,8}_relaxation_dt_limit: 2*min(tau) (anelastic_q8_model.f90:148-156)."""
+    return 2.0 * float(np.min(tau))
+
+
+def block_time_step(h, cfl, cs, cp):
+    """block_time_step (block.f90:495-510): cfl*h/sqrt(cs^2 + cp^2)."""
+    return cfl * h / math.sqrt(cs * cs + cp * cp)
+
+
+# ---------------------------------------------------------------------------
+# face frames (unit_normals.f90:8-284), vectorised
+# ---------------------------------------------------------------------------
+def face_frames(metx, mety, metz):
+    """local_orth_vectors_{q,r,s} for the six faces.
+
+    Returns lists n_l, n_m, n_n of arrays (3, n2, n1) per face:
+    q faces (3, nz, ny), r faces (3, nz, nx), s faces (3, ny, nx)."""
+    nl, nm, nn = [], [], []
+    for f in range(6):
+        ax = f // 2
+        idx = 0 if f % 2 == 0 else -1
+        sl = [slice(None)] * 3
+        sl[2 - ax] = idx                       # array dims are (z, y, x)
+        sl = tuple(sl)
+        l, m, n = face_frames_from_gradient(metx[ax][sl], mety[ax][sl], metz[ax][sl])
+        nl.append(np.ascontiguousarray(l))
+        nm.append(np.ascontiguousarray(m))
+        nn.append(np.ascontiguousarray(n))
+    return nl, nm, nn
+
+
+# ---------------------------------------------------------------------------
+# sources and receivers
+# ---------------------------------------------------------------------------
+def nearest_node(X, p):
+    """Nearest node to point p (1-based), ties to the last node in Fortran
+    loop order as in Find_Coordinates (seismogram.f90:767-876)."""
+    d2 = (X[0] - p[0]) ** 2 + (X[1] - p[1]) ** 2 + (X[2] - p[2]) ** 2
+    flat = d2.ravel()
+    best = flat.min()
+    k = np.flatnonzero(flat == best)[-1]
+    z, y, x = np.unravel_index(k, d2.shape)
+    return (int(x) + 1, int(y) + 1, int(z) + 1), tuple(float(X[c, z, y, x]) for c in range(3))
+
+
+def point_source(X, p, kind, t_init, duration, order=4, **m):
+    near, phys = nearest_node(X, p)
+    return SourceSpec(kind=kind, t_init=t_init, duration=duration, near=near, near_phys=phys,
+                      location=tuple(p), order=order, **m)
+
+
+def surface_receivers(n, count, seed, margin):
+    """Seeded receiver nodes on the r-min (free-surface) face."""
+    rng = np.random.default_rng(seed)
+    nq, nr, ns = n
+    xi = rng.integers(1 + margin, nq - margin + 1, count)
+    zi = rng.integers(1 + margin, ns - margin + 1, count)
+    return np.stack([xi, np.ones(count, dtype=np.int64), zi], axis=1)
+
+
+# ---------------------------------------------------------------------------
+# configurations
+# ---------------------------------------------------------------------------
+def kat_cartesian_41():
+    """inputfile/test_elastic_traditional_o6.in: 41^3 Cartesian [-2,2]^3 (km),
+    rho_s_p = (2.7, 3.464, 6), characteristic SAT on all faces, Cos-Bell
+    moment source (order 4) at the origin, CFL 0.2, t_final 0.02 (6 steps)."""
+    n = (41, 41, 41)
+    X = cartesian_grid(n, (-2.0, -2.0, -2.0), (2.0, 2.0, 2.0))
+    M = material_from_rho_s_p(n, 2.7, 3.464, 6.0)
+    spat = 4.0 / float(41 - 1)
+    dt = block_time_step(spat, 0.2, 3.464, 6.0)
+    nsteps = int(math.floor(0.02 / dt))
+    src = point_source(X, (0.0, 0.0, 0.0), L.WQL_SRC_COSBELL, t_init=0.05, duration=0.0,
+                       order=4, mxz=-10.0)
+    return Problem(n=n, mesh=L.WQL_CARTESIAN, X=X, M=M, bc=(1,) * 6, dt=dt, nsteps=nsteps,
+                   sources=[src], name="kat41")
+
+
+def cartesian_bump(n=65, h=100.0, nsteps=200, seed=1234, cfl=0.5, nrec=8):
+    """Plain FP64 Cartesian block: n^3 nodes, spacing h (m); rho = 2670,
+    cp = 6000, cs = 3464; free surface on r-min, characteristic SAT elsewhere;
+    zero stress and a Gaussian vx bump (sigma 300 m, 1 m/s) at the centre plus
+    seeded U[-1e-6, 1e-6] noise on all 9 fields."""
+    nn = (n, n, n)
+    L_ = (n - 1) * h
+    X = cartesian_grid(nn, (0.0, 0.0, 0.0), (L_, L_, L_))
+    rho, cp, cs = 2670.0, 6000.0, 3464.0
+    M = material_from_rho_s_p(nn, rho, cs, cp)
+    c = L_ / 2.0
+    r2 = (X[0] - c) ** 2 + (X[1] - c) ** 2 + (X[2] - c) ** 2
+    rng = np.random.default_rng(seed)
+    F0 = rng.uniform(-1.0e-6, 1.0e-6, size=(9,) + X.shape[1:])
+    F0[0] += np.exp(-r2 / (2.0 * 300.0 ** 2))
+    dt = block_time_step(h, cfl, cs, cp)
+    bc = (1, 1, 2, 1, 1, 1)
+    rec = surface_receivers(nn, nrec, seed + 1, margin=2)
+    return Problem(n=nn, mesh=L.WQL_CARTESIAN, X=X, M=M, bc=bc, dt=dt, nsteps=nsteps, F0=F0,
+                   receivers=rec, name=f"cartesian_bump_{n}")
+
+
+def curvilinear_q(n=257, h=50.0, nsteps=500, nmech=8, npml=20, seed=2024, cfl=0.5,
+                  nrec=64, f0=2.0, vk=True, hills=True, layered=True, qs_scale=50.0):
+    """Curvilinear block with topography (12 seeded Gaussian hills), layered
+    cs from 2000 to 3500 m/s with depth, cp = sqrt(3)*cs, rho = 2700, seeded
+    5 % von Karman perturbation (1 km, H 0.3), anelastic Q (nmech = 4 or 8)
+    with Qs = qs_scale*cs[km/s], Qp = 2 Qs; PML of npml nodes on the four
+    sides and the bottom; free surface on top (r-min); Ricker moment-tensor
+    source (f0) and nrec surface receivers."""
+    nn = (n, n, n)
+    X = topography_grid(nn, h, seed=seed) if hills else cartesian_grid(
+        nn, (0.0, 0.0, 0.0), ((n - 1) * h,) * 3)
+    depth = X[1] - X[1][:, :1, :]                  # depth below local surface
+    Ldep = (n - 1) * h
+    cs = 2000.0 + 1500.0 * np.clip(depth / Ldep, 0.0, 1.0) if layered else np.full(X.shape[1:], 3000.0)
+    if vk:
+        cs = cs * (1.0 + von_karman(X.shape[1:], h, seed + 11))
+    cp = math.sqrt(3.0) * cs
+    rho = np.full(X.shape[1:], 2700.0)
+    M = material_from_speeds(rho, cs, cp)
+    tau = weight = Qs_inv = Qp_inv = None
+    if nmech:
+        tau, weight = q_coefficients(nmech)
+        Qs = qs_scale * cs / 1000.0
+        Qs_inv = 1.0 / Qs
+        Qp_inv = 1.0 / (2.0 * Qs)
+        M = modulus_correction(M, Qs_inv, Qp_inv, tau, weight)
+    cs_ref, cp_ref = 3500.0, math.sqrt(3.0) * 3500.0
+    dt = block_time_step(h, cfl, cs_ref * 1.2, cp_ref * 1.2)
+    if nmech:
+        dt = min(dt, relaxation_dt_limit(tau))
+    if npml:
+        pml = (1, 1, 0, 1, 1, 1)
+        npm = (npml, npml, 0, npml, npml, npml)
+    else:
+        pml, npm = (0,) * 6, (0,) * 6
+    bc = (1, 1, 2, 1, 1, 1)
+    Lh = (n - 1) * h
+    src_pt = (0.5 * Lh, 0.35 * Lh, 0.5 * Lh)
+    src = point_source(X, src_pt, L.WQL_SRC_RICKER, t_init=1.2 / f0, duration=f0, order=4,
+                       mxx=1.0e15, myy=-0.5e15, mzz=-0.5e15, mxy=0.3e15)
+    rec = surface_receivers(nn, nrec, seed + 5, margin=max(npml, 6) + 1)
+    return Problem(n=nn, mesh=L.WQL_CURVILINEAR, X=X, M=M, bc=bc, dt=dt, nsteps=nsteps,
+                   pml=pml, npml=npm, nmech=nmech, tau=tau, weight=weight, Qs_inv=Qs_inv,
+                   Qp_inv=Qp_inv, sources=[src], receivers=rec,
+                   name=f"curvilinear_q{nmech}_{n}")
+
+
+# ---------------------------------------------------------------------------
+def make_device_block(prob: Problem, precision=L.WQL_FP64, device="cuda"):
+    """Create a :class:`Block` whose set-up arrays are float64 torch tensors on
+    the GPU: metric terms from the device pipeline (grid_metrics), face
+    frames from the device metrics, material and Q from the problem.  The
+    state starts at the init_* values (F = 0, DF = 1e40, memory = 0)."""
+    import torch
+
+    from .block import Block, grid_metrics
+    dev = torch.device(device)
+    X = torch.from_numpy(np.ascontiguousarray(prob.X)).to(dev)
+    metx, mety, metz, J = grid_metrics(X)
+    del X
+    # face frames from the device metrics (small face slices copied to host)
+    fm = []
+    for m in (metx, mety, metz):
+        fm.append(m)
+    nl, nm_, nn_ = [], [], []
+    for f in range(6):
+        ax = f // 2
+        idx = 0 if f % 2 == 0 else -1
+        sl = [slice(None)] * 3
+        sl[2 - ax] = idx
+        sl = tuple(sl)
+        q = [m[ax][sl].cpu().numpy() for m in fm]
+        parts = face_frames_from_gradient(*q)
+        for lst, arr in zip((nl, nm_, nn_), parts):
+            lst.append(torch.from_numpy(np.ascontiguousarray(arr)).to(dev))
+    kw = {}
+    if prob.nmech:
+        kw.update(Qs_inv=torch.from_numpy(np.ascontiguousarray(prob.Qs_inv)).to(dev),
+                  Qp_inv=torch.from_numpy(np.ascontiguousarray(prob.Qp_inv)).to(dev))
+    nq, nr, ns = prob.n
+    blk = Block(nq=nq, nr=nr, ns=ns, mesh=prob.mesh, precision=precision, nmech=prob.nmech,
+                bc=prob.bc, pml=prob.pml, npml=prob.npml,
+                tau=tuple(prob.tau) if prob.tau is not None else (),
+                weight=tuple(prob.weight) if prob.weight is not None else (),
+                metricx=metx, metricy=mety, metricz=metz, J=J,
+                M=torch.from_numpy(np.ascontiguousarray(prob.M)).to(dev),
+                n_l=nl, n_m=nm_, n_n=nn_, on_device=True, **kw)
+    blk.upload(L.WQL_GEOMETRY | L.WQL_MATERIAL | L.WQL_QINV | L.WQL_FACES)
+    blk.set_sources(prob.sources)
+    return blk
+
+
+def face_frames_from_gradient(qx, qy, qz):
+    """gen_orth_vectors applied to n = grad(xi)/|grad(xi)| on one face
+    (unit_normals.f90:8-284); returns (l, m, n), each (3,) + face shape."""
+    q = np.stack([qx, qy, qz])
+    nrm = np.sqrt(q[0] ** 2 + q[1] ** 2 + q[2] ** 2)
+    n = q / nrm
+    tol = 1.0e-12
+    m = np.zeros_like(n)
+    d1 = np.sqrt(n[0] ** 2 + (n[1] - 1.0) ** 2 + n[2] ** 2)
+    d2 = np.sqrt(n[0] ** 2 + (n[1] + 1.0) ** 2 + n[2] ** 2)
+    alt = ~((d1 >= tol) & (d2 >= tol))
+    m[1] = np.where(alt, 0.0, 1.0)
+    m[2] = np.where(alt, 1.0, 0.0)
+    a = n[0] * m[0] + n[1] * m[1] + n[2] * m[2]
+    m = m - a * n
+    m = (1.0 / np.sqrt(m[0] ** 2 + m[1] ** 2 + m[2] ** 2)) * m
+    l = np.stack([n[1] * m[2] - n[2] * m[1],
+                  n[2] * m[0] - n[0] * m[2],
+                  n[0] * m[1] - n[1] * m[0]])
+    return l, m, n
+
+
+def make_block(prob: Problem, metrics, precision=L.WQL_FP64, frames=None, upload=True):
+    """Create a host-array :class:`Block` from a problem and its metrics and
+    upload it (F from prob.F0 if given, otherwise the init_* state)."""
+    from .block import Block
+    metx, mety, metz, J = metrics
+    if frames is None:
+        frames = face_frames(metx, mety, metz)
+    nl, nm, nn = frames
+    nq, nr, ns = prob.n
+    kw = {}
+    if prob.nmech:
+        shp = (6, prob.nmech, ns, nr, nq)
+        kw.update(eta=np.zeros(shp), Deta=np.zeros(shp),
+                  Qs_inv=np.ascontiguousarray(prob.Qs_inv), Qp_inv=np.ascontiguousarray(prob.Qp_inv))
+    pq, pdq = [None] * 6, [None] * 6
+    from .block import pml_slab_box
+    for f in range(6):
+        if prob.pml[f]:
+            lo, hi = pml_slab_box(f, prob.n, prob.npml[f])
+            shp = (9, hi[2] - lo[2] + 1, hi[1] - lo[1] + 1, hi[0] - lo[0] + 1)
+            pq[f] = np.zeros(shp)
+            pdq[f] = np.full(shp, 1.0e40)
+    F = np.ascontiguousarray(prob.F0) if prob.F0 is not None else np.zeros((9, ns, nr, nq))
+    blk = Block(nq=nq, nr=nr, ns=ns, mesh=prob.mesh, precision=precision, nmech=prob.nmech,
+                bc=prob.bc, pml=prob.pml, npml=prob.npml,
+                tau=tuple(prob.tau) if prob.tau is not None else (),
+                weight=tuple(prob.weight) if prob.weight is not None else (),
+                metricx=np.ascontiguousarray(metx), metricy=np.ascontiguousarray(mety),
+                metricz=np.ascontiguousarray(metz), J=np.ascontiguousarray(J),
+                M=np.ascontiguousarray(prob.M), F=F, DF=np.full((9, ns, nr, nq), 1.0e40),
+                pml_Q=pq, pml_DQ=pdq, n_l=nl, n_m=nm, n_n=nn, **kw)
+    if upload:
+        blk.upload(L.WQL_GEOMETRY | L.WQL_MATERIAL | L.WQL_QINV | L.WQL_FACES | L.WQL_F)
+        blk.set_sources(prob.sources)
+    return blk
