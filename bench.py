@@ -120,11 +120,25 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=97)
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="only run a device copy of this many MiB 5 times (scale factor "
+                         "for FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one MI355X; multi-GPU runs are out of scope "
               "(one block per GPU, no halo exchange). Use --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration_mib:
+        import torch
+        n = args.copy_calibration_mib * 2 ** 20 // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n,
+                          "launches": 5}), flush=True)
+        return
 
     import ctypes
 

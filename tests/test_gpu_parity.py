@@ -11,7 +11,7 @@ import pytest
 
 from waveqlab3d_q_gpu_amd import _lib as L
 from waveqlab3d_q_gpu_amd import problems as PB
-from waveqlab3d_q_gpu_amd.block import time_step_RK
+from waveqlab3d_q_gpu_amd.block import pml_slab_box, time_step_RK
 
 from helpers import (field_errors, gpu_block, oracle_block, receiver_flat, rel_l2,
                      small_curvilinear)
@@ -90,7 +90,7 @@ def test_curvilinear_q_pml(oracle, nmech, fused):
         assert rel_l2(blk.Deta, orc.Deta) <= TOL64
     for f in range(6):
         if prob.pml[f]:
-            lo, hi = PB.pml_slab_box(f, prob.n, prob.npml[f])
+            lo, hi = pml_slab_box(f, prob.n, prob.npml[f])
             sl = (slice(None), slice(lo[2] - 1, hi[2]), slice(lo[1] - 1, hi[1]),
                   slice(lo[0] - 1, hi[0]))
             assert rel_l2(blk.pml_Q[f], orc.pQ[f][sl]) <= TOL64, f
@@ -116,7 +116,7 @@ def test_stage_level_rates(oracle):
     blk.eta[...] = orc.eta
     for f in range(6):
         if prob.pml[f]:
-            lo, hi = PB.pml_slab_box(f, prob.n, prob.npml[f])
+            lo, hi = pml_slab_box(f, prob.n, prob.npml[f])
             blk.pml_Q[f][...] = orc.pQ[f][:, lo[2] - 1:hi[2], lo[1] - 1:hi[1], lo[0] - 1:hi[0]]
     blk.upload(L.WQL_MEMORY | L.WQL_PML)
     t = 0.13

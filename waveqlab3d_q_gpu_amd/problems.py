@@ -109,7 +109,7 @@ def von_karman(shape, h, seed, std=0.05, corr=1000.0, hurst=0.3):
     kx = np.fft.rfftfreq(shape[2], d=h) * 2 * np.pi
     k2 = (kz[:, None, None] ** 2 + ky[None, :, None] ** 2 + kx[None, None, :] ** 2)
     amp = (1.0 + k2 * corr * corr) ** (-(hurst + 1.5) / 2.0)
-    field_ = np.fft.irfftn(spec * amp.astype(np.float32), s=shape).astype(np.float64)
+    field_ = np.fft.irfftn(spec * amp.astype(np.float32), s=shape, axes=(0, 1, 2)).astype(np.float64)
     field_ -= field_.mean()
     sd = field_.std()
     return field_ * (std / sd) if sd > 0 else field_
