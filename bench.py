@@ -136,7 +136,17 @@ def main():
         for _ in range(5):
             b.copy_(a)
         torch.cuda.synchronize()
+        # the library's own streaming kernels on n/2 doubles: k_scale reads and
+        # writes 4n bytes, k_update reads 8n and writes 4n bytes
+        from waveqlab3d_q_gpu_amd import scale_rate_array, update_state_array
+        x = torch.ones(n // 2, dtype=torch.float64, device="cuda")
+        y = torch.ones(n // 2, dtype=torch.float64, device="cuda")
+        for _ in range(5):
+            scale_rate_array(x, 0.5)
+            update_state_array(y, x, 0.25)
+        torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n,
+                          "k_scale_bytes_read": 4 * n, "k_update_bytes_read": 8 * n,
                           "launches": 5}), flush=True)
         return
 

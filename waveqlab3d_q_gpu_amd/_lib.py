@@ -97,6 +97,14 @@ def lib():
     """Load the library (raises if it has not been built)."""
     global _lib
     if _lib is None:
+        # One HIP runtime per process: when PyTorch is importable its bundled
+        # libamdhip64.so.7 is loaded first, and the library's own dependency
+        # on libamdhip64.so.7 then binds to that same runtime (two runtimes
+        # in one process would each claim the device).
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(
                 f"{LIB_PATH} is missing: build it with `python -m waveqlab3d_q_gpu_amd.build`")

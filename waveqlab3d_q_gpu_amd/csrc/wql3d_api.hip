@@ -80,6 +80,7 @@ struct wql_block {
   double* seis;
   int slot;
   long long bytes;
+  int tune_nt, tune_xcd, tune_band;  // node-kernel launch shape
 };
 
 static int g_device = -1;
@@ -190,6 +191,19 @@ extern "C" int wql_block_create(const wql_block_desc* d, wql_block** out) {
   b->nx = d->nq; b->ny = d->nr; b->nz = d->ns;
   b->P = (long long)b->nx * b->ny * b->nz;
   b->esz = d->precision == WQL_FP64 ? sizeof(double) : sizeof(float);
+  // launch shape of the node kernel: "nontemporal,xcd_remap,band_rows"
+  // (env WQL3D_NODE_TUNING); band_rows 0 = whole y extent
+  b->tune_nt = 1;
+  b->tune_xcd = 1;
+  b->tune_band = 0;
+  if (const char* tv = getenv("WQL3D_NODE_TUNING")) {
+    int v0 = 1, v1 = 1, v2 = 0;
+    if (sscanf(tv, "%d,%d,%d", &v0, &v1, &v2) == 3) {
+      b->tune_nt = v0 != 0;
+      b->tune_xcd = v1 != 0;
+      b->tune_band = v2 > 0 ? v2 : 0;
+    }
+  }
   hipError_t e = hipStreamCreate(&b->stream);  // default priority
   if (e != hipSuccess) { free(b); return fail(WQL_EHIP, "hipStreamCreate: %s", hipGetErrorString(e)); }
   const long long P = b->P;
@@ -622,6 +636,10 @@ static StepArgs<T> make_args(wql_block* b, double t) {
   a.ix0 = lo(0); a.ix1 = b->nx - 1 - lo(1);
   a.iy0 = lo(2); a.iy1 = b->ny - 1 - lo(3);
   a.iz0 = lo(4); a.iz1 = b->nz - 1 - lo(5);
+  a.band_rows = (b->tune_band > 0 && b->tune_band < b->ny) ? b->tune_band : b->ny;
+  a.segs = (int)(((long long)a.band_rows * b->nx + 255) / 256);
+  a.xcd_remap = b->tune_xcd;
+  a.nontemporal = b->tune_nt;
   return a;
 }
 

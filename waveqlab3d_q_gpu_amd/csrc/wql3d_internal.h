@@ -60,6 +60,9 @@ struct StepArgs {
   DevSource src[kMaxSources];
   double A, Bdt;    // fused stage coefficients
   int ix0, ix1, iy0, iy1, iz0, iz1;  // strict interior [lo, hi] 0-based (RHS_Center)
+  // node-kernel launch shape (see map_node): y-band height, 256-node segments
+  // per band slab, XCD-contiguous remap, non-temporal streams
+  int band_rows, segs, xcd_remap, nontemporal;
 };
 
 // Launchers (wql3d_kernels.hip).  All return hipError_t of the launch.

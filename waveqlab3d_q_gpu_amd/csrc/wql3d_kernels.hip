@@ -321,26 +321,62 @@ __device__ __forceinline__ bool source_at(const StepArgs<T>& a, int x, int y, in
 //   MODE 2: rates only, strict interior nodes (RHS_Center)
 // In MODE 0 rates include the moment-tensor source and the SAT penalty.
 // ---------------------------------------------------------------------------
-template <typename T, int MESH, int NMECH, int MODE>
+// Streams read and written once per stage (DF, memory variables, PML
+// auxiliaries, F_out, material/Q at the node) may bypass cache residency with
+// non-temporal accesses so the stencil operands (F_in, geometry) keep it.
+template <bool NT, typename V>
+__device__ __forceinline__ V ld_stream(const V* p) {
+  if constexpr (NT) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+template <bool NT, typename V>
+__device__ __forceinline__ void st_stream(V* p, V v) {
+  if constexpr (NT) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
+
+// Workgroup -> node mapping.  Workgroups are issued round-robin over the 8
+// XCDs; with xcd_remap the hardware index is permuted so each XCD receives a
+// contiguous range of logical indices.  Logical order is band-major: bands of
+// band_rows y-rows, then z, then 256-node segments of the band's (x,y) slab,
+// so consecutive work on one XCD shares y and z stencil neighbours in its L2.
+template <typename T>
+__device__ __forceinline__ bool map_node(const StepArgs<T>& a, int& x, int& y, int& z) {
+  const unsigned h = blockIdx.x;
+  unsigned L = h;
+  if (a.xcd_remap) {
+    const unsigned B = gridDim.x, q = B >> 3, r = B & 7u, j = h & 7u, k = h >> 3;
+    L = j * q + (j < r ? j : r) + k;
+  }
+  const unsigned seg = L % (unsigned)a.segs;
+  const unsigned bz = L / (unsigned)a.segs;
+  z = (int)(bz % (unsigned)a.nz);
+  const int band = (int)(bz / (unsigned)a.nz);
+  const int tb = (int)seg * 256 + (int)threadIdx.x;
+  const int yb = tb / a.nx;
+  x = tb - yb * a.nx;
+  y = band * a.band_rows + yb;
+  return yb < a.band_rows && y < a.ny;
+}
+
+template <typename T, int MESH, int NMECH, int MODE, bool NT>
 __global__ __launch_bounds__(256) void k_node(const StepArgs<T> a) {
-  const long long plane = (long long)a.nx * a.ny;
-  const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (t >= plane) return;
-  const int z = blockIdx.y;
-  const int y = (int)(t / a.nx);
-  const int x = (int)(t - (long long)y * a.nx);
+  int x, y, z;
+  if (!map_node(a, x, y, z)) return;
   if (MODE == 2) {
     if (x < a.ix0 || x > a.ix1 || y < a.iy0 || y > a.iy1 || z < a.iz0 || z > a.iz1) return;
   }
+  const long long plane = (long long)a.nx * a.ny;
   const long long P = a.P;
-  const long long id = t + (long long)z * plane;
+  const long long id = x + (long long)a.nx * y + (long long)z * plane;
   const int pos[3] = {x, y, z};
 
   T U[3][9];
   if (MESH) derivs_curv<T>(a, pos, id, U);
   else derivs_cart<T>(a, pos, id, U);
 
-  const T lam = a.M[id], mu = a.M[P + id], rho = a.M[2 * P + id];
+  const T lam = ld_stream<NT>(a.M + id), mu = ld_stream<NT>(a.M + P + id),
+          rho = ld_stream<NT>(a.M + 2 * P + id);
   const T Jp = MESH ? a.G[9 * P + id] : T(1);
   const T rhoJ_inv = T(1) / (rho * Jp);
   const T l2m = lam + T(2) * mu;
@@ -410,7 +446,7 @@ __global__ __launch_bounds__(256) void k_node(const StepArgs<T> a) {
       e[5] = e[5] - dd[1] * qy[2] - dd[2] * qz[2];
     }
     const T tr = e[0] + e[1] + e[2];
-    const T Qs = a.Qs[id], Qp = a.Qp[id];
+    const T Qs = ld_stream<NT>(a.Qs + id), Qp = ld_stream<NT>(a.Qp + id);
     const T twomuQs = T(2) * mu * Qs;
     const T pfac = l2m * Qp - twomuQs;
     const T muQs = mu * Qs;
@@ -420,7 +456,7 @@ __global__ __launch_bounds__(256) void k_node(const StepArgs<T> a) {
       T sum = T(0);
 #pragma unroll
       for (int i = 0; i < NMECH; ++i) {
-        ev[i] = a.eta[((long long)(c * NMECH + i)) * P + id];
+        ev[i] = ld_stream<NT>(a.eta + ((long long)(c * NMECH + i)) * P + id);
         sum += ev[i];
       }
       DU[3 + c] -= sum;
@@ -432,9 +468,9 @@ __global__ __launch_bounds__(256) void k_node(const StepArgs<T> a) {
         const T r = (src - ev[i]) * ti;
         const long long k = ((long long)(c * NMECH + i)) * P + id;
         if (MODE == 0) {
-          const T dn = scale_rate(a.A, a.Deta[k]) + r;
-          a.Deta[k] = dn;
-          a.eta[k] = ev[i] + (T)a.Bdt * dn;
+          const T dn = scale_rate(a.A, ld_stream<NT>(a.Deta + k)) + r;
+          st_stream<NT>(a.Deta + k, dn);
+          st_stream<NT>(a.eta + k, ev[i] + (T)a.Bdt * dn);
         } else {
           a.Deta[k] = a.Deta[k] + r;
         }
@@ -490,7 +526,7 @@ __global__ __launch_bounds__(256) void k_node(const StepArgs<T> a) {
       const long long si = sidx[ax];
       T q[9];
 #pragma unroll
-      for (int c = 0; c < 9; ++c) q[c] = Q[c * Ps + si];
+      for (int c = 0; c < 9; ++c) q[c] = ld_stream<NT>(Q + c * Ps + si);
       DU[0] -= d * Jp * q[0] * rhoJ_inv;
       DU[1] -= d * Jp * q[1] * rhoJ_inv;
       DU[2] -= d * Jp * q[2] * rhoJ_inv;
@@ -531,9 +567,9 @@ __global__ __launch_bounds__(256) void k_node(const StepArgs<T> a) {
         T r = s[c] - (d + cfs) * q[c];
         if (fsat) r -= pen * sat[c];
         if (MODE == 0) {
-          const T dn = scale_rate(a.A, DQ[c * Ps + si]) + r;
-          DQ[c * Ps + si] = dn;
-          Q[c * Ps + si] = q[c] + (T)a.Bdt * dn;
+          const T dn = scale_rate(a.A, ld_stream<NT>(DQ + c * Ps + si)) + r;
+          st_stream<NT>(DQ + c * Ps + si, dn);
+          st_stream<NT>(Q + c * Ps + si, q[c] + (T)a.Bdt * dn);
         } else {
           DQ[c * Ps + si] = DQ[c * Ps + si] + r;
         }
@@ -546,11 +582,11 @@ __global__ __launch_bounds__(256) void k_node(const StepArgs<T> a) {
     const bool has_src = a.nsrc > 0 && source_at<T, MESH>(a, x, y, z, id, srcv);
 #pragma unroll
     for (int c = 0; c < 9; ++c) {
-      double df = scale_rate(a.A, a.DF[c * P + id]) + (double)DU[c];
+      double df = scale_rate(a.A, ld_stream<NT>(a.DF + c * P + id)) + (double)DU[c];
       if (has_src && c >= 3) df -= srcv[c - 3];
       if (has_sat) df -= satDF[c];
-      a.DF[c * P + id] = df;
-      a.Fout[c * P + id] = a.Fin[c * P + id] + a.Bdt * df;
+      st_stream<NT>(a.DF + c * P + id, df);
+      st_stream<NT>(a.Fout + c * P + id, a.Fin[c * P + id] + a.Bdt * df);
     }
   } else {
 #pragma unroll
@@ -735,19 +771,26 @@ static inline dim3 plane_grid(int nx, int ny, int nz) {
   return dim3((unsigned)((plane + 255) / 256), (unsigned)nz, 1);
 }
 
-template <typename T, int MODE>
-static hipError_t launch_node(const StepArgs<T>& a, int mesh, hipStream_t s) {
-  const dim3 g = plane_grid(a.nx, a.ny, a.nz);
+template <typename T, int MODE, bool NT>
+static hipError_t launch_node_nt(const StepArgs<T>& a, int mesh, hipStream_t s) {
+  const long long nbands = (a.ny + a.band_rows - 1) / a.band_rows;
+  const dim3 g((unsigned)(nbands * a.nz * a.segs));
   const dim3 b(256);
 #define WQL_CASE(MS, NM)                                                   \
   if (mesh == MS && a.nmech == NM) {                                       \
-    hipLaunchKernelGGL((k_node<T, MS, NM, MODE>), g, b, 0, s, a);          \
+    hipLaunchKernelGGL((k_node<T, MS, NM, MODE, NT>), g, b, 0, s, a);      \
     return hipGetLastError();                                              \
   }
   WQL_CASE(0, 0) WQL_CASE(0, 4) WQL_CASE(0, 8)
   WQL_CASE(1, 0) WQL_CASE(1, 4) WQL_CASE(1, 8)
 #undef WQL_CASE
   return hipErrorInvalidValue;
+}
+
+template <typename T, int MODE>
+static hipError_t launch_node(const StepArgs<T>& a, int mesh, hipStream_t s) {
+  if (MODE == 0 && a.nontemporal) return launch_node_nt<T, MODE, true>(a, mesh, s);
+  return launch_node_nt<T, MODE, false>(a, mesh, s);
 }
 
 template <typename T>
