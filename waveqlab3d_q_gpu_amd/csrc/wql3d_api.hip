@@ -80,7 +80,9 @@ struct wql_block {
   double* seis;
   int slot;
   long long bytes;
-  int tune_nt, tune_xcd, tune_band;  // node-kernel launch shape
+  int tune_nt, tune_xcd, tune_band, tune_split;  // node-kernel launch shape
+  void* W;        // 15 planes of T: split anelastic stage hand-off
+  int mech_zero;  // mechanisms whose eta and Deta are known to be all zero
 };
 
 static int g_device = -1;
@@ -168,7 +170,7 @@ extern "C" void wql_block_destroy(wql_block* b) {
   if (!b) return;
   (void)hipStreamSynchronize(b->stream);
   void* ptrs[] = {b->F[0], b->F[1], b->DF, b->G, b->M, b->eta, b->Deta, b->Qs, b->Qp,
-                  b->rec_idx, b->seis};
+                  b->rec_idx, b->seis, b->W};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (int f = 0; f < 6; ++f) {
@@ -196,12 +198,14 @@ extern "C" int wql_block_create(const wql_block_desc* d, wql_block** out) {
   b->tune_nt = 1;
   b->tune_xcd = 1;
   b->tune_band = 0;
+  b->tune_split = 1;
   if (const char* tv = getenv("WQL3D_NODE_TUNING")) {
-    int v0 = 1, v1 = 1, v2 = 0;
-    if (sscanf(tv, "%d,%d,%d", &v0, &v1, &v2) == 3) {
+    int v0 = 1, v1 = 1, v2 = 0, v3 = 1;
+    if (sscanf(tv, "%d,%d,%d,%d", &v0, &v1, &v2, &v3) >= 3) {
       b->tune_nt = v0 != 0;
       b->tune_xcd = v1 != 0;
       b->tune_band = v2 > 0 ? v2 : 0;
+      b->tune_split = v3 != 0;
     }
   }
   hipError_t e = hipStreamCreate(&b->stream);  // default priority
@@ -223,6 +227,7 @@ extern "C" int wql_block_create(const wql_block_desc* d, wql_block** out) {
     ALLOC(b->Deta, 6LL * d->nmech * P * b->esz);
     ALLOC(b->Qs, P * b->esz);
     ALLOC(b->Qp, P * b->esz);
+    ALLOC(b->W, 15 * P * b->esz);
   }
   const int n[3] = {b->nx, b->ny, b->nz};
   for (int f = 0; f < 6; ++f) {
@@ -455,6 +460,7 @@ static int transfer(wql_block* b, uint32_t mask, bool up) {
     for (int c = 0; c < 3 && !rc; ++c) rc = xfer((T*)b->M + c * P, d.M, c, "M", lo1, n, body);
   if (rc) return rc;
   if (d.nmech && (mask & WQL_MEMORY)) {
+    if (up) b->mech_zero = 0;  // uploaded memory variables may be non-zero
     for (int c6 = 0; c6 < 6 && !rc; ++c6)
       for (int i = 0; i < d.nmech && !rc; ++i) {
         rc = xfer((T*)b->eta + (c6 * d.nmech + i) * P, d.eta[c6], i, "eta", lo1, n, body);
@@ -522,6 +528,7 @@ static int init_state_t(wql_block* b) {
 extern "C" int wql_block_init_state(wql_block* b) {
   if (!b) return fail(WQL_EINVAL, "null block");
   b->cur = 0;
+  b->mech_zero = (1 << b->d.nmech) - 1;  // eta = Deta = 0 after the fill below
   b->slot = 0;
   return b->d.precision == WQL_FP64 ? init_state_t<double>(b) : init_state_t<float>(b);
 }
@@ -640,6 +647,14 @@ static StepArgs<T> make_args(wql_block* b, double t) {
   a.segs = (int)(((long long)a.band_rows * b->nx + 255) / 256);
   a.xcd_remap = b->tune_xcd;
   a.nontemporal = b->tune_nt;
+  a.W = (T*)b->W;
+  a.split = b->tune_split;
+  // a zero-weight mechanism whose state is all zero stays zero:
+  // Deta = A*0 + (0 - 0)/tau = 0, eta = 0 + Bdt*0 = 0
+  int zero_weight = 0;
+  for (int i = 0; i < d.nmech; ++i)
+    if (d.weight[i] == 0.0) zero_weight |= 1 << i;
+  a.mech_skip = zero_weight & b->mech_zero;
   return a;
 }
 

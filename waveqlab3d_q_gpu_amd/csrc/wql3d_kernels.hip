@@ -413,8 +413,9 @@ __global__ __launch_bounds__(256) void k_node(const StepArgs<T> a) {
   const bool inpml = fa[0] >= 0 || fa[1] >= 0 || fa[2] >= 0;
 
   // anelastic Q4/Q8 (RHS_Interior.f90:2599-2906; interior JU_xJU_yJU_z6.f90:16299-16376)
+  constexpr bool FUSED = (MODE == 0 || MODE == 3);
+  T e[6];
   if (NMECH > 0) {
-    T e[6];
     e[0] = U[0][0];
     e[1] = U[1][1];
     e[2] = U[2][2];
@@ -445,6 +446,8 @@ __global__ __launch_bounds__(256) void k_node(const StepArgs<T> a) {
       e[4] = e[4] - dd[0] * qx[2] - dd[2] * qz[1];
       e[5] = e[5] - dd[1] * qy[2] - dd[2] * qz[2];
     }
+  }
+  if (NMECH > 0 && MODE != 3) {
     const T tr = e[0] + e[1] + e[2];
     const T Qs = ld_stream<NT>(a.Qs + id), Qp = ld_stream<NT>(a.Qp + id);
     const T twomuQs = T(2) * mu * Qs;
@@ -481,7 +484,7 @@ __global__ __launch_bounds__(256) void k_node(const StepArgs<T> a) {
   // SAT penalty terms at face nodes (impose_boundary_condition), fused mode
   double satDF[9];
   bool has_sat = false;
-  if (MODE == 0) {
+  if (FUSED) {
 #pragma unroll
     for (int f = 0; f < 6; ++f) {
       if (a.bc[f] <= 0) continue;
@@ -551,7 +554,7 @@ __global__ __launch_bounds__(256) void k_node(const StepArgs<T> a) {
       // SAT on the slab's own face adds to this slab's DQ (RHS_Interior.f90:1640-1643)
       T sat[9];
       bool fsat = false;
-      if (MODE == 0 && a.bc[f] > 0) {
+      if (FUSED && a.bc[f] > 0) {
         long long fidx, NF;
         if (on_face(a, f, x, y, z, &fidx, &NF)) {
           T u[9];
@@ -566,7 +569,7 @@ __global__ __launch_bounds__(256) void k_node(const StepArgs<T> a) {
       for (int c = 0; c < 9; ++c) {
         T r = s[c] - (d + cfs) * q[c];
         if (fsat) r -= pen * sat[c];
-        if (MODE == 0) {
+        if (FUSED) {
           const T dn = scale_rate(a.A, ld_stream<NT>(DQ + c * Ps + si)) + r;
           st_stream<NT>(DQ + c * Ps + si, dn);
           st_stream<NT>(Q + c * Ps + si, q[c] + (T)a.Bdt * dn);
@@ -588,9 +591,78 @@ __global__ __launch_bounds__(256) void k_node(const StepArgs<T> a) {
       st_stream<NT>(a.DF + c * P + id, df);
       st_stream<NT>(a.Fout + c * P + id, a.Fin[c * P + id] + a.Bdt * df);
     }
+  } else if (MODE == 3) {
+    // hand the complete non-memory-variable rates and the (PML-modified)
+    // strain rates to k_memory_update through the W planes
+    double srcv[6];
+    const bool has_src = a.nsrc > 0 && source_at<T, MESH>(a, x, y, z, id, srcv);
+#pragma unroll
+    for (int c = 0; c < 9; ++c) {
+      T r = DU[c];
+      if (has_src && c >= 3) r -= (T)srcv[c - 3];
+      if (has_sat) r -= (T)satDF[c];
+      st_stream<NT>(a.W + c * P + id, r);
+    }
+#pragma unroll
+    for (int c = 0; c < 6; ++c) st_stream<NT>(a.W + (9 + c) * P + id, e[c]);
   } else {
 #pragma unroll
     for (int c = 0; c < 9; ++c) a.DF[c * P + id] = a.DF[c * P + id] + (double)DU[c];
+  }
+}
+
+// Memory-variable part of a fused anelastic stage, one thread per node over
+// the flat node index (pure streaming, unit stride): W holds the rates
+// without the memory-variable terms (planes 0..8) and the strain rates used
+// by the relaxation equations (planes 9..14), both from k_node MODE 3.
+//   DU(4..9) -= sum_i eta_i;  Deta_i = A*Deta_i + (src_i(eps) - eta_i)/tau_i;
+//   eta_i += Bdt*Deta_i;  DF = A*DF + DU;  F_out = F_in + Bdt*DF
+template <typename T, int NMECH, bool NT>
+__global__ __launch_bounds__(256) void k_memory_update(const StepArgs<T> a) {
+  const long long id = (long long)blockIdx.x * 256 + threadIdx.x;
+  const long long P = a.P;
+  if (id >= P) return;
+  T DU[9], e[6];
+#pragma unroll
+  for (int c = 0; c < 9; ++c) DU[c] = ld_stream<NT>(a.W + c * P + id);
+#pragma unroll
+  for (int c = 0; c < 6; ++c) e[c] = ld_stream<NT>(a.W + (9 + c) * P + id);
+  const T lam = ld_stream<NT>(a.M + id), mu = ld_stream<NT>(a.M + P + id);
+  const T l2m = lam + T(2) * mu;
+  const T tr = e[0] + e[1] + e[2];
+  const T Qs = ld_stream<NT>(a.Qs + id), Qp = ld_stream<NT>(a.Qp + id);
+  const T twomuQs = T(2) * mu * Qs;
+  const T pfac = l2m * Qp - twomuQs;
+  const T muQs = mu * Qs;
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    T ev[NMECH];
+    T sum = T(0);
+#pragma unroll
+    for (int i = 0; i < NMECH; ++i) {
+      if (a.mech_skip & (1 << i)) { ev[i] = T(0); continue; }
+      ev[i] = ld_stream<NT>(a.eta + ((long long)(c * NMECH + i)) * P + id);
+      sum += ev[i];
+    }
+    DU[3 + c] -= sum;
+#pragma unroll
+    for (int i = 0; i < NMECH; ++i) {
+      if (a.mech_skip & (1 << i)) continue;
+      const T w = (T)a.wgt[i];
+      const T ti = (T)a.tau_inv[i];
+      const T src = c < 3 ? (w * twomuQs) * e[c] + (w * pfac) * tr : (w * muQs) * e[c];
+      const T r = (src - ev[i]) * ti;
+      const long long k = ((long long)(c * NMECH + i)) * P + id;
+      const T dn = scale_rate(a.A, ld_stream<NT>(a.Deta + k)) + r;
+      st_stream<NT>(a.Deta + k, dn);
+      st_stream<NT>(a.eta + k, ev[i] + (T)a.Bdt * dn);
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 9; ++c) {
+    const double df = scale_rate(a.A, ld_stream<NT>(a.DF + c * P + id)) + (double)DU[c];
+    st_stream<NT>(a.DF + c * P + id, df);
+    st_stream<NT>(a.Fout + c * P + id, ld_stream<NT>(a.Fin + c * P + id) + a.Bdt * df);
   }
 }
 
@@ -789,12 +861,26 @@ static hipError_t launch_node_nt(const StepArgs<T>& a, int mesh, hipStream_t s) 
 
 template <typename T, int MODE>
 static hipError_t launch_node(const StepArgs<T>& a, int mesh, hipStream_t s) {
-  if (MODE == 0 && a.nontemporal) return launch_node_nt<T, MODE, true>(a, mesh, s);
+  if ((MODE == 0 || MODE == 3) && a.nontemporal) return launch_node_nt<T, MODE, true>(a, mesh, s);
   return launch_node_nt<T, MODE, false>(a, mesh, s);
+}
+
+template <typename T, bool NT>
+static hipError_t launch_memory_update(const StepArgs<T>& a, hipStream_t s) {
+  const dim3 g((unsigned)((a.P + 255) / 256));
+  if (a.nmech == 4) hipLaunchKernelGGL((k_memory_update<T, 4, NT>), g, dim3(256), 0, s, a);
+  else if (a.nmech == 8) hipLaunchKernelGGL((k_memory_update<T, 8, NT>), g, dim3(256), 0, s, a);
+  else return hipErrorInvalidValue;
+  return hipGetLastError();
 }
 
 template <typename T>
 hipError_t launch_stage_fused(const StepArgs<T>& a, int mesh, hipStream_t s) {
+  if (a.nmech > 0 && a.split && a.W) {
+    hipError_t e = launch_node<T, 3>(a, mesh, s);
+    if (e != hipSuccess) return e;
+    return a.nontemporal ? launch_memory_update<T, true>(a, s) : launch_memory_update<T, false>(a, s);
+  }
   return launch_node<T, 0>(a, mesh, s);
 }
 template hipError_t launch_stage_fused<double>(const StepArgs<double>&, int, hipStream_t);
