@@ -195,7 +195,7 @@ extern "C" int wql_block_create(const wql_block_desc* d, wql_block** out) {
   b->esz = d->precision == WQL_FP64 ? sizeof(double) : sizeof(float);
   // launch shape of the node kernel: "nontemporal,xcd_remap,band_rows"
   // (env WQL3D_NODE_TUNING); band_rows 0 = whole y extent
-  b->tune_nt = 1;
+  b->tune_nt = 0;
   b->tune_xcd = 1;
   b->tune_band = 0;
   b->tune_split = 1;

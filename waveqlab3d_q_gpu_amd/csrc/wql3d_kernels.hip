@@ -11,6 +11,8 @@
 // the 6x9 closure table (reference JU_xJU_yJU_z6.f90:74-299), other rows the
 // 6-tap central stencil (JU_xJU_yJU_z6.f90:53-59).  Closure rows differ only
 // per (y,z) line for the y/z axes, so wavefronts stay convergent there.
+#include <type_traits>
+
 #include "wql3d_internal.h"
 
 namespace wql {
@@ -74,12 +76,32 @@ __device__ __forceinline__ float scale_rate(double A, float x) {
 // (0 metricx, 1 metricy, 2 metricz): c<3 -> sum_d X_m(pt,d)/h_d D_d(v_c);
 // c>=3 -> sum_d 1/h_d D_d(J X_m(.,d) sigma_c).
 // ---------------------------------------------------------------------------
+// Node offsets are 32-bit (a plane holds < 2^29 nodes, so byte offsets fit in
+// 32 bits) and every plane is addressed from its own wavefront-uniform base
+// pointer: the load then takes a scalar base plus one 32-bit vector offset,
+// shared by all plane loads of a tap.
+template <typename V>
+__device__ __forceinline__ V ldp(const V* base, unsigned idx) {
+  return *reinterpret_cast<const V*>(reinterpret_cast<const char*>(base) + idx * (unsigned)sizeof(V));
+}
+template <typename V>
+__device__ __forceinline__ V* atp(V* base, unsigned idx) {
+  using Byte = typename std::conditional<std::is_const<V>::value, const char, char>::type;
+  return reinterpret_cast<V*>(reinterpret_cast<Byte*>(base) + idx * (unsigned)sizeof(V));
+}
 template <typename T>
 __device__ __forceinline__ void derivs_curv(const StepArgs<T>& a, const int pos[3],
-                                            long long id, T U[3][9]) {
+                                            long long id64, T U[3][9]) {
   const long long P = a.P;
-  const long long st[3] = {1, a.nx, (long long)a.nx * a.ny};
+  const unsigned id = (unsigned)id64;
+  const unsigned st[3] = {1u, (unsigned)a.nx, (unsigned)a.nx * (unsigned)a.ny};
   const int n[3] = {a.nx, a.ny, a.nz};
+  const double* Fc[9];
+#pragma unroll
+  for (int c = 0; c < 9; ++c) Fc[c] = a.Fin + c * P;
+  const T* Gk[10];
+#pragma unroll
+  for (int k = 0; k < 10; ++k) Gk[k] = a.G + k * P;
   T dv[3][3];
   T S[3][6];
 #pragma unroll
@@ -91,18 +113,18 @@ __device__ __forceinline__ void derivs_curv(const StepArgs<T>& a, const int pos[
     const T ih = (T)a.ih[d];
     T v0 = T(0), v1 = T(0), v2 = T(0);
     for_taps(pos[d], n[d], [&](int off, double cd) {
-      const long long p = id + (long long)off * st[d];
+      const unsigned p = id + (unsigned)off * st[d];
       const T ck = ih * (T)cd;
-      v0 += ck * (T)a.Fin[p];
-      v1 += ck * (T)a.Fin[P + p];
-      v2 += ck * (T)a.Fin[2 * P + p];
-      const T j = a.G[9 * P + p];
-      const T jm0 = j * a.G[(0 + d) * P + p];
-      const T jm1 = j * a.G[(3 + d) * P + p];
-      const T jm2 = j * a.G[(6 + d) * P + p];
+      v0 += ck * (T)ldp(Fc[0], p);
+      v1 += ck * (T)ldp(Fc[1], p);
+      v2 += ck * (T)ldp(Fc[2], p);
+      const T j = ldp(Gk[9], p);
+      const T jm0 = j * ldp(Gk[0 + d], p);
+      const T jm1 = j * ldp(Gk[3 + d], p);
+      const T jm2 = j * ldp(Gk[6 + d], p);
 #pragma unroll
       for (int c = 0; c < 6; ++c) {
-        const T t = ck * (T)a.Fin[(3 + c) * P + p];
+        const T t = ck * (T)ldp(Fc[3 + c], p);
         S[0][c] += jm0 * t;
         S[1][c] += jm1 * t;
         S[2][c] += jm2 * t;
@@ -114,8 +136,8 @@ __device__ __forceinline__ void derivs_curv(const StepArgs<T>& a, const int pos[
   }
 #pragma unroll
   for (int m = 0; m < 3; ++m) {
-    const T X0 = a.G[(3 * m + 0) * P + id], X1 = a.G[(3 * m + 1) * P + id],
-            X2 = a.G[(3 * m + 2) * P + id];
+    const T X0 = ldp(Gk[3 * m + 0], id), X1 = ldp(Gk[3 * m + 1], id),
+            X2 = ldp(Gk[3 * m + 2], id);
 #pragma unroll
     for (int c = 0; c < 3; ++c) U[m][c] = X0 * dv[0][c] + X1 * dv[1][c] + X2 * dv[2][c];
 #pragma unroll
@@ -127,20 +149,24 @@ __device__ __forceinline__ void derivs_curv(const StepArgs<T>& a, const int pos[
 // (JU_xJU_yJU_z6.f90:16426-16470): U[d][c] = X_d(pt) / h_d * D_d(F_c).
 template <typename T>
 __device__ __forceinline__ void derivs_cart(const StepArgs<T>& a, const int pos[3],
-                                            long long id, T U[3][9]) {
+                                            long long id64, T U[3][9]) {
   const long long P = a.P;
-  const long long st[3] = {1, a.nx, (long long)a.nx * a.ny};
+  const unsigned id = (unsigned)id64;
+  const unsigned st[3] = {1u, (unsigned)a.nx, (unsigned)a.nx * (unsigned)a.ny};
   const int n[3] = {a.nx, a.ny, a.nz};
+  const double* Fc[9];
+#pragma unroll
+  for (int c = 0; c < 9; ++c) Fc[c] = a.Fin + c * P;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     T acc[9];
 #pragma unroll
     for (int c = 0; c < 9; ++c) acc[c] = T(0);
     for_taps(pos[d], n[d], [&](int off, double cd) {
-      const long long p = id + (long long)off * st[d];
+      const unsigned p = id + (unsigned)off * st[d];
       const T ck = (T)cd;
 #pragma unroll
-      for (int c = 0; c < 9; ++c) acc[c] += ck * (T)a.Fin[c * P + p];
+      for (int c = 0; c < 9; ++c) acc[c] += ck * (T)ldp(Fc[c], p);
     });
     const T s = a.G[d * P + id] * (T)a.ih[d];
 #pragma unroll
@@ -375,8 +401,9 @@ __global__ __launch_bounds__(256) void k_node(const StepArgs<T> a) {
   if (MESH) derivs_curv<T>(a, pos, id, U);
   else derivs_cart<T>(a, pos, id, U);
 
-  const T lam = ld_stream<NT>(a.M + id), mu = ld_stream<NT>(a.M + P + id),
-          rho = ld_stream<NT>(a.M + 2 * P + id);
+  const unsigned idu = (unsigned)id;
+  const T lam = ld_stream<NT>(atp(a.M, idu)), mu = ld_stream<NT>(atp(a.M + P, idu)),
+          rho = ld_stream<NT>(atp(a.M + 2 * P, idu));
   const T Jp = MESH ? a.G[9 * P + id] : T(1);
   const T rhoJ_inv = T(1) / (rho * Jp);
   const T l2m = lam + T(2) * mu;
@@ -449,7 +476,7 @@ __global__ __launch_bounds__(256) void k_node(const StepArgs<T> a) {
   }
   if (NMECH > 0 && MODE != 3) {
     const T tr = e[0] + e[1] + e[2];
-    const T Qs = ld_stream<NT>(a.Qs + id), Qp = ld_stream<NT>(a.Qp + id);
+    const T Qs = ld_stream<NT>(atp(a.Qs, idu)), Qp = ld_stream<NT>(atp(a.Qp, idu));
     const T twomuQs = T(2) * mu * Qs;
     const T pfac = l2m * Qp - twomuQs;
     const T muQs = mu * Qs;
@@ -459,7 +486,7 @@ __global__ __launch_bounds__(256) void k_node(const StepArgs<T> a) {
       T sum = T(0);
 #pragma unroll
       for (int i = 0; i < NMECH; ++i) {
-        ev[i] = ld_stream<NT>(a.eta + ((long long)(c * NMECH + i)) * P + id);
+        ev[i] = ld_stream<NT>(atp(a.eta + (long long)(c * NMECH + i) * P, idu));
         sum += ev[i];
       }
       DU[3 + c] -= sum;
@@ -469,13 +496,14 @@ __global__ __launch_bounds__(256) void k_node(const StepArgs<T> a) {
         const T ti = (T)a.tau_inv[i];
         const T src = c < 3 ? (w * twomuQs) * e[c] + (w * pfac) * tr : (w * muQs) * e[c];
         const T r = (src - ev[i]) * ti;
-        const long long k = ((long long)(c * NMECH + i)) * P + id;
+        T* De = atp(a.Deta + (long long)(c * NMECH + i) * P, idu);
+        T* Et = atp(a.eta + (long long)(c * NMECH + i) * P, idu);
         if (MODE == 0) {
-          const T dn = scale_rate(a.A, ld_stream<NT>(a.Deta + k)) + r;
-          st_stream<NT>(a.Deta + k, dn);
-          st_stream<NT>(a.eta + k, ev[i] + (T)a.Bdt * dn);
+          const T dn = scale_rate(a.A, ld_stream<NT>(De)) + r;
+          st_stream<NT>(De, dn);
+          st_stream<NT>(Et, ev[i] + (T)a.Bdt * dn);
         } else {
-          a.Deta[k] = a.Deta[k] + r;
+          *De = *De + r;
         }
       }
     }
@@ -585,11 +613,11 @@ __global__ __launch_bounds__(256) void k_node(const StepArgs<T> a) {
     const bool has_src = a.nsrc > 0 && source_at<T, MESH>(a, x, y, z, id, srcv);
 #pragma unroll
     for (int c = 0; c < 9; ++c) {
-      double df = scale_rate(a.A, ld_stream<NT>(a.DF + c * P + id)) + (double)DU[c];
+      double df = scale_rate(a.A, ld_stream<NT>(atp(a.DF + c * P, idu))) + (double)DU[c];
       if (has_src && c >= 3) df -= srcv[c - 3];
       if (has_sat) df -= satDF[c];
-      st_stream<NT>(a.DF + c * P + id, df);
-      st_stream<NT>(a.Fout + c * P + id, a.Fin[c * P + id] + a.Bdt * df);
+      st_stream<NT>(atp(a.DF + c * P, idu), df);
+      st_stream<NT>(atp(a.Fout + c * P, idu), ldp(a.Fin + c * P, idu) + a.Bdt * df);
     }
   } else if (MODE == 3) {
     // hand the complete non-memory-variable rates and the (PML-modified)
@@ -601,10 +629,10 @@ __global__ __launch_bounds__(256) void k_node(const StepArgs<T> a) {
       T r = DU[c];
       if (has_src && c >= 3) r -= (T)srcv[c - 3];
       if (has_sat) r -= (T)satDF[c];
-      st_stream<NT>(a.W + c * P + id, r);
+      st_stream<NT>(atp(a.W + c * P, idu), r);
     }
 #pragma unroll
-    for (int c = 0; c < 6; ++c) st_stream<NT>(a.W + (9 + c) * P + id, e[c]);
+    for (int c = 0; c < 6; ++c) st_stream<NT>(atp(a.W + (9 + c) * P, idu), e[c]);
   } else {
 #pragma unroll
     for (int c = 0; c < 9; ++c) a.DF[c * P + id] = a.DF[c * P + id] + (double)DU[c];
@@ -619,18 +647,18 @@ __global__ __launch_bounds__(256) void k_node(const StepArgs<T> a) {
 //   eta_i += Bdt*Deta_i;  DF = A*DF + DU;  F_out = F_in + Bdt*DF
 template <typename T, int NMECH, bool NT>
 __global__ __launch_bounds__(256) void k_memory_update(const StepArgs<T> a) {
-  const long long id = (long long)blockIdx.x * 256 + threadIdx.x;
+  const unsigned id = blockIdx.x * 256u + threadIdx.x;
   const long long P = a.P;
-  if (id >= P) return;
+  if (id >= (unsigned long long)P) return;
   T DU[9], e[6];
 #pragma unroll
-  for (int c = 0; c < 9; ++c) DU[c] = ld_stream<NT>(a.W + c * P + id);
+  for (int c = 0; c < 9; ++c) DU[c] = ld_stream<NT>(atp(a.W + c * P, id));
 #pragma unroll
-  for (int c = 0; c < 6; ++c) e[c] = ld_stream<NT>(a.W + (9 + c) * P + id);
-  const T lam = ld_stream<NT>(a.M + id), mu = ld_stream<NT>(a.M + P + id);
+  for (int c = 0; c < 6; ++c) e[c] = ld_stream<NT>(atp(a.W + (9 + c) * P, id));
+  const T lam = ld_stream<NT>(atp(a.M, id)), mu = ld_stream<NT>(atp(a.M + P, id));
   const T l2m = lam + T(2) * mu;
   const T tr = e[0] + e[1] + e[2];
-  const T Qs = ld_stream<NT>(a.Qs + id), Qp = ld_stream<NT>(a.Qp + id);
+  const T Qs = ld_stream<NT>(atp(a.Qs, id)), Qp = ld_stream<NT>(atp(a.Qp, id));
   const T twomuQs = T(2) * mu * Qs;
   const T pfac = l2m * Qp - twomuQs;
   const T muQs = mu * Qs;
@@ -641,7 +669,7 @@ __global__ __launch_bounds__(256) void k_memory_update(const StepArgs<T> a) {
 #pragma unroll
     for (int i = 0; i < NMECH; ++i) {
       if (a.mech_skip & (1 << i)) { ev[i] = T(0); continue; }
-      ev[i] = ld_stream<NT>(a.eta + ((long long)(c * NMECH + i)) * P + id);
+      ev[i] = ld_stream<NT>(atp(a.eta + (long long)(c * NMECH + i) * P, id));
       sum += ev[i];
     }
     DU[3 + c] -= sum;
@@ -652,17 +680,19 @@ __global__ __launch_bounds__(256) void k_memory_update(const StepArgs<T> a) {
       const T ti = (T)a.tau_inv[i];
       const T src = c < 3 ? (w * twomuQs) * e[c] + (w * pfac) * tr : (w * muQs) * e[c];
       const T r = (src - ev[i]) * ti;
-      const long long k = ((long long)(c * NMECH + i)) * P + id;
-      const T dn = scale_rate(a.A, ld_stream<NT>(a.Deta + k)) + r;
-      st_stream<NT>(a.Deta + k, dn);
-      st_stream<NT>(a.eta + k, ev[i] + (T)a.Bdt * dn);
+      T* De = a.Deta + (long long)(c * NMECH + i) * P;
+      T* Et = a.eta + (long long)(c * NMECH + i) * P;
+      const T dn = scale_rate(a.A, ld_stream<NT>(atp(De, id))) + r;
+      st_stream<NT>(atp(De, id), dn);
+      st_stream<NT>(atp(Et, id), ev[i] + (T)a.Bdt * dn);
     }
   }
 #pragma unroll
   for (int c = 0; c < 9; ++c) {
-    const double df = scale_rate(a.A, ld_stream<NT>(a.DF + c * P + id)) + (double)DU[c];
-    st_stream<NT>(a.DF + c * P + id, df);
-    st_stream<NT>(a.Fout + c * P + id, ld_stream<NT>(a.Fin + c * P + id) + a.Bdt * df);
+    double* DFc = a.DF + c * P;
+    const double df = scale_rate(a.A, ld_stream<NT>(atp(DFc, id))) + (double)DU[c];
+    st_stream<NT>(atp(DFc, id), df);
+    st_stream<NT>(atp(a.Fout + c * P, id), ld_stream<NT>(atp(a.Fin + c * P, id)) + a.Bdt * df);
   }
 }
 
