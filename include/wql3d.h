@@ -181,6 +181,17 @@ const char *wql_last_error(void);
 int wql_block_sync(wql_block *b);
 /* the block's HIP stream (hipStream_t) for event timing by the caller */
 void *wql_block_stream(wql_block *b);
+/* Kernel timing of fused stages (the reference times each step with
+ * cpu_time around time_step_RK, main.f90:120-132; here HIP events on the
+ * block's stream time the kernels of each stage).  enable != 0 records, for
+ * every following wql_stage_fused / wql_time_step stage, the durations of
+ * ms[0] the strict-interior kernel, ms[1] the node kernel over the rest of
+ * the block and ms[2] the memory-variable kernel (0 when a part does not
+ * run); wql_block_kernel_times synchronises, returns the sums in
+ * milliseconds and the number of stages recorded since the last call, and
+ * starts a new count. */
+int wql_block_set_timing(wql_block *b, int32_t enable);
+int wql_block_kernel_times(wql_block *b, double ms[3], int32_t *stages);
 
 #ifdef __cplusplus
 }

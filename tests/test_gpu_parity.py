@@ -99,6 +99,51 @@ def test_curvilinear_q_pml(oracle, nmech, fused):
         assert rel_l2(seis_g, seis_o) <= TOL64
 
 
+# Launch shapes of the fused stage (WQL3D_NODE_TUNING = "nontemporal,xcd,band,
+# split,interior,zchunk"): strict-interior kernel off, or tiles 64x8 / 32x16 /
+# 64x4 with z chunks that cut the interior box unevenly; shell memory
+# variables in their own kernel (split 1) or in the node kernel (split 0).
+@pytest.mark.parametrize("tuning", ["0,1,0,1,0,64", "0,1,0,1,1,7", "0,1,0,1,2,5",
+                                    "0,1,0,1,3,64", "0,1,0,0,1,16", "0,0,0,1,1,9"])
+def test_fused_stage_launch_shapes(oracle, monkeypatch, tuning):
+    monkeypatch.setenv("WQL3D_NODE_TUNING", tuning)
+    prob = small_curvilinear(n=40, nmech=8, nsteps=3, seed=11)
+    met = oracle.metrics(prob.X)
+    rng = np.random.default_rng(7)
+    F0 = rng.uniform(-1, 1, (9,) + prob.X.shape[1:])
+    F0[3:] *= 1e6
+    prob.F0 = F0
+    orc = oracle_block(oracle, prob, met)
+    orc.eta[...] = rng.uniform(-1, 1, orc.eta.shape)
+    for f in range(6):
+        if orc.pQ[f] is not None:
+            orc.pQ[f][...] = rng.uniform(-1, 1, orc.pQ[f].shape)
+    blk = gpu_block(prob, met)
+    blk.eta[...] = orc.eta
+    for f in range(6):
+        if prob.pml[f]:
+            lo, hi = pml_slab_box(f, prob.n, prob.npml[f])
+            blk.pml_Q[f][...] = orc.pQ[f][:, lo[2] - 1:hi[2], lo[1] - 1:hi[1], lo[0] - 1:hi[0]]
+    blk.upload(L.WQL_MEMORY | L.WQL_PML)
+    _, seis_o = orc.time_step(prob.dt, prob.nsteps, receivers=receiver_flat(prob))
+    blk.set_receivers(prob.receivers, prob.nsteps)
+    time_step_RK(blk, prob.dt, prob.nsteps, fused=True)
+    _download_all(blk)
+    errs = field_errors(blk.F, orc.F)
+    print(f"tuning={tuning} field rel L2:", ["%.2e" % e for e in errs], flush=True)
+    assert max(errs) <= TOL64
+    assert rel_l2(blk.DF, orc.DF) <= TOL64
+    assert rel_l2(blk.eta, orc.eta) <= TOL64
+    assert rel_l2(blk.Deta, orc.Deta) <= TOL64
+    for f in range(6):
+        if prob.pml[f]:
+            lo, hi = pml_slab_box(f, prob.n, prob.npml[f])
+            sl = (slice(None), slice(lo[2] - 1, hi[2]), slice(lo[1] - 1, hi[1]),
+                  slice(lo[0] - 1, hi[0]))
+            assert rel_l2(blk.pml_Q[f], orc.pQ[f][sl]) <= TOL64, f
+    assert rel_l2(blk.seismograms(), seis_o) <= TOL64
+
+
 def test_stage_level_rates(oracle):
     """One stage of the reference order: DF after scale+rhs+source+sat."""
     prob = small_curvilinear(nmech=8, nsteps=1)

@@ -82,6 +82,8 @@ SIGNATURES = {
     "wql_last_error": (C.c_char_p, []),
     "wql_block_sync": (C.c_int, [_P]),
     "wql_block_stream": (_P, [_P]),
+    "wql_block_set_timing": (C.c_int, [_P, C.c_int32]),
+    "wql_block_kernel_times": (C.c_int, [_P, _D, C.POINTER(C.c_int32)]),
 }
 
 _lib = None

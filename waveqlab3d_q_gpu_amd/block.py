@@ -261,6 +261,16 @@ class Block:
     def stream(self):
         return lib().wql_block_stream(self._h)
 
+    def set_timing(self, enable: bool = True):
+        """Record per-stage kernel durations (HIP events on the block stream)."""
+        check(lib().wql_block_set_timing(self._h, 1 if enable else 0))
+
+    def kernel_times(self):
+        """([interior_ms, shell_ms, memory_ms], stages) summed since the last call."""
+        ms, n = (C.c_double * 3)(), C.c_int32()
+        check(lib().wql_block_kernel_times(self._h, ms, C.byref(n)))
+        return list(ms), n.value
+
 
 def time_step_RK(block: Block, dt: float, nsteps: int, t: float = 0.0, fused: bool = True):
     """time_step_RK (time_step.f90:68-150) for ``nsteps`` steps.

@@ -81,8 +81,15 @@ struct wql_block {
   int slot;
   long long bytes;
   int tune_nt, tune_xcd, tune_band, tune_split;  // node-kernel launch shape
+  int tune_zm, tune_zchunk;                      // z-marching kernel, z levels per chunk
   void* W;        // 15 planes of T: split anelastic stage hand-off
   int mech_zero;  // mechanisms whose eta and Deta are known to be all zero
+  // kernel timing of fused stages (wql_block_set_timing): 4 events per stage
+  // around the interior, shell and memory-variable kernels
+  int timing;
+  int nev_used;
+  int nev_alloc;
+  hipEvent_t* ev;
 };
 
 static int g_device = -1;
@@ -133,6 +140,10 @@ static int validate(const wql_block_desc* d) {
   const int n[3] = {d->nq, d->nr, d->ns};
   for (int k = 0; k < 3; ++k)
     if (n[k] < 12) return fail(WQL_EINVAL, "axis %d has %d nodes; SBP-6 needs >= 12", k, n[k]);
+  // kernels address planes with 32-bit byte offsets: 8 * nodes < 2^32
+  if ((long long)n[0] * n[1] * n[2] >= (1LL << 29))
+    return fail(WQL_EINVAL, "block of %lld nodes exceeds 2^29 nodes per block",
+                (long long)n[0] * n[1] * n[2]);
   if (d->mesh != WQL_CARTESIAN && d->mesh != WQL_CURVILINEAR)
     return fail(WQL_EINVAL, "invalid mesh %d", d->mesh);
   if (d->precision != WQL_FP64 && d->precision != WQL_FP32_MIXED)
@@ -178,6 +189,8 @@ extern "C" void wql_block_destroy(wql_block* b) {
     if (b->pDQ[f]) (void)hipFree(b->pDQ[f]);
     if (b->face[f]) (void)hipFree(b->face[f]);
   }
+  for (int i = 0; i < b->nev_alloc; ++i) (void)hipEventDestroy(b->ev[i]);
+  free(b->ev);
   (void)hipStreamDestroy(b->stream);
   free(b);
 }
@@ -193,19 +206,26 @@ extern "C" int wql_block_create(const wql_block_desc* d, wql_block** out) {
   b->nx = d->nq; b->ny = d->nr; b->nz = d->ns;
   b->P = (long long)b->nx * b->ny * b->nz;
   b->esz = d->precision == WQL_FP64 ? sizeof(double) : sizeof(float);
-  // launch shape of the node kernel: "nontemporal,xcd_remap,band_rows"
-  // (env WQL3D_NODE_TUNING); band_rows 0 = whole y extent
+  // launch shape (env WQL3D_NODE_TUNING = "nontemporal,xcd_remap,band_rows,
+  // split,interior,zchunk"): band_rows 0 = whole y extent; split = memory
+  // variables of the shell in their own kernel; interior = tile of the
+  // strict-interior kernel (0 off, 1 64x8, 2 32x16, 3 64x4); zchunk = z
+  // levels per interior workgroup
   b->tune_nt = 0;
   b->tune_xcd = 1;
   b->tune_band = 0;
   b->tune_split = 1;
+  b->tune_zm = 1;
+  b->tune_zchunk = 32;
   if (const char* tv = getenv("WQL3D_NODE_TUNING")) {
-    int v0 = 1, v1 = 1, v2 = 0, v3 = 1;
-    if (sscanf(tv, "%d,%d,%d,%d", &v0, &v1, &v2, &v3) >= 3) {
+    int v0 = 0, v1 = 1, v2 = 0, v3 = 1, v4 = 1, v5 = 32;
+    if (sscanf(tv, "%d,%d,%d,%d,%d,%d", &v0, &v1, &v2, &v3, &v4, &v5) >= 3) {
       b->tune_nt = v0 != 0;
       b->tune_xcd = v1 != 0;
       b->tune_band = v2 > 0 ? v2 : 0;
       b->tune_split = v3 != 0;
+      b->tune_zm = (v4 >= 0 && v4 <= 3) ? v4 : 1;
+      b->tune_zchunk = v5 > 0 ? v5 : 32;
     }
   }
   hipError_t e = hipStreamCreate(&b->stream);  // default priority
@@ -649,6 +669,8 @@ static StepArgs<T> make_args(wql_block* b, double t) {
   a.nontemporal = b->tune_nt;
   a.W = (T*)b->W;
   a.split = b->tune_split;
+  a.zm = b->tune_zm;
+  a.zchunk = b->tune_zchunk;
   // a zero-weight mechanism whose state is all zero stays zero:
   // Deta = A*0 + (0 - 0)/tau = 0, eta = 0 + Bdt*0 = 0
   int zero_weight = 0;
@@ -749,18 +771,75 @@ static int record(wql_block* b) {
   return WQL_OK;
 }
 
+// next free timing event of the block (grows the pool by 128 events at a time)
+static int next_event(wql_block* b, hipEvent_t* out) {
+  if (b->nev_used == b->nev_alloc) {
+    const int n = b->nev_alloc + 128;
+    hipEvent_t* ev = (hipEvent_t*)realloc(b->ev, sizeof(hipEvent_t) * n);
+    if (!ev) return fail(WQL_ENOMEM, "host allocation failed");
+    b->ev = ev;
+    for (int i = b->nev_alloc; i < n; ++i) {
+      HIP_TRY(hipEventCreate(&b->ev[i]));
+      b->nev_alloc = i + 1;
+    }
+  }
+  *out = b->ev[b->nev_used++];
+  return WQL_OK;
+}
+
+template <typename T>
+static int stage_fused_t(wql_block* b, double A, double Bdt, double t) {
+  StepArgs<T> a = make_args<T>(b, t);
+  a.A = A; a.Bdt = Bdt;
+  if (!b->timing) {
+    HIP_TRY(launch_stage_fused<T>(a, b->d.mesh, b->stream));
+    return WQL_OK;
+  }
+  hipEvent_t e[4];
+  for (int k = 0; k < 4; ++k) {
+    int rc = next_event(b, &e[k]);
+    if (rc) return rc;
+  }
+  HIP_TRY(hipEventRecord(e[0], b->stream));
+  const StagePart parts[3] = {StagePart::kInterior, StagePart::kShell, StagePart::kMemory};
+  for (int k = 0; k < 3; ++k) {
+    HIP_TRY(launch_stage_part<T>(a, b->d.mesh, parts[k], b->stream));
+    HIP_TRY(hipEventRecord(e[k + 1], b->stream));
+  }
+  return WQL_OK;
+}
+
 extern "C" int wql_stage_fused(wql_block* b, double A, double Bdt, double t) {
   if (!b) return fail(WQL_EINVAL, "null block");
-  if (b->d.precision == WQL_FP64) {
-    StepArgs<double> a = make_args<double>(b, t);
-    a.A = A; a.Bdt = Bdt;
-    HIP_TRY(launch_stage_fused<double>(a, b->d.mesh, b->stream));
-  } else {
-    StepArgs<float> a = make_args<float>(b, t);
-    a.A = A; a.Bdt = Bdt;
-    HIP_TRY(launch_stage_fused<float>(a, b->d.mesh, b->stream));
-  }
+  int rc = b->d.precision == WQL_FP64 ? stage_fused_t<double>(b, A, Bdt, t)
+                                      : stage_fused_t<float>(b, A, Bdt, t);
+  if (rc) return rc;
   b->cur ^= 1;
+  return WQL_OK;
+}
+
+extern "C" int wql_block_set_timing(wql_block* b, int32_t enable) {
+  if (!b) return fail(WQL_EINVAL, "null block");
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  b->timing = enable != 0;
+  b->nev_used = 0;
+  return WQL_OK;
+}
+
+extern "C" int wql_block_kernel_times(wql_block* b, double* ms, int32_t* stages) {
+  if (!b || !ms || !stages) return fail(WQL_EINVAL, "null argument");
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (int k = 0; k + 3 < b->nev_used; k += 4) {
+    for (int j = 0; j < 3; ++j) {
+      float m = 0.f;
+      HIP_TRY(hipEventElapsedTime(&m, b->ev[k + j], b->ev[k + j + 1]));
+      acc[j] += m;
+    }
+  }
+  for (int j = 0; j < 3; ++j) ms[j] = acc[j];
+  *stages = b->nev_used / 4;
+  b->nev_used = 0;
   return WQL_OK;
 }
 

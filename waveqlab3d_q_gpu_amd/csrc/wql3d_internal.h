@@ -69,13 +69,30 @@ struct StepArgs {
   T* W;
   int split;
   int mech_skip;
+  // strict-interior z-marching kernel (k_interior) of the fused stage: tile
+  // shape (0 = off, 1 = 64x8, 2 = 32x16, 3 = 64x4) and z levels per chunk
+  int zm;
+  int zchunk;
+  // box list of k_node: when nbox > 0 the kernel covers these boxes
+  // ([x0,x1,y0,y1,z0,z1] inclusive, 0-based) instead of the whole block;
+  // box_start[b] = nodes of the boxes before b, box_start[nbox] = total
+  int nbox;
+  int box[6][6];
+  long long box_start[7];
 };
 
 // Launchers (wql3d_kernels.hip).  All return hipError_t of the launch.
 enum class RhsPart { kAll = 0, kInterior = 1 };
 
+// fused stage = the strict-interior kernel (if the box is not empty and the
+// interior kernel is enabled), the node kernel over the rest of the block
+// (the shell), and for anelastic stages with the W hand-off the
+// memory-variable kernel over every node
+enum class StagePart { kInterior = 0, kShell = 1, kMemory = 2 };
 template <typename T>
 hipError_t launch_stage_fused(const StepArgs<T>& a, int mesh, hipStream_t s);
+template <typename T>
+hipError_t launch_stage_part(const StepArgs<T>& a, int mesh, StagePart part, hipStream_t s);
 hipError_t launch_rhs(const StepArgs<double>& a, int mesh, RhsPart part, hipStream_t s);
 hipError_t launch_sat(const StepArgs<double>& a, hipStream_t s);
 hipError_t launch_source(const StepArgs<double>& a, int mesh, hipStream_t s);

@@ -66,6 +66,17 @@ __device__ __forceinline__ void for_taps(int i, int n, Fn&& f) {
   }
 }
 
+// The central taps only (rows 6..n-7), with compile-time offsets.
+template <typename Fn>
+__device__ __forceinline__ void for_taps_interior(Fn&& f) {
+  f(-3, -1.0 / 60.0);
+  f(-2, 0.15);
+  f(-1, -0.75);
+  f(1, 0.75);
+  f(2, -0.15);
+  f(3, 1.0 / 60.0);
+}
+
 __device__ __forceinline__ double scale_rate(double A, double x) { return A == 0.0 ? 0.0 : A * x; }
 __device__ __forceinline__ float scale_rate(double A, float x) {
   return A == 0.0 ? 0.0f : (float)A * x;
@@ -385,22 +396,61 @@ __device__ __forceinline__ bool map_node(const StepArgs<T>& a, int& x, int& y, i
   return yb < a.band_rows && y < a.ny;
 }
 
+template <typename T, int MESH, int NMECH, int MODE, bool NT, bool SHELL>
+__device__ __forceinline__ void node_epilogue(const StepArgs<T>& a, int x, int y, int z,
+                                              long long id, const T (&U)[3][9]);
+
+// Node of the shell boxes (nbox > 0): the global thread index runs over the
+// concatenated boxes, x fastest inside each box.
+template <typename T>
+__device__ __forceinline__ bool map_box_node(const StepArgs<T>& a, int& x, int& y, int& z) {
+  unsigned h = blockIdx.x;
+  if (a.xcd_remap) {
+    const unsigned B = gridDim.x, q = B >> 3, r = B & 7u, j = h & 7u, k = h >> 3;
+    h = j * q + (j < r ? j : r) + k;
+  }
+  const long long g = (long long)h * 256 + threadIdx.x;
+  if (g >= a.box_start[a.nbox]) return false;
+  int b = 0;
+  while (b + 1 < a.nbox && g >= a.box_start[b + 1]) ++b;
+  const long long t = g - a.box_start[b];
+  const int bx = a.box[b][1] - a.box[b][0] + 1, by = a.box[b][3] - a.box[b][2] + 1;
+  const long long row = t / bx;
+  x = a.box[b][0] + (int)(t - row * bx);
+  y = a.box[b][2] + (int)(row % by);
+  z = a.box[b][4] + (int)(row / by);
+  return true;
+}
+
 template <typename T, int MESH, int NMECH, int MODE, bool NT>
 __global__ __launch_bounds__(256) void k_node(const StepArgs<T> a) {
   int x, y, z;
-  if (!map_node(a, x, y, z)) return;
+  if (a.nbox > 0) {
+    if (!map_box_node(a, x, y, z)) return;
+  } else if (!map_node(a, x, y, z)) {
+    return;
+  }
   if (MODE == 2) {
     if (x < a.ix0 || x > a.ix1 || y < a.iy0 || y > a.iy1 || z < a.iz0 || z > a.iz1) return;
   }
   const long long plane = (long long)a.nx * a.ny;
-  const long long P = a.P;
   const long long id = x + (long long)a.nx * y + (long long)z * plane;
   const int pos[3] = {x, y, z};
 
   T U[3][9];
   if (MESH) derivs_curv<T>(a, pos, id, U);
   else derivs_cart<T>(a, pos, id, U);
+  node_epilogue<T, MESH, NMECH, MODE, NT, true>(a, x, y, z, id, U);
+}
 
+// Everything after the derivatives at one node: elastic rates, memory
+// variables (MODE 0/1/2) or the W hand-off (MODE 3), PML, SAT, source and
+// the RK update or rate accumulation.  SHELL = false is the strict interior
+// (RHS_Center): no PML slab and no face, so those paths are compiled out.
+template <typename T, int MESH, int NMECH, int MODE, bool NT, bool SHELL>
+__device__ __forceinline__ void node_epilogue(const StepArgs<T>& a, int x, int y, int z,
+                                              long long id, const T (&U)[3][9]) {
+  const long long P = a.P;
   const unsigned idu = (unsigned)id;
   const T lam = ld_stream<NT>(atp(a.M, idu)), mu = ld_stream<NT>(atp(a.M + P, idu)),
           rho = ld_stream<NT>(atp(a.M + 2 * P, idu));
@@ -424,7 +474,7 @@ __global__ __launch_bounds__(256) void k_node(const StepArgs<T> a) {
   int fa[3] = {-1, -1, -1};
   long long sidx[3] = {0, 0, 0};
   T dd[3] = {T(0), T(0), T(0)};
-  if (MODE != 2) {
+  if (MODE != 2 && SHELL) {
 #pragma unroll
     for (int f = 0; f < 6; ++f) {
       if (!a.pml[f]) continue;
@@ -512,7 +562,7 @@ __global__ __launch_bounds__(256) void k_node(const StepArgs<T> a) {
   // SAT penalty terms at face nodes (impose_boundary_condition), fused mode
   double satDF[9];
   bool has_sat = false;
-  if (FUSED) {
+  if (FUSED && SHELL) {
 #pragma unroll
     for (int f = 0; f < 6; ++f) {
       if (a.bc[f] <= 0) continue;
@@ -645,18 +695,13 @@ __global__ __launch_bounds__(256) void k_node(const StepArgs<T> a) {
 // by the relaxation equations (planes 9..14), both from k_node MODE 3.
 //   DU(4..9) -= sum_i eta_i;  Deta_i = A*Deta_i + (src_i(eps) - eta_i)/tau_i;
 //   eta_i += Bdt*Deta_i;  DF = A*DF + DU;  F_out = F_in + Bdt*DF
+// Memory variables of one node in a fused stage: DU(4..9) -= sum_i eta_i and
+// the relaxation update of eta_i / Deta_i from the strain rates e
+// (RHS_Interior.f90:2599-2906).  Mechanisms in mech_skip are not streamed.
 template <typename T, int NMECH, bool NT>
-__global__ __launch_bounds__(256) void k_memory_update(const StepArgs<T> a) {
-  const unsigned id = blockIdx.x * 256u + threadIdx.x;
+__device__ __forceinline__ void memory_update_node(const StepArgs<T>& a, unsigned id, T mu, T l2m,
+                                                   const T (&e)[6], T (&DU)[9]) {
   const long long P = a.P;
-  if (id >= (unsigned long long)P) return;
-  T DU[9], e[6];
-#pragma unroll
-  for (int c = 0; c < 9; ++c) DU[c] = ld_stream<NT>(atp(a.W + c * P, id));
-#pragma unroll
-  for (int c = 0; c < 6; ++c) e[c] = ld_stream<NT>(atp(a.W + (9 + c) * P, id));
-  const T lam = ld_stream<NT>(atp(a.M, id)), mu = ld_stream<NT>(atp(a.M + P, id));
-  const T l2m = lam + T(2) * mu;
   const T tr = e[0] + e[1] + e[2];
   const T Qs = ld_stream<NT>(atp(a.Qs, id)), Qp = ld_stream<NT>(atp(a.Qp, id));
   const T twomuQs = T(2) * mu * Qs;
@@ -687,12 +732,323 @@ __global__ __launch_bounds__(256) void k_memory_update(const StepArgs<T> a) {
       st_stream<NT>(atp(Et, id), ev[i] + (T)a.Bdt * dn);
     }
   }
+}
+
+template <typename T, int NMECH, bool NT>
+__global__ __launch_bounds__(256) void k_memory_update(const StepArgs<T> a) {
+  const unsigned id = blockIdx.x * 256u + threadIdx.x;
+  const long long P = a.P;
+  if (id >= (unsigned long long)P) return;
+  T DU[9], e[6];
+#pragma unroll
+  for (int c = 0; c < 9; ++c) DU[c] = ld_stream<NT>(atp(a.W + c * P, id));
+#pragma unroll
+  for (int c = 0; c < 6; ++c) e[c] = ld_stream<NT>(atp(a.W + (9 + c) * P, id));
+  const T lam = ld_stream<NT>(atp(a.M, id)), mu = ld_stream<NT>(atp(a.M + P, id));
+  const T l2m = lam + T(2) * mu;
+  memory_update_node<T, NMECH, NT>(a, id, mu, l2m, e, DU);
 #pragma unroll
   for (int c = 0; c < 9; ++c) {
     double* DFc = a.DF + c * P;
     const double df = scale_rate(a.A, ld_stream<NT>(atp(DFc, id))) + (double)DU[c];
     st_stream<NT>(atp(DFc, id), df);
     st_stream<NT>(atp(a.Fout + c * P, id), ld_stream<NT>(atp(a.Fin + c * P, id)) + a.Bdt * df);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Fused stage over the strict interior box (the RHS_Center region,
+// JU_xJU_yJU_z6.f90:16158-16173: no closure row, no PML slab, no face), as a
+// z-marching kernel in flux form.  Per node and direction d the stencil
+// operands are the velocities v_c and the fluxes
+//   curvilinear  fl_d[c] = J * sum_m X_m(d) * sigma(m,c)
+//   Cartesian    fl_d[c] = sigma(d,c)
+// so that
+//   rho*J * dv_c/dt = sum_d 1/h_d D_d(fl_d[c])          (Cartesian: X_d/h_d)
+//   dv_c/dx_m       = sum_d X_m(pt,d)/h_d D_d(v_c)
+// which are the elastic rates of RHS_Interior.f90:238-254 with the three
+// metric terms of JJU_x6 summed before, instead of after, differencing.
+//
+// A workgroup owns a TX x TY tile of the (x,y) plane and a chunk of z levels.
+// Per level, v and fl_q / fl_r of the tile plus a 3-node halo are staged in
+// LDS for the x and y derivatives; the z derivative reads a 7-level register
+// queue of (v, fl_s) of the thread's own column.  The epilogue applies the
+// memory variables, the moment-tensor source and the RK update in place, so
+// each per-node stream is read and written once.
+// ---------------------------------------------------------------------------
+// stress component index in F of sigma(m,c): F(3..8) = sxx, syy, szz, sxy, sxz, syz
+__host__ __device__ constexpr int sig_comp(int m, int c) {
+  return m == c ? 3 + m : (m + c == 1 ? 6 : (m + c == 2 ? 7 : 8));
+}
+
+// operands of node p for direction D: v (3) and fl_D (3).  load_ops issues
+// the loads (F in fp64, J and X_m(D) in the storage type), make_ops forms the
+// operands, so the two can be placed apart to hide the load latency.
+template <typename T, int MESH>
+struct RawOps {
+  double f[9];
+  T g[MESH ? 4 : 1];
+};
+
+template <typename T, int MESH, int D>
+__device__ __forceinline__ void load_ops(const StepArgs<T>& a, unsigned p, RawOps<T, MESH>& r) {
+  const long long P = a.P;
+#pragma unroll
+  for (int c = 0; c < 9; ++c) r.f[c] = ldp(a.Fin + c * P, p);
+  if (MESH) {
+    r.g[0] = ldp(a.G + 9 * P, p);
+#pragma unroll
+    for (int m = 0; m < 3; ++m) r.g[1 + m] = ldp(a.G + (3 * m + D) * P, p);
+  }
+}
+
+template <typename T, int MESH, int D>
+__device__ __forceinline__ void make_ops(const RawOps<T, MESH>& r, T (&v)[3], T (&fl)[3]) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c) v[c] = (T)r.f[c];
+  if (MESH) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      fl[c] = r.g[0] * (r.g[1] * (T)r.f[sig_comp(0, c)] + r.g[2] * (T)r.f[sig_comp(1, c)] +
+                        r.g[3] * (T)r.f[sig_comp(2, c)]);
+  } else {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) fl[c] = (T)r.f[sig_comp(D, c)];
+  }
+}
+
+template <typename T, int MESH, int D>
+__device__ __forceinline__ void node_operands(const StepArgs<T>& a, unsigned p, T (&v)[3],
+                                              T (&fl)[3]) {
+  RawOps<T, MESH> r;
+  load_ops<T, MESH, D>(a, p, r);
+  make_ops<T, MESH, D>(r, v, fl);
+}
+
+// An empty asm statement that takes v in a VGPR and hands it back: the value
+// must be computed at this point of the program.
+template <typename V>
+__device__ __forceinline__ void pin(V& v) {
+  asm volatile("" : "+v"(v));
+}
+
+// minimum waves per SIMD the interior kernel is compiled for (register cap
+// 512 / waves): fp32 storage / fp64 storage
+#ifndef WQL_IW32
+#define WQL_IW32 2
+#endif
+#ifndef WQL_IW64
+#define WQL_IW64 2
+#endif
+#define WQL_INTERIOR_WAVES(T) (sizeof(T) == 4 ? WQL_IW32 : WQL_IW64)
+
+// MODE 0: fused stage (memory variables in the epilogue); MODE 3: rates and
+// strain rates to W, memory variables and RK update by k_memory_update
+template <typename T, int MESH, int NMECH, int MODE, bool NT, int TX, int TY>
+__global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WQL_INTERIOR_WAVES(T))))
+void k_interior(const StepArgs<T> a) {
+  constexpr int HX = TX + 6, HY = TY + 6, NTH = TX * TY;
+  // planes: v (0..2), fl_q (3..5), fl_r (6..8)
+  __shared__ T L[9][HY][HX];
+
+  const int bx = a.ix1 - a.ix0 + 1, by = a.iy1 - a.iy0 + 1;
+  const unsigned ntx = (unsigned)((bx + TX - 1) / TX), nty = (unsigned)((by + TY - 1) / TY);
+  unsigned Lidx = blockIdx.x;
+  if (a.xcd_remap) {
+    const unsigned B = gridDim.x, q = B >> 3, r = B & 7u, j = Lidx & 7u, k = Lidx >> 3;
+    Lidx = j * q + (j < r ? j : r) + k;
+  }
+  const int tix = (int)(Lidx % ntx);
+  const int tiy = (int)((Lidx / ntx) % nty);
+  const int zc = (int)(Lidx / (ntx * nty));
+  const int z0 = a.iz0 + zc * a.zchunk;
+  const int z1 = min(a.iz1 + 1, z0 + a.zchunk);
+  const int tx = (int)threadIdx.x % TX, ty = (int)threadIdx.x / TX;
+  const int xt0 = a.ix0 + tix * TX, yt0 = a.iy0 + tiy * TY;
+  const int x = xt0 + tx, y = yt0 + ty;
+  // every thread inside the block stages its node (the halo of the box edge
+  // rows); only threads inside the interior box compute
+  const bool inblk = x < a.nx && y < a.ny;
+  const bool active = x <= a.ix1 && y <= a.iy1;
+  const long long P = a.P;
+  const unsigned plane = (unsigned)a.nx * (unsigned)a.ny;
+  const unsigned col = (unsigned)x + (unsigned)a.nx * (unsigned)y;
+
+  // z queue: (v, fl_s) of levels z-3 .. z+3 (index k <-> level z-3+k); the
+  // interior box keeps every level in [3, nz-4].  The first six iterations
+  // (z0-6 .. z0-1) only fill the queue.
+  T qv[7][3], qf[7][3];
+#pragma unroll
+  for (int k = 0; k < 7; ++k)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) { qv[k][c] = T(0); qf[k][c] = T(0); }
+
+  for (int z = z0 - 6; z < z1; ++z) {
+    const unsigned p = col + (unsigned)z * plane;
+    // level z+3 of the own column enters the queue (loads issued here, the
+    // operands formed after the LDS staging of level z)
+    RawOps<T, MESH> rz;
+    if (inblk) load_ops<T, MESH, 2>(a, p + 3u * plane, rz);
+    if (z < z0) {
+      if (inblk) make_ops<T, MESH, 2>(rz, qv[6], qf[6]);
+#pragma unroll
+      for (int k = 0; k < 6; ++k)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) { qv[k][c] = qv[k + 1][c]; qf[k][c] = qf[k + 1][c]; }
+      continue;
+    }
+    // own node of level z: v from the queue, fl_q and fl_r
+    if (inblk) {
+      T vq[3], fq[3], fr[3];
+      node_operands<T, MESH, 0>(a, p, vq, fq);
+      node_operands<T, MESH, 1>(a, p, vq, fr);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        L[c][ty + 3][tx + 3] = qv[3][c];
+        L[3 + c][ty + 3][tx + 3] = fq[c];
+        L[6 + c][ty + 3][tx + 3] = fr[c];
+      }
+    }
+    // halo: x strips (6 x TY) with fl_q, y strips (TX x 6) with fl_r
+    constexpr int NHX = 6 * TY, NH = NHX + 6 * TX;
+    for (int h = (int)threadIdx.x; h < NH; h += NTH) {
+      int hx, hy;
+      const bool xs = h < NHX;
+      if (xs) {
+        const int k = h % 6;
+        hy = h / 6;
+        hx = k < 3 ? k - 3 : TX + (k - 3);
+      } else {
+        const int k = (h - NHX) / TX;
+        hx = (h - NHX) % TX;
+        hy = k < 3 ? k - 3 : TY + (k - 3);
+      }
+      const int gx = xt0 + hx, gy = yt0 + hy;
+      if (gx >= a.nx || gy >= a.ny) continue;   // gx, gy >= 3 by construction
+      const unsigned gp = (unsigned)gx + (unsigned)a.nx * (unsigned)gy + (unsigned)z * plane;
+      T hv[3], hf[3];
+      if (xs) node_operands<T, MESH, 0>(a, gp, hv, hf);
+      else node_operands<T, MESH, 1>(a, gp, hv, hf);
+      const int fo = xs ? 3 : 6;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        L[c][hy + 3][hx + 3] = hv[c];
+        L[fo + c][hy + 3][hx + 3] = hf[c];
+      }
+    }
+    __syncthreads();
+    // level z+3 joins the queue (its loads were issued at the top)
+    if (inblk) make_ops<T, MESH, 2>(rz, qv[6], qf[6]);
+
+    if (active) {
+      // central-stencil differences (JU_xJU_yJU_z6.f90:53-59), unscaled
+      T Dv[3][3], Df[3][3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) { Dv[d][c] = T(0); Df[d][c] = T(0); }
+      // one LDS operand plane at a time; pin() makes each sum complete where
+      // it is written, so the compiler neither sinks the sums towards their
+      // use nor keeps all 72 LDS values live at once
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        for_taps_interior([&](int off, double cd) {
+          const T ck = (T)cd;
+          Dv[0][c] += ck * L[c][ty + 3][tx + 3 + off];
+          Dv[1][c] += ck * L[c][ty + 3 + off][tx + 3];
+        });
+        pin(Dv[0][c]);
+        pin(Dv[1][c]);
+        for_taps_interior([&](int off, double cd) {
+          const T ck = (T)cd;
+          Df[0][c] += ck * L[3 + c][ty + 3][tx + 3 + off];
+          Df[1][c] += ck * L[6 + c][ty + 3 + off][tx + 3];
+        });
+        pin(Df[0][c]);
+        pin(Df[1][c]);
+      }
+      for_taps_interior([&](int off, double cd) {
+        const T ck = (T)cd;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          Dv[2][c] += ck * qv[3 + off][c];
+          Df[2][c] += ck * qf[3 + off][c];
+        }
+      });
+      const unsigned idu = p;
+      const T lam = ld_stream<NT>(atp(a.M, idu)), mu = ld_stream<NT>(atp(a.M + P, idu)),
+              rho = ld_stream<NT>(atp(a.M + 2 * P, idu));
+      const T l2m = lam + T(2) * mu;
+      T gr[3][3];   // gr[m][c] = dv_c/dx_m
+      T DU[9];
+      if (MESH) {
+        const T ihd[3] = {(T)a.ih[0], (T)a.ih[1], (T)a.ih[2]};
+        const T J = ldp(a.G + 9 * P, idu);
+        const T rhoJ_inv = T(1) / (rho * J);
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          DU[c] = (ihd[0] * Df[0][c] + ihd[1] * Df[1][c] + ihd[2] * Df[2][c]) * rhoJ_inv;
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+          const T X0 = ldp(a.G + (3 * m + 0) * P, idu) * ihd[0];
+          const T X1 = ldp(a.G + (3 * m + 1) * P, idu) * ihd[1];
+          const T X2 = ldp(a.G + (3 * m + 2) * P, idu) * ihd[2];
+#pragma unroll
+          for (int c = 0; c < 3; ++c) gr[m][c] = X0 * Dv[0][c] + X1 * Dv[1][c] + X2 * Dv[2][c];
+        }
+      } else {
+        T sd[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) sd[d] = ldp(a.G + d * P, idu) * (T)a.ih[d];
+        const T rho_inv = T(1) / rho;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          DU[c] = (sd[0] * Df[0][c] + sd[1] * Df[1][c] + sd[2] * Df[2][c]) * rho_inv;
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) gr[m][c] = sd[m] * Dv[m][c];
+      }
+      DU[3] = l2m * gr[0][0] + lam * (gr[1][1] + gr[2][2]);
+      DU[4] = l2m * gr[1][1] + lam * (gr[0][0] + gr[2][2]);
+      DU[5] = l2m * gr[2][2] + lam * (gr[0][0] + gr[1][1]);
+      DU[6] = mu * (gr[1][0] + gr[0][1]);
+      DU[7] = mu * (gr[2][0] + gr[0][2]);
+      DU[8] = mu * (gr[2][1] + gr[1][2]);
+      double srcv[6];
+      const bool has_src = a.nsrc > 0 && source_at<T, MESH>(a, x, y, z, (long long)idu, srcv);
+      if constexpr (MODE == 3) {
+        // rates and strain rates to W for k_memory_update (as k_node MODE 3)
+        const T e[6] = {gr[0][0], gr[1][1], gr[2][2], gr[1][0] + gr[0][1], gr[2][0] + gr[0][2],
+                        gr[2][1] + gr[1][2]};
+#pragma unroll
+        for (int c = 0; c < 9; ++c) {
+          T r = DU[c];
+          if (has_src && c >= 3) r -= (T)srcv[c - 3];
+          st_stream<NT>(atp(a.W + c * P, idu), r);
+        }
+#pragma unroll
+        for (int c = 0; c < 6; ++c) st_stream<NT>(atp(a.W + (9 + c) * P, idu), e[c]);
+      } else {
+        if constexpr (NMECH > 0) {
+          const T e[6] = {gr[0][0], gr[1][1], gr[2][2], gr[1][0] + gr[0][1],
+                          gr[2][0] + gr[0][2], gr[2][1] + gr[1][2]};
+          memory_update_node<T, NMECH, NT>(a, idu, mu, l2m, e, DU);
+        }
+#pragma unroll
+        for (int c = 0; c < 9; ++c) {
+          double df = scale_rate(a.A, ld_stream<NT>(atp(a.DF + c * P, idu))) + (double)DU[c];
+          if (has_src && c >= 3) df -= srcv[c - 3];
+          st_stream<NT>(atp(a.DF + c * P, idu), df);
+          st_stream<NT>(atp(a.Fout + c * P, idu), ldp(a.Fin + c * P, idu) + a.Bdt * df);
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) { qv[k][c] = qv[k + 1][c]; qf[k][c] = qf[k + 1][c]; }
   }
 }
 
@@ -875,8 +1231,13 @@ static inline dim3 plane_grid(int nx, int ny, int nz) {
 
 template <typename T, int MODE, bool NT>
 static hipError_t launch_node_nt(const StepArgs<T>& a, int mesh, hipStream_t s) {
-  const long long nbands = (a.ny + a.band_rows - 1) / a.band_rows;
-  const dim3 g((unsigned)(nbands * a.nz * a.segs));
+  dim3 g;
+  if (a.nbox > 0) {
+    g = dim3((unsigned)((a.box_start[a.nbox] + 255) / 256));
+  } else {
+    const long long nbands = (a.ny + a.band_rows - 1) / a.band_rows;
+    g = dim3((unsigned)(nbands * a.nz * a.segs));
+  }
   const dim3 b(256);
 #define WQL_CASE(MS, NM)                                                   \
   if (mesh == MS && a.nmech == NM) {                                       \
@@ -904,15 +1265,96 @@ static hipError_t launch_memory_update(const StepArgs<T>& a, hipStream_t s) {
   return hipGetLastError();
 }
 
+template <typename T, int TX, int TY>
+static hipError_t launch_interior_tile(const StepArgs<T>& a, int mesh, bool to_w, hipStream_t s) {
+  const long long bx = a.ix1 - a.ix0 + 1, by = a.iy1 - a.iy0 + 1, bz = a.iz1 - a.iz0 + 1;
+  const long long ntx = (bx + TX - 1) / TX, nty = (by + TY - 1) / TY;
+  const long long nzc = (bz + a.zchunk - 1) / a.zchunk;
+  const dim3 g((unsigned)(ntx * nty * nzc)), b(TX * TY);
+#define WQL_CASE(MS, NM, MD)                                                        \
+  if (mesh == MS && a.nmech == NM && (MD == 3) == to_w) {                           \
+    hipLaunchKernelGGL((k_interior<T, MS, NM, MD, false, TX, TY>), g, b, 0, s, a);  \
+    return hipGetLastError();                                                       \
+  }
+  WQL_CASE(0, 0, 0) WQL_CASE(0, 4, 3) WQL_CASE(0, 8, 3) WQL_CASE(0, 4, 0) WQL_CASE(0, 8, 0)
+  WQL_CASE(1, 0, 0) WQL_CASE(1, 4, 3) WQL_CASE(1, 8, 3) WQL_CASE(1, 4, 0) WQL_CASE(1, 8, 0)
+#undef WQL_CASE
+  return hipErrorInvalidValue;
+}
+
+// tile shape of the interior kernel: zm = 1 -> 64x8, 2 -> 32x16, 3 -> 64x4
+template <typename T>
+static hipError_t launch_interior(const StepArgs<T>& a, int mesh, bool to_w, hipStream_t s) {
+  switch (a.zm) {
+    case 2: return launch_interior_tile<T, 32, 16>(a, mesh, to_w, s);
+    case 3: return launch_interior_tile<T, 64, 4>(a, mesh, to_w, s);
+    default: return launch_interior_tile<T, 64, 8>(a, mesh, to_w, s);
+  }
+}
+
+// The six boxes of the block outside the strict interior box, full x rows
+// wherever possible: z slabs, then y slabs of the remaining z range, then x
+// slabs of the remaining (y, z) range.
+template <typename T>
+static void set_shell_boxes(StepArgs<T>& a) {
+  int nb = 0;
+  auto add = [&](int x0, int x1, int y0, int y1, int z0, int z1) {
+    if (x0 > x1 || y0 > y1 || z0 > z1) return;
+    const int v[6] = {x0, x1, y0, y1, z0, z1};
+    for (int k = 0; k < 6; ++k) a.box[nb][k] = v[k];
+    a.box_start[nb + 1] =
+        a.box_start[nb] + (long long)(x1 - x0 + 1) * (y1 - y0 + 1) * (long long)(z1 - z0 + 1);
+    ++nb;
+  };
+  a.box_start[0] = 0;
+  add(0, a.nx - 1, 0, a.ny - 1, 0, a.iz0 - 1);
+  add(0, a.nx - 1, 0, a.ny - 1, a.iz1 + 1, a.nz - 1);
+  add(0, a.nx - 1, 0, a.iy0 - 1, a.iz0, a.iz1);
+  add(0, a.nx - 1, a.iy1 + 1, a.ny - 1, a.iz0, a.iz1);
+  add(0, a.ix0 - 1, a.iy0, a.iy1, a.iz0, a.iz1);
+  add(a.ix1 + 1, a.nx - 1, a.iy0, a.iy1, a.iz0, a.iz1);
+  a.nbox = nb;
+}
+
+template <typename T>
+static bool interior_on(const StepArgs<T>& a) {
+  return a.zm > 0 && a.zchunk > 0 && a.ix0 <= a.ix1 && a.iy0 <= a.iy1 && a.iz0 <= a.iz1;
+}
+
+// anelastic stages with the W hand-off: node kernels write rates and strain
+// rates, k_memory_update finishes every node
+template <typename T>
+static bool split_stage(const StepArgs<T>& a) {
+  return a.nmech > 0 && a.split && a.W;
+}
+
+template <typename T>
+hipError_t launch_stage_part(const StepArgs<T>& a, int mesh, StagePart part, hipStream_t s) {
+  switch (part) {
+    case StagePart::kInterior:
+      return interior_on(a) ? launch_interior<T>(a, mesh, split_stage(a), s) : hipSuccess;
+    case StagePart::kShell: {
+      StepArgs<T> sh = a;
+      if (interior_on(a)) set_shell_boxes(sh);
+      return split_stage(a) ? launch_node<T, 3>(sh, mesh, s) : launch_node<T, 0>(sh, mesh, s);
+    }
+    default:
+      if (!split_stage(a)) return hipSuccess;
+      return a.nontemporal ? launch_memory_update<T, true>(a, s)
+                           : launch_memory_update<T, false>(a, s);
+  }
+}
+
 template <typename T>
 hipError_t launch_stage_fused(const StepArgs<T>& a, int mesh, hipStream_t s) {
-  if (a.nmech > 0 && a.split && a.W) {
-    hipError_t e = launch_node<T, 3>(a, mesh, s);
+  for (StagePart p : {StagePart::kInterior, StagePart::kShell, StagePart::kMemory}) {
+    hipError_t e = launch_stage_part<T>(a, mesh, p, s);
     if (e != hipSuccess) return e;
-    return a.nontemporal ? launch_memory_update<T, true>(a, s) : launch_memory_update<T, false>(a, s);
   }
-  return launch_node<T, 0>(a, mesh, s);
+  return hipSuccess;
 }
+template hipError_t launch_stage_part<double>(const StepArgs<double>&, int, StagePart, hipStream_t);
+template hipError_t launch_stage_part<float>(const StepArgs<float>&, int, StagePart, hipStream_t);
 template hipError_t launch_stage_fused<double>(const StepArgs<double>&, int, hipStream_t);
 template hipError_t launch_stage_fused<float>(const StepArgs<float>&, int, hipStream_t);
 
