@@ -1,21 +1,26 @@
-"""Throughput of the fused per-block RK step on one MI355X.
+"""Throughput of the per-block RK step on one MI355X.
 
 Workload (default): 385^3 curvilinear block with topography, layered +
 von Karman medium, anelastic Q8 (8 mechanisms), 20-node PML on four sides
 and the bottom, free surface on top, Ricker moment-tensor source.  One
-"step" = one Kennedy-Carpenter (5,4) RK step = 5 fused stage kernels.
+"step" = one Kennedy-Carpenter (5,4) RK step = 5 stages; one stage = the
+strict-interior stencil kernel, the node kernel over the rest of the block
+(closures, PML, SAT) and the streaming memory-variable + RK kernel.
 
 Prints ONE JSON line: metric/value = FP32-mixed Mpt-RK-stage/s (grid points
-x 5 stages x steps / wall time), with the FP64 path on the same block and
-GPU timed beside it, the roofline of the fused stage kernel (HBM bound;
-algorithmic bytes per node-stage / measured launch time), and the CPU
-restatement timed on a bounded sample of the same workload.
+x 5 stages x steps / time on the block stream), the FP64 path on the same
+block and GPU beside it, the per-kernel durations (HIP events on the block
+stream), the roofline of the dominant kernel (HBM bound; algorithmic bytes
+per launch / measured launch time; traffic from rocprofv3 counters when the
+committed profile is present), the FP32-mixed vs FP64 error per field and
+per receiver seismogram, and the CPU restatement timed on a bounded sample
+of the same workload.
 """
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
-import math
 import os
 import sys
 import time
@@ -27,27 +32,76 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Mpt-RK-stage/s & % HBM peak: FP32-mixed vs FP64 GPU at stated seismogram L2"
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = "traffic_memory_update_fp32_q8_385.json"
 
 
-def algorithmic_bytes_per_launch(prob, precision):
-    """Compulsory HBM bytes of one fused stage over the block: F read+write
-    (ping-pong), DF read+write in FP64; geometry (9 metrics + J), material
-    (3), memory variables eta/Deta read+write (4*6*nmech), Q^-1 (2) and PML
-    auxiliaries Q/DQ read+write (36 per slab node) in the storage type."""
+def active_mechanisms(prob):
+    """Mechanisms whose state can be non-zero: a zero-weight mechanism that
+    starts at zero stays zero and is never read or written."""
+    if not prob.nmech:
+        return 0
+    return int(np.count_nonzero(np.asarray(prob.weight)))
+
+
+def interior_box(prob):
+    """Strict interior (0-based inclusive) per axis: max(npml, 6) from PML
+    faces, 6 from the others."""
+    lo = [max(prob.npml[2 * a], 6) if prob.pml[2 * a] else 6 for a in range(3)]
+    hi = [prob.n[a] - 1 - (max(prob.npml[2 * a + 1], 6) if prob.pml[2 * a + 1] else 6)
+          for a in range(3)]
+    return lo, hi
+
+
+def slab_nodes(prob):
     nq, nr, ns = prob.n
-    P = nq * nr * ns
-    es = 8 if precision == 0 else 4
-    ngeo = 10 if prob.mesh == 1 else 3
-    per_node = 36 * 8 + (ngeo + 3) * es
-    if prob.nmech:
-        per_node += (24 * prob.nmech + 2) * es
-    slab = 0
+    tot = 0
     for f in range(6):
         if prob.pml[f]:
             nn = [nq, nr, ns]
             nn[f // 2] = prob.npml[f]
-            slab += nn[0] * nn[1] * nn[2]
-    return P * per_node + slab * 36 * es, per_node
+            tot += nn[0] * nn[1] * nn[2]
+    return tot
+
+
+def byte_model(prob, precision):
+    """Algorithmic HBM bytes of one stage.
+
+    End to end (what the step must move, intermediates excluded): F read and
+    write (ping-pong) and DF read and write in FP64 (36 x 8 B); geometry
+    (curvilinear 9 metrics + J, Cartesian 3) and material (3) in the storage
+    type; eta and Deta read and write for the active mechanisms and Q^-1 (2);
+    PML Q and DQ read and write (36 values per slab node).
+
+    Per kernel (what each kernel of the stage must move, its W hand-off
+    included): interior stencil = F read (9 x 8), geometry, material, W write
+    (15); shell = the same per node plus the PML streams; memory-variable
+    kernel = W read (15), lambda, mu, Q^-1 (4), eta/Deta read and write, DF
+    read and write, F read and write."""
+    nq, nr, ns = prob.n
+    P = nq * nr * ns
+    es = 8 if precision == 0 else 4
+    ngeo = 10 if prob.mesh == 1 else 3
+    na = active_mechanisms(prob)
+    mem = (24 * na + 2) * es if prob.nmech else 0
+    slab = slab_nodes(prob)
+    end_to_end = P * (36 * 8 + (ngeo + 3) * es + mem) + slab * 36 * es
+    lo, hi = interior_box(prob)
+    n_int = int(np.prod([max(0, hi[a] - lo[a] + 1) for a in range(3)]))
+    # elastic stages have no hand-off: the node kernels write F and DF
+    stencil_node = 9 * 8 + (ngeo + 3) * es + (15 * es if prob.nmech else 27 * 8)
+    kernels = {
+        "interior": n_int * stencil_node,
+        "shell": (P - n_int) * stencil_node + slab * 36 * es,
+        "memory_update": P * (15 * es + 4 * es + 24 * na * es + 36 * 8) if prob.nmech else 0,
+    }
+    return {"P": P, "interior_nodes": n_int, "active_mechanisms": na,
+            "end_to_end": end_to_end, "kernels": kernels}
+
+
+def near_source_receivers(prob, offsets=((4, 0, 0), (-4, 0, 0), (0, 4, 0), (0, -4, 0),
+                                         (0, 0, 4), (0, 0, -4), (3, 3, 3), (-3, -3, -3))):
+    near = np.asarray(prob.sources[0].near)
+    return np.asarray([near + np.asarray(o) for o in offsets], dtype=np.int64)
 
 
 def timed_run(blk, prob, steps, warmup, t0):
@@ -58,6 +112,7 @@ def timed_run(blk, prob, steps, warmup, t0):
     torch.cuda.synchronize()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
     w0 = time.perf_counter()
     e0.record(st)
     t = blk.time_step(prob.dt, steps, t)
@@ -66,8 +121,7 @@ def timed_run(blk, prob, steps, warmup, t0):
     blk.sync()
     torch.cuda.synchronize()
     wall = time.perf_counter() - w0
-    ms = e0.elapsed_time(e1)
-    return ms, wall, t
+    return e0.elapsed_time(e1), wall, t
 
 
 def cpu_baseline(n_sample, nmech, npml, seed):
@@ -92,10 +146,12 @@ def cpu_baseline(n_sample, nmech, npml, seed):
     b.time_step(prob.dt, 1, 0.0)
     el = time.perf_counter() - t0
     P = nq * nr * ns
-    return {"value": P * 5 / el / 1e6, "unit": "Mpt-RK-stage/s", "cores": 1, "kind": "port",
+    return {"value": round(P * 5 / el / 1e6, 4), "unit": "Mpt-RK-stage/s", "cores": 1,
+            "kind": "port",
             "sample": f"{n_sample}^3 sub-block of the same curvilinear Q{nmech} + PML{npml} "
-                      f"set-up, 1 step (5 stages), oracle/wql_oracle.c (gcc -O2, 1 thread)",
-            "seconds": el}
+                      f"set-up, 1 step (5 stages), oracle/wql_oracle.c (gcc -O2, 1 thread, "
+                      f"no MPI/OpenMP)",
+            "seconds": round(el, 3)}
 
 
 def read_traffic(name):
@@ -105,8 +161,16 @@ def read_traffic(name):
     try:
         with open(path) as fh:
             return json.load(fh)
-    except Exception:
+    except (OSError, ValueError):
         return None
+
+
+def rel_l2(a, b):
+    """||a - b|| / ||b||, scaled by max|b| first so tiny traces do not underflow."""
+    s = float(np.abs(b).max())
+    if not s > 0:
+        return None
+    return float(np.linalg.norm(((a - b) / s).ravel()) / np.linalg.norm((b / s).ravel()))
 
 
 def main():
@@ -119,14 +183,17 @@ def main():
     ap.add_argument("--npml", type=int, default=20)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true",
+                    help="skip the FP32-mixed vs FP64 field comparison (host copies)")
     ap.add_argument("--cpu-sample", type=int, default=97)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run a device copy of this many MiB 5 times (scale factor "
                          "for FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one MI355X; multi-GPU runs are out of scope "
-              "(one block per GPU, no halo exchange). Use --gpus 1.", file=sys.stderr)
+        print("bench.py: this build measures one MI355X (one block per GPU, no halo "
+              "exchange); runs on more than one GPU are out of scope. Use --gpus 1.",
+              file=sys.stderr)
         sys.exit(2)
     if args.copy_calibration_mib:
         import torch
@@ -150,9 +217,7 @@ def main():
                           "launches": 5}), flush=True)
         return
 
-    import ctypes
-
-    import torch
+    import torch  # noqa: F401
 
     from waveqlab3d_q_gpu_amd import _lib as L
     from waveqlab3d_q_gpu_amd import problems as PB
@@ -165,59 +230,92 @@ def main():
                             npml=args.npml, seed=2024, nrec=64)
     nq, nr, ns = prob.n
     P = nq * nr * ns
+    receivers = np.concatenate([prob.receivers, near_source_receivers(prob)])
+    nrec_surface = prob.receivers.shape[0]
+    nslots = args.steps + args.warmup
     # start so that the Ricker peak falls inside the timed window
-    t0 = prob.sources[0].t_init - 0.5 * (args.steps + args.warmup) * prob.dt
+    t0 = prob.sources[0].t_init - 0.5 * nslots * prob.dt
 
-    results = {}
-    finals = {}
+    results, finals = {}, {}
     for prec, name in ((L.WQL_FP32_MIXED, "fp32"), (L.WQL_FP64, "fp64")):
         if name == "fp64" and args.no_fp64:
             continue
         print(f"[bench] {name}: creating block", file=sys.stderr, flush=True)
-        blk = PB.make_device_block(prob, precision=prec)
-        blk.set_receivers(prob.receivers, args.steps + args.warmup)
+        accuracy = not args.no_accuracy and not args.no_fp64
+        blk = PB.make_device_block(prob, precision=prec, state_mirror=accuracy)
+        blk.set_receivers(receivers, nslots)
         print(f"[bench] {name}: device bytes {blk.device_bytes() / 2**30:.1f} GiB; timing "
               f"{args.warmup}+{args.steps} steps", file=sys.stderr, flush=True)
         ms, wall, _ = timed_run(blk, prob, args.steps, args.warmup, t0)
-        nbytes, per_node = algorithmic_bytes_per_launch(prob, prec)
-        launch_ms = ms / (5 * args.steps)
+        seis = blk.seismograms()
+        # per-kernel durations on the block stream over 2 more steps
+        blk.set_timing(True)
+        blk.time_step(prob.dt, 2, 0.0)
+        kt, nst = blk.kernel_times()
+        blk.set_timing(False)
+        bm = byte_model(prob, prec)
+        stage_ms = ms / (5 * args.steps)
+        kernels = {}
+        for k, key in enumerate(("interior", "shell", "memory_update")):
+            kms = kt[k] / nst
+            kb = bm["kernels"][key]
+            kernels[key] = {"ms": round(kms, 4), "bytes": kb,
+                            "GBs": round(kb / (kms / 1e3) / 1e9, 1) if kms > 0 else None}
         results[name] = {
             "value": P * 5 * args.steps / (ms / 1e3) / 1e6,
             "ms_per_step": ms / args.steps,
             "wall_ms_per_step": wall * 1e3 / args.steps,
-            "launch_ms": launch_ms,
-            "bytes_per_launch": nbytes,
-            "bytes_per_node_stage": nbytes / P,
-            "achieved_GBs": nbytes / (launch_ms / 1e3) / 1e9,
+            "stage_ms": stage_ms,
+            "stage_bytes": bm["end_to_end"],
+            "stage_GBs": bm["end_to_end"] / (stage_ms / 1e3) / 1e9,
+            "kernels": kernels,
+            "byte_model": bm,
         }
-        print(f"[bench] {name}: {results[name]['value']:.1f} Mpt-RK-stage/s, "
-              f"{results[name]['achieved_GBs']:.0f} GB/s", file=sys.stderr, flush=True)
-        blk.F = np.zeros((9, ns, nr, nq))
-        blk._desc.F = blk.F.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
-        blk._desc.arrays_on_device = 0
-        finals[name] = (blk.seismograms(), blk)
-        if name == "fp32":
-            # keep only what the comparison needs
-            pass
+        print(f"[bench] {name}: {results[name]['value']:.1f} Mpt-RK-stage/s, stage "
+              f"{stage_ms:.3f} ms, kernels {[kernels[k]['ms'] for k in kernels]}",
+              file=sys.stderr, flush=True)
+        F = None
+        if accuracy:
+            # state after warmup + steps + the 2 timing steps
+            blk.download(L.WQL_F)
+            F = blk.F.cpu().numpy()
+        finals[name] = (seis, F)
+        blk.close()
+        del blk
+        torch.cuda.empty_cache()
 
-    # FP32-mixed vs FP64 on the same GPU, same state and time window
+    # FP32-mixed vs FP64: same inputs, same GPU, same time window
     acc = {}
     if "fp64" in finals:
-        s32, b32 = finals["fp32"]
-        s64, b64 = finals["fp64"]
-        seis_err = []
-        for r in range(s64.shape[1]):
-            den = np.linalg.norm(s64[:, r].ravel())
-            if den > 0:
-                seis_err.append(float(np.linalg.norm((s32[:, r] - s64[:, r]).ravel()) / den))
-        acc["seismogram_rel_l2_max_fp32_vs_fp64"] = max(seis_err) if seis_err else None
-        acc["receivers_with_signal"] = len(seis_err)
+        s32, F32 = finals["fp32"]
+        s64, F64 = finals["fp64"]
+        peaks = np.abs(s64).max(axis=(0, 2))
+        top = float(peaks.max()) if peaks.size else 0.0
+        errs = [rel_l2(s32[:, r], s64[:, r]) for r in range(s64.shape[1])
+                if top > 0 and peaks[r] >= 1e-3 * top]
+        acc["seismogram_rel_l2_max"] = max(errs) if errs else None
+        acc["seismogram_rel_l2_per_receiver"] = [float("%.3e" % e) for e in errs]
+        acc["receivers_with_signal"] = len(errs)
+        acc["receivers"] = (f"{nrec_surface} seeded surface receivers + 8 receivers 3-4 "
+                            "nodes from the source; a receiver counts when its FP64 peak "
+                            "is >= 1e-3 of the largest receiver peak in the window")
+        if F32 is not None:
+            acc["field_rel_l2"] = [float("%.3e" % rel_l2(F32[c], F64[c])) for c in range(9)]
+        acc["window"] = (f"{args.warmup + args.steps} steps from rest around the Ricker "
+                         "peak")
 
     r32 = results["fp32"]
-    traffic = read_traffic("traffic_stage_fused_fp32_q8_385.json")
-    roof = {"bound": "hbm", "achieved": round(r32["achieved_GBs"], 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(r32["achieved_GBs"] / HBM_PEAK_GBS, 4),
-            "traffic": traffic.get("bytes_per_launch") if traffic else None}
+    k32 = r32["kernels"]
+    dom = max(k32, key=lambda k: k32[k]["ms"])
+    traffic = read_traffic(TRAFFIC_FILE)
+    roof = {"bound": "hbm", "kernel": dom, "achieved": k32[dom]["GBs"], "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(k32[dom]["GBs"] / HBM_PEAK_GBS, 4),
+            "traffic": (traffic.get("bytes_per_launch")
+                        if traffic and traffic.get("kernel") == dom else None),
+            "algorithmic_bytes_per_launch": k32[dom]["bytes"],
+            "stage_end_to_end": {"achieved": round(r32["stage_GBs"], 1),
+                                 "frac": round(r32["stage_GBs"] / HBM_PEAK_GBS, 4),
+                                 "bytes_per_stage": r32["stage_bytes"]}}
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_sample, args.nmech, args.npml, 2024)
     out = {
         "metric": METRIC, "value": round(r32["value"], 2), "unit": "Mpt-RK-stage/s",
@@ -227,20 +325,24 @@ def main():
         "data": "synthetic (seeded topography, von Karman medium, Ricker source)",
         "config": {"workload": f"{nq}^3 curvilinear block, anelastic Q{args.nmech}, PML "
                                f"{args.npml} on 4 sides + bottom, free surface, h=50 m, "
-                               "fused RHS+Q+PML+SAT+source+RK stage",
+                               "RK stage = interior stencil + shell + memory-variable kernels",
                    "nodes": P, "precision": "fp32-mixed"},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "kernels_fp32": k32,
+        "stage_ms_fp32": round(r32["stage_ms"], 4),
     }
     if "fp64" in results:
         r64 = results["fp64"]
         out["fp64"] = {"value": round(r64["value"], 2), "ms_per_step": round(r64["ms_per_step"], 4),
-                       "achieved_GBs": round(r64["achieved_GBs"], 1),
-                       "frac": round(r64["achieved_GBs"] / HBM_PEAK_GBS, 4),
-                       "bytes_per_node_stage": r64["bytes_per_node_stage"]}
+                       "stage_ms": round(r64["stage_ms"], 4),
+                       "stage_GBs": round(r64["stage_GBs"], 1),
+                       "stage_frac": round(r64["stage_GBs"] / HBM_PEAK_GBS, 4),
+                       "kernels": r64["kernels"]}
         out["speedup_fp32_mixed_vs_fp64"] = round(r32["value"] / r64["value"], 3)
-    out["fp32_bytes_per_node_stage"] = r32["bytes_per_node_stage"]
-    out["accuracy"] = acc
+    out["bytes_per_node_stage_fp32"] = round(r32["stage_bytes"] / P, 1)
+    out["active_mechanisms"] = r32["byte_model"]["active_mechanisms"]
+    out["accuracy_fp32_mixed_vs_fp64"] = acc
     print(json.dumps(out), flush=True)
 
 

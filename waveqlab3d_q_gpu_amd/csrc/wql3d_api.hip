@@ -53,6 +53,9 @@ long long box_extent(const wql_box& b, int d) { return (long long)b.hi[d] - b.lo
 struct wql_block {
   wql_block_desc d;
   int nx, ny, nz;
+  // plane stride: nodes rounded up to a multiple of kPlaneAlign, so every
+  // plane starts on a 1 KiB (fp32) / 2 KiB (fp64) boundary and the streaming
+  // kernels' wavefront accesses never straddle extra cache lines
   long long P;
   size_t esz;                // sizeof(T) of geometry/material/memory arrays
   int cur;                   // current F buffer
@@ -140,8 +143,8 @@ static int validate(const wql_block_desc* d) {
   const int n[3] = {d->nq, d->nr, d->ns};
   for (int k = 0; k < 3; ++k)
     if (n[k] < 12) return fail(WQL_EINVAL, "axis %d has %d nodes; SBP-6 needs >= 12", k, n[k]);
-  // kernels address planes with 32-bit byte offsets: 8 * nodes < 2^32
-  if ((long long)n[0] * n[1] * n[2] >= (1LL << 29))
+  // kernels address planes with 32-bit byte offsets: 8 * plane stride < 2^32
+  if ((long long)n[0] * n[1] * n[2] + kPlaneAlign >= (1LL << 29))
     return fail(WQL_EINVAL, "block of %lld nodes exceeds 2^29 nodes per block",
                 (long long)n[0] * n[1] * n[2]);
   if (d->mesh != WQL_CARTESIAN && d->mesh != WQL_CURVILINEAR)
@@ -204,7 +207,7 @@ extern "C" int wql_block_create(const wql_block_desc* d, wql_block** out) {
   if (!b) return fail(WQL_ENOMEM, "host allocation failed");
   b->d = *d;
   b->nx = d->nq; b->ny = d->nr; b->nz = d->ns;
-  b->P = (long long)b->nx * b->ny * b->nz;
+  b->P = ((long long)b->nx * b->ny * b->nz + kPlaneAlign - 1) / kPlaneAlign * kPlaneAlign;
   b->esz = d->precision == WQL_FP64 ? sizeof(double) : sizeof(float);
   // launch shape (env WQL3D_NODE_TUNING = "nontemporal,xcd_remap,band_rows,
   // split,interior,zchunk"): band_rows 0 = whole y extent; split = memory

@@ -2,7 +2,9 @@
 // (wql3d_kernels.hip).
 //
 // Device layout of one block (no ghost nodes; node (x,y,z) 0-based, x fastest):
-//   idx = x + nx*(y + ny*z),  P = nx*ny*nz
+//   idx = x + nx*(y + ny*z);  plane stride P = nx*ny*nz rounded up to a
+//   multiple of kPlaneAlign nodes (the tail of each plane is padding that
+//   streaming kernels may read and write but no node reads)
 //   F[2]  : 9 planes of double each (ping-pong for the fused stage)
 //   DF    : 9 planes of double
 //   G     : geometry planes of type T: cartesian 3 (q_x, r_y, s_z);
@@ -23,6 +25,8 @@
 namespace wql {
 
 constexpr int kMaxSources = 8;
+// plane stride granularity in nodes (see the device layout above)
+constexpr long long kPlaneAlign = 256;
 
 struct DevSource {
   int near[3];      // 0-based nearest node
@@ -34,7 +38,7 @@ struct DevSource {
 template <typename T>
 struct StepArgs {
   int nx, ny, nz;
-  long long P;
+  long long P;      // plane stride (>= nx*ny*nz, see the device layout)
   int nmech;
   double ih[3];     // 1/hq, 1/hr, 1/hs
   double h[3];      // hq, hr, hs

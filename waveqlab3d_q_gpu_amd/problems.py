@@ -332,11 +332,13 @@ def curvilinear_q(n=257, h=50.0, nsteps=500, nmech=8, npml=20, seed=2024, cfl=0.
 
 
 # ---------------------------------------------------------------------------
-def make_device_block(prob: Problem, precision=L.WQL_FP64, device="cuda"):
+def make_device_block(prob: Problem, precision=L.WQL_FP64, device="cuda", state_mirror=False):
     """Create a :class:`Block` whose set-up arrays are float64 torch tensors on
     the GPU: metric terms from the device pipeline (grid_metrics), face
     frames from the device metrics, material and Q from the problem.  The
-    state starts at the init_* values (F = 0, DF = 1e40, memory = 0)."""
+    state starts at the init_* values (F = 0, DF = 1e40, memory = 0).  With
+    state_mirror the block also gets a device tensor ``F`` that
+    ``download(WQL_F)`` fills."""
     import torch
 
     from .block import Block, grid_metrics
@@ -364,6 +366,8 @@ def make_device_block(prob: Problem, precision=L.WQL_FP64, device="cuda"):
         kw.update(Qs_inv=torch.from_numpy(np.ascontiguousarray(prob.Qs_inv)).to(dev),
                   Qp_inv=torch.from_numpy(np.ascontiguousarray(prob.Qp_inv)).to(dev))
     nq, nr, ns = prob.n
+    if state_mirror:
+        kw["F"] = torch.zeros((9, ns, nr, nq), dtype=torch.float64, device=dev)
     blk = Block(nq=nq, nr=nr, ns=ns, mesh=prob.mesh, precision=precision, nmech=prob.nmech,
                 bc=prob.bc, pml=prob.pml, npml=prob.npml,
                 tau=tuple(prob.tau) if prob.tau is not None else (),
