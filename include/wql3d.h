@@ -193,6 +193,34 @@ void *wql_block_stream(wql_block *b);
 int wql_block_set_timing(wql_block *b, int32_t enable);
 int wql_block_kernel_times(wql_block *b, double ms[3], int32_t *stages);
 
+/* ---- discrete adjoint of the fused stage --------------------------------
+ * No reference counterpart (the reference has no adjoint); these serve the
+ * gradient of a receiver misfit with respect to per-node rho, lambda, mu.
+ * State u = (F, eta), rates r = (DF, Deta); a stage is r = A r + L(u; m),
+ * u = u + Bdt r.  wql_adjoint_stage reverses one stage whose INPUT state is
+ * the block's current state (restore it with wql_state_load):
+ *   rb += Bdt ub;  g += (dL/dm)^T rb;  ub += K^T rb;  rb = A rb
+ * with ub, rb the adjoints of u, r, K = dL/du and g the gradient planes.
+ * PML slabs are not differentiated (wql_adjoint_create fails with PML).
+ * Dense arrays below are DEVICE pointers, node order x fastest, no ghosts. */
+/* adjoint state, rate adjoints, gradients (zeroed) and nslots state slots */
+int wql_adjoint_create(wql_block *b, int32_t nslots);
+/* ub = rb = 0, g = 0 */
+int wql_adjoint_reset(wql_block *b);
+/* copy the current state (F, eta) into slot / back from slot (checkpoints) */
+int wql_state_save(wql_block *b, int32_t slot);
+int wql_state_load(wql_block *b, int32_t slot);
+/* reverse one stage with RK coefficients A (a_s) and Bdt (b_s * dt) */
+int wql_adjoint_stage(wql_block *b, double A, double Bdt);
+/* ub(F) at the receiver nodes += g[r][0..8] (device array of nrec x 9): the
+ * adjoint of one receiver sample */
+int wql_adjoint_add_receivers(wql_block *b, const double *g);
+/* read / write ub: F part [9][nodes], eta part [6*nmech][nodes] */
+int wql_adjoint_get_state(wql_block *b, double *Fbar, double *etabar);
+int wql_adjoint_set_state(wql_block *b, const double *Fbar, const double *etabar);
+/* accumulated gradients [nodes] each */
+int wql_adjoint_gradient(wql_block *b, double *g_rho, double *g_lambda, double *g_mu);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,122 @@
+"""Discrete adjoint of the fused RK step (libwql3d_hip.so wql_adjoint_*).
+
+There is no reference counterpart; the adjoint is pinned against the
+oracle-pinned forward path by
+  * the dot-product identity <Phi x, y> = <x, Phi^T y> for Phi = N source-free
+    RK steps (FP64, relative 1e-10), and
+  * FP64 central finite differences of a receiver misfit along seeded random
+    material directions (relative 1e-6).
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from waveqlab3d_q_gpu_amd import _lib as L
+from waveqlab3d_q_gpu_amd import problems as PB
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _runtime():
+    dev = ctypes.c_int32()
+    L.check(L.lib().wql_runtime_init(0, ctypes.byref(dev)))
+    yield
+
+
+def _problem(kind, n=24, nsteps=3, f0=20.0):
+    if kind == "cart_elastic":
+        prob = PB.curvilinear_q(n=n, h=100.0, nsteps=nsteps, nmech=0, npml=0, seed=31, nrec=4,
+                                f0=f0, hills=False)
+        prob.mesh = L.WQL_CARTESIAN
+        return prob
+    nmech = {"curv_elastic": 0, "curv_q4": 4, "curv_q8": 8}[kind]
+    return PB.curvilinear_q(n=n, h=100.0, nsteps=nsteps, nmech=nmech, npml=0, seed=31, nrec=4,
+                            f0=f0)
+
+
+@pytest.mark.parametrize("kind", ["cart_elastic", "curv_elastic", "curv_q4", "curv_q8"])
+def test_adjoint_dot_product(kind):
+    import torch
+    from waveqlab3d_q_gpu_amd.adjoint import AdjointRunner
+    prob = _problem(kind)
+    K = 3
+    blk = PB.make_device_block(prob, precision=L.WQL_FP64, state_mirror=True)
+    blk.set_sources([])
+    run = AdjointRunner(blk, prob.dt, K, prob.receivers)
+    rng = np.random.default_rng(17)
+    shp = blk.F.shape
+    dev = blk.F.device
+    xF = torch.from_numpy(rng.uniform(-1, 1, shp)).to(dev)
+    yF = torch.from_numpy(rng.uniform(-1, 1, shp)).to(dev)
+    # stresses scaled so that every component contributes to the products
+    xF[3:] *= 1e6
+    yF[3:] *= 1e-6
+    blk.F.copy_(xF)
+    mask = L.WQL_F
+    if prob.nmech:
+        xE = torch.from_numpy(rng.uniform(-1, 1, blk.eta.shape) * 1e3).to(dev)
+        yE = torch.from_numpy(rng.uniform(-1, 1, blk.eta.shape) * 1e-3).to(dev)
+        blk.eta.copy_(xE)
+        blk.Deta.zero_()
+        mask |= L.WQL_MEMORY
+    blk.upload(mask)
+    blk.time_step(prob.dt, K, 0.0)
+    blk.download(mask)
+    lhs = float((blk.F * yF).sum())
+    if prob.nmech:
+        lhs += float((blk.eta * yE).sum())
+    Fb = yF.clone().contiguous()
+    Eb = yE.clone().contiguous() if prob.nmech else None
+    run.adjoint_steps(Fb, Eb, K)
+    rhs = float((xF * Fb).sum())
+    if prob.nmech:
+        rhs += float((xE * Eb).sum())
+    rel = abs(lhs - rhs) / abs(lhs)
+    print(f"{kind}: <Phi x, y> = {lhs!r}  <x, Phi^T y> = {rhs!r}  rel {rel:.2e}", flush=True)
+    assert rel <= 1e-10
+
+
+def test_adjoint_gradient_matches_central_differences():
+    import torch
+    from waveqlab3d_q_gpu_amd.adjoint import AdjointRunner, receiver_traces
+    N = 40
+    prob = _problem("curv_q4", n=28, nsteps=N, f0=20.0)
+    prob.sources[0].t_init = 0.06
+    near = np.asarray(prob.sources[0].near)
+    rec = np.asarray([near + o for o in ((5, 0, 0), (-5, 0, 0), (0, -5, 0), (0, 5, 0),
+                                          (0, 0, 5), (0, 0, -5), (4, -4, 4), (-4, 4, -4))])
+    blk = PB.make_device_block(prob, precision=L.WQL_FP64, state_mirror=True)
+    run = AdjointRunner(blk, prob.dt, N, rec, every=7)
+    dev = blk.M.device
+    M0 = blk.M.clone()
+    lam0, mu0, rho0 = M0[0].clone(), M0[1].clone(), M0[2].clone()
+    rng = np.random.default_rng(5)
+    pert = [torch.from_numpy(1.0 + 0.02 * rng.uniform(-1, 1, lam0.shape)).to(dev) for _ in range(3)]
+    with torch.no_grad():
+        obs = receiver_traces(rho0 * pert[0], lam0 * pert[1], mu0 * pert[2], run).clone()
+
+    def misfit(rho, lam, mu):
+        tr = receiver_traces(rho, lam, mu, run)
+        return 0.5 * ((tr[..., :3] - obs[..., :3]) ** 2).sum()
+
+    rho = rho0.clone().requires_grad_(True)
+    lam = lam0.clone().requires_grad_(True)
+    mu = mu0.clone().requires_grad_(True)
+    J = misfit(rho, lam, mu)
+    J.backward()
+    assert float(J) > 0
+    grads = (rho.grad, lam.grad, mu.grad)
+    for k in range(4):
+        d = [torch.from_numpy(rng.uniform(-1, 1, lam0.shape)).to(dev) * m
+             for m in (rho0, lam0, mu0)]
+        adj = sum(float((g * dk).sum()) for g, dk in zip(grads, d))
+        eps = 1e-5
+        with torch.no_grad():
+            jp = float(misfit(rho0 + eps * d[0], lam0 + eps * d[1], mu0 + eps * d[2]))
+            jm = float(misfit(rho0 - eps * d[0], lam0 - eps * d[1], mu0 - eps * d[2]))
+        fd = (jp - jm) / (2 * eps)
+        rel = abs(fd - adj) / abs(fd)
+        print(f"direction {k}: fd {fd!r} adjoint {adj!r} rel {rel:.2e}", flush=True)
+        assert rel <= 1e-6

@@ -84,6 +84,15 @@ SIGNATURES = {
     "wql_block_stream": (_P, [_P]),
     "wql_block_set_timing": (C.c_int, [_P, C.c_int32]),
     "wql_block_kernel_times": (C.c_int, [_P, _D, C.POINTER(C.c_int32)]),
+    "wql_adjoint_create": (C.c_int, [_P, C.c_int32]),
+    "wql_adjoint_reset": (C.c_int, [_P]),
+    "wql_state_save": (C.c_int, [_P, C.c_int32]),
+    "wql_state_load": (C.c_int, [_P, C.c_int32]),
+    "wql_adjoint_stage": (C.c_int, [_P, C.c_double, C.c_double]),
+    "wql_adjoint_add_receivers": (C.c_int, [_P, _P]),
+    "wql_adjoint_get_state": (C.c_int, [_P, _P, _P]),
+    "wql_adjoint_set_state": (C.c_int, [_P, _P, _P]),
+    "wql_adjoint_gradient": (C.c_int, [_P, _P, _P, _P]),
 }
 
 _lib = None

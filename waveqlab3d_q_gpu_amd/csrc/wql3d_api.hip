@@ -9,6 +9,7 @@
 
 #include <vector>
 
+#include "wql3d_adjoint.h"
 #include "wql3d_internal.h"
 
 using namespace wql;
@@ -85,6 +86,7 @@ struct wql_block {
   long long bytes;
   int tune_nt, tune_xcd, tune_band, tune_split;  // node-kernel launch shape
   int tune_zm, tune_zchunk;                      // z-marching kernel, z levels per chunk
+  void* arena;    // one allocation holding F[2], DF, G, M, eta, Deta, Qs, Qp, W
   void* W;        // 15 planes of T: split anelastic stage hand-off
   int mech_zero;  // mechanisms whose eta and Deta are known to be all zero
   // kernel timing of fused stages (wql_block_set_timing): 4 events per stage
@@ -93,6 +95,18 @@ struct wql_block {
   int nev_used;
   int nev_alloc;
   hipEvent_t* ev;
+  // discrete adjoint (wql_adjoint_create): adjoint of (F, eta) and of
+  // (DF, Deta), gradient planes (rho, lambda, mu), workspace, state slots
+  int adj_nslots;
+  double* adj_Fb;
+  void* adj_Eb;
+  double* adj_DFb;
+  void* adj_DEb;
+  double* adj_g;
+  void* adj_Z;
+  double* slot_F;
+  void* slot_E;
+  int* slot_mz;
 };
 
 static int g_device = -1;
@@ -183,8 +197,8 @@ static int dalloc(wql_block* b, void** p, size_t bytes) {
 extern "C" void wql_block_destroy(wql_block* b) {
   if (!b) return;
   (void)hipStreamSynchronize(b->stream);
-  void* ptrs[] = {b->F[0], b->F[1], b->DF, b->G, b->M, b->eta, b->Deta, b->Qs, b->Qp,
-                  b->rec_idx, b->seis, b->W};
+  void* ptrs[] = {b->arena, b->rec_idx, b->seis, b->adj_Fb, b->adj_Eb, b->adj_DFb, b->adj_DEb,
+                  b->adj_g, b->adj_Z, b->slot_F, b->slot_E};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (int f = 0; f < 6; ++f) {
@@ -194,6 +208,7 @@ extern "C" void wql_block_destroy(wql_block* b) {
   }
   for (int i = 0; i < b->nev_alloc; ++i) (void)hipEventDestroy(b->ev[i]);
   free(b->ev);
+  free(b->slot_mz);
   (void)hipStreamDestroy(b->stream);
   free(b);
 }
@@ -240,17 +255,33 @@ extern "C" int wql_block_create(const wql_block_desc* d, wql_block** out) {
     int r_ = dalloc(b, (void**)&(ptr), (size_t)(bytes));   \
     if (r_) { wql_block_destroy(b); return r_; }           \
   } while (0)
-  ALLOC(b->F[0], 9 * P * sizeof(double));
-  ALLOC(b->F[1], 9 * P * sizeof(double));
-  ALLOC(b->DF, 9 * P * sizeof(double));
-  ALLOC(b->G, ngeo * P * b->esz);
-  ALLOC(b->M, 3 * P * b->esz);
-  if (d->nmech) {
-    ALLOC(b->eta, 6LL * d->nmech * P * b->esz);
-    ALLOC(b->Deta, 6LL * d->nmech * P * b->esz);
-    ALLOC(b->Qs, P * b->esz);
-    ALLOC(b->Qp, P * b->esz);
-    ALLOC(b->W, 15 * P * b->esz);
+  {
+    // all per-node plane arrays in one arena: array k starts on a 2 MiB
+    // boundary plus (k * stagger) mod 2 MiB bytes (env WQL3D_ARENA_STAGGER)
+    struct Part { void** ptr; long long bytes; };
+    const long long nm = d->nmech;
+    const Part parts[] = {
+        {(void**)&b->F[0], 9 * P * 8}, {(void**)&b->F[1], 9 * P * 8}, {(void**)&b->DF, 9 * P * 8},
+        {&b->G, ngeo * P * (long long)b->esz}, {&b->M, 3 * P * (long long)b->esz},
+        {&b->eta, 6 * nm * P * (long long)b->esz}, {&b->Deta, 6 * nm * P * (long long)b->esz},
+        {&b->Qs, nm ? P * (long long)b->esz : 0}, {&b->Qp, nm ? P * (long long)b->esz : 0},
+        {&b->W, nm ? 15 * P * (long long)b->esz : 0}};
+    const long long kPage = 2LL << 20;
+    long long stagger = 0;
+    if (const char* sv = getenv("WQL3D_ARENA_STAGGER")) stagger = atoll(sv);
+    long long off[10], end = 0;
+    int k = 0;
+    for (const Part& p : parts) {
+      off[k] = (end + kPage - 1) / kPage * kPage + ((k * stagger) % kPage + kPage) % kPage / 256 * 256;
+      end = off[k] + p.bytes;
+      ++k;
+    }
+    ALLOC(b->arena, end);
+    k = 0;
+    for (const Part& p : parts) {
+      *p.ptr = p.bytes ? (void*)((char*)b->arena + off[k]) : nullptr;
+      ++k;
+    }
   }
   const int n[3] = {b->nx, b->ny, b->nz};
   for (int f = 0; f < 6; ++f) {
@@ -869,5 +900,182 @@ extern "C" int wql_grid_metrics(int32_t nq, int32_t nr, int32_t ns, const double
   if (!X || !mx || !my || !mz || !J) return fail(WQL_EINVAL, "null array");
   HIP_TRY(launch_metrics(nq, nr, ns, X, mx, my, mz, J, 0));
   HIP_TRY(hipStreamSynchronize(0));
+  return WQL_OK;
+}
+
+// ---------------------------------------------------------------------------
+// discrete adjoint of the fused stage (wql3d_adjoint.hip)
+// ---------------------------------------------------------------------------
+static void adjoint_free(wql_block* b) {
+  void* ptrs[] = {b->adj_Fb, b->adj_Eb, b->adj_DFb, b->adj_DEb, b->adj_g, b->adj_Z, b->slot_F,
+                  b->slot_E};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  b->adj_Fb = b->adj_DFb = b->adj_g = b->slot_F = nullptr;
+  b->adj_Eb = b->adj_DEb = b->adj_Z = b->slot_E = nullptr;
+  free(b->slot_mz);
+  b->slot_mz = nullptr;
+  b->adj_nslots = 0;
+}
+
+extern "C" int wql_adjoint_create(wql_block* b, int32_t nslots) {
+  if (!b) return fail(WQL_EINVAL, "null block");
+  if (nslots < 0) return fail(WQL_EINVAL, "nslots must be >= 0");
+  for (int f = 0; f < 6; ++f)
+    if (b->d.pml[f]) return fail(WQL_EINVAL, "the adjoint does not differentiate PML slabs (face %d)", f);
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  adjoint_free(b);
+  const long long P = b->P, ne = 6LL * b->d.nmech * P;
+  const size_t es = b->esz;
+  int rc = 0;
+  if (!rc) rc = dalloc(b, (void**)&b->adj_Fb, 9 * P * sizeof(double));
+  if (!rc) rc = dalloc(b, &b->adj_Eb, ne * es);
+  if (!rc) rc = dalloc(b, (void**)&b->adj_DFb, 9 * P * sizeof(double));
+  if (!rc) rc = dalloc(b, &b->adj_DEb, ne * es);
+  if (!rc) rc = dalloc(b, (void**)&b->adj_g, 3 * P * sizeof(double));
+  if (!rc) rc = dalloc(b, &b->adj_Z, 18 * P * es);
+  if (!rc) rc = dalloc(b, (void**)&b->slot_F, (size_t)nslots * 9 * P * sizeof(double));
+  if (!rc) rc = dalloc(b, &b->slot_E, (size_t)nslots * ne * es);
+  if (!rc) {
+    b->slot_mz = (int*)calloc(nslots > 0 ? nslots : 1, sizeof(int));
+    if (!b->slot_mz) rc = fail(WQL_ENOMEM, "host allocation failed");
+  }
+  if (rc) { adjoint_free(b); return rc; }
+  b->adj_nslots = nslots;
+  return wql_adjoint_reset(b);
+}
+
+extern "C" int wql_adjoint_reset(wql_block* b) {
+  if (!b || !b->adj_Fb) return fail(WQL_ESTATE, "no adjoint context (wql_adjoint_create)");
+  const long long P = b->P, ne = 6LL * b->d.nmech * P;
+  HIP_TRY(launch_fill<double>(b->adj_Fb, 9 * P, 0.0, b->stream));
+  HIP_TRY(launch_fill<double>(b->adj_DFb, 9 * P, 0.0, b->stream));
+  HIP_TRY(launch_fill<double>(b->adj_g, 3 * P, 0.0, b->stream));
+  if (ne) {
+    if (b->d.precision == WQL_FP64) {
+      HIP_TRY(launch_fill<double>((double*)b->adj_Eb, ne, 0.0, b->stream));
+      HIP_TRY(launch_fill<double>((double*)b->adj_DEb, ne, 0.0, b->stream));
+    } else {
+      HIP_TRY(launch_fill<float>((float*)b->adj_Eb, ne, 0.0, b->stream));
+      HIP_TRY(launch_fill<float>((float*)b->adj_DEb, ne, 0.0, b->stream));
+    }
+  }
+  return WQL_OK;
+}
+
+static int check_slot(wql_block* b, int slot) {
+  if (!b || !b->adj_Fb) return fail(WQL_ESTATE, "no adjoint context (wql_adjoint_create)");
+  if (slot < 0 || slot >= b->adj_nslots)
+    return fail(WQL_EINVAL, "slot %d outside 0..%d", slot, b->adj_nslots - 1);
+  return WQL_OK;
+}
+
+extern "C" int wql_state_save(wql_block* b, int32_t slot) {
+  int rc = check_slot(b, slot);
+  if (rc) return rc;
+  const long long P = b->P, ne = 6LL * b->d.nmech * P;
+  HIP_TRY(hipMemcpyAsync(b->slot_F + (size_t)slot * 9 * P, b->F[b->cur], 9 * P * sizeof(double),
+                         hipMemcpyDeviceToDevice, b->stream));
+  if (ne)
+    HIP_TRY(hipMemcpyAsync((char*)b->slot_E + (size_t)slot * ne * b->esz, b->eta, ne * b->esz,
+                           hipMemcpyDeviceToDevice, b->stream));
+  b->slot_mz[slot] = b->mech_zero;
+  return WQL_OK;
+}
+
+extern "C" int wql_state_load(wql_block* b, int32_t slot) {
+  int rc = check_slot(b, slot);
+  if (rc) return rc;
+  const long long P = b->P, ne = 6LL * b->d.nmech * P;
+  HIP_TRY(hipMemcpyAsync(b->F[b->cur], b->slot_F + (size_t)slot * 9 * P, 9 * P * sizeof(double),
+                         hipMemcpyDeviceToDevice, b->stream));
+  if (ne)
+    HIP_TRY(hipMemcpyAsync(b->eta, (char*)b->slot_E + (size_t)slot * ne * b->esz, ne * b->esz,
+                           hipMemcpyDeviceToDevice, b->stream));
+  // the slot's zero mechanisms; Deta of those is zero as well (it only ever
+  // holds A*0 + 0)
+  b->mech_zero = b->slot_mz[slot];
+  return WQL_OK;
+}
+
+template <typename T>
+static int adjoint_stage_t(wql_block* b, double A, double Bdt) {
+  StepArgs<T> a = make_args<T>(b, 0.0);
+  AdjArgs<T> j;
+  j.Fb = b->adj_Fb;
+  j.Eb = (T*)b->adj_Eb;
+  j.DFb = b->adj_DFb;
+  j.DEb = (T*)b->adj_DEb;
+  for (int k = 0; k < 3; ++k) j.g[k] = b->adj_g + k * b->P;
+  j.Z = (T*)b->adj_Z;
+  HIP_TRY(launch_adjoint_stage<T>(a, j, b->d.mesh, A, Bdt, b->stream));
+  return WQL_OK;
+}
+
+extern "C" int wql_adjoint_stage(wql_block* b, double A, double Bdt) {
+  if (!b || !b->adj_Fb) return fail(WQL_ESTATE, "no adjoint context (wql_adjoint_create)");
+  return b->d.precision == WQL_FP64 ? adjoint_stage_t<double>(b, A, Bdt)
+                                    : adjoint_stage_t<float>(b, A, Bdt);
+}
+
+extern "C" int wql_adjoint_add_receivers(wql_block* b, const double* g) {
+  if (!b || !b->adj_Fb) return fail(WQL_ESTATE, "no adjoint context (wql_adjoint_create)");
+  if (!g) return fail(WQL_EINVAL, "null array");
+  if (!b->nrec) return fail(WQL_ESTATE, "no receivers (wql_block_set_receivers)");
+  HIP_TRY(launch_adjoint_inject(b->adj_Fb, b->P, b->rec_idx, b->nrec, g, b->stream));
+  return WQL_OK;
+}
+
+// dense device arrays (x fastest, no padding) <-> the adjoint planes
+template <typename T>
+static int adj_copy(wql_block* b, double* dense, void* planes, int ncomp, bool to_dense) {
+  const long long P = b->P;
+  const long long nodes = (long long)b->nx * b->ny * b->nz;
+  for (int c = 0; c < ncomp; ++c) {
+    T* pl = (T*)planes + c * P;
+    double* dn = dense + c * nodes;
+    if (to_dense)
+      HIP_TRY(launch_scatter<T>(dn, pl, b->nx, b->ny, b->nz, 1, 1, b->nx, (long long)b->nx * b->ny,
+                                0, 0, b->stream));
+    else
+      HIP_TRY(launch_gather<T>(pl, dn, b->nx, b->ny, b->nz, 1, 1, b->nx, (long long)b->nx * b->ny,
+                               0, 0, b->stream));
+  }
+  return WQL_OK;
+}
+
+static int adj_state_io(wql_block* b, double* Fbar, double* etabar, bool to_dense) {
+  if (!b || !b->adj_Fb) return fail(WQL_ESTATE, "no adjoint context (wql_adjoint_create)");
+  if (!Fbar) return fail(WQL_EINVAL, "null array");
+  int rc = adj_copy<double>(b, Fbar, b->adj_Fb, 9, to_dense);
+  if (rc) return rc;
+  if (b->d.nmech) {
+    if (!etabar) return fail(WQL_EINVAL, "null eta array");
+    rc = b->d.precision == WQL_FP64
+             ? adj_copy<double>(b, etabar, b->adj_Eb, 6 * b->d.nmech, to_dense)
+             : adj_copy<float>(b, etabar, b->adj_Eb, 6 * b->d.nmech, to_dense);
+    if (rc) return rc;
+  }
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  return WQL_OK;
+}
+
+extern "C" int wql_adjoint_get_state(wql_block* b, double* Fbar, double* etabar) {
+  return adj_state_io(b, Fbar, etabar, true);
+}
+
+extern "C" int wql_adjoint_set_state(wql_block* b, const double* Fbar, const double* etabar) {
+  return adj_state_io(b, const_cast<double*>(Fbar), const_cast<double*>(etabar), false);
+}
+
+extern "C" int wql_adjoint_gradient(wql_block* b, double* g_rho, double* g_lambda, double* g_mu) {
+  if (!b || !b->adj_Fb) return fail(WQL_ESTATE, "no adjoint context (wql_adjoint_create)");
+  if (!g_rho || !g_lambda || !g_mu) return fail(WQL_EINVAL, "null array");
+  double* out[3] = {g_rho, g_lambda, g_mu};
+  for (int k = 0; k < 3; ++k) {
+    int rc = adj_copy<double>(b, out[k], b->adj_g + k * b->P, 1, true);
+    if (rc) return rc;
+  }
+  HIP_TRY(hipStreamSynchronize(b->stream));
   return WQL_OK;
 }

@@ -337,8 +337,8 @@ def make_device_block(prob: Problem, precision=L.WQL_FP64, device="cuda", state_
     the GPU: metric terms from the device pipeline (grid_metrics), face
     frames from the device metrics, material and Q from the problem.  The
     state starts at the init_* values (F = 0, DF = 1e40, memory = 0).  With
-    state_mirror the block also gets a device tensor ``F`` that
-    ``download(WQL_F)`` fills."""
+    state_mirror the block also gets device tensors ``F`` (and ``eta``,
+    ``Deta`` when anelastic) that ``download`` fills and ``upload`` reads."""
     import torch
 
     from .block import Block, grid_metrics
@@ -368,6 +368,9 @@ def make_device_block(prob: Problem, precision=L.WQL_FP64, device="cuda", state_
     nq, nr, ns = prob.n
     if state_mirror:
         kw["F"] = torch.zeros((9, ns, nr, nq), dtype=torch.float64, device=dev)
+        if prob.nmech:
+            for name in ("eta", "Deta"):
+                kw[name] = torch.zeros((6, prob.nmech, ns, nr, nq), dtype=torch.float64, device=dev)
     blk = Block(nq=nq, nr=nr, ns=ns, mesh=prob.mesh, precision=precision, nmech=prob.nmech,
                 bc=prob.bc, pml=prob.pml, npml=prob.npml,
                 tau=tuple(prob.tau) if prob.tau is not None else (),
