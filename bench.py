@@ -80,19 +80,24 @@ def byte_model(prob, precision):
     nq, nr, ns = prob.n
     P = nq * nr * ns
     es = 8 if precision == 0 else 4
+    # RK state bytes per component: F read + write and DF read + write; the
+    # FP32-Kahan state adds the compensation plane (read + write)
+    kahan = precision == 2
+    ss = 4 if kahan else 8
+    state = 4 * ss + (8 if kahan else 0)
     ngeo = 10 if prob.mesh == 1 else 3
     na = active_mechanisms(prob)
     mem = (24 * na + 2) * es if prob.nmech else 0
     slab = slab_nodes(prob)
-    end_to_end = P * (36 * 8 + (ngeo + 3) * es + mem) + slab * 36 * es
+    end_to_end = P * (9 * state + (ngeo + 3) * es + mem) + slab * 36 * es
     lo, hi = interior_box(prob)
     n_int = int(np.prod([max(0, hi[a] - lo[a] + 1) for a in range(3)]))
     # elastic stages have no hand-off: the node kernels write F and DF
-    stencil_node = 9 * 8 + (ngeo + 3) * es + (15 * es if prob.nmech else 27 * 8)
+    stencil_node = 9 * ss + (ngeo + 3) * es + (15 * es if prob.nmech else 9 * (state - ss))
     kernels = {
         "interior": n_int * stencil_node,
         "shell": (P - n_int) * stencil_node + slab * 36 * es,
-        "memory_update": P * (15 * es + 4 * es + 24 * na * es + 36 * 8) if prob.nmech else 0,
+        "memory_update": P * (15 * es + 4 * es + 24 * na * es + 9 * state) if prob.nmech else 0,
     }
     return {"P": P, "interior_nodes": n_int, "active_mechanisms": na,
             "end_to_end": end_to_end, "kernels": kernels}
@@ -182,6 +187,8 @@ def main():
     ap.add_argument("--nmech", type=int, default=8)
     ap.add_argument("--npml", type=int, default=20)
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-kahan", action="store_true",
+                    help="skip the FP32 state with Kahan-compensated RK update")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true",
                     help="skip the FP32-mixed vs FP64 field comparison (host copies)")
@@ -237,12 +244,15 @@ def main():
     t0 = prob.sources[0].t_init - 0.5 * nslots * prob.dt
 
     results, finals = {}, {}
-    for prec, name in ((L.WQL_FP32_MIXED, "fp32"), (L.WQL_FP64, "fp64")):
+    for prec, name in ((L.WQL_FP32_MIXED, "fp32"), (L.WQL_FP32_KAHAN, "fp32_kahan"),
+                       (L.WQL_FP64, "fp64")):
         if name == "fp64" and args.no_fp64:
             continue
+        if name == "fp32_kahan" and args.no_kahan:
+            continue
         print(f"[bench] {name}: creating block", file=sys.stderr, flush=True)
-        accuracy = not args.no_accuracy and not args.no_fp64
-        blk = PB.make_device_block(prob, precision=prec, state_mirror=accuracy)
+        want_fields = not args.no_accuracy and not args.no_fp64
+        blk = PB.make_device_block(prob, precision=prec, state_mirror=want_fields)
         blk.set_receivers(receivers, nslots)
         print(f"[bench] {name}: device bytes {blk.device_bytes() / 2**30:.1f} GiB; timing "
               f"{args.warmup}+{args.steps} steps", file=sys.stderr, flush=True)
@@ -275,7 +285,7 @@ def main():
               f"{stage_ms:.3f} ms, kernels {[kernels[k]['ms'] for k in kernels]}",
               file=sys.stderr, flush=True)
         F = None
-        if accuracy:
+        if want_fields:
             # state after warmup + steps + the 2 timing steps
             blk.download(L.WQL_F)
             F = blk.F.cpu().numpy()
@@ -284,25 +294,24 @@ def main():
         del blk
         torch.cuda.empty_cache()
 
-    # FP32-mixed vs FP64: same inputs, same GPU, same time window
-    acc = {}
-    if "fp64" in finals:
-        s32, F32 = finals["fp32"]
+    # FP32 variants vs FP64: same inputs, same GPU, same time window
+    def accuracy(name):
+        s32, F32 = finals[name]
         s64, F64 = finals["fp64"]
         peaks = np.abs(s64).max(axis=(0, 2))
         top = float(peaks.max()) if peaks.size else 0.0
         errs = [rel_l2(s32[:, r], s64[:, r]) for r in range(s64.shape[1])
                 if top > 0 and peaks[r] >= 1e-3 * top]
-        acc["seismogram_rel_l2_max"] = max(errs) if errs else None
-        acc["seismogram_rel_l2_per_receiver"] = [float("%.3e" % e) for e in errs]
-        acc["receivers_with_signal"] = len(errs)
-        acc["receivers"] = (f"{nrec_surface} seeded surface receivers + 8 receivers 3-4 "
-                            "nodes from the source; a receiver counts when its FP64 peak "
-                            "is >= 1e-3 of the largest receiver peak in the window")
+        acc = {"seismogram_rel_l2_max": max(errs) if errs else None,
+               "seismogram_rel_l2_per_receiver": [float("%.3e" % e) for e in errs],
+               "receivers_with_signal": len(errs),
+               "receivers": (f"{nrec_surface} seeded surface receivers + 8 receivers 3-4 nodes "
+                             "from the source; a receiver counts when its FP64 peak is >= 1e-3 "
+                             "of the largest receiver peak in the window"),
+               "window": f"{args.warmup + args.steps} steps from rest around the Ricker peak"}
         if F32 is not None:
             acc["field_rel_l2"] = [float("%.3e" % rel_l2(F32[c], F64[c])) for c in range(9)]
-        acc["window"] = (f"{args.warmup + args.steps} steps from rest around the Ricker "
-                         "peak")
+        return acc
 
     r32 = results["fp32"]
     k32 = r32["kernels"]
@@ -342,7 +351,19 @@ def main():
         out["speedup_fp32_mixed_vs_fp64"] = round(r32["value"] / r64["value"], 3)
     out["bytes_per_node_stage_fp32"] = round(r32["stage_bytes"] / P, 1)
     out["active_mechanisms"] = r32["byte_model"]["active_mechanisms"]
-    out["accuracy_fp32_mixed_vs_fp64"] = acc
+    if "fp64" in finals:
+        out["accuracy_fp32_mixed_vs_fp64"] = accuracy("fp32")
+    if "fp32_kahan" in results:
+        rk = results["fp32_kahan"]
+        out["fp32_kahan"] = {"value": round(rk["value"], 2), "ms_per_step": round(rk["ms_per_step"], 4),
+                             "stage_ms": round(rk["stage_ms"], 4),
+                             "stage_GBs": round(rk["stage_GBs"], 1),
+                             "stage_frac": round(rk["stage_GBs"] / HBM_PEAK_GBS, 4),
+                             "bytes_per_node_stage": round(rk["stage_bytes"] / P, 1),
+                             "kernels": rk["kernels"]}
+        if "fp64" in finals:
+            out["fp32_kahan"]["accuracy_vs_fp64"] = accuracy("fp32_kahan")
+            out["fp32_kahan"]["speedup_vs_fp64"] = round(rk["value"] / results["fp64"]["value"], 3)
     print(json.dumps(out), flush=True)
 
 

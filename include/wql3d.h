@@ -43,7 +43,10 @@ enum {
 };
 
 enum { WQL_CARTESIAN = 0, WQL_CURVILINEAR = 1 };       /* type_of_mesh */
-enum { WQL_FP64 = 0, WQL_FP32_MIXED = 1 };             /* arithmetic mode */
+/* arithmetic mode: FP64; FP32 stencil/metric/memory arithmetic with the RK
+ * state F, DF in FP64 (FP32_MIXED); or all FP32 with a Kahan-compensated F
+ * update (FP32_KAHAN, fused stage only) */
+enum { WQL_FP64 = 0, WQL_FP32_MIXED = 1, WQL_FP32_KAHAN = 2 };
 enum { WQL_BC_NONE = 0, WQL_BC_CHARACTERISTIC = 1,     /* BoundaryConditions.f90:28-30 */
        WQL_BC_FREE_SURFACE = 2, WQL_BC_CLAMPED = 3 };
 enum { WQL_SRC_GAUSSIAN = 0, WQL_SRC_COSBELL = 1, WQL_SRC_RICKER = 2 };

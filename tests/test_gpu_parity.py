@@ -223,21 +223,24 @@ def test_grid_metrics_kernel(oracle):
         assert rel_l2(g.cpu().numpy(), r) < 1e-14
 
 
-def test_fp32_mixed_error_budget(oracle):
-    """FP32-mixed (FP64 RK accumulation of F/DF): stated error against the
-    FP64 oracle, per field and per receiver."""
+@pytest.mark.parametrize("precision", [L.WQL_FP32_MIXED, L.WQL_FP32_KAHAN])
+def test_fp32_error_budget(oracle, precision):
+    """FP32 stencil/metric/memory arithmetic with the RK state in FP64
+    (FP32-mixed) or in FP32 with a Kahan-compensated update (FP32-Kahan):
+    stated error against the FP64 oracle, per field and per receiver."""
+    name = "fp32-mixed" if precision == L.WQL_FP32_MIXED else "fp32-kahan"
     prob = small_curvilinear(nmech=8, nsteps=30)
     met = oracle.metrics(prob.X)
     orc = oracle_block(oracle, prob, met)
     _, seis_o = orc.time_step(prob.dt, prob.nsteps, receivers=receiver_flat(prob))
-    blk = gpu_block(prob, met, precision=L.WQL_FP32_MIXED)
+    blk = gpu_block(prob, met, precision=precision)
     blk.set_receivers(prob.receivers, prob.nsteps)
     time_step_RK(blk, prob.dt, prob.nsteps)
     _download_all(blk)
     errs = field_errors(blk.F, orc.F)
     seis_g = blk.seismograms()
     serr = [rel_l2(seis_g[:, r], seis_o[:, r]) for r in range(seis_o.shape[1])]
-    print("fp32-mixed field rel L2:", ["%.2e" % e for e in errs], flush=True)
-    print("fp32-mixed seismogram rel L2:", ["%.2e" % e for e in serr], flush=True)
+    print(f"{name} field rel L2:", ["%.2e" % e for e in errs], flush=True)
+    print(f"{name} seismogram rel L2:", ["%.2e" % e for e in serr], flush=True)
     assert max(errs) <= 1e-4
     assert max(serr) <= 1e-5

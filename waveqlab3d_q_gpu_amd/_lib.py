@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(HERE, "libwql3d_hip.so")
 # enums of wql3d.h
 WQL_OK, WQL_EINVAL, WQL_EHIP, WQL_ENOMEM, WQL_ESTATE, WQL_ENODEV = range(6)
 WQL_CARTESIAN, WQL_CURVILINEAR = 0, 1
-WQL_FP64, WQL_FP32_MIXED = 0, 1
+WQL_FP64, WQL_FP32_MIXED, WQL_FP32_KAHAN = 0, 1, 2
 WQL_SRC_GAUSSIAN, WQL_SRC_COSBELL, WQL_SRC_RICKER = 0, 1, 2
 WQL_F, WQL_DF, WQL_GEOMETRY, WQL_MATERIAL = 1, 2, 4, 8
 WQL_MEMORY, WQL_QINV, WQL_PML, WQL_FACES, WQL_ALL = 16, 32, 64, 128, 255

@@ -59,9 +59,11 @@ struct wql_block {
   // kernels' wavefront accesses never straddle extra cache lines
   long long P;
   size_t esz;                // sizeof(T) of geometry/material/memory arrays
+  size_t ssz;                // sizeof(S) of the RK state F, DF (8, or 4 in FP32-Kahan)
   int cur;                   // current F buffer
-  double* F[2];
-  double* DF;
+  void* F[2];                // S planes
+  void* DF;
+  void* Fc;                  // FP32-Kahan: compensation of F (float planes)
   void* G;
   void* M;
   void* eta;
@@ -163,7 +165,8 @@ static int validate(const wql_block_desc* d) {
                 (long long)n[0] * n[1] * n[2]);
   if (d->mesh != WQL_CARTESIAN && d->mesh != WQL_CURVILINEAR)
     return fail(WQL_EINVAL, "invalid mesh %d", d->mesh);
-  if (d->precision != WQL_FP64 && d->precision != WQL_FP32_MIXED)
+  if (d->precision != WQL_FP64 && d->precision != WQL_FP32_MIXED &&
+      d->precision != WQL_FP32_KAHAN)
     return fail(WQL_EINVAL, "invalid precision %d", d->precision);
   if (d->nmech != 0 && d->nmech != 4 && d->nmech != 8)
     return fail(WQL_EINVAL, "nmech must be 0, 4 or 8 (got %d)", d->nmech);
@@ -224,6 +227,7 @@ extern "C" int wql_block_create(const wql_block_desc* d, wql_block** out) {
   b->nx = d->nq; b->ny = d->nr; b->nz = d->ns;
   b->P = ((long long)b->nx * b->ny * b->nz + kPlaneAlign - 1) / kPlaneAlign * kPlaneAlign;
   b->esz = d->precision == WQL_FP64 ? sizeof(double) : sizeof(float);
+  b->ssz = d->precision == WQL_FP32_KAHAN ? sizeof(float) : sizeof(double);
   // launch shape (env WQL3D_NODE_TUNING = "nontemporal,xcd_remap,band_rows,
   // split,interior,zchunk"): band_rows 0 = whole y extent; split = memory
   // variables of the shell in their own kernel; interior = tile of the
@@ -261,7 +265,9 @@ extern "C" int wql_block_create(const wql_block_desc* d, wql_block** out) {
     struct Part { void** ptr; long long bytes; };
     const long long nm = d->nmech;
     const Part parts[] = {
-        {(void**)&b->F[0], 9 * P * 8}, {(void**)&b->F[1], 9 * P * 8}, {(void**)&b->DF, 9 * P * 8},
+        {&b->F[0], 9 * P * (long long)b->ssz}, {&b->F[1], 9 * P * (long long)b->ssz},
+        {&b->DF, 9 * P * (long long)b->ssz},
+        {&b->Fc, d->precision == WQL_FP32_KAHAN ? 9 * P * 4 : 0},
         {&b->G, ngeo * P * (long long)b->esz}, {&b->M, 3 * P * (long long)b->esz},
         {&b->eta, 6 * nm * P * (long long)b->esz}, {&b->Deta, 6 * nm * P * (long long)b->esz},
         {&b->Qs, nm ? P * (long long)b->esz : 0}, {&b->Qp, nm ? P * (long long)b->esz : 0},
@@ -269,7 +275,7 @@ extern "C" int wql_block_create(const wql_block_desc* d, wql_block** out) {
     const long long kPage = 2LL << 20;
     long long stagger = 0;
     if (const char* sv = getenv("WQL3D_ARENA_STAGGER")) stagger = atoll(sv);
-    long long off[10], end = 0;
+    long long off[11], end = 0;
     int k = 0;
     for (const Part& p : parts) {
       off[k] = (end + kPage - 1) / kPage * kPage + ((k * stagger) % kPage + kPage) % kPage / 256 * 256;
@@ -484,16 +490,24 @@ static int transfer(wql_block* b, uint32_t mask, bool up) {
     v.base = host;
     return up ? put_comp<T>(b, (T*)dev, v, c, l, nn, nm) : get_comp<T>(b, (const T*)dev, v, c, l, nn, nm);
   };
-  auto xfer64 = [&](double* dev, const double* host, int c, const char* nm) -> int {
+  // the RK state: double planes, or float planes in FP32-Kahan
+  const bool kahan = d.precision == WQL_FP32_KAHAN;
+  auto xfer_state = [&](void* base, const double* host, int c, const char* nm) -> int {
     View v = body;
     v.base = host;
-    return up ? put_comp<double>(b, dev, v, c, lo1, n, nm)
-              : get_comp<double>(b, dev, v, c, lo1, n, nm);
+    if (kahan) {
+      float* dev = (float*)base + c * P;
+      return up ? put_comp<float>(b, dev, v, c, lo1, n, nm) : get_comp<float>(b, dev, v, c, lo1, n, nm);
+    }
+    double* dev = (double*)base + c * P;
+    return up ? put_comp<double>(b, dev, v, c, lo1, n, nm) : get_comp<double>(b, dev, v, c, lo1, n, nm);
   };
-  if (mask & WQL_F)
-    for (int c = 0; c < 9 && !rc; ++c) rc = xfer64(b->F[b->cur] + c * P, d.F, c, "F");
+  if (mask & WQL_F) {
+    for (int c = 0; c < 9 && !rc; ++c) rc = xfer_state(b->F[b->cur], d.F, c, "F");
+    if (!rc && up && kahan) HIP_TRY(launch_fill<float>((float*)b->Fc, 9 * P, 0.0, b->stream));
+  }
   if (mask & WQL_DF)
-    for (int c = 0; c < 9 && !rc; ++c) rc = xfer64(b->DF + c * P, d.DF, c, "DF");
+    for (int c = 0; c < 9 && !rc; ++c) rc = xfer_state(b->DF, d.DF, c, "DF");
   if (rc) return rc;
   if (mask & WQL_GEOMETRY) {
     T* G = (T*)b->G;
@@ -562,9 +576,17 @@ template <typename T>
 static int init_state_t(wql_block* b) {
   const long long P = b->P;
   const int nm = b->d.nmech;
-  HIP_TRY(launch_fill<double>(b->F[0], 9 * P, 0.0, b->stream));
-  HIP_TRY(launch_fill<double>(b->F[1], 9 * P, 0.0, b->stream));
-  HIP_TRY(launch_fill<double>(b->DF, 9 * P, 1.0e40, b->stream));   // fields.f90:39
+  if (b->d.precision == WQL_FP32_KAHAN) {
+    HIP_TRY(launch_fill<float>((float*)b->F[0], 9 * P, 0.0, b->stream));
+    HIP_TRY(launch_fill<float>((float*)b->F[1], 9 * P, 0.0, b->stream));
+    HIP_TRY(launch_fill<float>((float*)b->Fc, 9 * P, 0.0, b->stream));
+    // 1e40 (fields.f90:39) is +inf in float; the A(1) = 0 scaling maps it to 0
+    HIP_TRY(launch_fill<float>((float*)b->DF, 9 * P, 1.0e40, b->stream));
+  } else {
+    HIP_TRY(launch_fill<double>((double*)b->F[0], 9 * P, 0.0, b->stream));
+    HIP_TRY(launch_fill<double>((double*)b->F[1], 9 * P, 0.0, b->stream));
+    HIP_TRY(launch_fill<double>((double*)b->DF, 9 * P, 1.0e40, b->stream));   // fields.f90:39
+  }
   if (nm) {
     HIP_TRY(launch_fill<T>((T*)b->eta, 6LL * nm * P, 0.0, b->stream));  // material.f90:389-404
     HIP_TRY(launch_fill<T>((T*)b->Deta, 6LL * nm * P, 0.0, b->stream));
@@ -653,19 +675,20 @@ static double source_time(int type, double t, double t0, double w) {
   return 0.0;
 }
 
-template <typename T>
-static StepArgs<T> make_args(wql_block* b, double t) {
+template <typename T, typename S = double>
+static StepArgs<T, S> make_args(wql_block* b, double t) {
   const wql_block_desc& d = b->d;
-  StepArgs<T> a;
+  StepArgs<T, S> a;
   memset(&a, 0, sizeof a);
   a.nx = b->nx; a.ny = b->ny; a.nz = b->nz;
   a.P = b->P;
   a.nmech = d.nmech;
   a.h[0] = d.hq; a.h[1] = d.hr; a.h[2] = d.hs;
   for (int k = 0; k < 3; ++k) { a.ih[k] = 1.0 / a.h[k]; a.pen[k] = d.tau0 / a.h[k]; }
-  a.Fin = b->F[b->cur];
-  a.Fout = b->F[b->cur ^ 1];
-  a.DF = b->DF;
+  a.Fin = (const S*)b->F[b->cur];
+  a.Fout = (S*)b->F[b->cur ^ 1];
+  a.DF = (S*)b->DF;
+  a.Fc = (S*)b->Fc;
   a.G = (const T*)b->G;
   a.M = (const T*)b->M;
   a.eta = (T*)b->eta;
@@ -683,13 +706,13 @@ static StepArgs<T> make_args(wql_block* b, double t) {
   }
   a.nsrc = b->nsrc;
   for (int s = 0; s < b->nsrc; ++s) {
-    const wql_source& S = b->src[s];
+    const wql_source& Sr = b->src[s];
     DevSource& D = a.src[s];
-    for (int k = 0; k < 3; ++k) { D.near[k] = S.near[k] - 1; D.off[k] = S.location[k] - S.near_phys[k]; }
-    D.order = S.order;
-    D.alpha = S.alpha;
-    const double A = source_time(S.type, t, S.t_init, S.duration);
-    const double m[6] = {S.mXX, S.mYY, S.mZZ, S.mXY, S.mXZ, S.mYZ};
+    for (int k = 0; k < 3; ++k) { D.near[k] = Sr.near[k] - 1; D.off[k] = Sr.location[k] - Sr.near_phys[k]; }
+    D.order = Sr.order;
+    D.alpha = Sr.alpha;
+    const double A = source_time(Sr.type, t, Sr.t_init, Sr.duration);
+    const double m[6] = {Sr.mXX, Sr.mYY, Sr.mZZ, Sr.mXY, Sr.mXZ, Sr.mYZ};
     for (int c = 0; c < 6; ++c) D.am[c] = A * m[c];
   }
   // strict interior (JU_xJU_yJU_z6.f90:16158-16173), 0-based inclusive
@@ -717,7 +740,7 @@ static StepArgs<T> make_args(wql_block* b, double t) {
 template <typename T>
 static int scale_t(wql_block* b, double A) {
   const long long P = b->P;
-  HIP_TRY(launch_scale<double>(b->DF, 9 * P, A, b->stream));
+  HIP_TRY(launch_scale<double>((double*)b->DF, 9 * P, A, b->stream));
   if (b->d.nmech) HIP_TRY(launch_scale<T>((T*)b->Deta, 6LL * b->d.nmech * P, A, b->stream));
   for (int f = 0; f < 6; ++f)
     if (b->pDQ[f]) HIP_TRY(launch_scale<T>((T*)b->pDQ[f], 9 * b->Ps[f], A, b->stream));
@@ -727,7 +750,7 @@ static int scale_t(wql_block* b, double A) {
 template <typename T>
 static int update_t(wql_block* b, double inc) {
   const long long P = b->P;
-  HIP_TRY(launch_update<double>(b->F[b->cur], b->DF, 9 * P, inc, b->stream));
+  HIP_TRY(launch_update<double>((double*)b->F[b->cur], (const double*)b->DF, 9 * P, inc, b->stream));
   if (b->d.nmech)
     HIP_TRY(launch_update<T>((T*)b->eta, (const T*)b->Deta, 6LL * b->d.nmech * P, inc, b->stream));
   for (int f = 0; f < 6; ++f)
@@ -735,13 +758,21 @@ static int update_t(wql_block* b, double inc) {
   return WQL_OK;
 }
 
-extern "C" int wql_rk_scale(wql_block* b, double A) {
+static int no_kahan(wql_block* b, const char* what) {
   if (!b) return fail(WQL_EINVAL, "null block");
+  if (b->d.precision == WQL_FP32_KAHAN)
+    return fail(WQL_ESTATE, "%s needs an FP64 RK state; the FP32-Kahan state runs through "
+                "wql_stage_fused / wql_time_step", what);
+  return WQL_OK;
+}
+
+extern "C" int wql_rk_scale(wql_block* b, double A) {
+  if (int rc = no_kahan(b, "wql_rk_scale")) return rc;
   return b->d.precision == WQL_FP64 ? scale_t<double>(b, A) : scale_t<float>(b, A);
 }
 
 extern "C" int wql_rk_update(wql_block* b, double inc) {
-  if (!b) return fail(WQL_EINVAL, "null block");
+  if (int rc = no_kahan(b, "wql_rk_update")) return rc;
   return b->d.precision == WQL_FP64 ? update_t<double>(b, inc) : update_t<float>(b, inc);
 }
 
@@ -762,7 +793,8 @@ extern "C" int wql_update_state_array(double* state, const double* rate, int64_t
 static int need_fp64(wql_block* b, const char* what) {
   if (!b) return fail(WQL_EINVAL, "null block");
   if (b->d.precision != WQL_FP64)
-    return fail(WQL_ESTATE, "%s is the FP64 reference-order path; use wql_stage_fused for FP32-mixed", what);
+    return fail(WQL_ESTATE, "%s is the FP64 reference-order path; use wql_stage_fused for FP32 modes",
+                what);
   return WQL_OK;
 }
 
@@ -798,7 +830,11 @@ extern "C" int wql_sat(wql_block* b) {
 
 static int record(wql_block* b) {
   if (b->nrec && b->slot < b->nslots) {
-    HIP_TRY(launch_sample(b->F[b->cur], b->P, b->rec_idx, b->nrec,
+    if (b->d.precision == WQL_FP32_KAHAN)
+      HIP_TRY(launch_sample<float>((const float*)b->F[b->cur], b->P, b->rec_idx, b->nrec,
+                                   b->seis + (size_t)9 * b->nrec * b->slot, b->stream));
+    else
+      HIP_TRY(launch_sample<double>((const double*)b->F[b->cur], b->P, b->rec_idx, b->nrec,
                           b->seis + (size_t)9 * b->nrec * b->slot, b->stream));
     ++b->slot;
   }
@@ -821,9 +857,9 @@ static int next_event(wql_block* b, hipEvent_t* out) {
   return WQL_OK;
 }
 
-template <typename T>
+template <typename T, typename S = double>
 static int stage_fused_t(wql_block* b, double A, double Bdt, double t) {
-  StepArgs<T> a = make_args<T>(b, t);
+  StepArgs<T, S> a = make_args<T, S>(b, t);
   a.A = A; a.Bdt = Bdt;
   if (!b->timing) {
     HIP_TRY(launch_stage_fused<T>(a, b->d.mesh, b->stream));
@@ -845,8 +881,9 @@ static int stage_fused_t(wql_block* b, double A, double Bdt, double t) {
 
 extern "C" int wql_stage_fused(wql_block* b, double A, double Bdt, double t) {
   if (!b) return fail(WQL_EINVAL, "null block");
-  int rc = b->d.precision == WQL_FP64 ? stage_fused_t<double>(b, A, Bdt, t)
-                                      : stage_fused_t<float>(b, A, Bdt, t);
+  int rc = b->d.precision == WQL_FP64   ? stage_fused_t<double>(b, A, Bdt, t)
+           : b->d.precision == WQL_FP32_MIXED ? stage_fused_t<float>(b, A, Bdt, t)
+                                              : stage_fused_t<float, float>(b, A, Bdt, t);
   if (rc) return rc;
   b->cur ^= 1;
   return WQL_OK;
@@ -921,6 +958,7 @@ static void adjoint_free(wql_block* b) {
 extern "C" int wql_adjoint_create(wql_block* b, int32_t nslots) {
   if (!b) return fail(WQL_EINVAL, "null block");
   if (nslots < 0) return fail(WQL_EINVAL, "nslots must be >= 0");
+  if (int rc = no_kahan(b, "the adjoint")) return rc;
   for (int f = 0; f < 6; ++f)
     if (b->d.pml[f]) return fail(WQL_EINVAL, "the adjoint does not differentiate PML slabs (face %d)", f);
   HIP_TRY(hipStreamSynchronize(b->stream));
@@ -974,7 +1012,7 @@ extern "C" int wql_state_save(wql_block* b, int32_t slot) {
   int rc = check_slot(b, slot);
   if (rc) return rc;
   const long long P = b->P, ne = 6LL * b->d.nmech * P;
-  HIP_TRY(hipMemcpyAsync(b->slot_F + (size_t)slot * 9 * P, b->F[b->cur], 9 * P * sizeof(double),
+  HIP_TRY(hipMemcpyAsync(b->slot_F + (size_t)slot * 9 * P, (const double*)b->F[b->cur], 9 * P * sizeof(double),
                          hipMemcpyDeviceToDevice, b->stream));
   if (ne)
     HIP_TRY(hipMemcpyAsync((char*)b->slot_E + (size_t)slot * ne * b->esz, b->eta, ne * b->esz,
@@ -987,7 +1025,7 @@ extern "C" int wql_state_load(wql_block* b, int32_t slot) {
   int rc = check_slot(b, slot);
   if (rc) return rc;
   const long long P = b->P, ne = 6LL * b->d.nmech * P;
-  HIP_TRY(hipMemcpyAsync(b->F[b->cur], b->slot_F + (size_t)slot * 9 * P, 9 * P * sizeof(double),
+  HIP_TRY(hipMemcpyAsync((double*)b->F[b->cur], b->slot_F + (size_t)slot * 9 * P, 9 * P * sizeof(double),
                          hipMemcpyDeviceToDevice, b->stream));
   if (ne)
     HIP_TRY(hipMemcpyAsync(b->eta, (char*)b->slot_E + (size_t)slot * ne * b->esz, ne * b->esz,

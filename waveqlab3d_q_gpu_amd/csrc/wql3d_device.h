@@ -97,25 +97,25 @@ __device__ __forceinline__ V* atp(V* base, unsigned idx) {
   using Byte = typename std::conditional<std::is_const<V>::value, const char, char>::type;
   return reinterpret_cast<V*>(reinterpret_cast<Byte*>(base) + idx * (unsigned)sizeof(V));
 }
-template <typename T>
-__device__ __forceinline__ void derivs_curv(const StepArgs<T>& a, const int pos[3],
+template <typename T, typename S>
+__device__ __forceinline__ void derivs_curv(const StepArgs<T, S>& a, const int pos[3],
                                             long long id64, T U[3][9]) {
   const long long P = a.P;
   const unsigned id = (unsigned)id64;
   const unsigned st[3] = {1u, (unsigned)a.nx, (unsigned)a.nx * (unsigned)a.ny};
   const int n[3] = {a.nx, a.ny, a.nz};
-  const double* Fc[9];
+  const S* Fc[9];
 #pragma unroll
   for (int c = 0; c < 9; ++c) Fc[c] = a.Fin + c * P;
   const T* Gk[10];
 #pragma unroll
   for (int k = 0; k < 10; ++k) Gk[k] = a.G + k * P;
   T dv[3][3];
-  T S[3][6];
+  T Sf[3][6];
 #pragma unroll
   for (int m = 0; m < 3; ++m)
 #pragma unroll
-    for (int c = 0; c < 6; ++c) S[m][c] = T(0);
+    for (int c = 0; c < 6; ++c) Sf[m][c] = T(0);
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     const T ih = (T)a.ih[d];
@@ -133,9 +133,9 @@ __device__ __forceinline__ void derivs_curv(const StepArgs<T>& a, const int pos[
 #pragma unroll
       for (int c = 0; c < 6; ++c) {
         const T t = ck * (T)ldp(Fc[3 + c], p);
-        S[0][c] += jm0 * t;
-        S[1][c] += jm1 * t;
-        S[2][c] += jm2 * t;
+        Sf[0][c] += jm0 * t;
+        Sf[1][c] += jm1 * t;
+        Sf[2][c] += jm2 * t;
       }
     });
     dv[d][0] = v0;
@@ -149,20 +149,20 @@ __device__ __forceinline__ void derivs_curv(const StepArgs<T>& a, const int pos[
 #pragma unroll
     for (int c = 0; c < 3; ++c) U[m][c] = X0 * dv[0][c] + X1 * dv[1][c] + X2 * dv[2][c];
 #pragma unroll
-    for (int c = 0; c < 6; ++c) U[m][3 + c] = S[m][c];
+    for (int c = 0; c < 6; ++c) U[m][3 + c] = Sf[m][c];
   }
 }
 
 // Cartesian mesh: diagonal metrics only, J-free form of the interior branch
 // (JU_xJU_yJU_z6.f90:16426-16470): U[d][c] = X_d(pt) / h_d * D_d(F_c).
-template <typename T>
-__device__ __forceinline__ void derivs_cart(const StepArgs<T>& a, const int pos[3],
+template <typename T, typename S>
+__device__ __forceinline__ void derivs_cart(const StepArgs<T, S>& a, const int pos[3],
                                             long long id64, T U[3][9]) {
   const long long P = a.P;
   const unsigned id = (unsigned)id64;
   const unsigned st[3] = {1u, (unsigned)a.nx, (unsigned)a.nx * (unsigned)a.ny};
   const int n[3] = {a.nx, a.ny, a.nz};
-  const double* Fc[9];
+  const S* Fc[9];
 #pragma unroll
   for (int c = 0; c < 9; ++c) Fc[c] = a.Fin + c * P;
 #pragma unroll
@@ -185,8 +185,8 @@ __device__ __forceinline__ void derivs_cart(const StepArgs<T>& a, const int pos[
 // ---------------------------------------------------------------------------
 // PML geometry (pml.f90:11-168; RHS_Interior.f90:1846-2451)
 // ---------------------------------------------------------------------------
-template <typename T>
-__device__ __forceinline__ bool pml_slab(const StepArgs<T>& a, int f, int x, int y, int z,
+template <typename T, typename S>
+__device__ __forceinline__ bool pml_slab(const StepArgs<T, S>& a, int f, int x, int y, int z,
                                          long long* sidx, double* d) {
   const int np = a.npml[f];
   const double d0 = 3.0 * 6.907755278982137;  // 6/(2*1)*log(1/Rf), Rf = 1e-3
@@ -227,8 +227,8 @@ __device__ __forceinline__ bool pml_slab(const StepArgs<T>& a, int f, int x, int
   return true;
 }
 
-template <typename T>
-__device__ __forceinline__ long long slab_plane(const StepArgs<T>& a, int f) {
+template <typename T, typename S>
+__device__ __forceinline__ long long slab_plane(const StepArgs<T, S>& a, int f) {
   const long long np = a.npml[f];
   return f < 2 ? np * a.ny * a.nz : (f < 4 ? (long long)a.nx * np * a.nz : (long long)a.nx * a.ny * np);
 }
@@ -237,8 +237,8 @@ __device__ __forceinline__ long long slab_plane(const StepArgs<T>& a, int f) {
 // SAT boundary data ubc at a face node: BCm/BCp (BoundaryConditions.f90:783-924)
 // ---------------------------------------------------------------------------
 // Frame of face f at face index fidx: l[3], m[3], n[3] and |grad xi|.
-template <typename T>
-__device__ __forceinline__ void face_frame(const StepArgs<T>& a, int f, long long fidx, long long NF,
+template <typename T, typename S>
+__device__ __forceinline__ void face_frame(const StepArgs<T, S>& a, int f, long long fidx, long long NF,
                                            T fr[10]) {
   const T* fd = a.face[f];
 #pragma unroll
@@ -293,8 +293,8 @@ __device__ __forceinline__ void sat_ubc_frame(const R fr[10], int code, int f, c
   ubc[8] = sn * (n2 * Gy + n1 * Gz);
 }
 
-template <typename T>
-__device__ __forceinline__ void sat_ubc(const StepArgs<T>& a, int f, long long fidx, long long NF,
+template <typename T, typename S>
+__device__ __forceinline__ void sat_ubc(const StepArgs<T, S>& a, int f, long long fidx, long long NF,
                                         const T u[9], T lam, T mu, T rho, T ubc[9]) {
   T fr[10];
   face_frame(a, f, fidx, NF, fr);
@@ -316,8 +316,8 @@ __device__ __forceinline__ void sat_structure(const S ubc[9], S lam, S mu, S rho
 }
 
 // face index of node (x,y,z) on face f; returns false if not on the face
-template <typename T>
-__device__ __forceinline__ bool on_face(const StepArgs<T>& a, int f, int x, int y, int z,
+template <typename T, typename S>
+__device__ __forceinline__ bool on_face(const StepArgs<T, S>& a, int f, int x, int y, int z,
                                         long long* fidx, long long* NF) {
   switch (f) {
     case 0: *fidx = y + (long long)a.ny * z; *NF = (long long)a.ny * a.nz; return x == 0;
@@ -350,24 +350,24 @@ __device__ __forceinline__ double singular_source(int i, int i0, double h, doubl
 }
 
 // source contribution (to subtract from DF(4..9)) at node (x,y,z)
-template <typename T, int MESH>
-__device__ __forceinline__ bool source_at(const StepArgs<T>& a, int x, int y, int z, long long id,
+template <typename T, int MESH, typename S>
+__device__ __forceinline__ bool source_at(const StepArgs<T, S>& a, int x, int y, int z, long long id,
                                           double out[6]) {
   bool any = false;
   for (int s = 0; s < a.nsrc; ++s) {
-    const DevSource& S = a.src[s];
-    const int r = S.order == 2 ? 2 : 4;
-    if (x < S.near[0] - r || x > S.near[0] + r || y < S.near[1] - r || y > S.near[1] + r ||
-        z < S.near[2] - r || z > S.near[2] + r)
+    const DevSource& Sr = a.src[s];
+    const int r = Sr.order == 2 ? 2 : 4;
+    if (x < Sr.near[0] - r || x > Sr.near[0] + r || y < Sr.near[1] - r || y > Sr.near[1] + r ||
+        z < Sr.near[2] - r || z > Sr.near[2] + r)
       continue;
     const long long P = a.P;
     const double qx = (double)a.G[0 * P + id];
     const double ry = (double)a.G[(MESH ? 4 : 1) * P + id];
     const double sz = (double)a.G[(MESH ? 8 : 2) * P + id];
     const double hx = a.h[0] / qx, hy = a.h[1] / ry, hz = a.h[2] / sz;
-    const double dx = singular_source(x, S.near[0], hx, S.off[0] / hx, S.order, S.alpha);
-    const double dy = singular_source(y, S.near[1], hy, S.off[1] / hy, S.order, S.alpha);
-    const double dz = singular_source(z, S.near[2], hz, S.off[2] / hz, S.order, S.alpha);
+    const double dx = singular_source(x, Sr.near[0], hx, Sr.off[0] / hx, Sr.order, Sr.alpha);
+    const double dy = singular_source(y, Sr.near[1], hy, Sr.off[1] / hy, Sr.order, Sr.alpha);
+    const double dz = singular_source(z, Sr.near[2], hz, Sr.off[2] / hz, Sr.order, Sr.alpha);
     const double delta = dx * dy * dz;
     if (!any) {
 #pragma unroll
@@ -375,7 +375,7 @@ __device__ __forceinline__ bool source_at(const StepArgs<T>& a, int x, int y, in
       any = true;
     }
 #pragma unroll
-    for (int c = 0; c < 6; ++c) out[c] += S.am[c] * delta;
+    for (int c = 0; c < 6; ++c) out[c] += Sr.am[c] * delta;
   }
   return any;
 }
@@ -392,6 +392,27 @@ template <bool NT, typename V>
 __device__ __forceinline__ void st_stream(V* p, V v) {
   if constexpr (NT) __builtin_nontemporal_store(v, p);
   else *p = v;
+}
+
+// DF = df and the RK state update F_out = F_in + Bdt*DF of component c at
+// node idu: FP64 state, or FP32 state with a Kahan-compensated sum whose
+// compensation planes Fc are updated in place.
+template <bool NT, typename T, typename S>
+__device__ __forceinline__ void rk_finish(const StepArgs<T, S>& a, int c, unsigned idu, double df) {
+  const long long P = a.P;
+  if constexpr (std::is_same<S, double>::value) {
+    st_stream<NT>(atp(a.DF + c * P, idu), df);
+    st_stream<NT>(atp(a.Fout + c * P, idu), ldp(a.Fin + c * P, idu) + a.Bdt * df);
+  } else {
+    const float d = (float)df;
+    st_stream<NT>(atp(a.DF + c * P, idu), d);
+    const float f = ldp(a.Fin + c * P, idu);
+    float* cp = atp(a.Fc + c * P, idu);
+    const float y = (float)a.Bdt * d - ld_stream<NT>(cp);
+    const float t = f + y;
+    st_stream<NT>(cp, (t - f) - y);
+    st_stream<NT>(atp(a.Fout + c * P, idu), t);
+  }
 }
 
 }  // namespace wql

@@ -35,7 +35,10 @@ struct DevSource {
   double am[6];     // A(t) * (mXX, mYY, mZZ, mXY, mXZ, mYZ) for the stage
 };
 
-template <typename T>
+// T: storage type of geometry, material, memory variables and PML fields;
+// S: type of the RK state F and rates DF (double, or float with the Kahan
+// compensation planes Fc).
+template <typename T, typename S = double>
 struct StepArgs {
   int nx, ny, nz;
   long long P;      // plane stride (>= nx*ny*nz, see the device layout)
@@ -43,9 +46,10 @@ struct StepArgs {
   double ih[3];     // 1/hq, 1/hr, 1/hs
   double h[3];      // hq, hr, hs
   double pen[3];    // tau0/h per axis (SAT)
-  const double* Fin;
-  double* Fout;
-  double* DF;
+  const S* Fin;
+  S* Fout;
+  S* DF;
+  S* Fc;            // Kahan compensation of F (S = float only), updated in place
   const T* G;
   const T* M;
   T* eta;
@@ -93,10 +97,10 @@ enum class RhsPart { kAll = 0, kInterior = 1 };
 // (the shell), and for anelastic stages with the W hand-off the
 // memory-variable kernel over every node
 enum class StagePart { kInterior = 0, kShell = 1, kMemory = 2 };
-template <typename T>
-hipError_t launch_stage_fused(const StepArgs<T>& a, int mesh, hipStream_t s);
-template <typename T>
-hipError_t launch_stage_part(const StepArgs<T>& a, int mesh, StagePart part, hipStream_t s);
+template <typename T, typename S>
+hipError_t launch_stage_fused(const StepArgs<T, S>& a, int mesh, hipStream_t s);
+template <typename T, typename S>
+hipError_t launch_stage_part(const StepArgs<T, S>& a, int mesh, StagePart part, hipStream_t s);
 hipError_t launch_rhs(const StepArgs<double>& a, int mesh, RhsPart part, hipStream_t s);
 hipError_t launch_sat(const StepArgs<double>& a, hipStream_t s);
 hipError_t launch_source(const StepArgs<double>& a, int mesh, hipStream_t s);
@@ -115,8 +119,9 @@ template <typename T>
 hipError_t launch_scatter(double* dst, const T* src, int nx, int ny, int nz, int ncomp,
                           long long sx, long long sy, long long sz, long long sc,
                           long long off0, hipStream_t s);
-hipError_t launch_sample(const double* F, long long P, const long long* idx, int n,
-                         double* out, hipStream_t s);
+template <typename S>
+hipError_t launch_sample(const S* F, long long P, const long long* idx, int n, double* out,
+                         hipStream_t s);
 hipError_t launch_metrics(int nx, int ny, int nz, const double* X, double* mx, double* my,
                           double* mz, double* J, hipStream_t s);
 

@@ -25,8 +25,8 @@ namespace wql {
 // contiguous range of logical indices.  Logical order is band-major: bands of
 // band_rows y-rows, then z, then 256-node segments of the band's (x,y) slab,
 // so consecutive work on one XCD shares y and z stencil neighbours in its L2.
-template <typename T>
-__device__ __forceinline__ bool map_node(const StepArgs<T>& a, int& x, int& y, int& z) {
+template <typename T, typename S>
+__device__ __forceinline__ bool map_node(const StepArgs<T, S>& a, int& x, int& y, int& z) {
   const unsigned h = blockIdx.x;
   unsigned L = h;
   if (a.xcd_remap) {
@@ -44,14 +44,14 @@ __device__ __forceinline__ bool map_node(const StepArgs<T>& a, int& x, int& y, i
   return yb < a.band_rows && y < a.ny;
 }
 
-template <typename T, int MESH, int NMECH, int MODE, bool NT, bool SHELL>
-__device__ __forceinline__ void node_epilogue(const StepArgs<T>& a, int x, int y, int z,
+template <typename T, int MESH, int NMECH, int MODE, bool NT, bool SHELL, typename S>
+__device__ __forceinline__ void node_epilogue(const StepArgs<T, S>& a, int x, int y, int z,
                                               long long id, const T (&U)[3][9]);
 
 // Node of the shell boxes (nbox > 0): the global thread index runs over the
 // concatenated boxes, x fastest inside each box.
-template <typename T>
-__device__ __forceinline__ bool map_box_node(const StepArgs<T>& a, int& x, int& y, int& z) {
+template <typename T, typename S>
+__device__ __forceinline__ bool map_box_node(const StepArgs<T, S>& a, int& x, int& y, int& z) {
   unsigned h = blockIdx.x;
   if (a.xcd_remap) {
     const unsigned B = gridDim.x, q = B >> 3, r = B & 7u, j = h & 7u, k = h >> 3;
@@ -70,8 +70,8 @@ __device__ __forceinline__ bool map_box_node(const StepArgs<T>& a, int& x, int& 
   return true;
 }
 
-template <typename T, int MESH, int NMECH, int MODE, bool NT>
-__global__ __launch_bounds__(256) void k_node(const StepArgs<T> a) {
+template <typename T, int MESH, int NMECH, int MODE, bool NT, typename S>
+__global__ __launch_bounds__(256) void k_node(const StepArgs<T, S> a) {
   int x, y, z;
   if (a.nbox > 0) {
     if (!map_box_node(a, x, y, z)) return;
@@ -95,8 +95,8 @@ __global__ __launch_bounds__(256) void k_node(const StepArgs<T> a) {
 // variables (MODE 0/1/2) or the W hand-off (MODE 3), PML, SAT, source and
 // the RK update or rate accumulation.  SHELL = false is the strict interior
 // (RHS_Center): no PML slab and no face, so those paths are compiled out.
-template <typename T, int MESH, int NMECH, int MODE, bool NT, bool SHELL>
-__device__ __forceinline__ void node_epilogue(const StepArgs<T>& a, int x, int y, int z,
+template <typename T, int MESH, int NMECH, int MODE, bool NT, bool SHELL, typename S>
+__device__ __forceinline__ void node_epilogue(const StepArgs<T, S>& a, int x, int y, int z,
                                               long long id, const T (&U)[3][9]) {
   const long long P = a.P;
   const unsigned idu = (unsigned)id;
@@ -306,8 +306,7 @@ __device__ __forceinline__ void node_epilogue(const StepArgs<T>& a, int x, int y
       double df = scale_rate(a.A, ld_stream<NT>(atp(a.DF + c * P, idu))) + (double)DU[c];
       if (has_src && c >= 3) df -= srcv[c - 3];
       if (has_sat) df -= satDF[c];
-      st_stream<NT>(atp(a.DF + c * P, idu), df);
-      st_stream<NT>(atp(a.Fout + c * P, idu), ldp(a.Fin + c * P, idu) + a.Bdt * df);
+      rk_finish<NT>(a, c, idu, df);
     }
   } else if (MODE == 3) {
     // hand the complete non-memory-variable rates and the (PML-modified)
@@ -338,8 +337,8 @@ __device__ __forceinline__ void node_epilogue(const StepArgs<T>& a, int x, int y
 // Memory variables of one node in a fused stage: DU(4..9) -= sum_i eta_i and
 // the relaxation update of eta_i / Deta_i from the strain rates e
 // (RHS_Interior.f90:2599-2906).  Mechanisms in mech_skip are not streamed.
-template <typename T, int NMECH, bool NT>
-__device__ __forceinline__ void memory_update_node(const StepArgs<T>& a, unsigned id, T mu, T l2m,
+template <typename T, int NMECH, bool NT, typename S>
+__device__ __forceinline__ void memory_update_node(const StepArgs<T, S>& a, unsigned id, T mu, T l2m,
                                                    const T (&e)[6], T (&DU)[9]) {
   const long long P = a.P;
   const T tr = e[0] + e[1] + e[2];
@@ -374,8 +373,8 @@ __device__ __forceinline__ void memory_update_node(const StepArgs<T>& a, unsigne
   }
 }
 
-template <typename T, int NMECH, bool NT>
-__global__ __launch_bounds__(256) void k_memory_update(const StepArgs<T> a) {
+template <typename T, int NMECH, bool NT, typename S>
+__global__ __launch_bounds__(256) void k_memory_update(const StepArgs<T, S> a) {
   const unsigned id = blockIdx.x * 256u + threadIdx.x;
   const long long P = a.P;
   if (id >= (unsigned long long)P) return;
@@ -389,10 +388,8 @@ __global__ __launch_bounds__(256) void k_memory_update(const StepArgs<T> a) {
   memory_update_node<T, NMECH, NT>(a, id, mu, l2m, e, DU);
 #pragma unroll
   for (int c = 0; c < 9; ++c) {
-    double* DFc = a.DF + c * P;
-    const double df = scale_rate(a.A, ld_stream<NT>(atp(DFc, id))) + (double)DU[c];
-    st_stream<NT>(atp(DFc, id), df);
-    st_stream<NT>(atp(a.Fout + c * P, id), ld_stream<NT>(atp(a.Fin + c * P, id)) + a.Bdt * df);
+    const double df = scale_rate(a.A, ld_stream<NT>(atp(a.DF + c * P, id))) + (double)DU[c];
+    rk_finish<NT>(a, c, id, df);
   }
 }
 
@@ -424,14 +421,15 @@ __host__ __device__ constexpr int sig_comp(int m, int c) {
 // operands of node p for direction D: v (3) and fl_D (3).  load_ops issues
 // the loads (F in fp64, J and X_m(D) in the storage type), make_ops forms the
 // operands, so the two can be placed apart to hide the load latency.
-template <typename T, int MESH>
+template <typename T, int MESH, typename S = double>
 struct RawOps {
-  double f[9];
+  S f[9];
   T g[MESH ? 4 : 1];
 };
 
-template <typename T, int MESH, int D>
-__device__ __forceinline__ void load_ops(const StepArgs<T>& a, unsigned p, RawOps<T, MESH>& r) {
+template <typename T, int MESH, int D, typename S>
+__device__ __forceinline__ void load_ops(const StepArgs<T, S>& a, unsigned p,
+                                         RawOps<T, MESH, S>& r) {
   const long long P = a.P;
 #pragma unroll
   for (int c = 0; c < 9; ++c) r.f[c] = ldp(a.Fin + c * P, p);
@@ -442,8 +440,8 @@ __device__ __forceinline__ void load_ops(const StepArgs<T>& a, unsigned p, RawOp
   }
 }
 
-template <typename T, int MESH, int D>
-__device__ __forceinline__ void make_ops(const RawOps<T, MESH>& r, T (&v)[3], T (&fl)[3]) {
+template <typename T, int MESH, int D, typename S>
+__device__ __forceinline__ void make_ops(const RawOps<T, MESH, S>& r, T (&v)[3], T (&fl)[3]) {
 #pragma unroll
   for (int c = 0; c < 3; ++c) v[c] = (T)r.f[c];
   if (MESH) {
@@ -457,10 +455,10 @@ __device__ __forceinline__ void make_ops(const RawOps<T, MESH>& r, T (&v)[3], T 
   }
 }
 
-template <typename T, int MESH, int D>
-__device__ __forceinline__ void node_operands(const StepArgs<T>& a, unsigned p, T (&v)[3],
+template <typename T, int MESH, int D, typename S>
+__device__ __forceinline__ void node_operands(const StepArgs<T, S>& a, unsigned p, T (&v)[3],
                                               T (&fl)[3]) {
-  RawOps<T, MESH> r;
+  RawOps<T, MESH, S> r;
   load_ops<T, MESH, D>(a, p, r);
   make_ops<T, MESH, D>(r, v, fl);
 }
@@ -484,9 +482,9 @@ __device__ __forceinline__ void pin(V& v) {
 
 // MODE 0: fused stage (memory variables in the epilogue); MODE 3: rates and
 // strain rates to W, memory variables and RK update by k_memory_update
-template <typename T, int MESH, int NMECH, int MODE, bool NT, int TX, int TY>
+template <typename T, int MESH, int NMECH, int MODE, bool NT, int TX, int TY, typename S>
 __global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WQL_INTERIOR_WAVES(T))))
-void k_interior(const StepArgs<T> a) {
+void k_interior(const StepArgs<T, S> a) {
   constexpr int HX = TX + 6, HY = TY + 6, NTH = TX * TY;
   // planes: v (0..2), fl_q (3..5), fl_r (6..8)
   __shared__ T L[9][HY][HX];
@@ -527,7 +525,7 @@ void k_interior(const StepArgs<T> a) {
     const unsigned p = col + (unsigned)z * plane;
     // level z+3 of the own column enters the queue (loads issued here, the
     // operands formed after the LDS staging of level z)
-    RawOps<T, MESH> rz;
+    RawOps<T, MESH, S> rz;
     if (inblk) load_ops<T, MESH, 2>(a, p + 3u * plane, rz);
     if (z < z0) {
       if (inblk) make_ops<T, MESH, 2>(rz, qv[6], qf[6]);
@@ -679,8 +677,7 @@ void k_interior(const StepArgs<T> a) {
         for (int c = 0; c < 9; ++c) {
           double df = scale_rate(a.A, ld_stream<NT>(atp(a.DF + c * P, idu))) + (double)DU[c];
           if (has_src && c >= 3) df -= srcv[c - 3];
-          st_stream<NT>(atp(a.DF + c * P, idu), df);
-          st_stream<NT>(atp(a.Fout + c * P, idu), ldp(a.Fin + c * P, idu) + a.Bdt * df);
+          rk_finish<NT>(a, c, idu, df);
         }
       }
     }
@@ -802,7 +799,8 @@ __global__ __launch_bounds__(256) void k_scatter(double* dst, const T* src, int 
       (double)src[(long long)c * plane * nz + t + z * plane];
 }
 
-__global__ void k_sample(const double* F, long long P, const long long* idx, int n, double* out) {
+template <typename S>
+__global__ void k_sample(const S* F, long long P, const long long* idx, int n, double* out) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= 9 * n) return;
   const int r = t / 9, c = t - 9 * (t / 9);
@@ -861,8 +859,8 @@ static inline dim3 plane_grid(int nx, int ny, int nz) {
   return dim3((unsigned)((plane + 255) / 256), (unsigned)nz, 1);
 }
 
-template <typename T, int MODE, bool NT>
-static hipError_t launch_node_nt(const StepArgs<T>& a, int mesh, hipStream_t s) {
+template <typename T, int MODE, bool NT, typename S>
+static hipError_t launch_node_nt(const StepArgs<T, S>& a, int mesh, hipStream_t s) {
   dim3 g;
   if (a.nbox > 0) {
     g = dim3((unsigned)((a.box_start[a.nbox] + 255) / 256));
@@ -873,7 +871,7 @@ static hipError_t launch_node_nt(const StepArgs<T>& a, int mesh, hipStream_t s) 
   const dim3 b(256);
 #define WQL_CASE(MS, NM)                                                   \
   if (mesh == MS && a.nmech == NM) {                                       \
-    hipLaunchKernelGGL((k_node<T, MS, NM, MODE, NT>), g, b, 0, s, a);      \
+    hipLaunchKernelGGL((k_node<T, MS, NM, MODE, NT, S>), g, b, 0, s, a);   \
     return hipGetLastError();                                              \
   }
   WQL_CASE(0, 0) WQL_CASE(0, 4) WQL_CASE(0, 8)
@@ -882,30 +880,30 @@ static hipError_t launch_node_nt(const StepArgs<T>& a, int mesh, hipStream_t s) 
   return hipErrorInvalidValue;
 }
 
-template <typename T, int MODE>
-static hipError_t launch_node(const StepArgs<T>& a, int mesh, hipStream_t s) {
+template <typename T, int MODE, typename S>
+static hipError_t launch_node(const StepArgs<T, S>& a, int mesh, hipStream_t s) {
   if ((MODE == 0 || MODE == 3) && a.nontemporal) return launch_node_nt<T, MODE, true>(a, mesh, s);
   return launch_node_nt<T, MODE, false>(a, mesh, s);
 }
 
-template <typename T, bool NT>
-static hipError_t launch_memory_update(const StepArgs<T>& a, hipStream_t s) {
+template <typename T, bool NT, typename S>
+static hipError_t launch_memory_update(const StepArgs<T, S>& a, hipStream_t s) {
   const dim3 g((unsigned)((a.P + 255) / 256));
-  if (a.nmech == 4) hipLaunchKernelGGL((k_memory_update<T, 4, NT>), g, dim3(256), 0, s, a);
-  else if (a.nmech == 8) hipLaunchKernelGGL((k_memory_update<T, 8, NT>), g, dim3(256), 0, s, a);
+  if (a.nmech == 4) hipLaunchKernelGGL((k_memory_update<T, 4, NT, S>), g, dim3(256), 0, s, a);
+  else if (a.nmech == 8) hipLaunchKernelGGL((k_memory_update<T, 8, NT, S>), g, dim3(256), 0, s, a);
   else return hipErrorInvalidValue;
   return hipGetLastError();
 }
 
-template <typename T, int TX, int TY>
-static hipError_t launch_interior_tile(const StepArgs<T>& a, int mesh, bool to_w, hipStream_t s) {
+template <typename T, int TX, int TY, typename S>
+static hipError_t launch_interior_tile(const StepArgs<T, S>& a, int mesh, bool to_w, hipStream_t s) {
   const long long bx = a.ix1 - a.ix0 + 1, by = a.iy1 - a.iy0 + 1, bz = a.iz1 - a.iz0 + 1;
   const long long ntx = (bx + TX - 1) / TX, nty = (by + TY - 1) / TY;
   const long long nzc = (bz + a.zchunk - 1) / a.zchunk;
   const dim3 g((unsigned)(ntx * nty * nzc)), b(TX * TY);
 #define WQL_CASE(MS, NM, MD)                                                        \
   if (mesh == MS && a.nmech == NM && (MD == 3) == to_w) {                           \
-    hipLaunchKernelGGL((k_interior<T, MS, NM, MD, false, TX, TY>), g, b, 0, s, a);  \
+    hipLaunchKernelGGL((k_interior<T, MS, NM, MD, false, TX, TY, S>), g, b, 0, s, a);  \
     return hipGetLastError();                                                       \
   }
   WQL_CASE(0, 0, 0) WQL_CASE(0, 4, 3) WQL_CASE(0, 8, 3) WQL_CASE(0, 4, 0) WQL_CASE(0, 8, 0)
@@ -915,8 +913,8 @@ static hipError_t launch_interior_tile(const StepArgs<T>& a, int mesh, bool to_w
 }
 
 // tile shape of the interior kernel: zm = 1 -> 64x8, 2 -> 32x16, 3 -> 64x4
-template <typename T>
-static hipError_t launch_interior(const StepArgs<T>& a, int mesh, bool to_w, hipStream_t s) {
+template <typename T, typename S>
+static hipError_t launch_interior(const StepArgs<T, S>& a, int mesh, bool to_w, hipStream_t s) {
   switch (a.zm) {
     case 2: return launch_interior_tile<T, 32, 16>(a, mesh, to_w, s);
     case 3: return launch_interior_tile<T, 64, 4>(a, mesh, to_w, s);
@@ -927,8 +925,8 @@ static hipError_t launch_interior(const StepArgs<T>& a, int mesh, bool to_w, hip
 // The six boxes of the block outside the strict interior box, full x rows
 // wherever possible: z slabs, then y slabs of the remaining z range, then x
 // slabs of the remaining (y, z) range.
-template <typename T>
-static void set_shell_boxes(StepArgs<T>& a) {
+template <typename T, typename S>
+static void set_shell_boxes(StepArgs<T, S>& a) {
   int nb = 0;
   auto add = [&](int x0, int x1, int y0, int y1, int z0, int z1) {
     if (x0 > x1 || y0 > y1 || z0 > z1) return;
@@ -948,25 +946,25 @@ static void set_shell_boxes(StepArgs<T>& a) {
   a.nbox = nb;
 }
 
-template <typename T>
-static bool interior_on(const StepArgs<T>& a) {
+template <typename T, typename S>
+static bool interior_on(const StepArgs<T, S>& a) {
   return a.zm > 0 && a.zchunk > 0 && a.ix0 <= a.ix1 && a.iy0 <= a.iy1 && a.iz0 <= a.iz1;
 }
 
 // anelastic stages with the W hand-off: node kernels write rates and strain
 // rates, k_memory_update finishes every node
-template <typename T>
-static bool split_stage(const StepArgs<T>& a) {
+template <typename T, typename S>
+static bool split_stage(const StepArgs<T, S>& a) {
   return a.nmech > 0 && a.split && a.W;
 }
 
-template <typename T>
-hipError_t launch_stage_part(const StepArgs<T>& a, int mesh, StagePart part, hipStream_t s) {
+template <typename T, typename S>
+hipError_t launch_stage_part(const StepArgs<T, S>& a, int mesh, StagePart part, hipStream_t s) {
   switch (part) {
     case StagePart::kInterior:
       return interior_on(a) ? launch_interior<T>(a, mesh, split_stage(a), s) : hipSuccess;
     case StagePart::kShell: {
-      StepArgs<T> sh = a;
+      StepArgs<T, S> sh = a;
       if (interior_on(a)) set_shell_boxes(sh);
       return split_stage(a) ? launch_node<T, 3>(sh, mesh, s) : launch_node<T, 0>(sh, mesh, s);
     }
@@ -977,8 +975,8 @@ hipError_t launch_stage_part(const StepArgs<T>& a, int mesh, StagePart part, hip
   }
 }
 
-template <typename T>
-hipError_t launch_stage_fused(const StepArgs<T>& a, int mesh, hipStream_t s) {
+template <typename T, typename S>
+hipError_t launch_stage_fused(const StepArgs<T, S>& a, int mesh, hipStream_t s) {
   for (StagePart p : {StagePart::kInterior, StagePart::kShell, StagePart::kMemory}) {
     hipError_t e = launch_stage_part<T>(a, mesh, p, s);
     if (e != hipSuccess) return e;
@@ -989,6 +987,9 @@ template hipError_t launch_stage_part<double>(const StepArgs<double>&, int, Stag
 template hipError_t launch_stage_part<float>(const StepArgs<float>&, int, StagePart, hipStream_t);
 template hipError_t launch_stage_fused<double>(const StepArgs<double>&, int, hipStream_t);
 template hipError_t launch_stage_fused<float>(const StepArgs<float>&, int, hipStream_t);
+template hipError_t launch_stage_part<float, float>(const StepArgs<float, float>&, int, StagePart,
+                                                    hipStream_t);
+template hipError_t launch_stage_fused<float, float>(const StepArgs<float, float>&, int, hipStream_t);
 
 hipError_t launch_rhs(const StepArgs<double>& a, int mesh, RhsPart part, hipStream_t s) {
   return part == RhsPart::kAll ? launch_node<double, 1>(a, mesh, s)
@@ -1077,12 +1078,17 @@ template hipError_t launch_scatter<double>(double*, const double*, int, int, int
 template hipError_t launch_scatter<float>(double*, const float*, int, int, int, int, long long,
                                           long long, long long, long long, long long, hipStream_t);
 
-hipError_t launch_sample(const double* F, long long P, const long long* idx, int n, double* out,
+template <typename S>
+hipError_t launch_sample(const S* F, long long P, const long long* idx, int n, double* out,
                          hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_sample, dim3((9 * n + 255) / 256), dim3(256), 0, s, F, P, idx, n, out);
+  hipLaunchKernelGGL(k_sample<S>, dim3((9 * n + 255) / 256), dim3(256), 0, s, F, P, idx, n, out);
   return hipGetLastError();
 }
+template hipError_t launch_sample<double>(const double*, long long, const long long*, int, double*,
+                                          hipStream_t);
+template hipError_t launch_sample<float>(const float*, long long, const long long*, int, double*,
+                                         hipStream_t);
 
 hipError_t launch_metrics(int nx, int ny, int nz, const double* X, double* mx, double* my,
                           double* mz, double* J, hipStream_t s) {
