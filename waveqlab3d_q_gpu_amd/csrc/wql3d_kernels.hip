@@ -337,40 +337,54 @@ __device__ __forceinline__ void node_epilogue(const StepArgs<T, S>& a, int x, in
 // Memory variables of one node in a fused stage: DU(4..9) -= sum_i eta_i and
 // the relaxation update of eta_i / Deta_i from the strain rates e
 // (RHS_Interior.f90:2599-2906).  Mechanisms in mech_skip are not streamed.
-template <typename T, int NMECH, bool NT, typename S>
-__device__ __forceinline__ void memory_update_node(const StepArgs<T, S>& a, unsigned id, T mu, T l2m,
-                                                   const T (&e)[6], T (&DU)[9]) {
-  const long long P = a.P;
+// The memory-variable update proper.  eta and Deta are passed as restrict
+// pointers so that the loads of one component can be issued ahead of the
+// stores of the previous one; each component loads all of its mechanisms
+// before it stores any.
+template <typename T, int NMECH, bool NT>
+__device__ __forceinline__ void memory_update_planes(T* __restrict__ eta, T* __restrict__ Deta,
+                                                     const T* __restrict__ Qsp,
+                                                     const T* __restrict__ Qpp, long long P,
+                                                     int mech_skip, const double* wgt,
+                                                     const double* tau_inv, double A, double Bdt,
+                                                     unsigned id, T mu, T l2m, const T (&e)[6],
+                                                     T (&DU)[9]) {
   const T tr = e[0] + e[1] + e[2];
-  const T Qs = ld_stream<NT>(atp(a.Qs, id)), Qp = ld_stream<NT>(atp(a.Qp, id));
+  const T Qs = ld_stream<NT>(atp(Qsp, id)), Qp = ld_stream<NT>(atp(Qpp, id));
   const T twomuQs = T(2) * mu * Qs;
   const T pfac = l2m * Qp - twomuQs;
   const T muQs = mu * Qs;
 #pragma unroll
   for (int c = 0; c < 6; ++c) {
-    T ev[NMECH];
+    T ev[NMECH], dv[NMECH];
     T sum = T(0);
 #pragma unroll
     for (int i = 0; i < NMECH; ++i) {
-      if (a.mech_skip & (1 << i)) { ev[i] = T(0); continue; }
-      ev[i] = ld_stream<NT>(atp(a.eta + (long long)(c * NMECH + i) * P, id));
+      if (mech_skip & (1 << i)) { ev[i] = T(0); dv[i] = T(0); continue; }
+      ev[i] = ld_stream<NT>(atp(eta + (long long)(c * NMECH + i) * P, id));
+      dv[i] = ld_stream<NT>(atp(Deta + (long long)(c * NMECH + i) * P, id));
       sum += ev[i];
     }
     DU[3 + c] -= sum;
 #pragma unroll
     for (int i = 0; i < NMECH; ++i) {
-      if (a.mech_skip & (1 << i)) continue;
-      const T w = (T)a.wgt[i];
-      const T ti = (T)a.tau_inv[i];
+      if (mech_skip & (1 << i)) continue;
+      const T w = (T)wgt[i];
+      const T ti = (T)tau_inv[i];
       const T src = c < 3 ? (w * twomuQs) * e[c] + (w * pfac) * tr : (w * muQs) * e[c];
       const T r = (src - ev[i]) * ti;
-      T* De = a.Deta + (long long)(c * NMECH + i) * P;
-      T* Et = a.eta + (long long)(c * NMECH + i) * P;
-      const T dn = scale_rate(a.A, ld_stream<NT>(atp(De, id))) + r;
-      st_stream<NT>(atp(De, id), dn);
-      st_stream<NT>(atp(Et, id), ev[i] + (T)a.Bdt * dn);
+      const T dn = scale_rate(A, dv[i]) + r;
+      st_stream<NT>(atp(Deta + (long long)(c * NMECH + i) * P, id), dn);
+      st_stream<NT>(atp(eta + (long long)(c * NMECH + i) * P, id), ev[i] + (T)Bdt * dn);
     }
   }
+}
+
+template <typename T, int NMECH, bool NT, typename S>
+__device__ __forceinline__ void memory_update_node(const StepArgs<T, S>& a, unsigned id, T mu, T l2m,
+                                                   const T (&e)[6], T (&DU)[9]) {
+  memory_update_planes<T, NMECH, NT>(a.eta, a.Deta, a.Qs, a.Qp, a.P, a.mech_skip, a.wgt, a.tau_inv,
+                                     a.A, a.Bdt, id, mu, l2m, e, DU);
 }
 
 template <typename T, int NMECH, bool NT, typename S>
