@@ -47,7 +47,7 @@ class _Block(C.Structure):
     _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
                 ("curvilinear", C.c_int), ("nmech", C.c_int), ("nsrc", C.c_int),
                 ("bc", C.c_int * 6), ("pml", C.c_int * 6), ("npml", C.c_int * 6),
-                ("hq", C.c_double), ("hr", C.c_double), ("hs", C.c_double),
+                ("sat_form", C.c_int), ("hq", C.c_double), ("hr", C.c_double), ("hs", C.c_double),
                 ("tau0", C.c_double),
                 ("tau", C.c_double * 8), ("weight", C.c_double * 8),
                 ("F", _P), ("DF", _P),
@@ -58,6 +58,14 @@ class _Block(C.Structure):
                 ("nl", _P * 6), ("nm", _P * 6), ("nn", _P * 6),
                 ("src", C.POINTER(_Source))]
 
+
+class _Iface(C.Structure):
+    _fields_ = [("ny", C.c_int), ("nz", C.c_int), ("coupling", C.c_int),
+                ("yf", _P), ("zf", _P), ("S", _P), ("DS", _P), ("W", _P), ("DW", _P),
+                ("Uhat", _P), ("Vhat", _P), ("Svel", _P), ("trup", _P)]
+
+
+IFACE_LOCKED, IFACE_SLIP_WEAKENING_TPV5 = 0, 1
 
 _lib = None
 
@@ -84,6 +92,10 @@ def lib():
                                     C.c_int, C.POINTER(C.c_long), _P]
         L.orc_source_time.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double]
         L.orc_source_time.restype = C.c_double
+        L.orc_iface_enforce.argtypes = [C.POINTER(_Iface), C.POINTER(_Block), C.POINTER(_Block),
+                                        C.c_double, C.c_int, C.c_int]
+        L.orc_iface_scale.argtypes = [C.POINTER(_Iface), C.c_double]
+        L.orc_iface_update.argtypes = [C.POINTER(_Iface), C.c_double]
         _lib = L
     return _lib
 
@@ -155,6 +167,9 @@ class OracleBlock:
         self.bc = [-1] * 6
         self.pml = [0] * 6
         self.npml = [0] * 6
+        # 0: the SAT penalty of the current source; 1: the alpha/theta form
+        # behind the archived golden files (wql_oracle.h, orc_block.sat_form)
+        self.sat_form = 0
         self.nmech = 0
         self.tau = np.zeros(8); self.weight = np.zeros(8)
         self.eta = None; self.Deta = None; self.Qs_inv = None; self.Qp_inv = None
@@ -207,6 +222,9 @@ class OracleBlock:
         b.nmech = self.nmech
         b.nsrc = len(self.sources)
         b.bc[:] = list(self.bc); b.pml[:] = list(self.pml); b.npml[:] = list(self.npml)
+        if self.sat_form and any(self.pml):
+            raise ValueError("the alpha/theta SAT form is defined without PML faces")
+        b.sat_form = int(self.sat_form)
         b.hq, b.hr, b.hs, b.tau0 = self.hq, self.hr, self.hs, self.tau0
         b.tau[:] = list(self.tau); b.weight[:] = list(self.weight)
         for name in ("F", "DF", "metx", "mety", "metz", "J", "M", "eta", "Deta",
@@ -252,3 +270,80 @@ class OracleBlock:
         lib().orc_time_step(C.byref(self._sync()), dt, nsteps, _ptr(tt), nrec,
                             rec.ctypes.data_as(C.POINTER(C.c_long)), _ptr(seis))
         return float(tt[0]), seis[:, :nrec] if nrec else None
+
+
+class OracleIface:
+    """The q-interface between the x-high face of block L and the x-low face
+    of block R (wql_oracle_iface.c).  yf, zf: coordinates of the face nodes
+    (nz, ny)."""
+
+    def __init__(self, ny, nz, yf, zf, coupling=IFACE_SLIP_WEAKENING_TPV5):
+        self.ny, self.nz, self.coupling = ny, nz, coupling
+        shp = (nz, ny)
+        self.yf = np.ascontiguousarray(yf, dtype=np.float64).reshape(shp)
+        self.zf = np.ascontiguousarray(zf, dtype=np.float64).reshape(shp)
+        self.S = np.zeros((4,) + shp)
+        self.DS = np.full((4,) + shp, 1.0e40)       # iface.f90:38-39
+        self.W = np.zeros(shp)
+        self.DW = np.full(shp, 1.0e40)
+        self.Uhat = np.zeros((6,) + shp)
+        self.Vhat = np.zeros((6,) + shp)
+        self.Svel = np.zeros((3,) + shp)
+        self.trup = np.full(shp, 1.0e9)
+
+    def _sync(self):
+        s = _Iface()
+        s.ny, s.nz, s.coupling = self.ny, self.nz, self.coupling
+        for name in ("yf", "zf", "S", "DS", "W", "DW", "Uhat", "Vhat", "Svel", "trup"):
+            setattr(s, name, _ptr(getattr(self, name)))
+        self._struct = s
+        return s
+
+    def enforce(self, L, R, t, stage, side):
+        """side 2: block L's evaluation (SAT into L); side 1: block R's."""
+        lib().orc_iface_enforce(C.byref(self._sync()), C.byref(L._sync()), C.byref(R._sync()),
+                                t, stage, side)
+
+    def scale_rates(self, A):
+        lib().orc_iface_scale(C.byref(self._sync()), A)
+
+    def update_fields(self, inc):
+        lib().orc_iface_update(C.byref(self._sync()), inc)
+
+
+def two_block_time_step(L, R, I, dt, nsteps, t=0.0):
+    """time_step_RK (time_step.f90:68-150) of a two-block domain: per stage
+    scale_rates, set_rates, boundary conditions (block L, then R) and the
+    interface (enforce_bound_iface_conditions, domain.f90:615-644), the
+    stage-1 output, update_fields (blocks, then interface).  Returns (t,
+    frames) with the fault output of every step (write_fault_output /
+    write_hat_output, domain.f90:782-847): Uface, Vface (block faces, 9),
+    S (4), Svel (3), Uhat, Vhat (6), trup, each (nsteps, ncomp, nz, ny)."""
+    A, B, Cc = rk_coefficients()
+    keys = {"Uface": 9, "Vface": 9, "S": 4, "Svel": 3, "Uhat": 6, "Vhat": 6, "trup": 1}
+    frames = {k: np.zeros((nsteps, n, I.nz, I.ny)) for k, n in keys.items()}
+    for n in range(nsteps):
+        t0 = t
+        for s in range(5):
+            for b in (L, R):
+                b.scale_rates(A[s])
+            I.scale_rates(A[s])
+            for b in (L, R):
+                b.set_rates()
+            L.impose_boundary_condition()
+            I.enforce(L, R, t, s + 1, 2)
+            R.impose_boundary_condition()
+            I.enforce(L, R, t, s + 1, 1)
+            if s == 0:
+                frames["Uface"][n] = L.F[:, :, :, -1]
+                frames["Vface"][n] = R.F[:, :, :, 0]
+                frames["S"][n] = I.S
+                frames["Svel"][n] = I.Svel
+                frames["Uhat"][n] = I.Uhat
+                frames["Vhat"][n] = I.Vhat
+                frames["trup"][n, 0] = I.trup
+            t = t0 + Cc[s + 1] * dt
+            for b in (L, R):
+                b.update_fields(B[s] * dt)
+            I.update_fields(B[s] * dt)
+    return t, frames

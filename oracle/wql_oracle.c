@@ -722,6 +722,50 @@ static void bc_char(double ubc[9], const double u[9], const double *l, const dou
   ubc[8] = sg * norm * (n[2] * F_y + n[1] * F_z);
 }
 
+/* the alpha/theta form of the penalty (commented out at
+ * BoundaryConditions.f90:832-840 for BCp and 904-912 for BCm): the rate
+ * vector itself, without structure(); alpha, theta per code as set in
+ * BC_Lx..BC_Rz (BoundaryConditions.f90:129-160) */
+static void bc_alpha_theta(double Ux[9], const double *l, const double *m, const double *n,
+                           const double u[9], double norm, double Zs, double Zp, double r,
+                           int code, int plus, double lam, double mu, double rho) {
+  const double alpha = code == 1 ? 2.0 : (code == 2 ? 1.0 : 0.0);
+  const double theta = code == 3 ? -1.0 : 0.0;
+  double T[3], v[3] = {u[0], u[1], u[2]};
+  T[0] = n[0] * u[3] + n[1] * u[6] + n[2] * u[7];
+  T[1] = n[0] * u[6] + n[1] * u[4] + n[2] * u[8];
+  T[2] = n[0] * u[7] + n[1] * u[8] + n[2] * u[5];
+  double vl = l[0] * v[0] + l[1] * v[1] + l[2] * v[2];
+  double vm = m[0] * v[0] + m[1] * v[1] + m[2] * v[2];
+  double vn = n[0] * v[0] + n[1] * v[1] + n[2] * v[2];
+  double Tl = l[0] * T[0] + l[1] * T[1] + l[2] * T[2];
+  double Tm = m[0] * T[0] + m[1] * T[1] + m[2] * T[2];
+  double Tn = n[0] * T[0] + n[1] * T[1] + n[2] * T[2];
+  const double sg = plus ? 1.0 : -1.0;
+  double pl = 0.5 * ((1.0 - r) * Zs * vl + sg * (1.0 + r) * Tl);
+  double pm = 0.5 * ((1.0 - r) * Zs * vm + sg * (1.0 + r) * Tm);
+  double pn = 0.5 * ((1.0 - r) * Zp * vn + sg * (1.0 + r) * Tn);
+  double ql = 0.5 * ((1.0 - r) * vl + sg * (1.0 + r) / Zs * Tl);
+  double qm = 0.5 * ((1.0 - r) * vm + sg * (1.0 + r) / Zs * Tm);
+  double qn = 0.5 * ((1.0 - r) * vn + sg * (1.0 + r) / Zp * Tn);
+  double Fx = l[0] * pl + m[0] * pm + n[0] * pn;
+  double Fy = l[1] * pl + m[1] * pm + n[1] * pn;
+  double Fz = l[2] * pl + m[2] * pm + n[2] * pn;
+  double F_x = l[0] * ql + m[0] * qm + n[0] * qn;
+  double F_y = l[1] * ql + m[1] * qm + n[1] * qn;
+  double F_z = l[2] * ql + m[2] * qm + n[2] * qn;
+  double l2m = lam + 2.0 * mu;
+  Ux[0] = alpha * norm / rho * Fx;
+  Ux[1] = alpha * norm / rho * Fy;
+  Ux[2] = alpha * norm / rho * Fz;
+  Ux[3] = theta * norm * (l2m * n[0] * F_x + lam * (n[1] * F_y + n[2] * F_z));
+  Ux[4] = theta * norm * (l2m * n[1] * F_y + lam * (n[0] * F_x + n[2] * F_z));
+  Ux[5] = theta * norm * (l2m * n[2] * F_z + lam * (n[0] * F_x + n[1] * F_y));
+  Ux[6] = theta * mu * norm * (n[1] * F_x + n[0] * F_y);
+  Ux[7] = theta * mu * norm * (n[2] * F_x + n[0] * F_z);
+  Ux[8] = theta * mu * norm * (n[2] * F_y + n[1] * F_z);
+}
+
 void orc_impose_boundary_condition(orc_block *b) {
   long N = (long)b->nx * b->ny * b->nz;
   double h[3] = {b->hq, b->hr, b->hs};
@@ -764,6 +808,7 @@ void orc_impose_boundary_condition(orc_block *b) {
         Ux[6] = mu * ubc[6];
         Ux[7] = mu * ubc[7];
         Ux[8] = mu * ubc[8];
+        if (b->sat_form == 1) bc_alpha_theta(Ux, l, m, n, u, norm, Zs, Zp, r, code, plus, lam, mu, rho);
         double pen = b->tau0 / h[axis];
         for (int c = 0; c < 9; ++c) b->DF[c * N + id] = b->DF[c * N + id] - pen * Ux[c];
         if (b->pml[f])

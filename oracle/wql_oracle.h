@@ -44,6 +44,13 @@ typedef struct orc_block {
   int bc[6];          /* Lx,Rx,Ly,Ry,Lz,Rz; 1 char., 2 free surface, 3 clamped, <=0 none */
   int pml[6];
   int npml[6];
+  /* SAT form: 0 = BCm/BCp + structure as in BoundaryConditions.f90:783-924
+   * and RHS_Interior.f90:1584-1768; 1 = the alpha/theta form kept in
+   * comments at BoundaryConditions.f90:832-840 / 904-912 (velocity penalty
+   * alpha*norm/rho*F with alpha = 2, 1, 0 for codes 1, 2, 3; stress penalty
+   * theta*norm*(...) with theta = 0, 0, -1; no structure), which produced
+   * the archived golden files test_problems/truth/1; no PML faces */
+  int sat_form;
   double hq, hr, hs, tau0;
   double tau[8], weight[8];
   double *F, *DF;
@@ -94,6 +101,30 @@ void orc_time_step(orc_block *b, double dt, int nsteps, double *t,
 
 /* source_time (moment_tensor.f90:813-872) + Ricker (new) */
 double orc_source_time(int type, double t, double t0, double w);
+
+/* ---- two-block q-interface (wql_oracle_iface.c) ------------------------ */
+enum { ORC_IFACE_LOCKED = 0, ORC_IFACE_SLIP_WEAKENING_TPV5 = 1 };
+
+/* Interface between the x-high face of block L and the x-low face of block
+ * R (same ny, nz).  Face arrays [k][NF], fidx = (y-1) + ny*(z-1).
+ *   S, DS: slip (3 local components n, m, l) and slip path length, and their
+ *   rates (iface.f90:36-44: S = 0, DS = 1e40 initially); W, DW: state;
+ *   yf, zf: y and z coordinates of the face nodes (prestress, strength);
+ *   stage-1 outputs Uhat, Vhat [6] (slip velocity (local), hat tractions +
+ *   prestress), Svel [3], trup (1e9 initially). */
+typedef struct orc_iface {
+  int ny, nz;
+  int coupling;
+  double *yf, *zf;
+  double *S, *DS, *W, *DW;
+  double *Uhat, *Vhat, *Svel, *trup;
+} orc_iface;
+
+/* enforce_iface_conditions for one side: side 2 = block L's call (SAT into
+ * L), side 1 = block R's call (SAT into R); both add to DS */
+void orc_iface_enforce(orc_iface *I, orc_block *L, orc_block *R, double t, int stage, int side);
+void orc_iface_scale(orc_iface *I, double A);     /* scale_rates_iface (iface.f90:63-73) */
+void orc_iface_update(orc_iface *I, double inc);  /* update_fields_iface (iface.f90:76-88) */
 
 #ifdef __cplusplus
 }

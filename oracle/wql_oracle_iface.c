@@ -1,0 +1,287 @@
+/*
+ * wql_oracle_iface.c -- CPU restatement (FP64) of the two-block interface of
+ * WaveQLab3D-Q: the q-direction interface between the x-high face of block L
+ * and the x-low face of block R, locked or TPV5 linear slip-weakening.
+ * TEST INFRASTRUCTURE ONLY (see wql_oracle.h).
+ *
+ * Follows, step by step and in the same order of operations:
+ *   enforce_iface_conditions (domain.f90:615-680): one evaluation per block,
+ *     block L first (its side 2), then block R (its side 1), both on the same
+ *     interface object;
+ *   Impose_Interface_Condition (RHS_Interior.f90:1771-1843): SAT forcing
+ *     DF(face) -= tau0/hq * forcing of the calling block;
+ *   Couple_Interface_x (CouplingForcing.f90:9-463): tractions, rotation to
+ *     (n, m, l), hat variables, stage-1 outputs, rotation back, characteristic
+ *     SAT forcing u_bc / v_bc;
+ *   Interface_RHS (Interface_Condition.f90:12-513): hats by Welded_Interface
+ *     (:515-590) or Slip_Weakening_Friction (:745-916) with solve (:1095-1168),
+ *     Tau_strength TPV5 (:1283-1299), prestress TPV5 (:1802-1830), boxcar
+ *     (:2438-2457); DS(1:3) += Q - P, DS(4) += |slip rate|;
+ *   rupture_front (CouplingForcing.f90:580-600);
+ *   scale_rates_iface / update_fields_iface (iface.f90:63-88).
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "wql_oracle.h"
+
+static double boxcar(double x, double w) {
+  const double tol = 100.0 * 2.220446049250313e-16;
+  if (-w + tol < x && x < w - tol) return 1.0;
+  if (fabs(-w - x) <= tol || fabs(x - w) <= tol) return 0.5;
+  return 0.0;
+}
+
+/* prestress, problem TPV5 (Interface_Condition.f90:1821-1830) */
+static void prestress_tpv5(double y, double z, double *T0n, double *T0m, double *T0l) {
+  *T0n = -120.0;
+  *T0m = 0.0;
+  double l = 70.0;
+  l = l + 8.0 * boxcar(y - 7.5, 1.5) * boxcar(z + 7.5, 1.5);
+  l = l + 11.6 * boxcar(y - 7.5, 1.5) * boxcar(z, 1.5);
+  l = l - 8.0 * boxcar(y - 7.5, 1.5) * boxcar(z - 7.5, 1.5);
+  *T0l = l;
+}
+
+/* Tau_strength, problem TPV5 (Interface_Condition.f90:1283-1299) */
+static double tau_strength_tpv5(double sigma_n, double D, double y, double z) {
+  double fs = 0.677;
+  const double fd = 0.525, Dc = 0.4;
+  fs = fs + 1.0e10 * (1.0 - boxcar(y, 15.0) * boxcar(z, 15.0));
+  const double f = fs - (fs - fd) * fmin(D, Dc) / Dc;
+  return sigma_n * f;
+}
+
+/* hats P (left) and Q (right) in local (n, m, l) coordinates from the rotated
+ * grid values U, V; returns the slip-rate magnitude of the friction solve */
+static void hats(int coupling, const double U[6], const double V[6], double Z1s, double Z2s,
+                 double Z1p, double Z2p, double slip, double y, double z, double P[6],
+                 double Q[6]) {
+  const double p_n = Z1p * U[0] - U[3], p_m = Z1s * U[1] - U[4], p_l = Z1s * U[2] - U[5];
+  const double q_n = Z2p * V[0] + V[3], q_m = Z2s * V[1] + V[4], q_l = Z2s * V[2] + V[5];
+  if (coupling == ORC_IFACE_LOCKED) {
+    /* Welded_Interface (Interface_Condition.f90:515-590) */
+    const double eta_p = 1.0 / (1.0 / Z1p + 1.0 / Z2p);
+    const double eta_s = 1.0 / (1.0 / Z1s + 1.0 / Z2s);
+    const double phi_n = eta_p * (q_n / Z2p - p_n / Z1p);
+    const double phi_m = eta_s * (q_m / Z2s - p_m / Z1s);
+    const double phi_l = eta_s * (q_l / Z2s - p_l / Z1s);
+    P[0] = (q_n - phi_n) / Z2p; P[3] = phi_n;
+    Q[0] = (p_n + phi_n) / Z1p; Q[3] = phi_n;
+    P[2] = (q_l - phi_l) / Z2s; P[5] = phi_l;
+    Q[2] = (p_l + phi_l) / Z1s; Q[5] = phi_l;
+    P[1] = (q_m - phi_m) / Z2s; P[4] = phi_m;
+    Q[1] = (p_m + phi_m) / Z1s; Q[4] = phi_m;
+    return;
+  }
+  /* Slip_Weakening_Friction (Interface_Condition.f90:745-916) */
+  const double eta_s = 1.0 / (1.0 / Z1s + 1.0 / Z2s);
+  const double eta_p = 1.0 / (1.0 / Z1p + 1.0 / Z2p);
+  double T0n, T0m, T0l;
+  prestress_tpv5(y, z, &T0n, &T0m, &T0l);
+  const double phi_n = eta_p * (q_n / Z2p - p_n / Z1p);
+  const double phi_m = eta_s * (q_m / Z2s - p_m / Z1s);
+  const double phi_l = eta_s * (q_l / Z2s - p_l / Z1s);
+  const double u_n_hat = (q_n - phi_n) / Z2p, Tu_n_hat = phi_n;
+  const double v_n_hat = (p_n + phi_n) / Z1p, Tv_n_hat = phi_n;
+  const double tau_lock = sqrt((T0l + phi_l) * (T0l + phi_l) + (T0m + phi_m) * (T0m + phi_m));
+  const double sigma_n = fmax(0.0, -(T0n + phi_n));
+  const double tau_str = tau_strength_tpv5(sigma_n, slip, y, z);
+  double Tu_m_hat, Tu_l_hat, Tv_m_hat, Tv_l_hat, vv_m, vv_l;
+  if (tau_lock > tau_str) {
+    /* solve, slip-weakening branch (Interface_Condition.f90:1143-1160) */
+    const double phi_1 = T0m + phi_m, phi_2 = T0l + phi_l;
+    const double Phi = sqrt(phi_1 * phi_1 + phi_2 * phi_2);
+    const double V = (Phi - tau_str) / eta_s;
+    vv_m = phi_1 / (eta_s + tau_str / V);
+    vv_l = phi_2 / (eta_s + tau_str / V);
+    const double tau_m = phi_1 - eta_s * vv_m;
+    const double tau_l = phi_2 - eta_s * vv_l;
+    Tu_m_hat = tau_m - T0m;
+    Tu_l_hat = tau_l - T0l;
+    Tv_m_hat = tau_m - T0m;
+    Tv_l_hat = tau_l - T0l;
+  } else {
+    Tu_m_hat = phi_m;
+    Tu_l_hat = phi_l;
+    Tv_m_hat = phi_m;
+    Tv_l_hat = phi_l;
+    vv_m = 0.0;
+    vv_l = 0.0;
+  }
+  const double v_m_hat = (Tu_m_hat + p_m) / Z1s + vv_m;
+  const double v_l_hat = (Tu_l_hat + p_l) / Z1s + vv_l;
+  const double u_m_hat = (q_m - Tv_m_hat) / Z2s - vv_m;
+  const double u_l_hat = (q_l - Tv_l_hat) / Z2s - vv_l;
+  P[0] = u_n_hat; P[1] = u_m_hat; P[2] = u_l_hat;
+  P[3] = Tu_n_hat; P[4] = Tu_m_hat; P[5] = Tu_l_hat;
+  Q[0] = v_n_hat; Q[1] = v_m_hat; Q[2] = v_l_hat;
+  Q[3] = Tv_n_hat; Q[4] = Tv_m_hat; Q[5] = Tv_l_hat;
+}
+
+/* to local (n, m, l): Rotate_in_Local_Cordinates (CouplingForcing.f90:466-522) */
+static void rot_in(const double *n, const double *m, const double *l, double *u) {
+  const double u1 = u[0], u2 = u[1], u3 = u[2], T1 = u[3], T2 = u[4], T3 = u[5];
+  u[0] = n[0] * u1 + n[1] * u2 + n[2] * u3;
+  u[1] = m[0] * u1 + m[1] * u2 + m[2] * u3;
+  u[2] = l[0] * u1 + l[1] * u2 + l[2] * u3;
+  u[3] = n[0] * T1 + n[1] * T2 + n[2] * T3;
+  u[4] = m[0] * T1 + m[1] * T2 + m[2] * T3;
+  u[5] = l[0] * T1 + l[1] * T2 + l[2] * T3;
+}
+
+/* back to x, y, z: Rotate_Back_Physical_Cordinates (CouplingForcing.f90:525-578) */
+static void rot_back(const double *n, const double *m, const double *l, double *u) {
+  const double u1 = u[0], u2 = u[1], u3 = u[2], T1 = u[3], T2 = u[4], T3 = u[5];
+  u[0] = n[0] * u1 + m[0] * u2 + l[0] * u3;
+  u[1] = n[1] * u1 + m[1] * u2 + l[1] * u3;
+  u[2] = n[2] * u1 + m[2] * u2 + l[2] * u3;
+  u[3] = n[0] * T1 + m[0] * T2 + l[0] * T3;
+  u[4] = n[1] * T1 + m[1] * T2 + l[1] * T3;
+  u[5] = n[2] * T1 + m[2] * T2 + l[2] * T3;
+}
+
+/* S^T Z S and S^T Z^-1 S as written in CouplingForcing.f90:316-341 */
+static void char_mats(const double *n, const double *m, const double *l, double Zp, double Zs,
+                      double Mz[3][3], double Mi[3][3]) {
+  const double *r[3] = {n, m, l};
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) {
+      /* entry (a,b): r_a(1) r_b(1) Zp + r_a(2) r_b(2) Zs + r_a(3) r_b(3) Zs */
+      Mz[a][b] = r[a][0] * r[b][0] * Zp + r[a][1] * r[b][1] * Zs + r[a][2] * r[b][2] * Zs;
+      Mi[a][b] = r[a][0] * r[b][0] * 1.0 / Zp + r[a][1] * r[b][1] * 1.0 / Zs +
+                 r[a][2] * r[b][2] * 1.0 / Zs;
+    }
+}
+
+static void eval_side(orc_iface *I, orc_block *L, orc_block *R, int ib, double t, int stage) {
+  orc_block *B = ib == 2 ? L : R;
+  const int ny = I->ny, nz = I->nz, NF = ny * nz;
+  const int fL = 1, fR = 0;            /* Rx face of L, Lx face of R */
+  const int fB = ib == 2 ? fL : fR;
+  const long long NL = (long long)L->nx * L->ny * L->nz, NR = (long long)R->nx * R->ny * R->nz;
+  const long long NB = ib == 2 ? NL : NR;
+  for (int z = 0; z < nz; ++z)
+    for (int y = 0; y < ny; ++y) {
+      const int fi = y + ny * z;
+      const long long iL = (L->nx - 1) + (long long)L->nx * (y + (long long)L->ny * z);
+      const long long iR = 0 + (long long)R->nx * (y + (long long)R->ny * z);
+      const long long iB = ib == 2 ? iL : iR;
+      double n[3], m[3], l[3];
+      for (int k = 0; k < 3; ++k) {
+        n[k] = B->nn[fB][k * NF + fi];
+        m[k] = B->nm[fB][k * NF + fi];
+        l[k] = B->nl[fB][k * NF + fi];
+      }
+      const double *u = L->F, *v = R->F;
+      /* tractions and velocities in x, y, z (CouplingForcing.f90:107-140) */
+      double U[6], V[6];
+      U[3] = n[0] * u[3 * NL + iL] + n[1] * u[6 * NL + iL] + n[2] * u[7 * NL + iL];
+      U[4] = n[0] * u[6 * NL + iL] + n[1] * u[4 * NL + iL] + n[2] * u[8 * NL + iL];
+      U[5] = n[0] * u[7 * NL + iL] + n[1] * u[8 * NL + iL] + n[2] * u[5 * NL + iL];
+      V[3] = n[0] * v[3 * NR + iR] + n[1] * v[6 * NR + iR] + n[2] * v[7 * NR + iR];
+      V[4] = n[0] * v[6 * NR + iR] + n[1] * v[4 * NR + iR] + n[2] * v[8 * NR + iR];
+      V[5] = n[0] * v[7 * NR + iR] + n[1] * v[8 * NR + iR] + n[2] * v[5 * NR + iR];
+      for (int c = 0; c < 3; ++c) {
+        U[c] = u[c * NL + iL];
+        V[c] = v[c * NR + iR];
+      }
+      rot_in(n, m, l, U);
+      rot_in(n, m, l, V);
+      /* Interface_RHS: materials of both sides at the node */
+      const double lam1 = L->M[iL], mu1 = L->M[NL + iL], rho1 = L->M[2 * NL + iL];
+      const double lam2 = R->M[iR], mu2 = R->M[NR + iR], rho2 = R->M[2 * NR + iR];
+      const double c1p = sqrt((2.0 * mu1 + lam1) / rho1), c1s = sqrt(mu1 / rho1);
+      const double c2p = sqrt((2.0 * mu2 + lam2) / rho2), c2s = sqrt(mu2 / rho2);
+      const double Z1s = rho1 * c1s, Z2s = rho2 * c2s, Z1p = rho1 * c1p, Z2p = rho2 * c2p;
+      double P[6], Q[6];
+      hats(I->coupling, U, V, Z1s, Z2s, Z1p, Z2p, I->S[3 * NF + fi], I->yf[fi], I->zf[fi], P,
+           Q);
+      const double slip_rate = sqrt((Q[1] - P[1]) * (Q[1] - P[1]) + (Q[2] - P[2]) * (Q[2] - P[2]));
+      /* the interface state belongs to block L's side: one DS increment per
+       * stage, as when each block keeps its own interface copy */
+      if (ib == 2) {
+        for (int c = 0; c < 3; ++c) I->DS[c * NF + fi] = I->DS[c * NF + fi] + Q[c] - P[c];
+        I->DS[3 * NF + fi] = I->DS[3 * NF + fi] + slip_rate;
+      }
+      if (stage == 1) {
+        double T0n = 0.0, T0m = 0.0, T0l = 0.0;
+        if (I->coupling != ORC_IFACE_LOCKED) prestress_tpv5(I->yf[fi], I->zf[fi], &T0n, &T0m, &T0l);
+        const double T0[3] = {T0n, T0m, T0l};
+        for (int c = 0; c < 3; ++c) {
+          I->Uhat[(3 + c) * NF + fi] = P[3 + c] + T0[c];
+          I->Vhat[(3 + c) * NF + fi] = Q[3 + c] + T0[c];
+          /* CouplingForcing.f90:210-219: slip velocity in local coordinates */
+          I->Svel[c * NF + fi] = Q[c] - P[c];
+          I->Uhat[c * NF + fi] = Q[c] - P[c];
+          I->Vhat[c * NF + fi] = Q[c] - P[c];
+        }
+        if (sqrt(I->Svel[1 * NF + fi] * I->Svel[1 * NF + fi] +
+                 I->Svel[2 * NF + fi] * I->Svel[2 * NF + fi]) > 1e-3)
+          I->trup[fi] = fmin(I->trup[fi], t);
+      }
+      rot_back(n, m, l, U);
+      rot_back(n, m, l, V);
+      rot_back(n, m, l, P);
+      rot_back(n, m, l, Q);
+      /* characteristic SAT forcing (CouplingForcing.f90:228-457); both
+       * blocks use the metric column of the calling block */
+      const double qx = B->metx[iB], qy = B->mety[iB], qz = B->metz[iB];
+      const double norm = sqrt(qx * qx + qy * qy + qz * qz);
+      (void)NB;
+      double Mz[3][3], Pi[3][3], Nz[3][3], Qi[3][3];
+      char_mats(n, m, l, Z1p, Z1s, Mz, Pi);
+      char_mats(n, m, l, Z2p, Z2s, Nz, Qi);
+      double dp[3], dq[3];
+      for (int a = 0; a < 3; ++a) {
+        const double ph = 0.5 * (Mz[a][0] * P[0] + Mz[a][1] * P[1] + Mz[a][2] * P[2] + P[3 + a]);
+        const double pp = 0.5 * (Mz[a][0] * U[0] + Mz[a][1] * U[1] + Mz[a][2] * U[2] + U[3 + a]);
+        const double qh = 0.5 * (Nz[a][0] * Q[0] + Nz[a][1] * Q[1] + Nz[a][2] * Q[2] - Q[3 + a]);
+        const double qm = 0.5 * (Nz[a][0] * V[0] + Nz[a][1] * V[1] + Nz[a][2] * V[2] - V[3 + a]);
+        dp[a] = pp - ph;
+        dq[a] = qm - qh;
+      }
+      double bc[9];
+      if (ib == 2) {
+        double A[3];
+        for (int a = 0; a < 3; ++a) A[a] = Pi[a][0] * dp[0] + Pi[a][1] * dp[1] + Pi[a][2] * dp[2];
+        for (int a = 0; a < 3; ++a) bc[a] = norm / rho1 * dp[a];
+        bc[3] = (2.0 * mu1 + lam1) * qx * A[0] + lam1 * qy * A[1] + lam1 * qz * A[2];
+        bc[4] = (2.0 * mu1 + lam1) * qy * A[1] + lam1 * qz * A[2] + lam1 * qx * A[0];
+        bc[5] = (2.0 * mu1 + lam1) * qz * A[2] + lam1 * qy * A[1] + lam1 * qx * A[0];
+        bc[6] = mu1 * (qy * A[0] + qx * A[1]);
+        bc[7] = mu1 * (qz * A[0] + qx * A[2]);
+        bc[8] = mu1 * (qz * A[1] + qy * A[2]);
+      } else {
+        double Bq[3];
+        for (int a = 0; a < 3; ++a) Bq[a] = Qi[a][0] * dq[0] + Qi[a][1] * dq[1] + Qi[a][2] * dq[2];
+        for (int a = 0; a < 3; ++a) bc[a] = norm / rho2 * dq[a];
+        bc[3] = -(2.0 * mu2 + lam2) * qx * Bq[0] - lam2 * qy * Bq[1] - lam2 * qz * Bq[2];
+        bc[4] = -(2.0 * mu2 + lam2) * qy * Bq[1] - lam2 * qz * Bq[2] - lam2 * qx * Bq[0];
+        bc[5] = -(2.0 * mu2 + lam2) * qz * Bq[2] - lam2 * qy * Bq[1] - lam2 * qx * Bq[0];
+        bc[6] = -mu2 * (qy * Bq[0] + qx * Bq[1]);
+        bc[7] = -mu2 * (qz * Bq[0] + qx * Bq[2]);
+        bc[8] = -mu2 * (qz * Bq[1] + qy * Bq[2]);
+      }
+      /* Impose_Interface_Condition: DF(face) -= tau0/hq * forcing */
+      const double pen = B->tau0 / B->hq;
+      for (int c = 0; c < 9; ++c) B->DF[c * NB + iB] = B->DF[c * NB + iB] - pen * bc[c];
+    }
+}
+
+void orc_iface_enforce(orc_iface *I, orc_block *L, orc_block *R, double t, int stage, int side) {
+  eval_side(I, L, R, side == 2 ? 2 : 1, t, stage);
+}
+
+void orc_iface_scale(orc_iface *I, double A) {
+  const int NF = I->ny * I->nz;
+  for (int k = 0; k < 4 * NF; ++k) I->DS[k] = A * I->DS[k];
+  for (int k = 0; k < NF; ++k) I->DW[k] = A * I->DW[k];
+}
+
+void orc_iface_update(orc_iface *I, double inc) {
+  const int NF = I->ny * I->nz;
+  for (int k = 0; k < 4 * NF; ++k) I->S[k] = I->S[k] + inc * I->DS[k];
+  for (int k = 0; k < NF; ++k) I->W[k] = I->W[k] + inc * I->DW[k];
+}

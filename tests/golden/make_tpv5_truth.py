@@ -1,0 +1,35 @@
+"""Writes tpv5_cart_truth.npz: the reference's archived TPV5 interface
+outputs test_problems/truth/1/test_rup_cart_fric_interface.{S,Svel,Uface,
+Vface,Uhat,Vhat,trup} (float32, 46 frames of a 21 x 21 face; layout and
+checker python/read_binary.py:7-37, writer fault_output.f90:73-123).
+
+The files are data of the reference's own test (test_problems/
+test_rup_cart_fric.in, serial_test1 / mpi_test1 in src/CMakeLists.txt);
+they are stored unchanged, reshaped to (frame, component, z, y).
+
+    python tests/golden/make_tpv5_truth.py /root/reference
+"""
+import os
+import sys
+
+import numpy as np
+
+N, NT = 21, 46
+KEYS = (("S", 4), ("Svel", 3), ("Uface", 9), ("Vface", 9), ("Uhat", 6), ("Vhat", 6), ("trup", 1))
+
+
+def main(ref):
+    base = os.path.join(ref, "test_problems", "truth", "1", "test_rup_cart_fric_interface.")
+    out = {}
+    for key, nc in KEYS:
+        a = np.fromfile(base + key, dtype="<f4")
+        if a.size != NT * nc * N * N:
+            raise SystemExit(f"{key}: {a.size} values, expected {NT * nc * N * N}")
+        out[key] = a.reshape(NT, nc, N, N)
+    dst = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tpv5_cart_truth.npz")
+    np.savez_compressed(dst, **out)
+    print("wrote", dst, os.path.getsize(dst), "bytes")
+
+
+if __name__ == "__main__":
+    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")

@@ -12,6 +12,7 @@ def oracle_block(O, prob, metrics):
     b.metx, b.mety, b.metz, b.J = metrics
     b.M[...] = prob.M
     b.bc = list(prob.bc)
+    b.sat_form = prob.sat_form
     for f in range(6):
         if prob.pml[f]:
             b.set_pml(f, prob.npml[f])
@@ -53,3 +54,24 @@ def small_curvilinear(n=30, nmech=8, npml=8, nsteps=12, seed=5):
     prob = PB.curvilinear_q(n=n, h=50.0 * 256 / (n - 1) / 4, nsteps=nsteps, nmech=nmech,
                             npml=npml, seed=seed, nrec=6, f0=4.0, qs_scale=50.0)
     return prob
+
+
+def oracle_two_block(O, tp):
+    """Oracle blocks L, R and interface of a TwoBlockProblem (metrics from
+    the oracle's own pipeline)."""
+    L_ = oracle_block(O, tp.left, O.metrics(tp.left.X))
+    R_ = oracle_block(O, tp.right, O.metrics(tp.right.X))
+    nz, ny = tp.yf.shape
+    I = O.OracleIface(ny, nz, tp.yf, tp.zf, coupling=tp.coupling)
+    return L_, R_, I
+
+
+TPV5_KEYS = (("S", 4), ("Svel", 3), ("Uface", 9), ("Vface", 9), ("Uhat", 6), ("Vhat", 6),
+             ("trup", 1))
+
+
+def tpv5_truth():
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "tpv5_cart_truth.npz")
+    with np.load(path, allow_pickle=False) as z:
+        return {k: z[k].astype(np.float64) for k, _ in TPV5_KEYS}

@@ -1,0 +1,96 @@
+"""The oracle's two-block interface (wql_oracle_iface.c) against the
+reference's archived TPV5 golden files and against what the interface
+conditions fix.
+
+Golden files: test_problems/truth/1/test_rup_cart_fric_interface.* (stored
+as tests/golden/tpv5_cart_truth.npz), checked by python/read_binary.py:9-37
+with max |difference| <= 1e-3 over 46 frames of S, Svel, Uface, Vface, Uhat,
+Vhat.  The files predate two changes of the current source: they carry the
+alpha/theta SAT form (commented out at BoundaryConditions.f90:832-840 and
+904-912) and one slip-rate increment per stage (each MPI rank owns its copy of
+the interface; on one rank both block evaluations of
+enforce_bound_iface_conditions, domain.f90:615-644, add to the shared DS).
+The oracle reproduces them with sat_form = 1 and one increment per stage.
+"""
+import numpy as np
+import pytest
+
+from tests.helpers import TPV5_KEYS, oracle_two_block, tpv5_truth
+from waveqlab3d_q_gpu_amd import _lib as L
+from waveqlab3d_q_gpu_amd import problems as PB
+
+
+def test_tpv5_cartesian_golden_files(oracle):
+    tp = PB.tpv5_cartesian(21, sat_form=1)
+    assert tp.nsteps == 46
+    Lb, Rb, I = oracle_two_block(oracle, tp)
+    _, fr = oracle.two_block_time_step(Lb, Rb, I, tp.dt, tp.nsteps, 0.0)
+    truth = tpv5_truth()
+    for key, nc in TPV5_KEYS:
+        mine, ref = fr[key], truth[key]
+        assert mine.shape == ref.shape, key
+        d = float(np.abs(mine - ref).max())
+        # the reference's own tolerance (read_binary.py:37) ...
+        assert d <= 1.0e-3, (key, d)
+        # ... and float32 rounding of the stored values (|values| <= 120)
+        scale = max(1.0, float(np.abs(ref[ref < 1e8]).max()))
+        assert d <= 4e-7 * scale + 1e-7 or key == "trup", (key, d)
+    # the rupture front reached the free surface and the nucleation patch
+    assert np.isfinite(fr["trup"]).all() and (fr["trup"][-1] < 1e9).any()
+
+
+def test_tpv5_current_sat_form_differs(oracle):
+    """The golden files pin the alpha/theta form: the current BCm/BCp form
+    differs from them by far more than the tolerance (so the comparison above
+    would catch the wrong form)."""
+    tp = PB.tpv5_cartesian(21, sat_form=0)
+    Lb, Rb, I = oracle_two_block(oracle, tp)
+    _, fr = oracle.two_block_time_step(Lb, Rb, I, tp.dt, 10, 0.0)
+    truth = tpv5_truth()
+    assert float(np.abs(fr["Uface"] - truth["Uface"][:10]).max()) > 1e-2
+
+
+def _uniform_pair(oracle, coupling, jump=0.0, seed=3):
+    """L and R with different materials, fields constant in space: velocity
+    and traction n.sigma (n = x) continuous across the interface; the other
+    stress components differ."""
+    rng = np.random.default_rng(seed)
+    tp = PB.tpv5_cartesian(16, sat_form=0, coupling=coupling)
+    tp.right.M = PB.material_from_rho_s_p(tp.right.n, 2.2, 2.9, 5.1)
+    Lb, Rb, I = oracle_two_block(oracle, tp)
+    v = rng.normal(size=3)
+    sxx, sxy, sxz = rng.normal(size=3)
+    for b, other in ((Lb, rng.normal(size=3)), (Rb, rng.normal(size=3))):
+        vals = [v[0], v[1], v[2], sxx, other[0], other[1], sxy, sxz, other[2]]
+        for c in range(9):
+            b.F[c] = vals[c]
+    Rb.F[1] += jump
+    for b in (Lb, Rb):
+        b.DF[...] = 0.0
+    I.DS[...] = 0.0
+    return Lb, Rb, I
+
+
+def test_locked_sat_vanishes_when_interface_conditions_hold(oracle):
+    Lb, Rb, I = _uniform_pair(oracle, L.WQL_IFACE_LOCKED)
+    I.enforce(Lb, Rb, 0.0, 1, 2)
+    I.enforce(Lb, Rb, 0.0, 1, 1)
+    scale = float(np.abs(Lb.F).max()) * Lb.tau0 / Lb.hq * 10.0
+    assert float(np.abs(Lb.DF).max()) <= 1e-13 * scale
+    assert float(np.abs(Rb.DF).max()) <= 1e-13 * scale
+    # no slip rate either
+    assert float(np.abs(I.DS[:3]).max()) <= 1e-14
+
+
+def test_locked_sat_opposes_a_velocity_jump(oracle):
+    """A tangential velocity jump [v_y] = V_R - V_L > 0: the penalty on the
+    velocity rate pushes v_y of L up and v_y of R down (DF = -tau0/h * F
+    with F of opposite signs), and the hats see slip rate |jump|."""
+    Lb, Rb, I = _uniform_pair(oracle, L.WQL_IFACE_LOCKED, jump=0.5)
+    I.enforce(Lb, Rb, 0.0, 1, 2)
+    I.enforce(Lb, Rb, 0.0, 1, 1)
+    fL = Lb.DF[1][:, :, -1]
+    fR = Rb.DF[1][:, :, 0]
+    assert (fL > 0).all() and (fR < 0).all()
+    # welded hats are continuous: no slip rate even with a jump in the grid values
+    assert float(np.abs(I.DS[:3]).max()) <= 1e-14

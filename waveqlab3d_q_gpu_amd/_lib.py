@@ -16,6 +16,8 @@ WQL_OK, WQL_EINVAL, WQL_EHIP, WQL_ENOMEM, WQL_ESTATE, WQL_ENODEV = range(6)
 WQL_CARTESIAN, WQL_CURVILINEAR = 0, 1
 WQL_FP64, WQL_FP32_MIXED, WQL_FP32_KAHAN = 0, 1, 2
 WQL_SRC_GAUSSIAN, WQL_SRC_COSBELL, WQL_SRC_RICKER = 0, 1, 2
+WQL_SAT_BC, WQL_SAT_ALPHA_THETA = 0, 1
+WQL_IFACE_LOCKED, WQL_IFACE_SLIP_WEAKENING_TPV5 = 0, 1
 WQL_F, WQL_DF, WQL_GEOMETRY, WQL_MATERIAL = 1, 2, 4, 8
 WQL_MEMORY, WQL_QINV, WQL_PML, WQL_FACES, WQL_ALL = 16, 32, 64, 128, 255
 

@@ -44,6 +44,7 @@ class Problem:
     sources: list = field(default_factory=list)
     receivers: np.ndarray = None   # (nrec, 3) 1-based node indices
     name: str = ""
+    sat_form: int = 0              # WQL_SAT_BC / WQL_SAT_ALPHA_THETA
 
 
 # ---------------------------------------------------------------------------
@@ -260,6 +261,48 @@ def kat_cartesian_41():
                        order=4, mxz=-10.0)
     return Problem(n=n, mesh=L.WQL_CARTESIAN, X=X, M=M, bc=(1,) * 6, dt=dt, nsteps=nsteps,
                    sources=[src], name="kat41")
+
+
+@dataclass
+class TwoBlockProblem:
+    """Two blocks coupled across the x-high face of ``left`` and the x-low
+    face of ``right`` (nblocks = 2, interface direction 'q'), with the
+    interface coupling and the face coordinates (y, z) of every interface
+    node as (nz, ny) arrays."""
+    left: Problem
+    right: Problem
+    coupling: int
+    yf: np.ndarray
+    zf: np.ndarray
+    dt: float
+    nsteps: int
+    name: str = ""
+
+
+def tpv5_cartesian(n=21, sat_form=1, coupling=None):
+    """test_problems/test_rup_cart_fric.in: TPV5 on two n^3 Cartesian blocks,
+    block 1 [-3,0] x [6,9] x [-1.5,1.5], block 2 [0,3] x [6,9] x [-1.5,1.5]
+    (km), rho_s_p = (2.67, 3.464, 6), lqrs/rqrs = (1,2,1)/(0,1,1) and
+    (0,2,1)/(1,1,1), slip-weakening friction, CFL 0.5, t_final 0.5 (46 steps
+    at n = 21).  sat_form 1 selects the SAT form of the archived golden files
+    (test_problems/truth/1)."""
+    if coupling is None:
+        coupling = L.WQL_IFACE_SLIP_WEAKENING_TPV5
+    nn = (n, n, n)
+    XL = cartesian_grid(nn, (-3.0, 6.0, -1.5), (0.0, 9.0, 1.5))
+    XR = cartesian_grid(nn, (0.0, 6.0, -1.5), (3.0, 9.0, 1.5))
+    M = material_from_rho_s_p(nn, 2.67, 3.464, 6.0)
+    spat = 3.0 / float(n - 1)
+    dt = block_time_step(spat, 0.5, 3.464, 6.0)
+    nsteps = int(math.floor(0.5 / dt))
+    left = Problem(n=nn, mesh=L.WQL_CARTESIAN, X=XL, M=M.copy(), bc=(1, 0, 2, 1, 1, 1), dt=dt,
+                   nsteps=nsteps, name="tpv5_block1", sat_form=sat_form)
+    right = Problem(n=nn, mesh=L.WQL_CARTESIAN, X=XR, M=M.copy(), bc=(0, 1, 2, 1, 1, 1), dt=dt,
+                    nsteps=nsteps, name="tpv5_block2", sat_form=sat_form)
+    return TwoBlockProblem(left=left, right=right, coupling=coupling,
+                           yf=np.ascontiguousarray(XL[1][:, :, -1]),
+                           zf=np.ascontiguousarray(XL[2][:, :, -1]), dt=dt, nsteps=nsteps,
+                           name=f"tpv5_cartesian_{n}")
 
 
 def cartesian_bump(n=65, h=100.0, nsteps=200, seed=1234, cfl=0.5, nrec=8):
