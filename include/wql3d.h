@@ -50,6 +50,14 @@ enum { WQL_FP64 = 0, WQL_FP32_MIXED = 1, WQL_FP32_KAHAN = 2 };
 enum { WQL_BC_NONE = 0, WQL_BC_CHARACTERISTIC = 1,     /* BoundaryConditions.f90:28-30 */
        WQL_BC_FREE_SURFACE = 2, WQL_BC_CLAMPED = 3 };
 enum { WQL_SRC_GAUSSIAN = 0, WQL_SRC_COSBELL = 1, WQL_SRC_RICKER = 2 };
+/* SAT penalty form of the boundary faces: BCm/BCp followed by structure()
+ * (BoundaryConditions.f90:783-924, RHS_Interior.f90:1584-1768), or the
+ * alpha/theta form kept in comments at BoundaryConditions.f90:832-840 /
+ * 904-912 (velocity penalty alpha*norm/rho*F, alpha = 2, 1, 0 for codes 1, 2,
+ * 3; stress penalty theta*norm*(...), theta = 0, 0, -1; no structure), the
+ * form that produced the archived golden files test_problems/truth/1.  The
+ * alpha/theta form takes no PML faces. */
+enum { WQL_SAT_BC = 0, WQL_SAT_ALPHA_THETA = 1 };
 
 /* upload/download masks */
 enum {
@@ -93,7 +101,9 @@ typedef struct wql_block_desc {
   wql_box pml_box[6];                         /* bounds of PMLB(f)%Q / %DQ */
   double *pml_Q[6], *pml_DQ[6];               /* pml_box x 9 */
   wql_box face_box[6];                        /* bounds of B(f)%n_l/n_m/n_n (2-D) */
-  const double *n_l[6], *n_m[6], *n_n[6];     /* face_box x 3 */
+  const double *n_l[6], *n_m[6], *n_n[6];     /* face_box x 3; needed on faces with
+                                                 bc > 0 and on interface faces */
+  int32_t sat_form;                           /* WQL_SAT_BC / WQL_SAT_ALPHA_THETA */
 } wql_block_desc;
 
 /* moment tensor point source (moment_tensor.f90: moment_tensor type) */
@@ -196,6 +206,51 @@ void *wql_block_stream(wql_block *b);
 int wql_block_set_timing(wql_block *b, int32_t enable);
 int wql_block_kernel_times(wql_block *b, double ms[3], int32_t *stages);
 
+/* ---- two-block interface (nblocks = 2, interface direction 'q') ----------
+ * The x-high face (Rx, bc 0) of block L is coupled to the x-low face (Lx,
+ * bc 0) of block R; both blocks have the same ny, nz and live on this GPU.
+ * Per stage, as enforce_bound_iface_conditions (domain.f90:615-644) with
+ * Impose_Interface_Condition (RHS_Interior.f90:1771-1843), Couple_Interface_x
+ * (CouplingForcing.f90:9-463) and Interface_RHS (Interface_Condition.f90:
+ * 12-513): hat variables from both faces (welded, or TPV5 linear slip-weakening
+ * friction with Tau_strength / prestress TPV5, Interface_Condition.f90:
+ * 1283-1299 / 1802-1830), the characteristic SAT forcing of block L (its side
+ * 2) and of block R (its side 1), the slip rate DS(1:3) = Q - P and
+ * DS(4) = |slip rate| (one increment per stage, as each MPI rank holds its own
+ * copy of the interface), S += Bdt*DS (scale_rates_iface / update_fields_iface,
+ * iface.f90:58-85).  The stage-1 fault output (write_fault_output /
+ * write_hat_output, domain.f90:782-847) is recorded into device slots.
+ * FP64 and FP32_MIXED blocks (the RK state is FP64 in both). */
+enum { WQL_IFACE_LOCKED = 0, WQL_IFACE_SLIP_WEAKENING_TPV5 = 1 };
+typedef struct wql_iface_desc {
+  int32_t coupling;          /* WQL_IFACE_*: coupling 'locked' / 'slip-weakening_friction'
+                                with problem 'TPV5' */
+  int32_t arrays_on_device;  /* 0: y, z are host pointers, 1: device pointers */
+  const double *y, *z;       /* coordinates of the interface nodes, [nz][ny] (y fastest):
+                                the y and z of block L's x-high face (G%X(pq, :, :, 2:3)) */
+} wql_iface_desc;
+typedef struct wql_iface wql_iface;
+/* init_iface (iface.f90:13-55): S = 0, DS = 1e40, rupture time 1e9 */
+int wql_iface_create(wql_block *L, wql_block *R, const wql_iface_desc *desc, wql_iface **out);
+void wql_iface_destroy(wql_iface *I);
+/* one RK stage of both blocks and the interface; stage1 != 0 records the fault
+ * output of time t into the next slot (if slots are set) */
+int wql_iface_stage(wql_iface *I, double A, double Bdt, double t, int32_t stage1);
+/* time_step_RK (time_step.f90:68-150) of the two-block domain: nsteps steps of
+ * 5 stages, *t advanced as D%t; receivers of both blocks and the fault output
+ * recorded at stage 1 */
+int wql_iface_time_step(wql_iface *I, double dt, int32_t nsteps, double *t);
+/* device slots for nslots steps of fault output; each slot holds, per node
+ * of the face ([nz][ny]), 38 planes: Uface (block L face fields, 9), Vface
+ * (block R face fields, 9), S (4), Svel (3), Uhat (6), Vhat (6), rupture time (1),
+ * in the order and meaning of the reference's .Uface/.Vface/.S/.Svel/.Uhat/.Vhat/
+ * .trup files (fault_output.f90:73-123) */
+int wql_iface_set_frames(wql_iface *I, int32_t nslots);
+/* copy recorded slots 0..nslots-1 ([slot][38][nz][ny], host) */
+int wql_iface_get_frames(wql_iface *I, int32_t nslots, double *out);
+/* interface state S[4][nz][ny], DS[4][nz][ny], rupture time [nz][ny] (host) */
+int wql_iface_get_state(wql_iface *I, double *S, double *DS, double *trup);
+
 /* ---- discrete adjoint of the fused stage --------------------------------
  * No reference counterpart (the reference has no adjoint); these serve the
  * gradient of a receiver misfit with respect to per-node rho, lambda, mu.
@@ -204,7 +259,8 @@ int wql_block_kernel_times(wql_block *b, double ms[3], int32_t *stages);
  * the block's current state (restore it with wql_state_load):
  *   rb += Bdt ub;  g += (dL/dm)^T rb;  ub += K^T rb;  rb = A rb
  * with ub, rb the adjoints of u, r, K = dL/du and g the gradient planes.
- * PML slabs are not differentiated (wql_adjoint_create fails with PML).
+ * PML slabs, interfaces and the alpha/theta SAT form are not differentiated
+ * (wql_adjoint_create fails with any of them).
  * Dense arrays below are DEVICE pointers, node order x fastest, no ghosts. */
 /* adjoint state, rate adjoints, gradients (zeroed) and nslots state slots */
 int wql_adjoint_create(wql_block *b, int32_t nslots);

@@ -8,6 +8,8 @@ from . import _lib
 from ._lib import WqlError
 from .block import (Block, SourceSpec, init_RK, time_step_RK, scale_rate_array,
                     update_state_array, grid_metrics, RK_A, RK_B, RK_C, TAU0)
+from .interface import Interface
 
 __all__ = ["Block", "SourceSpec", "init_RK", "time_step_RK", "scale_rate_array",
-           "update_state_array", "grid_metrics", "WqlError", "RK_A", "RK_B", "RK_C", "TAU0"]
+           "update_state_array", "grid_metrics", "WqlError", "RK_A", "RK_B", "RK_C", "TAU0",
+           "Interface"]

@@ -43,7 +43,13 @@ class BlockDesc(C.Structure):
         ("Qs_inv", _D), ("Qp_inv", _D),
         ("pml_box", Box * 6), ("pml_Q", _D * 6), ("pml_DQ", _D * 6),
         ("face_box", Box * 6), ("n_l", _D * 6), ("n_m", _D * 6), ("n_n", _D * 6),
+        ("sat_form", C.c_int32),
     ]
+
+
+class IfaceDesc(C.Structure):
+    _fields_ = [("coupling", C.c_int32), ("arrays_on_device", C.c_int32),
+                ("y", _D), ("z", _D)]
 
 
 class Source(C.Structure):
@@ -95,6 +101,13 @@ SIGNATURES = {
     "wql_adjoint_get_state": (C.c_int, [_P, _P, _P]),
     "wql_adjoint_set_state": (C.c_int, [_P, _P, _P]),
     "wql_adjoint_gradient": (C.c_int, [_P, _P, _P, _P]),
+    "wql_iface_create": (C.c_int, [_P, _P, C.POINTER(IfaceDesc), C.POINTER(_P)]),
+    "wql_iface_destroy": (None, [_P]),
+    "wql_iface_stage": (C.c_int, [_P, C.c_double, C.c_double, C.c_double, C.c_int32]),
+    "wql_iface_time_step": (C.c_int, [_P, C.c_double, C.c_int32, _D]),
+    "wql_iface_set_frames": (C.c_int, [_P, C.c_int32]),
+    "wql_iface_get_frames": (C.c_int, [_P, C.c_int32, _D]),
+    "wql_iface_get_state": (C.c_int, [_P, _D, _D, _D]),
 }
 
 _lib = None

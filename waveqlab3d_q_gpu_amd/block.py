@@ -131,6 +131,7 @@ class Block:
     n_m: list = field(default_factory=lambda: [None] * 6)
     n_n: list = field(default_factory=lambda: [None] * 6)
     tau0: float = TAU0
+    sat_form: int = L.WQL_SAT_BC
     on_device: bool = False
     _h: object = None
 
@@ -145,6 +146,7 @@ class Block:
         d.arrays_on_device = 1 if self.on_device else 0
         d.hq, d.hr, d.hs = (1.0 / (self.nq - 1), 1.0 / (self.nr - 1), 1.0 / (self.ns - 1))
         d.tau0 = self.tau0
+        d.sat_form = int(self.sat_form)
         d.tau[:] = list(self.tau) + [0.0] * (8 - len(self.tau))
         d.weight[:] = list(self.weight) + [0.0] * (8 - len(self.weight))
         d.body = _box((1, 1, 1), n)

@@ -12,7 +12,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libwql3d_hip.so")
-SOURCES = ["wql3d_kernels.hip", "wql3d_adjoint.hip", "wql3d_api.hip"]
+SOURCES = ["wql3d_kernels.hip", "wql3d_adjoint.hip", "wql3d_iface.hip", "wql3d_api.hip"]
 
 # -packed-fp32-ops: keep the compiler from emitting packed FP32 instructions.
 FLAGS = [

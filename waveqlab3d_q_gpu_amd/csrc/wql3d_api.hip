@@ -109,6 +109,26 @@ struct wql_block {
   double* slot_F;
   void* slot_E;
   int* slot_mz;
+  // two-block interface (wql_iface_create): the interface the block is
+  // coupled through and the SAT forcing planes of its coupled face
+  wql_iface* iface;
+  double* iface_sat[6];
+};
+
+// Two-block q-interface (wql_iface_create): block L's x-high face against
+// block R's x-low face, ny x nz nodes, node index y + ny*z.
+struct wql_iface {
+  wql_block* L;
+  wql_block* R;
+  int ny, nz;
+  long long NF;
+  int coupling;
+  double* buf;      // one allocation of 29 planes: S 4, DS 4, trup, y, z, satL 9, satR 9
+  double *S, *DS, *trup, *yf, *zf, *satL, *satR;
+  double* frames;   // nslots * kFramePlanes planes of fault output
+  int nslots, slot;
+  hipEvent_t ev_iface;  // on L's stream after k_iface: block R's stage waits for it
+  hipEvent_t ev_right;  // on R's stream after block R's stage: the next k_iface waits
 };
 
 static int g_device = -1;
@@ -185,6 +205,11 @@ static int validate(const wql_block_desc* d) {
     for (int i = 0; i < d->nmech; ++i)
       if (!(d->tau[i] > 0.0)) return fail(WQL_EINVAL, "tau[%d] must be positive", i);
   if (d->hq <= 0 || d->hr <= 0 || d->hs <= 0) return fail(WQL_EINVAL, "h must be positive");
+  if (d->sat_form != WQL_SAT_BC && d->sat_form != WQL_SAT_ALPHA_THETA)
+    return fail(WQL_EINVAL, "invalid sat_form %d", d->sat_form);
+  if (d->sat_form == WQL_SAT_ALPHA_THETA)
+    for (int f = 0; f < 6; ++f)
+      if (d->pml[f]) return fail(WQL_EINVAL, "the alpha/theta SAT form takes no PML faces (face %d)", f);
   return WQL_OK;
 }
 
@@ -200,6 +225,10 @@ static int dalloc(wql_block* b, void** p, size_t bytes) {
 extern "C" void wql_block_destroy(wql_block* b) {
   if (!b) return;
   (void)hipStreamSynchronize(b->stream);
+  if (b->iface) {  // the interface outlives the block: it refuses further stages
+    if (b->iface->L == b) b->iface->L = nullptr;
+    if (b->iface->R == b) b->iface->R = nullptr;
+  }
   void* ptrs[] = {b->arena, b->rec_idx, b->seis, b->adj_Fb, b->adj_Eb, b->adj_DFb, b->adj_DEb,
                   b->adj_g, b->adj_Z, b->slot_F, b->slot_E};
   for (void* p : ptrs)
@@ -300,9 +329,10 @@ extern "C" int wql_block_create(const wql_block_desc* d, wql_block** out) {
       ALLOC(b->pQ[f], 9 * b->Ps[f] * b->esz);
       ALLOC(b->pDQ[f], 9 * b->Ps[f] * b->esz);
     }
-    if (d->bc[f] > 0) {
+    // face frames on boundary faces, and on bc 0 faces given frames (interface faces)
+    if (d->bc[f] > 0 || (d->bc[f] == 0 && d->n_l[f] && d->n_m[f] && d->n_n[f])) {
       b->NF[f] = face_size(b, f);
-      ALLOC(b->face[f], 10 * b->NF[f] * b->esz);
+      ALLOC(b->face[f], 13 * b->NF[f] * b->esz);
     }
   }
 #undef ALLOC
@@ -409,8 +439,9 @@ static int get_comp(wql_block* b, const T* src, const View& v, int c, const int 
   return WQL_OK;
 }
 
-// face frame + |grad xi| planes of face f (unit_normals.f90:8-168 results,
-// norm as in BoundaryConditions.f90:114-118)
+// face frame + |grad xi| + metric column planes of face f (unit_normals.f90:
+// 8-168 results, norm as in BoundaryConditions.f90:114-118, the metric column
+// as read by Couple_Interface_x, CouplingForcing.f90:228-260)
 template <typename T>
 static int put_face(wql_block* b, int f) {
   const wql_block_desc& d = b->d;
@@ -466,11 +497,14 @@ static int put_face(wql_block* b, int f) {
                 mets[m][off0 + x * body0.sx + y * body0.sy + z * body0.sz];
     }
   }
-  std::vector<T> out(10 * NF);
+  std::vector<T> out(13 * NF);
   for (long long k = 0; k < 9 * NF; ++k) out[k] = (T)frame[k];
   for (long long k = 0; k < NF; ++k) {
     const double qx = met[k], qy = met[NF + k], qz = met[2 * NF + k];
     out[9 * NF + k] = (T)sqrt(qx * qx + qy * qy + qz * qz);
+    out[10 * NF + k] = (T)qx;
+    out[11 * NF + k] = (T)qy;
+    out[12 * NF + k] = (T)qz;
   }
   HIP_TRY(hipMemcpy(b->face[f], out.data(), out.size() * sizeof(T), hipMemcpyHostToDevice));
   return WQL_OK;
@@ -556,7 +590,7 @@ static int transfer(wql_block* b, uint32_t mask, bool up) {
   }
   if (up && (mask & WQL_FACES))
     for (int f = 0; f < 6 && !rc; ++f)
-      if (d.bc[f] > 0) rc = put_face<T>(b, f);
+      if (b->face[f]) rc = put_face<T>(b, f);
   return rc;
 }
 
@@ -703,7 +737,9 @@ static StepArgs<T, S> make_args(wql_block* b, double t) {
     a.pDQ[f] = (T*)b->pDQ[f];
     a.bc[f] = d.bc[f] > 0 ? d.bc[f] : 0;
     a.face[f] = (const T*)b->face[f];
+    a.iface[f] = d.bc[f] == 0 ? b->iface_sat[f] : nullptr;
   }
+  a.sat_form = d.sat_form;
   a.nsrc = b->nsrc;
   for (int s = 0; s < b->nsrc; ++s) {
     const wql_source& Sr = b->src[s];
@@ -931,6 +967,206 @@ extern "C" int wql_time_step(wql_block* b, double dt, int32_t nsteps, double* t)
   return WQL_OK;
 }
 
+// ---------------------------------------------------------------------------
+// two-block interface
+// ---------------------------------------------------------------------------
+static int iface_ok(wql_iface* I) {
+  if (!I) return fail(WQL_EINVAL, "null interface");
+  if (!I->L || !I->R) return fail(WQL_ESTATE, "a block of the interface was destroyed");
+  return WQL_OK;
+}
+
+extern "C" int wql_iface_create(wql_block* L, wql_block* R, const wql_iface_desc* d,
+                                wql_iface** out) {
+  if (!out) return fail(WQL_EINVAL, "null output pointer");
+  *out = nullptr;
+  if (!L || !R || !d || L == R) return fail(WQL_EINVAL, "need two distinct blocks and a descriptor");
+  if (d->coupling != WQL_IFACE_LOCKED && d->coupling != WQL_IFACE_SLIP_WEAKENING_TPV5)
+    return fail(WQL_EINVAL, "invalid coupling %d", d->coupling);
+  if (!d->y || !d->z) return fail(WQL_EINVAL, "face coordinates y, z are null");
+  if (L->iface || R->iface) return fail(WQL_ESTATE, "a block is already coupled to an interface");
+  if (L->d.precision != R->d.precision) return fail(WQL_EINVAL, "blocks differ in precision");
+  if (L->d.precision == WQL_FP32_KAHAN)
+    return fail(WQL_EINVAL, "the interface needs an FP64 RK state (FP64 or FP32_MIXED blocks)");
+  if (L->d.bc[1] != 0 || R->d.bc[0] != 0)
+    return fail(WQL_EINVAL, "the coupled faces (Rx of L, Lx of R) need bc 0 (got %d, %d)",
+                L->d.bc[1], R->d.bc[0]);
+  if (L->d.pml[1] || R->d.pml[0]) return fail(WQL_EINVAL, "the coupled faces cannot carry PML");
+  if (L->ny != R->ny || L->nz != R->nz)
+    return fail(WQL_EINVAL, "coupled faces differ: %dx%d vs %dx%d nodes", L->ny, L->nz, R->ny, R->nz);
+  if (!L->face[1] || !R->face[0])
+    return fail(WQL_EINVAL, "the coupled faces need n_l/n_m/n_n in the block descriptors");
+  wql_iface* I = (wql_iface*)calloc(1, sizeof(wql_iface));
+  if (!I) return fail(WQL_ENOMEM, "host allocation failed");
+  I->L = L; I->R = R;
+  I->ny = L->ny; I->nz = L->nz;
+  I->NF = (long long)I->ny * I->nz;
+  I->coupling = d->coupling;
+  const long long NF = I->NF;
+  hipError_t e = hipMalloc((void**)&I->buf, 29 * NF * sizeof(double));
+  if (e != hipSuccess) {
+    free(I);
+    return fail(WQL_ENOMEM, "hipMalloc(interface) failed: %s", hipGetErrorString(e));
+  }
+  I->S = I->buf; I->DS = I->S + 4 * NF; I->trup = I->DS + 4 * NF; I->yf = I->trup + NF;
+  I->zf = I->yf + NF; I->satL = I->zf + NF; I->satR = I->satL + 9 * NF;
+  auto cleanup = [&](int rc) {
+    (void)hipFree(I->buf);
+    if (I->ev_iface) (void)hipEventDestroy(I->ev_iface);
+    if (I->ev_right) (void)hipEventDestroy(I->ev_right);
+    free(I);
+    return rc;
+  };
+  const hipMemcpyKind kind = d->arrays_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  if ((e = hipMemcpy(I->yf, d->y, NF * sizeof(double), kind)) != hipSuccess ||
+      (e = hipMemcpy(I->zf, d->z, NF * sizeof(double), kind)) != hipSuccess)
+    return cleanup(fail(WQL_EHIP, "face coordinates: %s", hipGetErrorString(e)));
+  // init_iface: S = 0, DS = 1e40, trup = 1e9; the SAT planes start at zero
+  if ((e = launch_fill<double>(I->S, 4 * NF, 0.0, L->stream)) != hipSuccess ||
+      (e = launch_fill<double>(I->DS, 4 * NF, 1.0e40, L->stream)) != hipSuccess ||
+      (e = launch_fill<double>(I->trup, NF, 1.0e9, L->stream)) != hipSuccess ||
+      (e = launch_fill<double>(I->satL, 18 * NF, 0.0, L->stream)) != hipSuccess ||
+      (e = hipStreamSynchronize(L->stream)) != hipSuccess)
+    return cleanup(fail(WQL_EHIP, "interface init: %s", hipGetErrorString(e)));
+  if ((e = hipEventCreateWithFlags(&I->ev_iface, hipEventDisableTiming)) != hipSuccess ||
+      (e = hipEventCreateWithFlags(&I->ev_right, hipEventDisableTiming)) != hipSuccess ||
+      (e = hipEventRecord(I->ev_right, R->stream)) != hipSuccess)
+    return cleanup(fail(WQL_EHIP, "interface events: %s", hipGetErrorString(e)));
+  L->iface = I; R->iface = I;
+  L->iface_sat[1] = I->satL;
+  R->iface_sat[0] = I->satR;
+  *out = I;
+  return WQL_OK;
+}
+
+extern "C" void wql_iface_destroy(wql_iface* I) {
+  if (!I) return;
+  if (I->L) {
+    (void)hipStreamSynchronize(I->L->stream);
+    I->L->iface = nullptr;
+    I->L->iface_sat[1] = nullptr;
+  }
+  if (I->R) {
+    (void)hipStreamSynchronize(I->R->stream);
+    I->R->iface = nullptr;
+    I->R->iface_sat[0] = nullptr;
+  }
+  if (I->buf) (void)hipFree(I->buf);
+  if (I->frames) (void)hipFree(I->frames);
+  if (I->ev_iface) (void)hipEventDestroy(I->ev_iface);
+  if (I->ev_right) (void)hipEventDestroy(I->ev_right);
+  free(I);
+}
+
+// one stage of the two-block domain: k_iface on L's stream (after R's
+// previous stage), block L's fused stage behind it on L's stream, block R's on
+// R's stream after k_iface
+template <typename T>
+static int iface_stage_t(wql_iface* I, double A, double Bdt, double t, int stage1) {
+  wql_block* L = I->L;
+  wql_block* R = I->R;
+  HIP_TRY(hipStreamWaitEvent(L->stream, I->ev_right, 0));
+  IfaceArgs<T> a;
+  memset(&a, 0, sizeof a);
+  a.ny = I->ny; a.nz = I->nz;
+  a.coupling = I->coupling;
+  a.nxL = L->nx; a.nxR = R->nx;
+  a.FL = (const double*)L->F[L->cur];
+  a.FR = (const double*)R->F[R->cur];
+  a.PL = L->P; a.PR = R->P;
+  a.ML = (const T*)L->M; a.MR = (const T*)R->M;
+  a.frL = (const T*)L->face[1]; a.frR = (const T*)R->face[0];
+  a.penL = L->d.tau0 / L->d.hq;
+  a.penR = R->d.tau0 / R->d.hq;
+  a.S = I->S; a.DS = I->DS; a.trup = I->trup; a.yf = I->yf; a.zf = I->zf;
+  a.satL = I->satL; a.satR = I->satR;
+  a.A = A; a.Bdt = Bdt; a.t = t;
+  a.stage1 = stage1 ? 1 : 0;
+  a.frame = (stage1 && I->frames && I->slot < I->nslots)
+                ? I->frames + (size_t)I->slot * kFramePlanes * I->NF : nullptr;
+  HIP_TRY(launch_iface<T>(a, L->stream));
+  if (a.frame) ++I->slot;
+  HIP_TRY(hipEventRecord(I->ev_iface, L->stream));
+  int rc = stage_fused_t<T>(L, A, Bdt, t);
+  if (rc) return rc;
+  HIP_TRY(hipStreamWaitEvent(R->stream, I->ev_iface, 0));
+  rc = stage_fused_t<T>(R, A, Bdt, t);
+  if (rc) return rc;
+  HIP_TRY(hipEventRecord(I->ev_right, R->stream));
+  L->cur ^= 1;
+  R->cur ^= 1;
+  return WQL_OK;
+}
+
+extern "C" int wql_iface_stage(wql_iface* I, double A, double Bdt, double t, int32_t stage1) {
+  if (int rc = iface_ok(I)) return rc;
+  return I->L->d.precision == WQL_FP64 ? iface_stage_t<double>(I, A, Bdt, t, stage1)
+                                       : iface_stage_t<float>(I, A, Bdt, t, stage1);
+}
+
+extern "C" int wql_iface_time_step(wql_iface* I, double dt, int32_t nsteps, double* t) {
+  if (int rc = iface_ok(I)) return rc;
+  if (!t) return fail(WQL_EINVAL, "null time");
+  for (int n = 0; n < nsteps; ++n) {
+    const double t0 = *t;
+    for (int s = 0; s < 5; ++s) {
+      if (s == 0) {
+        int rc = record(I->L);
+        if (!rc) rc = record(I->R);
+        if (rc) return rc;
+      }
+      int rc = wql_iface_stage(I, kRkA[s], kRkB[s] * dt, *t, s == 0);
+      if (rc) return rc;
+      *t = t0 + kRkC[s + 1] * dt;
+    }
+  }
+  return WQL_OK;
+}
+
+static int iface_sync(wql_iface* I) {
+  HIP_TRY(hipStreamSynchronize(I->L->stream));
+  HIP_TRY(hipStreamSynchronize(I->R->stream));
+  return WQL_OK;
+}
+
+extern "C" int wql_iface_set_frames(wql_iface* I, int32_t nslots) {
+  if (int rc = iface_ok(I)) return rc;
+  if (nslots < 0) return fail(WQL_EINVAL, "nslots must be >= 0");
+  if (int rc = iface_sync(I)) return rc;
+  if (I->frames) (void)hipFree(I->frames);
+  I->frames = nullptr;
+  I->nslots = 0;
+  I->slot = 0;
+  if (nslots == 0) return WQL_OK;
+  const size_t bytes = (size_t)nslots * kFramePlanes * I->NF * sizeof(double);
+  hipError_t e = hipMalloc((void**)&I->frames, bytes);
+  if (e != hipSuccess) return fail(WQL_ENOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+  HIP_TRY(launch_fill<double>(I->frames, (long long)nslots * kFramePlanes * I->NF, 0.0, I->L->stream));
+  HIP_TRY(hipStreamSynchronize(I->L->stream));
+  I->nslots = nslots;
+  return WQL_OK;
+}
+
+extern "C" int wql_iface_get_frames(wql_iface* I, int32_t nslots, double* out) {
+  if (int rc = iface_ok(I)) return rc;
+  if (!out || nslots < 0 || nslots > I->nslots) return fail(WQL_EINVAL, "bad slot count or buffer");
+  if (int rc = iface_sync(I)) return rc;
+  if (nslots)
+    HIP_TRY(hipMemcpy(out, I->frames, (size_t)nslots * kFramePlanes * I->NF * sizeof(double),
+                      hipMemcpyDeviceToHost));
+  return WQL_OK;
+}
+
+extern "C" int wql_iface_get_state(wql_iface* I, double* S, double* DS, double* trup) {
+  if (int rc = iface_ok(I)) return rc;
+  if (int rc = iface_sync(I)) return rc;
+  const size_t nb = (size_t)I->NF * sizeof(double);
+  if (S) HIP_TRY(hipMemcpy(S, I->S, 4 * nb, hipMemcpyDeviceToHost));
+  if (DS) HIP_TRY(hipMemcpy(DS, I->DS, 4 * nb, hipMemcpyDeviceToHost));
+  if (trup) HIP_TRY(hipMemcpy(trup, I->trup, nb, hipMemcpyDeviceToHost));
+  return WQL_OK;
+}
+
 extern "C" int wql_grid_metrics(int32_t nq, int32_t nr, int32_t ns, const double* X, double* mx,
                                 double* my, double* mz, double* J) {
   if (nq < 12 || nr < 12 || ns < 12) return fail(WQL_EINVAL, "SBP-6 needs >= 12 nodes per axis");
@@ -961,6 +1197,9 @@ extern "C" int wql_adjoint_create(wql_block* b, int32_t nslots) {
   if (int rc = no_kahan(b, "the adjoint")) return rc;
   for (int f = 0; f < 6; ++f)
     if (b->d.pml[f]) return fail(WQL_EINVAL, "the adjoint does not differentiate PML slabs (face %d)", f);
+  if (b->iface) return fail(WQL_EINVAL, "the adjoint does not differentiate interfaces");
+  if (b->d.sat_form != WQL_SAT_BC)
+    return fail(WQL_EINVAL, "the adjoint differentiates the BCm/BCp SAT form only");
   HIP_TRY(hipStreamSynchronize(b->stream));
   adjoint_free(b);
   const long long P = b->P, ne = 6LL * b->d.nmech * P;

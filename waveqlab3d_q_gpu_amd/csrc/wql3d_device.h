@@ -315,6 +315,73 @@ __device__ __forceinline__ void sat_structure(const S ubc[9], S lam, S mu, S rho
   Ux[8] = mu * ubc[8];
 }
 
+// The alpha/theta form of the boundary penalty (commented out at
+// BoundaryConditions.f90:832-840 and 904-912; alpha, theta per code as set in
+// BC_Lx..BC_Rz, BoundaryConditions.f90:129-160): the rate vector itself,
+// velocity alpha*norm/rho*F, stresses theta*norm*(...), no structure().
+template <typename S, typename R>
+__device__ __forceinline__ void sat_alpha_theta_frame(const R fr[10], int code, int f,
+                                                      const S u[9], S lam, S mu, S rho,
+                                                      S Ux[9]) {
+  const R l0 = fr[0], l1 = fr[1], l2 = fr[2];
+  const R m0 = fr[3], m1 = fr[4], m2 = fr[5];
+  const R n0 = fr[6], n1 = fr[7], n2 = fr[8];
+  const R norm = fr[9];
+  const S r = code == 1 ? S(0) : (code == 2 ? S(1) : S(-1));
+  const S alpha = code == 1 ? S(2) : (code == 2 ? S(1) : S(0));
+  const S theta = code == 3 ? S(-1) : S(0);
+  const S cp = sqrt((S(2) * mu + lam) / rho), cs = sqrt(mu / rho);
+  const S Zp = rho * cp, Zs = rho * cs;
+  const S T0 = n0 * u[3] + n1 * u[6] + n2 * u[7];
+  const S T1 = n0 * u[6] + n1 * u[4] + n2 * u[8];
+  const S T2 = n0 * u[7] + n1 * u[8] + n2 * u[5];
+  const S vl = l0 * u[0] + l1 * u[1] + l2 * u[2];
+  const S vm = m0 * u[0] + m1 * u[1] + m2 * u[2];
+  const S vn = n0 * u[0] + n1 * u[1] + n2 * u[2];
+  const S Tl = l0 * T0 + l1 * T1 + l2 * T2;
+  const S Tm = m0 * T0 + m1 * T1 + m2 * T2;
+  const S Tn = n0 * T0 + n1 * T1 + n2 * T2;
+  const S sg = (f & 1) ? S(1) : S(-1);
+  const S a1 = S(0.5) * (S(1) - r), a2 = S(0.5) * (S(1) + r) * sg;
+  const S pl = a1 * Zs * vl + a2 * Tl;
+  const S pm = a1 * Zs * vm + a2 * Tm;
+  const S pn = a1 * Zp * vn + a2 * Tn;
+  const S ql = a1 * vl + a2 / Zs * Tl;
+  const S qm = a1 * vm + a2 / Zs * Tm;
+  const S qn = a1 * vn + a2 / Zp * Tn;
+  const S Fx = l0 * pl + m0 * pm + n0 * pn;
+  const S Fy = l1 * pl + m1 * pm + n1 * pn;
+  const S Fz = l2 * pl + m2 * pm + n2 * pn;
+  const S Gx = l0 * ql + m0 * qm + n0 * qn;
+  const S Gy = l1 * ql + m1 * qm + n1 * qn;
+  const S Gz = l2 * ql + m2 * qm + n2 * qn;
+  const S l2m = lam + S(2) * mu;
+  Ux[0] = alpha * norm / rho * Fx;
+  Ux[1] = alpha * norm / rho * Fy;
+  Ux[2] = alpha * norm / rho * Fz;
+  Ux[3] = theta * norm * (l2m * n0 * Gx + lam * (n1 * Gy + n2 * Gz));
+  Ux[4] = theta * norm * (l2m * n1 * Gy + lam * (n0 * Gx + n2 * Gz));
+  Ux[5] = theta * norm * (l2m * n2 * Gz + lam * (n0 * Gx + n1 * Gy));
+  Ux[6] = theta * mu * norm * (n1 * Gx + n0 * Gy);
+  Ux[7] = theta * mu * norm * (n2 * Gx + n0 * Gz);
+  Ux[8] = theta * mu * norm * (n2 * Gy + n1 * Gz);
+}
+
+// rate vector of the boundary penalty at a face node in the block's SAT form
+// (DF -= pen * Ux); ubc (the PML DQ penalty) is filled for the BCm/BCp form
+template <typename T, typename S>
+__device__ __forceinline__ void sat_rate(const StepArgs<T, S>& a, int f, long long fidx, long long NF,
+                                         const T u[9], T lam, T mu, T rho, T ubc[9], T Ux[9]) {
+  T fr[10];
+  face_frame(a, f, fidx, NF, fr);
+  if (a.sat_form == WQL_SAT_ALPHA_THETA) {
+    sat_alpha_theta_frame<T, T>(fr, a.bc[f], f, u, lam, mu, rho, Ux);
+  } else {
+    sat_ubc_frame<T, T>(fr, a.bc[f], f, u, lam, mu, rho, ubc);
+    sat_structure<T>(ubc, lam, mu, rho, Ux);
+  }
+}
+
 // face index of node (x,y,z) on face f; returns false if not on the face
 template <typename T, typename S>
 __device__ __forceinline__ bool on_face(const StepArgs<T, S>& a, int f, int x, int y, int z,

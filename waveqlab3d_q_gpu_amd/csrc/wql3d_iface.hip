@@ -1,0 +1,348 @@
+// Two-block q-interface of the fused stage: hat variables, characteristic SAT
+// forcing of both blocks, slip-rate update and stage-1 fault output, one
+// thread per interface node.
+//
+// Follows, per node and in the same order of operations:
+//   enforce_bound_iface_conditions (domain.f90:615-644): block L's evaluation
+//     (its side 2) first, then block R's (its side 1);
+//   Couple_Interface_x (CouplingForcing.f90:9-463): tractions n.sigma of both
+//     faces (:107-140), rotation into (n, m, l) (Rotate_in_Local_Cordinates,
+//     :466-522), hats, stage-1 outputs (:200-226), rotation back (:525-578),
+//     SAT forcing u_bc / v_bc with S^T Z S and S^T Z^-1 S (:228-457);
+//   Interface_RHS (Interface_Condition.f90:12-513): Welded_Interface (:515-590)
+//     or Slip_Weakening_Friction (:745-916) with solve (:1095-1168),
+//     Tau_strength TPV5 (:1283-1299), prestress TPV5 (:1802-1830), boxcar
+//     (:2438-2457); DS(1:3) += Q - P, DS(4) += |slip rate|;
+//   rupture_front (CouplingForcing.f90:580-600);
+//   scale_rates_iface / update_fields_iface (iface.f90:58-85).
+// The interface state is block L's copy (one slip-rate increment per stage,
+// as each MPI rank holds its own interface).  All interface arithmetic is
+// FP64; material and face frames are read in the blocks' storage type T.
+#include <math.h>
+
+#include "wql3d_internal.h"
+
+namespace wql {
+namespace {
+
+__device__ __forceinline__ double boxcar(double x, double w) {
+  const double tol = 100.0 * 2.220446049250313e-16;
+  if (-w + tol < x && x < w - tol) return 1.0;
+  if (fabs(-w - x) <= tol || fabs(x - w) <= tol) return 0.5;
+  return 0.0;
+}
+
+// prestress, problem TPV5: normal -120, strike 0, dip 70 plus the nucleation
+// and the two asperity patches (MPa)
+__device__ __forceinline__ void prestress_tpv5(double y, double z, double T0[3]) {
+  T0[0] = -120.0;
+  T0[1] = 0.0;
+  double l = 70.0;
+  l = l + 8.0 * boxcar(y - 7.5, 1.5) * boxcar(z + 7.5, 1.5);
+  l = l + 11.6 * boxcar(y - 7.5, 1.5) * boxcar(z, 1.5);
+  l = l - 8.0 * boxcar(y - 7.5, 1.5) * boxcar(z - 7.5, 1.5);
+  T0[2] = l;
+}
+
+// linear slip-weakening strength: fs = 0.677 inside |y|, |z| < 15 km (1e10
+// outside), fd = 0.525, Dc = 0.4 m
+__device__ __forceinline__ double tau_strength_tpv5(double sigma_n, double D, double y, double z) {
+  double fs = 0.677;
+  const double fd = 0.525, Dc = 0.4;
+  fs = fs + 1.0e10 * (1.0 - boxcar(y, 15.0) * boxcar(z, 15.0));
+  const double f = fs - (fs - fd) * fmin(D, Dc) / Dc;
+  return sigma_n * f;
+}
+
+// hats P (minus side) and Q (plus side) in (n, m, l) from the rotated grid
+// values U (block L) and V (block R): velocities [0..2], tractions [3..5]
+__device__ __forceinline__ void hats(int coupling, const double U[6], const double V[6],
+                                     double Z1s, double Z2s, double Z1p, double Z2p, double slip,
+                                     double y, double z, double P[6], double Q[6]) {
+  const double p_n = Z1p * U[0] - U[3], p_m = Z1s * U[1] - U[4], p_l = Z1s * U[2] - U[5];
+  const double q_n = Z2p * V[0] + V[3], q_m = Z2s * V[1] + V[4], q_l = Z2s * V[2] + V[5];
+  const double eta_p = 1.0 / (1.0 / Z1p + 1.0 / Z2p);
+  const double eta_s = 1.0 / (1.0 / Z1s + 1.0 / Z2s);
+  const double phi_n = eta_p * (q_n / Z2p - p_n / Z1p);
+  const double phi_m = eta_s * (q_m / Z2s - p_m / Z1s);
+  const double phi_l = eta_s * (q_l / Z2s - p_l / Z1s);
+  if (coupling == WQL_IFACE_LOCKED) {
+    P[0] = (q_n - phi_n) / Z2p; P[3] = phi_n;
+    Q[0] = (p_n + phi_n) / Z1p; Q[3] = phi_n;
+    P[2] = (q_l - phi_l) / Z2s; P[5] = phi_l;
+    Q[2] = (p_l + phi_l) / Z1s; Q[5] = phi_l;
+    P[1] = (q_m - phi_m) / Z2s; P[4] = phi_m;
+    Q[1] = (p_m + phi_m) / Z1s; Q[4] = phi_m;
+    return;
+  }
+  double T0[3];
+  prestress_tpv5(y, z, T0);
+  const double u_n_hat = (q_n - phi_n) / Z2p, Tu_n_hat = phi_n;
+  const double v_n_hat = (p_n + phi_n) / Z1p, Tv_n_hat = phi_n;
+  const double tau_lock =
+      sqrt((T0[2] + phi_l) * (T0[2] + phi_l) + (T0[1] + phi_m) * (T0[1] + phi_m));
+  const double sigma_n = fmax(0.0, -(T0[0] + phi_n));
+  const double tau_str = tau_strength_tpv5(sigma_n, slip, y, z);
+  double Tu_m, Tu_l, Tv_m, Tv_l, vv_m, vv_l;
+  if (tau_lock > tau_str) {
+    // sliding: |tau| = strength, slip rate V = (|phi| - strength) / eta_s
+    const double phi_1 = T0[1] + phi_m, phi_2 = T0[2] + phi_l;
+    const double Phi = sqrt(phi_1 * phi_1 + phi_2 * phi_2);
+    const double Vs = (Phi - tau_str) / eta_s;
+    vv_m = phi_1 / (eta_s + tau_str / Vs);
+    vv_l = phi_2 / (eta_s + tau_str / Vs);
+    const double tau_m = phi_1 - eta_s * vv_m;
+    const double tau_l = phi_2 - eta_s * vv_l;
+    Tu_m = tau_m - T0[1];
+    Tu_l = tau_l - T0[2];
+    Tv_m = tau_m - T0[1];
+    Tv_l = tau_l - T0[2];
+  } else {
+    Tu_m = phi_m; Tu_l = phi_l;
+    Tv_m = phi_m; Tv_l = phi_l;
+    vv_m = 0.0; vv_l = 0.0;
+  }
+  P[0] = u_n_hat;
+  P[1] = (q_m - Tv_m) / Z2s - vv_m;
+  P[2] = (q_l - Tv_l) / Z2s - vv_l;
+  P[3] = Tu_n_hat; P[4] = Tu_m; P[5] = Tu_l;
+  Q[0] = v_n_hat;
+  Q[1] = (Tu_m + p_m) / Z1s + vv_m;
+  Q[2] = (Tu_l + p_l) / Z1s + vv_l;
+  Q[3] = Tv_n_hat; Q[4] = Tv_m; Q[5] = Tv_l;
+}
+
+__device__ __forceinline__ void rot_in(const double n[3], const double m[3], const double l[3],
+                                       double u[6]) {
+  const double u1 = u[0], u2 = u[1], u3 = u[2], T1 = u[3], T2 = u[4], T3 = u[5];
+  u[0] = n[0] * u1 + n[1] * u2 + n[2] * u3;
+  u[1] = m[0] * u1 + m[1] * u2 + m[2] * u3;
+  u[2] = l[0] * u1 + l[1] * u2 + l[2] * u3;
+  u[3] = n[0] * T1 + n[1] * T2 + n[2] * T3;
+  u[4] = m[0] * T1 + m[1] * T2 + m[2] * T3;
+  u[5] = l[0] * T1 + l[1] * T2 + l[2] * T3;
+}
+
+__device__ __forceinline__ void rot_back(const double n[3], const double m[3], const double l[3],
+                                         double u[6]) {
+  const double u1 = u[0], u2 = u[1], u3 = u[2], T1 = u[3], T2 = u[4], T3 = u[5];
+  u[0] = n[0] * u1 + m[0] * u2 + l[0] * u3;
+  u[1] = n[1] * u1 + m[1] * u2 + l[1] * u3;
+  u[2] = n[2] * u1 + m[2] * u2 + l[2] * u3;
+  u[3] = n[0] * T1 + m[0] * T2 + l[0] * T3;
+  u[4] = n[1] * T1 + m[1] * T2 + l[1] * T3;
+  u[5] = n[2] * T1 + m[2] * T2 + l[2] * T3;
+}
+
+struct Side {
+  double lam, mu, rho;
+};
+
+// one evaluation (the calling block's frame and metric column): hats,
+// slip-rate magnitude and the calling block's SAT forcing bc[9]
+template <typename T>
+__device__ __forceinline__ void evaluate(const IfaceArgs<T>& a, int ib, const T* fr, long long fi,
+                                         long long NF, const double u[9], const double v[9],
+                                         const Side& s1, const Side& s2, double slip, double y,
+                                         double z, double P[6], double Q[6], double* slip_rate,
+                                         double bc[9]) {
+  double l[3], m[3], n[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    l[k] = (double)fr[k * NF + fi];
+    m[k] = (double)fr[(3 + k) * NF + fi];
+    n[k] = (double)fr[(6 + k) * NF + fi];
+  }
+  const double qx = (double)fr[10 * NF + fi], qy = (double)fr[11 * NF + fi],
+               qz = (double)fr[12 * NF + fi];
+  double U[6], V[6];
+  U[3] = n[0] * u[3] + n[1] * u[6] + n[2] * u[7];
+  U[4] = n[0] * u[6] + n[1] * u[4] + n[2] * u[8];
+  U[5] = n[0] * u[7] + n[1] * u[8] + n[2] * u[5];
+  V[3] = n[0] * v[3] + n[1] * v[6] + n[2] * v[7];
+  V[4] = n[0] * v[6] + n[1] * v[4] + n[2] * v[8];
+  V[5] = n[0] * v[7] + n[1] * v[8] + n[2] * v[5];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    U[c] = u[c];
+    V[c] = v[c];
+  }
+  rot_in(n, m, l, U);
+  rot_in(n, m, l, V);
+  const double c1p = sqrt((2.0 * s1.mu + s1.lam) / s1.rho), c1s = sqrt(s1.mu / s1.rho);
+  const double c2p = sqrt((2.0 * s2.mu + s2.lam) / s2.rho), c2s = sqrt(s2.mu / s2.rho);
+  const double Z1s = s1.rho * c1s, Z2s = s2.rho * c2s, Z1p = s1.rho * c1p, Z2p = s2.rho * c2p;
+  hats(a.coupling, U, V, Z1s, Z2s, Z1p, Z2p, slip, y, z, P, Q);
+  *slip_rate = sqrt((Q[1] - P[1]) * (Q[1] - P[1]) + (Q[2] - P[2]) * (Q[2] - P[2]));
+  double Pb[6], Qb[6];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    Pb[c] = P[c];
+    Qb[c] = Q[c];
+  }
+  rot_back(n, m, l, U);
+  rot_back(n, m, l, V);
+  rot_back(n, m, l, Pb);
+  rot_back(n, m, l, Qb);
+  const double norm = sqrt(qx * qx + qy * qy + qz * qz);
+  // rows n, m, l of the rotation
+  const double* r[3] = {n, m, l};
+  if (ib == 2) {
+    // minus side: p = 1/2 (S^T Zp S v + T), forcing from (p - p_hat)
+    double dp[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      double mz[3];
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        mz[j] = r[i][0] * r[j][0] * Z1p + r[i][1] * r[j][1] * Z1s + r[i][2] * r[j][2] * Z1s;
+      const double ph = 0.5 * (mz[0] * Pb[0] + mz[1] * Pb[1] + mz[2] * Pb[2] + Pb[3 + i]);
+      const double pp = 0.5 * (mz[0] * U[0] + mz[1] * U[1] + mz[2] * U[2] + U[3 + i]);
+      dp[i] = pp - ph;
+    }
+    double A[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      double mi[3];
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        mi[j] = r[i][0] * r[j][0] * 1.0 / Z1p + r[i][1] * r[j][1] * 1.0 / Z1s +
+                r[i][2] * r[j][2] * 1.0 / Z1s;
+      A[i] = mi[0] * dp[0] + mi[1] * dp[1] + mi[2] * dp[2];
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) bc[i] = norm / s1.rho * dp[i];
+    const double l2m = 2.0 * s1.mu + s1.lam, lam = s1.lam, mu = s1.mu;
+    bc[3] = l2m * qx * A[0] + lam * qy * A[1] + lam * qz * A[2];
+    bc[4] = l2m * qy * A[1] + lam * qz * A[2] + lam * qx * A[0];
+    bc[5] = l2m * qz * A[2] + lam * qy * A[1] + lam * qx * A[0];
+    bc[6] = mu * (qy * A[0] + qx * A[1]);
+    bc[7] = mu * (qz * A[0] + qx * A[2]);
+    bc[8] = mu * (qz * A[1] + qy * A[2]);
+  } else {
+    // plus side: q = 1/2 (S^T Zq S v - T), forcing from (q - q_hat)
+    double dq[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      double nz[3];
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        nz[j] = r[i][0] * r[j][0] * Z2p + r[i][1] * r[j][1] * Z2s + r[i][2] * r[j][2] * Z2s;
+      const double qh = 0.5 * (nz[0] * Qb[0] + nz[1] * Qb[1] + nz[2] * Qb[2] - Qb[3 + i]);
+      const double qm = 0.5 * (nz[0] * V[0] + nz[1] * V[1] + nz[2] * V[2] - V[3 + i]);
+      dq[i] = qm - qh;
+    }
+    double B[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      double qi[3];
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        qi[j] = r[i][0] * r[j][0] * 1.0 / Z2p + r[i][1] * r[j][1] * 1.0 / Z2s +
+                r[i][2] * r[j][2] * 1.0 / Z2s;
+      B[i] = qi[0] * dq[0] + qi[1] * dq[1] + qi[2] * dq[2];
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) bc[i] = norm / s2.rho * dq[i];
+    const double l2m = 2.0 * s2.mu + s2.lam, lam = s2.lam, mu = s2.mu;
+    bc[3] = -l2m * qx * B[0] - lam * qy * B[1] - lam * qz * B[2];
+    bc[4] = -l2m * qy * B[1] - lam * qz * B[2] - lam * qx * B[0];
+    bc[5] = -l2m * qz * B[2] - lam * qy * B[1] - lam * qx * B[0];
+    bc[6] = -mu * (qy * B[0] + qx * B[1]);
+    bc[7] = -mu * (qz * B[0] + qx * B[2]);
+    bc[8] = -mu * (qz * B[1] + qy * B[2]);
+  }
+}
+
+// stage-1 outputs of one evaluation (CouplingForcing.f90:200-226,
+// Interface_Condition.f90:494-503, rupture_front)
+template <typename T>
+__device__ __forceinline__ void stage1_output(const IfaceArgs<T>& a, const double P[6], const double Q[6], double y,
+                                              double z, double out[15], double* trup) {
+  double T0[3] = {0.0, 0.0, 0.0};
+  if (a.coupling != WQL_IFACE_LOCKED) prestress_tpv5(y, z, T0);
+  // out: Svel[3], Uhat[6], Vhat[6]
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    out[c] = Q[c] - P[c];
+    out[3 + c] = Q[c] - P[c];
+    out[6 + c] = P[3 + c] + T0[c];
+    out[9 + c] = Q[c] - P[c];
+    out[12 + c] = Q[3 + c] + T0[c];
+  }
+  if (sqrt(out[1] * out[1] + out[2] * out[2]) > 1e-3) *trup = fmin(*trup, a.t);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_iface(const IfaceArgs<T> a) {
+  const long long NF = (long long)a.ny * a.nz;
+  const long long fi = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (fi >= NF) return;
+  const int y = (int)(fi % a.ny), z = (int)(fi / a.ny);
+  const long long iL = (long long)(a.nxL - 1) + (long long)a.nxL * (y + (long long)a.ny * z);
+  const long long iR = (long long)a.nxR * (y + (long long)a.ny * z);  // block R's face is x = 0
+  double u[9], v[9];
+#pragma unroll
+  for (int c = 0; c < 9; ++c) {
+    u[c] = a.FL[c * a.PL + iL];
+    v[c] = a.FR[c * a.PR + iR];
+  }
+  Side s1, s2;
+  s1.lam = (double)a.ML[iL]; s1.mu = (double)a.ML[a.PL + iL]; s1.rho = (double)a.ML[2 * a.PL + iL];
+  s2.lam = (double)a.MR[iR]; s2.mu = (double)a.MR[a.PR + iR]; s2.rho = (double)a.MR[2 * a.PR + iR];
+  const double yc = a.yf[fi], zc = a.zf[fi];
+  const double slip = a.S[3 * NF + fi];
+  double P[6], Q[6], srate, bc[9];
+  double trup = a.trup[fi];
+  double out[15];
+  // block L, side 2: SAT into L, the slip-rate increment
+  evaluate(a, 2, a.frL, fi, NF, u, v, s1, s2, slip, yc, zc, P, Q, &srate, bc);
+  double ds[4];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) ds[c] = a.A * a.DS[c * NF + fi] + (Q[c] - P[c]);
+  ds[3] = a.A * a.DS[3 * NF + fi] + srate;
+  if (a.stage1) stage1_output(a, P, Q, yc, zc, out, &trup);
+#pragma unroll
+  for (int c = 0; c < 9; ++c) a.satL[c * NF + fi] = a.penL * bc[c];
+  // block R, side 1: SAT into R (its hats are the same up to rounding)
+  evaluate(a, 1, a.frR, fi, NF, u, v, s1, s2, slip, yc, zc, P, Q, &srate, bc);
+  if (a.stage1) stage1_output(a, P, Q, yc, zc, out, &trup);
+#pragma unroll
+  for (int c = 0; c < 9; ++c) a.satR[c * NF + fi] = a.penR * bc[c];
+  if (a.stage1) {
+    if (a.frame) {
+      double* f = a.frame;
+#pragma unroll
+      for (int c = 0; c < 9; ++c) {
+        f[c * NF + fi] = u[c];
+        f[(9 + c) * NF + fi] = v[c];
+      }
+#pragma unroll
+      for (int c = 0; c < 4; ++c) f[(18 + c) * NF + fi] = a.S[c * NF + fi];
+#pragma unroll
+      for (int c = 0; c < 15; ++c) f[(22 + c) * NF + fi] = out[c];
+      f[37 * NF + fi] = trup;
+    }
+    a.trup[fi] = trup;
+  }
+  // scale_rates_iface, then update_fields_iface after the blocks' stage
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    a.DS[c * NF + fi] = ds[c];
+    a.S[c * NF + fi] = a.S[c * NF + fi] + a.Bdt * ds[c];
+  }
+}
+
+}  // namespace
+
+template <typename T>
+hipError_t launch_iface(const IfaceArgs<T>& a, hipStream_t s) {
+  const long long NF = (long long)a.ny * a.nz;
+  hipLaunchKernelGGL(k_iface<T>, dim3((unsigned)((NF + 255) / 256)), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+template hipError_t launch_iface<double>(const IfaceArgs<double>&, hipStream_t);
+template hipError_t launch_iface<float>(const IfaceArgs<float>&, hipStream_t);
+
+}  // namespace wql

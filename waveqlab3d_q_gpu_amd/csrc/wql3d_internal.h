@@ -13,7 +13,9 @@
 //   eta   : 6*nmech planes of T, plane (c6*nmech + i); Deta likewise
 //   Qs,Qp : 1 plane each of T
 //   pQ/pDQ[f] : 9 planes over the PML slab of face f (slab-local index)
-//   face[f]   : 10 planes over the face (l[3], m[3], n[3], |grad xi|)
+//   face[f]   : 13 planes over the face (l[3], m[3], n[3], |grad xi|, and the
+//               metric column of the face axis: xi_x, xi_y, xi_z) on faces
+//               with a boundary condition or an interface
 // T = double (WQL_FP64) or float (WQL_FP32_MIXED).
 #pragma once
 
@@ -63,7 +65,12 @@ struct StepArgs {
   T* pQ[6];
   T* pDQ[6];
   int bc[6];
+  int sat_form;     // WQL_SAT_BC / WQL_SAT_ALPHA_THETA
   const T* face[6];
+  // interface SAT forcing of faces with bc 0 coupled to another block:
+  // 9 planes over the face, tau0/h * forcing, subtracted from DF like the
+  // boundary penalty (null: no interface on the face)
+  const double* iface[6];
   int nsrc;
   DevSource src[kMaxSources];
   double A, Bdt;    // fused stage coefficients
@@ -124,5 +131,38 @@ hipError_t launch_sample(const S* F, long long P, const long long* idx, int n, d
                          hipStream_t s);
 hipError_t launch_metrics(int nx, int ny, int nz, const double* X, double* mx, double* my,
                           double* mz, double* J, hipStream_t s);
+
+// Two-block q-interface: block L's x-high face against block R's x-low face,
+// both ny x nz nodes; face index fi = y + ny*z.  One thread per face node
+// evaluates both sides (block L's side 2, then block R's side 1), writes the
+// SAT forcing of each block, updates the interface state in place and, at
+// stage 1 with a slot, the fault output.
+constexpr int kFramePlanes = 38;  // Uface 9, Vface 9, S 4, Svel 3, Uhat 6, Vhat 6, trup 1
+template <typename T>
+struct IfaceArgs {
+  int ny, nz;
+  int coupling;
+  int nxL, nxR;            // x extents (L's face is x = nxL - 1, R's is x = 0)
+  const double* FL;        // block L / R fields at the start of the stage
+  const double* FR;
+  long long PL, PR;        // plane strides
+  const T* ML;             // lambda, mu, rho planes
+  const T* MR;
+  const T* frL;            // 13 face planes of L's face 1 and R's face 0
+  const T* frR;
+  double penL, penR;       // tau0/hq of each block
+  double* S;               // 4 planes: slip (3) and slip-path length
+  double* DS;              // 4 planes: rates of S
+  double* trup;            // rupture time
+  const double* yf;        // face node coordinates
+  const double* zf;
+  double* satL;            // 9 planes each: tau0/hq * SAT forcing
+  double* satR;
+  double A, Bdt, t;
+  int stage1;
+  double* frame;           // null, or kFramePlanes planes of the current output slot
+};
+template <typename T>
+hipError_t launch_iface(const IfaceArgs<T>& a, hipStream_t s);
 
 }  // namespace wql

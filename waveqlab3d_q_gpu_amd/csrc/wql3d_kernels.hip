@@ -207,27 +207,33 @@ __device__ __forceinline__ void node_epilogue(const StepArgs<T, S>& a, int x, in
     }
   }
 
-  // SAT penalty terms at face nodes (impose_boundary_condition), fused mode
+  // SAT penalty terms at face nodes (impose_boundary_condition) and the
+  // interface forcing of coupled faces (Impose_Interface_Condition,
+  // RHS_Interior.f90:1813-1836, computed by k_iface), fused mode
   double satDF[9];
   bool has_sat = false;
   if (FUSED && SHELL) {
 #pragma unroll
     for (int f = 0; f < 6; ++f) {
-      if (a.bc[f] <= 0) continue;
+      if (a.bc[f] < 0 || (a.bc[f] == 0 && !a.iface[f])) continue;
       long long fidx, NF;
       if (!on_face(a, f, x, y, z, &fidx, &NF)) continue;
-      T u[9], ubc[9];
-#pragma unroll
-      for (int c = 0; c < 9; ++c) u[c] = (T)a.Fin[c * P + id];
-      sat_ubc(a, f, fidx, NF, u, lam, mu, rho, ubc);
-      const T pen = (T)a.pen[f >> 1];
-      T Ux[9];
-      sat_structure<T>(ubc, lam, mu, rho, Ux);
       if (!has_sat) {
 #pragma unroll
         for (int c = 0; c < 9; ++c) satDF[c] = 0.0;
         has_sat = true;
       }
+      if (a.bc[f] == 0) {
+#pragma unroll
+        for (int c = 0; c < 9; ++c) satDF[c] += a.iface[f][c * NF + fidx];
+        continue;
+      }
+      T u[9], ubc[9];
+#pragma unroll
+      for (int c = 0; c < 9; ++c) u[c] = (T)a.Fin[c * P + id];
+      const T pen = (T)a.pen[f >> 1];
+      T Ux[9];
+      sat_rate(a, f, fidx, NF, u, lam, mu, rho, ubc, Ux);
 #pragma unroll
       for (int c = 0; c < 9; ++c) satDF[c] += (double)(pen * Ux[c]);
     }
@@ -720,12 +726,10 @@ __global__ __launch_bounds__(256) void k_sat_face(const StepArgs<double> a, int 
   const long long P = a.P;
   const long long id = x + (long long)a.nx * (y + (long long)a.ny * z);
   const double lam = a.M[id], mu = a.M[P + id], rho = a.M[2 * P + id];
-  double u[9], ubc[9];
+  double u[9], ubc[9], Ux[9];
   for (int c = 0; c < 9; ++c) u[c] = a.Fin[c * P + id];
-  sat_ubc(a, f, t, (long long)n1 * n2, u, lam, mu, rho, ubc);
+  sat_rate(a, f, t, (long long)n1 * n2, u, lam, mu, rho, ubc, Ux);
   const double pen = a.pen[axis];
-  double Ux[9];
-  sat_structure<double>(ubc, lam, mu, rho, Ux);
   for (int c = 0; c < 9; ++c) a.DF[c * P + id] = a.DF[c * P + id] - pen * Ux[c];
   if (a.pml[f]) {
     long long si;

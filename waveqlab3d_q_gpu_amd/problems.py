@@ -305,6 +305,61 @@ def tpv5_cartesian(n=21, sat_form=1, coupling=None):
                            name=f"tpv5_cartesian_{n}")
 
 
+def two_block_curvilinear(n=24, nmech=4, npml=6, coupling=None, cfl=0.4, nsteps=30):
+    """Two curvilinear n^3 blocks, block L over x in [-3, 0] and block R over
+    [0, 3] (y, z in [0, 3], km), both mapped by the same smooth map so the
+    interface x = 0 becomes a curved surface whose nodes coincide; different
+    materials on the two sides (2.7/3.464/6 and 2.4/2.9/5.1), anelastic Q
+    (nmech mechanisms, Qs = 50, Qp = 100), PML of npml nodes on the outer x
+    faces, free surface on y-min, characteristic SAT elsewhere; a Gaussian
+    velocity pulse in block L next to the interface."""
+    if coupling is None:
+        coupling = L.WQL_IFACE_LOCKED
+    nn = (n, n, n)
+
+    def mapped(a, b):
+        X = cartesian_grid(nn, a, b)
+        x, y, z = X[0].copy(), X[1].copy(), X[2].copy()
+        X[0] = x + 0.12 * np.sin(2.0 * np.pi * y / 3.0) * np.sin(2.0 * np.pi * z / 3.0)
+        X[1] = y + 0.08 * np.sin(np.pi * x / 3.0) * np.cos(np.pi * z / 3.0)
+        X[2] = z + 0.05 * np.sin(np.pi * y / 3.0) * np.sin(np.pi * (x + 3.0) / 6.0)
+        return X
+
+    XL = mapped((-3.0, 0.0, 0.0), (0.0, 3.0, 3.0))
+    XR = mapped((0.0, 0.0, 0.0), (3.0, 3.0, 3.0))
+    ML = material_from_rho_s_p(nn, 2.7, 3.464, 6.0)
+    MR = material_from_rho_s_p(nn, 2.4, 2.9, 5.1)
+    tau = weight = Qs_inv = Qp_inv = None
+    if nmech:
+        tau, weight = q_coefficients(nmech)
+        Qs_inv = np.full(XL.shape[1:], 1.0 / 50.0)
+        Qp_inv = np.full(XL.shape[1:], 1.0 / 100.0)
+        ML = modulus_correction(ML, Qs_inv, Qp_inv, tau, weight)
+        MR = modulus_correction(MR, Qs_inv, Qp_inv, tau, weight)
+    dt = block_time_step(3.0 / (n - 1) * 0.8, cfl, 3.464, 6.0)
+    if nmech:
+        dt = min(dt, relaxation_dt_limit(tau))
+    r2 = (XL[0] + 0.6) ** 2 + (XL[1] - 1.5) ** 2 + (XL[2] - 1.5) ** 2
+    g = np.exp(-r2 / 0.15)
+    F0 = np.zeros((9,) + XL.shape[1:])
+    F0[0] = g
+    F0[2] = 0.5 * g
+    pmlL = (1, 0, 0, 0, 0, 0) if npml else (0,) * 6
+    pmlR = (0, 1, 0, 0, 0, 0) if npml else (0,) * 6
+    nL = tuple(npml * p for p in pmlL)
+    nR = tuple(npml * p for p in pmlR)
+    kw = dict(mesh=L.WQL_CURVILINEAR, dt=dt, nsteps=nsteps, nmech=nmech, tau=tau, weight=weight,
+              Qs_inv=Qs_inv, Qp_inv=Qp_inv)
+    left = Problem(n=nn, X=XL, M=ML, bc=(1, 0, 2, 1, 1, 1), pml=pmlL, npml=nL, F0=F0,
+                   name="curv_block1", **kw)
+    right = Problem(n=nn, X=XR, M=MR, bc=(0, 1, 2, 1, 1, 1), pml=pmlR, npml=nR,
+                    name="curv_block2", **kw)
+    return TwoBlockProblem(left=left, right=right, coupling=coupling,
+                           yf=np.ascontiguousarray(XL[1][:, :, -1]),
+                           zf=np.ascontiguousarray(XL[2][:, :, -1]), dt=dt, nsteps=nsteps,
+                           name=f"two_block_curvilinear_{n}")
+
+
 def cartesian_bump(n=65, h=100.0, nsteps=200, seed=1234, cfl=0.5, nrec=8):
     """Plain FP64 Cartesian block: n^3 nodes, spacing h (m); rho = 2670,
     cp = 6000, cs = 3464; free surface on r-min, characteristic SAT elsewhere;
@@ -420,7 +475,7 @@ def make_device_block(prob: Problem, precision=L.WQL_FP64, device="cuda", state_
                 weight=tuple(prob.weight) if prob.weight is not None else (),
                 metricx=metx, metricy=mety, metricz=metz, J=J,
                 M=torch.from_numpy(np.ascontiguousarray(prob.M)).to(dev),
-                n_l=nl, n_m=nm_, n_n=nn_, on_device=True, **kw)
+                n_l=nl, n_m=nm_, n_n=nn_, on_device=True, sat_form=prob.sat_form, **kw)
     blk.upload(L.WQL_GEOMETRY | L.WQL_MATERIAL | L.WQL_QINV | L.WQL_FACES)
     blk.set_sources(prob.sources)
     return blk
@@ -478,7 +533,7 @@ def make_block(prob: Problem, metrics, precision=L.WQL_FP64, frames=None, upload
                 metricx=np.ascontiguousarray(metx), metricy=np.ascontiguousarray(mety),
                 metricz=np.ascontiguousarray(metz), J=np.ascontiguousarray(J),
                 M=np.ascontiguousarray(prob.M), F=F, DF=np.full((9, ns, nr, nq), 1.0e40),
-                pml_Q=pq, pml_DQ=pdq, n_l=nl, n_m=nm, n_n=nn, **kw)
+                pml_Q=pq, pml_DQ=pdq, n_l=nl, n_m=nm, n_n=nn, sat_form=prob.sat_form, **kw)
     if upload:
         blk.upload(L.WQL_GEOMETRY | L.WQL_MATERIAL | L.WQL_QINV | L.WQL_FACES | L.WQL_F)
         blk.set_sources(prob.sources)
