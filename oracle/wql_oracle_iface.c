@@ -18,7 +18,7 @@
  *     Tau_strength TPV5 (:1283-1299), prestress TPV5 (:1802-1830), boxcar
  *     (:2438-2457); DS(1:3) += Q - P, DS(4) += |slip rate|;
  *   rupture_front (CouplingForcing.f90:580-600);
- *   scale_rates_iface / update_fields_iface (iface.f90:63-88).
+ *   scale_rates_iface / update_fields_iface (iface.f90:58-85).
  */
 #include <math.h>
 #include <stdlib.h>
