@@ -52,8 +52,8 @@ enum { WQL_BC_NONE = 0, WQL_BC_CHARACTERISTIC = 1,     /* BoundaryConditions.f90
 enum { WQL_SRC_GAUSSIAN = 0, WQL_SRC_COSBELL = 1, WQL_SRC_RICKER = 2 };
 /* SAT penalty form of the boundary faces: BCm/BCp followed by structure()
  * (BoundaryConditions.f90:783-924, RHS_Interior.f90:1584-1768), or the
- * alpha/theta form kept in comments at BoundaryConditions.f90:832-840 /
- * 904-912 (velocity penalty alpha*norm/rho*F, alpha = 2, 1, 0 for codes 1, 2,
+ * alpha/theta form kept in comments at BoundaryConditions.f90:833-841 /
+ * 905-913 (velocity penalty alpha*norm/rho*F, alpha = 2, 1, 0 for codes 1, 2,
  * 3; stress penalty theta*norm*(...), theta = 0, 0, -1; no structure), the
  * form that produced the archived golden files test_problems/truth/1.  The
  * alpha/theta form takes no PML faces. */

@@ -313,7 +313,8 @@ class OracleIface:
 
 def two_block_time_step(L, R, I, dt, nsteps, t=0.0):
     """time_step_RK (time_step.f90:68-150) of a two-block domain: per stage
-    scale_rates, set_rates, boundary conditions (block L, then R) and the
+    scale_rates, set_rates with the moment tensor (domain.f90:951-979),
+    boundary conditions (block L, then R) and the
     interface (enforce_bound_iface_conditions, domain.f90:615-644), the
     stage-1 output, update_fields (blocks, then interface).  Returns (t,
     frames) with the fault output of every step (write_fault_output /
@@ -330,6 +331,8 @@ def two_block_time_step(L, R, I, dt, nsteps, t=0.0):
             I.scale_rates(A[s])
             for b in (L, R):
                 b.set_rates()
+                if b.sources:
+                    b.set_moment_tensor(t)
             L.impose_boundary_condition()
             I.enforce(L, R, t, s + 1, 2)
             R.impose_boundary_condition()

@@ -723,7 +723,7 @@ static void bc_char(double ubc[9], const double u[9], const double *l, const dou
 }
 
 /* the alpha/theta form of the penalty (commented out at
- * BoundaryConditions.f90:832-840 for BCp and 904-912 for BCm): the rate
+ * BoundaryConditions.f90:833-841 for BCp and 905-913 for BCm): the rate
  * vector itself, without structure(); alpha, theta per code as set in
  * BC_Lx..BC_Rz (BoundaryConditions.f90:129-160) */
 static void bc_alpha_theta(double Ux[9], const double *l, const double *m, const double *n,

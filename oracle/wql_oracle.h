@@ -46,7 +46,7 @@ typedef struct orc_block {
   int npml[6];
   /* SAT form: 0 = BCm/BCp + structure as in BoundaryConditions.f90:783-924
    * and RHS_Interior.f90:1584-1768; 1 = the alpha/theta form kept in
-   * comments at BoundaryConditions.f90:832-840 / 904-912 (velocity penalty
+   * comments at BoundaryConditions.f90:833-841 / 905-913 (velocity penalty
    * alpha*norm/rho*F with alpha = 2, 1, 0 for codes 1, 2, 3; stress penalty
    * theta*norm*(...) with theta = 0, 0, -1; no structure), which produced
    * the archived golden files test_problems/truth/1; no PML faces */

@@ -144,3 +144,17 @@ def test_interface_argument_errors(oracle):
     itf.close()
     bl.close()
     br.close()
+
+
+def test_locked_interface_kat_41(oracle):
+    """The reference's locked two-block regression (two 41^3 blocks, 103
+    steps): block maxima 2.0699E+03 / 1.6389E+03 as printed by close_domain."""
+    tp = PB.locked_interface_41()
+    bl, br, itf = _gpu_pair(oracle, tp)
+    itf.time_step(tp.dt, tp.nsteps, 0.0)
+    bl.download(L.WQL_F)
+    br.download(L.WQL_F)
+    m1, m2 = float(np.abs(bl.F).max()), float(np.abs(br.F).max())
+    print(f"locked KAT block maxima {m1!r} {m2!r}", flush=True)
+    assert "%.4E" % m1 == "2.0699E+03" and "%.4E" % m2 == "1.6389E+03"
+    itf.close()

@@ -6,8 +6,8 @@ Golden files: test_problems/truth/1/test_rup_cart_fric_interface.* (stored
 as tests/golden/tpv5_cart_truth.npz), checked by python/read_binary.py:9-37
 with max |difference| <= 1e-3 over 46 frames of S, Svel, Uface, Vface, Uhat,
 Vhat.  The files predate two changes of the current source: they carry the
-alpha/theta SAT form (commented out at BoundaryConditions.f90:832-840 and
-904-912) and one slip-rate increment per stage (each MPI rank owns its copy of
+alpha/theta SAT form (commented out at BoundaryConditions.f90:833-841 and
+905-913) and one slip-rate increment per stage (each MPI rank owns its copy of
 the interface; on one rank both block evaluations of
 enforce_bound_iface_conditions, domain.f90:615-644, add to the shared DS).
 The oracle reproduces them with sat_form = 1 and one increment per stage.
@@ -94,3 +94,17 @@ def test_locked_sat_opposes_a_velocity_jump(oracle):
     assert (fL > 0).all() and (fR < 0).all()
     # welded hats are continuous: no slip rate even with a jump in the grid values
     assert float(np.abs(I.DS[:3]).max()) <= 1e-14
+
+
+def test_locked_interface_kat_41(oracle):
+    """inputfile/test_elastic_locked_interface_o6.in: the reference prints
+    'block 1 max field: 2.0699E+03', 'block 2 max field: 1.6389E+03'
+    (PHASE2_SCRATCH_MEMORY_AUDIT.md:222-223; close_domain, domain.f90:583-586)
+    after 103 steps of two 41^3 blocks with a locked interface and a Cos-Bell
+    source on block 1's interface face."""
+    tp = PB.locked_interface_41()
+    assert tp.nsteps == 103 and tuple(tp.left.sources[0].near) == (41, 21, 21)
+    Lb, Rb, I = oracle_two_block(oracle, tp)
+    oracle.two_block_time_step(Lb, Rb, I, tp.dt, tp.nsteps, 0.0)
+    assert "%.4E" % np.abs(Lb.F).max() == "2.0699E+03"
+    assert "%.4E" % np.abs(Rb.F).max() == "1.6389E+03"

@@ -316,7 +316,7 @@ __device__ __forceinline__ void sat_structure(const S ubc[9], S lam, S mu, S rho
 }
 
 // The alpha/theta form of the boundary penalty (commented out at
-// BoundaryConditions.f90:832-840 and 904-912; alpha, theta per code as set in
+// BoundaryConditions.f90:833-841 and 905-913; alpha, theta per code as set in
 // BC_Lx..BC_Rz, BoundaryConditions.f90:129-160): the rate vector itself,
 // velocity alpha*norm/rho*F, stresses theta*norm*(...), no structure().
 template <typename S, typename R>

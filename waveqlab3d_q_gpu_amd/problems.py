@@ -305,6 +305,31 @@ def tpv5_cartesian(n=21, sat_form=1, coupling=None):
                            name=f"tpv5_cartesian_{n}")
 
 
+def locked_interface_41():
+    """inputfile/test_elastic_locked_interface_o6.in: two 41^3 Cartesian blocks,
+    [-2,0] x [-1,1]^2 and [0,2] x [-1,1]^2 (km), rho_s_p = (2.7, 3.464, 6),
+    locked interface, characteristic SAT on the outer faces, CFL 0.2,
+    t_final 0.15 (103 steps); block 1 holds an order-4 Cos-Bell moment
+    source at the origin (on its interface face) with mXX = -0.25,
+    mXZ = -10, t_init = 0.05."""
+    nn = (41, 41, 41)
+    XL = cartesian_grid(nn, (-2.0, -1.0, -1.0), (0.0, 1.0, 1.0))
+    XR = cartesian_grid(nn, (0.0, -1.0, -1.0), (2.0, 1.0, 1.0))
+    M = material_from_rho_s_p(nn, 2.7, 3.464, 6.0)
+    dt = block_time_step(2.0 / 40.0, 0.2, 3.464, 6.0)
+    nsteps = int(math.floor(0.15 / dt))
+    src = point_source(XL, (0.0, 0.0, 0.0), L.WQL_SRC_COSBELL, t_init=0.05, duration=0.0,
+                       order=4, mxx=-0.25, mxz=-10.0)
+    left = Problem(n=nn, mesh=L.WQL_CARTESIAN, X=XL, M=M.copy(), bc=(1, 0, 1, 1, 1, 1), dt=dt,
+                   nsteps=nsteps, sources=[src], name="locked41_block1")
+    right = Problem(n=nn, mesh=L.WQL_CARTESIAN, X=XR, M=M.copy(), bc=(0, 1, 1, 1, 1, 1), dt=dt,
+                    nsteps=nsteps, name="locked41_block2")
+    return TwoBlockProblem(left=left, right=right, coupling=L.WQL_IFACE_LOCKED,
+                           yf=np.ascontiguousarray(XL[1][:, :, -1]),
+                           zf=np.ascontiguousarray(XL[2][:, :, -1]), dt=dt, nsteps=nsteps,
+                           name="locked_interface_41")
+
+
 def two_block_curvilinear(n=24, nmech=4, npml=6, coupling=None, cfl=0.4, nsteps=30):
     """Two curvilinear n^3 blocks, block L over x in [-3, 0] and block R over
     [0, 3] (y, z in [0, 3], km), both mapped by the same smooth map so the
