@@ -103,10 +103,16 @@ def test_curvilinear_q_pml(oracle, nmech, fused):
 # split,interior,zchunk"): strict-interior kernel off, or tiles 64x8 / 32x16 /
 # 64x4 with z chunks that cut the interior box unevenly; shell memory
 # variables in their own kernel (split 1) or in the node kernel (split 0).
-@pytest.mark.parametrize("tuning", ["0,1,0,1,0,64", "0,1,0,1,1,7", "0,1,0,1,2,5",
-                                    "0,1,0,1,3,64", "0,1,0,0,1,16", "0,0,0,1,1,9"])
-def test_fused_stage_launch_shapes(oracle, monkeypatch, tuning):
+# WQL3D_STAGE_CHUNKS: z chunks of the two-stream pipeline of anelastic stages
+# (1 = one stream; 13 chunks of 3 z levels = the halo width; 5 uneven chunks).
+@pytest.mark.parametrize("tuning,chunks", [("0,1,0,1,0,64", "4"), ("0,1,0,1,1,7", "4"),
+                                           ("0,1,0,1,2,5", "5"), ("0,1,0,1,3,64", "4"),
+                                           ("0,1,0,0,1,16", "4"), ("0,0,0,1,1,9", "4"),
+                                           ("0,1,0,1,1,32", "1"), ("0,1,0,1,1,32", "13"),
+                                           ("0,1,0,1,0,32", "13")])
+def test_fused_stage_launch_shapes(oracle, monkeypatch, tuning, chunks):
     monkeypatch.setenv("WQL3D_NODE_TUNING", tuning)
+    monkeypatch.setenv("WQL3D_STAGE_CHUNKS", chunks)
     prob = small_curvilinear(n=40, nmech=8, nsteps=3, seed=11)
     met = oracle.metrics(prob.X)
     rng = np.random.default_rng(7)
@@ -130,7 +136,7 @@ def test_fused_stage_launch_shapes(oracle, monkeypatch, tuning):
     time_step_RK(blk, prob.dt, prob.nsteps, fused=True)
     _download_all(blk)
     errs = field_errors(blk.F, orc.F)
-    print(f"tuning={tuning} field rel L2:", ["%.2e" % e for e in errs], flush=True)
+    print(f"tuning={tuning} chunks={chunks} field rel L2:", ["%.2e" % e for e in errs], flush=True)
     assert max(errs) <= TOL64
     assert rel_l2(blk.DF, orc.DF) <= TOL64
     assert rel_l2(blk.eta, orc.eta) <= TOL64

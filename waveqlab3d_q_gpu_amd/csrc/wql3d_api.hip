@@ -88,6 +88,7 @@ struct wql_block {
   long long bytes;
   int tune_nt, tune_xcd, tune_band, tune_split;  // node-kernel launch shape
   int tune_zm, tune_zchunk;                      // z-marching kernel, z levels per chunk
+  int tune_mem_npt;                              // nodes per thread of k_memory_update
   void* arena;    // one allocation holding F[2], DF, G, M, eta, Deta, Qs, Qp, W
   void* W;        // 15 planes of T: split anelastic stage hand-off
   int mech_zero;  // mechanisms whose eta and Deta are known to be all zero
@@ -113,10 +114,26 @@ struct wql_block {
   // coupled through and the SAT forcing planes of its coupled face
   wql_iface* iface;
   double* iface_sat[6];
+  // z-chunk pipeline of anelastic stages (env WQL3D_STAGE_CHUNKS, default
+  // kDefaultChunks = 1, i.e. off: at 385^3 with Q8 and PML 2/4/8 chunks ran
+  // 3%/6%/14% slower, since the interior kernel's two waves per SIMD fill the
+  // register file and leave no room for memory-kernel waves on its CUs, so the
+  // two kernels do not overlap): the stencil kernels of chunk k run on `stream`, the
+  // memory-variable kernel of chunk k on `stream2` behind ev_sten[k]; the
+  // stencil of chunk k of the next stage waits for ev_mem[k + 1] (its z halo).
+  // `pending` = memory kernels on stream2 that `stream` has not yet waited for.
+  int nchunks;
+  hipStream_t stream2;
+  hipEvent_t ev_sten[kMaxChunks];
+  hipEvent_t ev_mem[kMaxChunks];
+  int pending;
 };
 
 // Two-block q-interface (wql_iface_create): block L's x-high face against
 // block R's x-low face, ny x nz nodes, node index y + ny*z.
+// make the block's stream wait for the pipelined memory-variable kernels
+static int join(wql_block* b);
+
 struct wql_iface {
   wql_block* L;
   wql_block* R;
@@ -224,6 +241,7 @@ static int dalloc(wql_block* b, void** p, size_t bytes) {
 
 extern "C" void wql_block_destroy(wql_block* b) {
   if (!b) return;
+  if (b->stream2) (void)hipStreamSynchronize(b->stream2);
   (void)hipStreamSynchronize(b->stream);
   if (b->iface) {  // the interface outlives the block: it refuses further stages
     if (b->iface->L == b) b->iface->L = nullptr;
@@ -241,6 +259,11 @@ extern "C" void wql_block_destroy(wql_block* b) {
   for (int i = 0; i < b->nev_alloc; ++i) (void)hipEventDestroy(b->ev[i]);
   free(b->ev);
   free(b->slot_mz);
+  for (int k = 0; k < kMaxChunks; ++k) {
+    if (b->ev_sten[k]) (void)hipEventDestroy(b->ev_sten[k]);
+    if (b->ev_mem[k]) (void)hipEventDestroy(b->ev_mem[k]);
+  }
+  if (b->stream2) (void)hipStreamDestroy(b->stream2);
   (void)hipStreamDestroy(b->stream);
   free(b);
 }
@@ -279,8 +302,29 @@ extern "C" int wql_block_create(const wql_block_desc* d, wql_block** out) {
       b->tune_zchunk = v5 > 0 ? v5 : 32;
     }
   }
+  b->tune_mem_npt = 2;
+  if (const char* mv = getenv("WQL3D_MEM_NPT")) {
+    const int v = atoi(mv);
+    b->tune_mem_npt = v == 1 ? 1 : (v == 4 ? 4 : 2);
+  }
+  b->nchunks = kDefaultChunks;
+  if (const char* cv = getenv("WQL3D_STAGE_CHUNKS")) {
+    const int v = atoi(cv);
+    b->nchunks = v < 1 ? 1 : (v > kMaxChunks ? kMaxChunks : v);
+  }
+  // a chunk must hold at least 3 z levels (the stencil's z halo)
+  while (b->nchunks > 1 && b->nz / b->nchunks < 3) --b->nchunks;
   hipError_t e = hipStreamCreate(&b->stream);  // default priority
   if (e != hipSuccess) { free(b); return fail(WQL_EHIP, "hipStreamCreate: %s", hipGetErrorString(e)); }
+  e = hipStreamCreate(&b->stream2);  // default priority
+  for (int k = 0; k < kMaxChunks && e == hipSuccess; ++k) {
+    e = hipEventCreateWithFlags(&b->ev_sten[k], hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_mem[k], hipEventDisableTiming);
+  }
+  if (e != hipSuccess) {
+    wql_block_destroy(b);
+    return fail(WQL_EHIP, "stream / event creation: %s", hipGetErrorString(e));
+  }
   const long long P = b->P;
   const int ngeo = d->mesh == WQL_CURVILINEAR ? 10 : 3;
 #define ALLOC(ptr, bytes)                                  \
@@ -764,6 +808,9 @@ static StepArgs<T, S> make_args(wql_block* b, double t) {
   a.split = b->tune_split;
   a.zm = b->tune_zm;
   a.zchunk = b->tune_zchunk;
+  a.node0 = 0;
+  a.node1 = b->P;
+  a.mem_npt = b->tune_mem_npt;
   // a zero-weight mechanism whose state is all zero stays zero:
   // Deta = A*0 + (0 - 0)/tau = 0, eta = 0 + Bdt*0 = 0
   int zero_weight = 0;
@@ -893,14 +940,54 @@ static int next_event(wql_block* b, hipEvent_t* out) {
   return WQL_OK;
 }
 
+static int join(wql_block* b) {
+  if (b->pending) {
+    HIP_TRY(hipStreamWaitEvent(b->stream, b->ev_mem[b->nchunks - 1], 0));
+    b->pending = 0;
+  }
+  return WQL_OK;
+}
+
+// One stage as a z-chunk pipeline: the stencil kernels (latency bound) of
+// chunk k on the block's stream overlap the memory-variable kernel (HBM
+// bound) of chunk k - 1 on stream2.  Hazards: the stencil of chunk k reads F
+// of levels z0-3 .. z1+2, written by the previous stage's memory kernels of
+// chunks k-1, k, k+1 (it waits for ev_mem[k+1], stream2 runs in order); it
+// writes W and the PML fields of chunk k only, which the previous stage's
+// memory kernel of chunk k has finished reading (same wait).  The memory
+// kernel of chunk k writes F_out, DF, eta of chunk k after the stencil of
+// chunk k (ev_sten[k]); F_out is the buffer this stage's stencils do not read.
+template <typename T, typename S>
+static int stage_pipelined(wql_block* b, const StepArgs<T, S>& a) {
+  const int K = b->nchunks, nz = b->nz;
+  for (int k = 0; k < K; ++k) {
+    const int z0 = (int)((long long)nz * k / K), z1 = (int)((long long)nz * (k + 1) / K);
+    HIP_TRY(hipStreamWaitEvent(b->stream, b->ev_mem[k + 1 < K ? k + 1 : K - 1], 0));
+    HIP_TRY(launch_stage_part_z<T>(a, b->d.mesh, StagePart::kInterior, z0, z1, b->stream));
+    HIP_TRY(launch_stage_part_z<T>(a, b->d.mesh, StagePart::kShell, z0, z1, b->stream));
+    HIP_TRY(hipEventRecord(b->ev_sten[k], b->stream));
+    HIP_TRY(hipStreamWaitEvent(b->stream2, b->ev_sten[k], 0));
+    HIP_TRY(launch_stage_part_z<T>(a, b->d.mesh, StagePart::kMemory, z0, z1, b->stream2));
+    HIP_TRY(hipEventRecord(b->ev_mem[k], b->stream2));
+  }
+  b->pending = 1;
+  return WQL_OK;
+}
+
 template <typename T, typename S = double>
-static int stage_fused_t(wql_block* b, double A, double Bdt, double t) {
+static int stage_fused_t(wql_block* b, double A, double Bdt, double t, bool keep_pending = false) {
   StepArgs<T, S> a = make_args<T, S>(b, t);
   a.A = A; a.Bdt = Bdt;
   if (!b->timing) {
+    if (b->nchunks > 1 && stage_has_memory_part(a)) {
+      if (int rc = stage_pipelined<T, S>(b, a)) return rc;
+      return keep_pending ? WQL_OK : join(b);
+    }
+    if (int rc = join(b)) return rc;
     HIP_TRY(launch_stage_fused<T>(a, b->d.mesh, b->stream));
     return WQL_OK;
   }
+  if (int rc = join(b)) return rc;
   hipEvent_t e[4];
   for (int k = 0; k < 4; ++k) {
     int rc = next_event(b, &e[k]);
@@ -915,14 +1002,21 @@ static int stage_fused_t(wql_block* b, double A, double Bdt, double t) {
   return WQL_OK;
 }
 
-extern "C" int wql_stage_fused(wql_block* b, double A, double Bdt, double t) {
-  if (!b) return fail(WQL_EINVAL, "null block");
-  int rc = b->d.precision == WQL_FP64   ? stage_fused_t<double>(b, A, Bdt, t)
-           : b->d.precision == WQL_FP32_MIXED ? stage_fused_t<float>(b, A, Bdt, t)
-                                              : stage_fused_t<float, float>(b, A, Bdt, t);
+// one fused stage; keep_pending leaves the pipelined memory kernels of the
+// stage unjoined (the next stage of the same block orders itself behind them)
+static int stage_fused(wql_block* b, double A, double Bdt, double t, bool keep_pending) {
+  int rc = b->d.precision == WQL_FP64 ? stage_fused_t<double>(b, A, Bdt, t, keep_pending)
+           : b->d.precision == WQL_FP32_MIXED
+               ? stage_fused_t<float>(b, A, Bdt, t, keep_pending)
+               : stage_fused_t<float, float>(b, A, Bdt, t, keep_pending);
   if (rc) return rc;
   b->cur ^= 1;
   return WQL_OK;
+}
+
+extern "C" int wql_stage_fused(wql_block* b, double A, double Bdt, double t) {
+  if (!b) return fail(WQL_EINVAL, "null block");
+  return stage_fused(b, A, Bdt, t, false);
 }
 
 extern "C" int wql_block_set_timing(wql_block* b, int32_t enable) {
@@ -955,16 +1049,17 @@ extern "C" int wql_time_step(wql_block* b, double dt, int32_t nsteps, double* t)
   for (int n = 0; n < nsteps; ++n) {
     const double t0 = *t;
     for (int s = 0; s < 5; ++s) {
-      if (s == 0) {
-        int rc = record(b);  // receivers sample F at t0 (time_step.f90:137)
+      if (s == 0 && b->nrec && b->slot < b->nslots) {
+        int rc = join(b);
+        if (!rc) rc = record(b);  // receivers sample F at t0 (time_step.f90:137)
         if (rc) return rc;
       }
-      int rc = wql_stage_fused(b, kRkA[s], kRkB[s] * dt, *t);
+      int rc = stage_fused(b, kRkA[s], kRkB[s] * dt, *t, true);
       if (rc) return rc;
       *t = t0 + kRkC[s + 1] * dt;
     }
   }
-  return WQL_OK;
+  return join(b);
 }
 
 // ---------------------------------------------------------------------------

@@ -27,6 +27,9 @@
 namespace wql {
 
 constexpr int kMaxSources = 8;
+// z-chunk pipeline of the anelastic fused stage: chunks per stage
+constexpr int kMaxChunks = 16;
+constexpr int kDefaultChunks = 1;
 // plane stride granularity in nodes (see the device layout above)
 constexpr long long kPlaneAlign = 256;
 
@@ -94,6 +97,11 @@ struct StepArgs {
   int nbox;
   int box[6][6];
   long long box_start[7];
+  // flat node range [node0, node1) of k_memory_update (0 and P: every node)
+  // and its nodes per thread (1, or 2 with vector accesses when node0 and
+  // node1 are even)
+  long long node0, node1;
+  int mem_npt;
 };
 
 // Launchers (wql3d_kernels.hip).  All return hipError_t of the launch.
@@ -108,6 +116,13 @@ template <typename T, typename S>
 hipError_t launch_stage_fused(const StepArgs<T, S>& a, int mesh, hipStream_t s);
 template <typename T, typename S>
 hipError_t launch_stage_part(const StepArgs<T, S>& a, int mesh, StagePart part, hipStream_t s);
+// the same part restricted to the z levels [z0, z1) of the block
+template <typename T, typename S>
+hipError_t launch_stage_part_z(const StepArgs<T, S>& a, int mesh, StagePart part, int z0, int z1,
+                               hipStream_t s);
+// true when the stage runs as stencil kernels + k_memory_update (W hand-off)
+template <typename T, typename S>
+bool stage_has_memory_part(const StepArgs<T, S>& a);
 hipError_t launch_rhs(const StepArgs<double>& a, int mesh, RhsPart part, hipStream_t s);
 hipError_t launch_sat(const StepArgs<double>& a, hipStream_t s);
 hipError_t launch_source(const StepArgs<double>& a, int mesh, hipStream_t s);

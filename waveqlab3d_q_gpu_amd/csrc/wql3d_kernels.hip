@@ -395,9 +395,9 @@ __device__ __forceinline__ void memory_update_node(const StepArgs<T, S>& a, unsi
 
 template <typename T, int NMECH, bool NT, typename S>
 __global__ __launch_bounds__(256) void k_memory_update(const StepArgs<T, S> a) {
-  const unsigned id = blockIdx.x * 256u + threadIdx.x;
+  const unsigned id = (unsigned)a.node0 + blockIdx.x * 256u + threadIdx.x;
   const long long P = a.P;
-  if (id >= (unsigned long long)P) return;
+  if (id >= (unsigned long long)a.node1) return;
   T DU[9], e[6];
 #pragma unroll
   for (int c = 0; c < 9; ++c) DU[c] = ld_stream<NT>(atp(a.W + c * P, id));
@@ -410,6 +410,155 @@ __global__ __launch_bounds__(256) void k_memory_update(const StepArgs<T, S> a) {
   for (int c = 0; c < 9; ++c) {
     const double df = scale_rate(a.A, ld_stream<NT>(atp(a.DF + c * P, id))) + (double)DU[c];
     rk_finish<NT>(a, c, id, df);
+  }
+}
+
+// The same update with N consecutive nodes per thread (N = 2): every plane
+// access is one N-wide vector load or store, so a wavefront moves twice the
+// bytes per memory instruction; the arithmetic per node is unchanged (same
+// operations in the same order as memory_update_planes / rk_finish).
+template <int N, typename V> struct VecN;
+template <> struct VecN<2, float> { using type = float2; };
+template <> struct VecN<2, double> { using type = double2; };
+template <> struct VecN<4, float> { using type = float4; };
+template <> struct VecN<4, double> { using type = double4; };
+
+template <int N, typename V>
+__device__ __forceinline__ void ldn(const V* base, unsigned id, V (&o)[N]) {
+  using VV = typename VecN<N, V>::type;
+  const VV t = ldp(reinterpret_cast<const VV*>(base), id / N);
+  o[0] = t.x;
+  o[1] = t.y;
+  if constexpr (N == 4) {
+    o[2] = t.z;
+    o[3] = t.w;
+  }
+}
+template <int N, typename V>
+__device__ __forceinline__ void stn(V* base, unsigned id, const V (&v)[N]) {
+  using VV = typename VecN<N, V>::type;
+  VV t;
+  t.x = v[0];
+  t.y = v[1];
+  if constexpr (N == 4) {
+    t.z = v[2];
+    t.w = v[3];
+  }
+  *atp(reinterpret_cast<VV*>(base), id / N) = t;
+}
+
+template <typename T, int NMECH, int N>
+__device__ __forceinline__ void memory_update_planes_n(
+    T* __restrict__ eta, T* __restrict__ Deta, const T* __restrict__ Qsp, const T* __restrict__ Qpp,
+    long long P, int mech_skip, const double* wgt, const double* tau_inv, double A, double Bdt,
+    unsigned id, const T (&mu)[N], const T (&l2m)[N], const T (&e)[N][6], T (&DU)[N][9]) {
+  T Qs[N], Qp[N];
+  ldn<N>(Qsp, id, Qs);
+  ldn<N>(Qpp, id, Qp);
+  T tr[N], twomuQs[N], pfac[N], muQs[N];
+#pragma unroll
+  for (int n = 0; n < N; ++n) {
+    tr[n] = e[n][0] + e[n][1] + e[n][2];
+    twomuQs[n] = T(2) * mu[n] * Qs[n];
+    pfac[n] = l2m[n] * Qp[n] - twomuQs[n];
+    muQs[n] = mu[n] * Qs[n];
+  }
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    T ev[NMECH][N], dv[NMECH][N];
+    T sum[N];
+#pragma unroll
+    for (int n = 0; n < N; ++n) sum[n] = T(0);
+#pragma unroll
+    for (int i = 0; i < NMECH; ++i) {
+      if (mech_skip & (1 << i)) {
+#pragma unroll
+        for (int n = 0; n < N; ++n) { ev[i][n] = T(0); dv[i][n] = T(0); }
+        continue;
+      }
+      ldn<N>(eta + (long long)(c * NMECH + i) * P, id, ev[i]);
+      ldn<N>(Deta + (long long)(c * NMECH + i) * P, id, dv[i]);
+#pragma unroll
+      for (int n = 0; n < N; ++n) sum[n] += ev[i][n];
+    }
+#pragma unroll
+    for (int n = 0; n < N; ++n) DU[n][3 + c] -= sum[n];
+#pragma unroll
+    for (int i = 0; i < NMECH; ++i) {
+      if (mech_skip & (1 << i)) continue;
+      const T w = (T)wgt[i];
+      const T ti = (T)tau_inv[i];
+      T dn[N], en[N];
+#pragma unroll
+      for (int n = 0; n < N; ++n) {
+        const T src = c < 3 ? (w * twomuQs[n]) * e[n][c] + (w * pfac[n]) * tr[n]
+                            : (w * muQs[n]) * e[n][c];
+        const T r = (src - ev[i][n]) * ti;
+        dn[n] = scale_rate(A, dv[i][n]) + r;
+        en[n] = ev[i][n] + (T)Bdt * dn[n];
+      }
+      stn<N>(Deta + (long long)(c * NMECH + i) * P, id, dn);
+      stn<N>(eta + (long long)(c * NMECH + i) * P, id, en);
+    }
+  }
+}
+
+template <typename T, int NMECH, int N, typename S>
+__global__ __launch_bounds__(256) void k_memory_update_n(const StepArgs<T, S> a) {
+  const unsigned id = (unsigned)a.node0 + (blockIdx.x * 256u + threadIdx.x) * (unsigned)N;
+  const long long P = a.P;
+  if (id >= (unsigned long long)a.node1) return;
+  T DU[N][9], e[N][6];
+#pragma unroll
+  for (int c = 0; c < 9; ++c) {
+    T t[N];
+    ldn<N>(a.W + c * P, id, t);
+#pragma unroll
+    for (int n = 0; n < N; ++n) DU[n][c] = t[n];
+  }
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    T t[N];
+    ldn<N>(a.W + (9 + c) * P, id, t);
+#pragma unroll
+    for (int n = 0; n < N; ++n) e[n][c] = t[n];
+  }
+  T lam[N], mu[N], l2m[N];
+  ldn<N>(a.M, id, lam);
+  ldn<N>(a.M + P, id, mu);
+#pragma unroll
+  for (int n = 0; n < N; ++n) l2m[n] = lam[n] + T(2) * mu[n];
+  memory_update_planes_n<T, NMECH, N>(a.eta, a.Deta, a.Qs, a.Qp, P, a.mech_skip, a.wgt,
+                                      a.tau_inv, a.A, a.Bdt, id, mu, l2m, e, DU);
+#pragma unroll
+  for (int c = 0; c < 9; ++c) {
+    S dfo[N], fin[N];
+    ldn<N>(a.DF + c * P, id, dfo);
+    ldn<N>(a.Fin + c * P, id, fin);
+    if constexpr (std::is_same<S, double>::value) {
+      double df[N], fo[N];
+#pragma unroll
+      for (int n = 0; n < N; ++n) {
+        df[n] = scale_rate(a.A, dfo[n]) + (double)DU[n][c];
+        fo[n] = fin[n] + a.Bdt * df[n];
+      }
+      stn<N>(a.DF + c * P, id, df);
+      stn<N>(a.Fout + c * P, id, fo);
+    } else {
+      float d[N], cn[N], fo[N], cpv[N];
+      ldn<N>(a.Fc + c * P, id, cpv);
+#pragma unroll
+      for (int n = 0; n < N; ++n) {
+        d[n] = (float)(scale_rate(a.A, dfo[n]) + (double)DU[n][c]);
+        const float y = (float)a.Bdt * d[n] - cpv[n];
+        const float t = fin[n] + y;
+        cn[n] = (t - fin[n]) - y;
+        fo[n] = t;
+      }
+      stn<N>(a.DF + c * P, id, d);
+      stn<N>(a.Fc + c * P, id, cn);
+      stn<N>(a.Fout + c * P, id, fo);
+    }
   }
 }
 
@@ -906,7 +1055,22 @@ static hipError_t launch_node(const StepArgs<T, S>& a, int mesh, hipStream_t s) 
 
 template <typename T, bool NT, typename S>
 static hipError_t launch_memory_update(const StepArgs<T, S>& a, hipStream_t s) {
-  const dim3 g((unsigned)((a.P + 255) / 256));
+  if (a.node1 <= a.node0) return hipSuccess;
+  if (!NT && a.mem_npt == 4 && a.node0 % 4 == 0 && a.node1 % 4 == 0) {
+    const dim3 g((unsigned)((a.node1 - a.node0 + 1023) / 1024));
+    if (a.nmech == 4) hipLaunchKernelGGL((k_memory_update_n<T, 4, 4, S>), g, dim3(256), 0, s, a);
+    else if (a.nmech == 8) hipLaunchKernelGGL((k_memory_update_n<T, 8, 4, S>), g, dim3(256), 0, s, a);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+  }
+  if (!NT && a.mem_npt >= 2 && a.node0 % 2 == 0 && a.node1 % 2 == 0) {
+    const dim3 g((unsigned)((a.node1 - a.node0 + 511) / 512));
+    if (a.nmech == 4) hipLaunchKernelGGL((k_memory_update_n<T, 4, 2, S>), g, dim3(256), 0, s, a);
+    else if (a.nmech == 8) hipLaunchKernelGGL((k_memory_update_n<T, 8, 2, S>), g, dim3(256), 0, s, a);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+  }
+  const dim3 g((unsigned)((a.node1 - a.node0 + 255) / 256));
   if (a.nmech == 4) hipLaunchKernelGGL((k_memory_update<T, 4, NT, S>), g, dim3(256), 0, s, a);
   else if (a.nmech == 8) hipLaunchKernelGGL((k_memory_update<T, 8, NT, S>), g, dim3(256), 0, s, a);
   else return hipErrorInvalidValue;
@@ -994,6 +1158,69 @@ hipError_t launch_stage_part(const StepArgs<T, S>& a, int mesh, StagePart part, 
 }
 
 template <typename T, typename S>
+bool stage_has_memory_part(const StepArgs<T, S>& a) {
+  return split_stage(a);
+}
+
+// One part of the stage over the z levels [z0, z1): the interior box and the
+// shell boxes cut to those levels, or the flat node range of those levels
+// (the last range also covers the plane padding).  The stencil parts read F
+// of levels z0-3 .. z1+2; every part writes only nodes of its levels.
+template <typename T, typename S>
+hipError_t launch_stage_part_z(const StepArgs<T, S>& a, int mesh, StagePart part, int z0, int z1,
+                               hipStream_t s) {
+  if (z0 >= z1) return hipSuccess;
+  switch (part) {
+    case StagePart::kInterior: {
+      if (!interior_on(a)) return hipSuccess;
+      StepArgs<T, S> b = a;
+      b.iz0 = a.iz0 > z0 ? a.iz0 : z0;
+      b.iz1 = a.iz1 < z1 - 1 ? a.iz1 : z1 - 1;
+      if (b.iz0 > b.iz1) return hipSuccess;
+      return launch_interior<T>(b, mesh, split_stage(a), s);
+    }
+    case StagePart::kShell: {
+      StepArgs<T, S> sh = a;
+      if (interior_on(a)) {
+        set_shell_boxes(sh);
+      } else {
+        sh.nbox = 1;
+        const int v[6] = {0, a.nx - 1, 0, a.ny - 1, 0, a.nz - 1};
+        for (int k = 0; k < 6; ++k) sh.box[0][k] = v[k];
+      }
+      // cut every box to [z0, z1 - 1]
+      StepArgs<T, S> cut = sh;
+      int nb = 0;
+      cut.box_start[0] = 0;
+      for (int i = 0; i < sh.nbox; ++i) {
+        int bz0 = sh.box[i][4] > z0 ? sh.box[i][4] : z0;
+        int bz1 = sh.box[i][5] < z1 - 1 ? sh.box[i][5] : z1 - 1;
+        if (bz0 > bz1) continue;
+        for (int k = 0; k < 4; ++k) cut.box[nb][k] = sh.box[i][k];
+        cut.box[nb][4] = bz0;
+        cut.box[nb][5] = bz1;
+        cut.box_start[nb + 1] = cut.box_start[nb] +
+                                (long long)(sh.box[i][1] - sh.box[i][0] + 1) *
+                                    (sh.box[i][3] - sh.box[i][2] + 1) * (long long)(bz1 - bz0 + 1);
+        ++nb;
+      }
+      if (nb == 0) return hipSuccess;
+      cut.nbox = nb;
+      return split_stage(a) ? launch_node<T, 3>(cut, mesh, s) : launch_node<T, 0>(cut, mesh, s);
+    }
+    default: {
+      if (!split_stage(a)) return hipSuccess;
+      StepArgs<T, S> b = a;
+      const long long plane = (long long)a.nx * a.ny;
+      b.node0 = z0 * plane;
+      b.node1 = z1 >= a.nz ? a.P : z1 * plane;
+      return a.nontemporal ? launch_memory_update<T, true>(b, s)
+                           : launch_memory_update<T, false>(b, s);
+    }
+  }
+}
+
+template <typename T, typename S>
 hipError_t launch_stage_fused(const StepArgs<T, S>& a, int mesh, hipStream_t s) {
   for (StagePart p : {StagePart::kInterior, StagePart::kShell, StagePart::kMemory}) {
     hipError_t e = launch_stage_part<T>(a, mesh, p, s);
@@ -1008,6 +1235,15 @@ template hipError_t launch_stage_fused<float>(const StepArgs<float>&, int, hipSt
 template hipError_t launch_stage_part<float, float>(const StepArgs<float, float>&, int, StagePart,
                                                     hipStream_t);
 template hipError_t launch_stage_fused<float, float>(const StepArgs<float, float>&, int, hipStream_t);
+template hipError_t launch_stage_part_z<double>(const StepArgs<double>&, int, StagePart, int, int,
+                                                hipStream_t);
+template hipError_t launch_stage_part_z<float>(const StepArgs<float>&, int, StagePart, int, int,
+                                               hipStream_t);
+template hipError_t launch_stage_part_z<float, float>(const StepArgs<float, float>&, int, StagePart,
+                                                      int, int, hipStream_t);
+template bool stage_has_memory_part<double>(const StepArgs<double>&);
+template bool stage_has_memory_part<float>(const StepArgs<float>&);
+template bool stage_has_memory_part<float, float>(const StepArgs<float, float>&);
 
 hipError_t launch_rhs(const StepArgs<double>& a, int mesh, RhsPart part, hipStream_t s) {
   return part == RhsPart::kAll ? launch_node<double, 1>(a, mesh, s)
