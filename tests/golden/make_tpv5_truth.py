@@ -1,7 +1,8 @@
 """Writes tpv5_cart_truth.npz: the reference's archived TPV5 interface
 outputs test_problems/truth/1/test_rup_cart_fric_interface.{S,Svel,Uface,
 Vface,Uhat,Vhat,trup} (float32, 46 frames of a 21 x 21 face; layout and
-checker python/read_binary.py:7-37, writer fault_output.f90:73-123).
+checker python/read_binary.py:7-37, writer fault_output.f90:73-123), and the
+station files test_rup_cart_fric_{21_11_11,1_11_11}.dat of the same run.
 
 The files are data of the reference's own test (test_problems/
 test_rup_cart_fric.in, serial_test1 / mpi_test1 in src/CMakeLists.txt);
@@ -26,6 +27,11 @@ def main(ref):
         if a.size != NT * nc * N * N:
             raise SystemExit(f"{key}: {a.size} values, expected {NT * nc * N * N}")
         out[key] = a.reshape(NT, nc, N, N)
+    # station files of the same run: block 1 node (21, 11, 11) on the fault
+    # and block 2 node (1, 11, 11), one line of t, vx, vy, vz per step
+    for label in ("21_11_11", "1_11_11"):
+        path = os.path.join(ref, "test_problems", "truth", "1", f"test_rup_cart_fric_{label}.dat")
+        out["station_" + label] = np.loadtxt(path)
     dst = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tpv5_cart_truth.npz")
     np.savez_compressed(dst, **out)
     print("wrote", dst, os.path.getsize(dst), "bytes")

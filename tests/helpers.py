@@ -74,4 +74,7 @@ def tpv5_truth():
     import os
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "tpv5_cart_truth.npz")
     with np.load(path, allow_pickle=False) as z:
-        return {k: z[k].astype(np.float64) for k, _ in TPV5_KEYS}
+        out = {k: z[k].astype(np.float64) for k, _ in TPV5_KEYS}
+        for label in ("21_11_11", "1_11_11"):
+            out["station_" + label] = z["station_" + label].astype(np.float64)
+        return out

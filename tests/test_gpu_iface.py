@@ -158,3 +158,39 @@ def test_locked_interface_kat_41(oracle):
     print(f"locked KAT block maxima {m1!r} {m2!r}", flush=True)
     assert "%.4E" % m1 == "2.0699E+03" and "%.4E" % m2 == "1.6389E+03"
     itf.close()
+
+
+def test_tpv5_output_files_match_reference_files(oracle, tmp_path):
+    """The two-block run on the GPU, written in the reference's formats
+    (output.py), against test_problems/truth/1: the fault files as
+    python/read_binary.py compares them (max |diff| <= 1e-3) and the station
+    files of the fault nodes (21, 11, 11) of block 1 and (1, 11, 11) of block 2
+    (6 printed decimals)."""
+    from waveqlab3d_q_gpu_amd import output as OUT
+    tp = PB.tpv5_cartesian(21, sat_form=L.WQL_SAT_ALPHA_THETA)
+    bl, br, itf = _gpu_pair(oracle, tp)
+    bl.set_receivers([[21, 11, 11]], tp.nsteps)
+    br.set_receivers([[1, 11, 11]], tp.nsteps)
+    itf.set_frames(tp.nsteps)
+    itf.time_step(tp.dt, tp.nsteps, 0.0)
+    name = str(tmp_path / "test_rup_cart_fric")
+    OUT.write_fault_files(str(tmp_path), "test_rup_cart_fric", itf.frames())
+    t = np.arange(tp.nsteps) * tp.dt
+    p1 = OUT.write_station_files(str(tmp_path), "test_rup_cart_fric", 1, [[21, 11, 11]], t,
+                                 bl.seismograms())
+    p2 = OUT.write_station_files(str(tmp_path), "test_rup_cart_fric", 2, [[1, 11, 11]], t,
+                                 br.seismograms())
+    truth = tpv5_truth()
+    for key, nc in (("S", 4), ("Svel", 3), ("Uface", 9), ("Vface", 9), ("Uhat", 6),
+                    ("Vhat", 6)):
+        got = OUT.read_fault_file(f"{name}_interface.{key}", nc, 21, 21)
+        d = float(np.abs(got - truth[key]).max())
+        print(f"file {key}: max|diff| {d:.3e}", flush=True)
+        assert d <= 1.0e-3, (key, d)
+    for path, label in ((p1[0], "21_11_11"), (p2[0], "1_11_11")):
+        st = OUT.read_station_file(path)
+        ref = truth["station_" + label]
+        d = float(np.abs(st - ref).max())
+        print(f"station {label}: max|diff| {d:.2e}", flush=True)
+        assert d <= 6e-7, (label, d)
+    itf.close()

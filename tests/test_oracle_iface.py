@@ -108,3 +108,20 @@ def test_locked_interface_kat_41(oracle):
     oracle.two_block_time_step(Lb, Rb, I, tp.dt, tp.nsteps, 0.0)
     assert "%.4E" % np.abs(Lb.F).max() == "2.0699E+03"
     assert "%.4E" % np.abs(Rb.F).max() == "1.6389E+03"
+
+
+def test_tpv5_station_files(oracle):
+    """test_problems/truth/1/test_rup_cart_fric_{21_11_11,1_11_11}.dat (t, vx,
+    vy, vz with 6 decimals) are the particle velocities of the fault nodes
+    (21, 11, 11) of block 1 and (1, 11, 11) of block 2: the oracle's face
+    fields at y = z = 11 reproduce them to the printed digits."""
+    tp = PB.tpv5_cartesian(21, sat_form=1)
+    Lb, Rb, I = oracle_two_block(oracle, tp)
+    _, fr = oracle.two_block_time_step(Lb, Rb, I, tp.dt, tp.nsteps, 0.0)
+    truth = tpv5_truth()
+    t = np.arange(tp.nsteps) * tp.dt
+    for key, label in (("Uface", "21_11_11"), ("Vface", "1_11_11")):
+        st = truth["station_" + label]
+        assert np.abs(st[:, 0] - t).max() <= 6e-7
+        v = fr[key][:, 0:3, 10, 10]
+        assert np.abs(v - st[:, 1:4]).max() <= 6e-7, (label, np.abs(v - st[:, 1:4]).max())
