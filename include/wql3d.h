@@ -221,16 +221,31 @@ int wql_block_kernel_times(wql_block *b, double ms[3], int32_t *stages);
  * iface.f90:58-85).  The stage-1 fault output (write_fault_output /
  * write_hat_output, domain.f90:782-847) is recorded into device slots.
  * FP64 and FP32_MIXED blocks (the RK state is FP64 in both). */
-enum { WQL_IFACE_LOCKED = 0, WQL_IFACE_SLIP_WEAKENING_TPV5 = 1 };
+/* Slip-weakening friction is evaluated from per-node tables: the prestress
+ * (T0_n, T0_m, T0_l) in each block's face frame (prestress(), Interface_Condition.f90:
+ * 1802-2440) and the parameters of the linear slip-weakening strength
+ * tau_str = sigma_n*(fs - (fs - fd)*min(D, Dc)/Dc) (Tau_strength, :1268-1797),
+ * fs including the problem's high-strength barrier.  The TPV5 and TPV28 tables
+ * are built at wql_iface_create (TPV5 :1283-1299, 1821-1830; TPV28 :1459-1475,
+ * 2049-2079 with initial_stress_condition.f90:78-84); WQL_IFACE_SLIP_WEAKENING
+ * takes them from the caller (e.g. filled by the reference's own prestress()). */
+enum { WQL_IFACE_LOCKED = 0, WQL_IFACE_SLIP_WEAKENING_TPV5 = 1,
+       WQL_IFACE_SLIP_WEAKENING_TPV28 = 2, WQL_IFACE_SLIP_WEAKENING = 3 };
 typedef struct wql_iface_desc {
   int32_t coupling;          /* WQL_IFACE_*: coupling 'locked' / 'slip-weakening_friction'
-                                with problem 'TPV5' */
-  int32_t arrays_on_device;  /* 0: y, z are host pointers, 1: device pointers */
+                                with problem 'TPV5' / 'TPV28' / caller tables */
+  int32_t arrays_on_device;  /* 0: the arrays below are host pointers, 1: device pointers */
   const double *y, *z;       /* coordinates of the interface nodes, [nz][ny] (y fastest):
                                 the y and z of block L's x-high face (G%X(pq, :, :, 2:3)) */
+  const double *s0;          /* TPV28 only: initial stress (xx, yy, zz, xy, xz, yz), host
+                                array of 6; null = (-60, 0, 60, 0, 29.38, 0) */
+  const double *T0L, *T0R;   /* WQL_IFACE_SLIP_WEAKENING only: prestress [3][nz][ny] in the
+                                (n, m, l) frame of block L's face / block R's face */
+  const double *fs, *fd, *Dc;  /* WQL_IFACE_SLIP_WEAKENING only: [nz][ny] each */
 } wql_iface_desc;
 typedef struct wql_iface wql_iface;
-/* init_iface (iface.f90:13-55): S = 0, DS = 1e40, rupture time 1e9 */
+/* init_iface (iface.f90:13-55): S = 0, DS = 1e40, rupture time 1e9; for
+ * TPV28 the blocks' face frames must be uploaded (WQL_FACES) before */
 int wql_iface_create(wql_block *L, wql_block *R, const wql_iface_desc *desc, wql_iface **out);
 void wql_iface_destroy(wql_iface *I);
 /* one RK stage of both blocks and the interface; stage1 != 0 records the fault

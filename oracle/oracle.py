@@ -61,11 +61,11 @@ class _Block(C.Structure):
 
 class _Iface(C.Structure):
     _fields_ = [("ny", C.c_int), ("nz", C.c_int), ("coupling", C.c_int),
-                ("yf", _P), ("zf", _P), ("S", _P), ("DS", _P), ("W", _P), ("DW", _P),
+                ("s0", C.c_double * 6), ("yf", _P), ("zf", _P), ("S", _P), ("DS", _P), ("W", _P), ("DW", _P),
                 ("Uhat", _P), ("Vhat", _P), ("Svel", _P), ("trup", _P)]
 
 
-IFACE_LOCKED, IFACE_SLIP_WEAKENING_TPV5 = 0, 1
+IFACE_LOCKED, IFACE_SLIP_WEAKENING_TPV5, IFACE_SLIP_WEAKENING_TPV28 = 0, 1, 2
 
 _lib = None
 
@@ -277,8 +277,12 @@ class OracleIface:
     of block R (wql_oracle_iface.c).  yf, zf: coordinates of the face nodes
     (nz, ny)."""
 
-    def __init__(self, ny, nz, yf, zf, coupling=IFACE_SLIP_WEAKENING_TPV5):
+    # TPV28 initial stress of the source (initial_stress_condition.f90:78-84)
+    TPV28_STRESS = (-60.0, 0.0, 60.0, 0.0, 29.38, 0.0)
+
+    def __init__(self, ny, nz, yf, zf, coupling=IFACE_SLIP_WEAKENING_TPV5, s0=None):
         self.ny, self.nz, self.coupling = ny, nz, coupling
+        self.s0 = tuple(self.TPV28_STRESS if s0 is None else s0)
         shp = (nz, ny)
         self.yf = np.ascontiguousarray(yf, dtype=np.float64).reshape(shp)
         self.zf = np.ascontiguousarray(zf, dtype=np.float64).reshape(shp)
@@ -294,6 +298,7 @@ class OracleIface:
     def _sync(self):
         s = _Iface()
         s.ny, s.nz, s.coupling = self.ny, self.nz, self.coupling
+        s.s0[:] = list(self.s0)
         for name in ("yf", "zf", "S", "DS", "W", "DW", "Uhat", "Vhat", "Svel", "trup"):
             setattr(s, name, _ptr(getattr(self, name)))
         self._struct = s

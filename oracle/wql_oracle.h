@@ -103,7 +103,7 @@ void orc_time_step(orc_block *b, double dt, int nsteps, double *t,
 double orc_source_time(int type, double t, double t0, double w);
 
 /* ---- two-block q-interface (wql_oracle_iface.c) ------------------------ */
-enum { ORC_IFACE_LOCKED = 0, ORC_IFACE_SLIP_WEAKENING_TPV5 = 1 };
+enum { ORC_IFACE_LOCKED = 0, ORC_IFACE_SLIP_WEAKENING_TPV5 = 1, ORC_IFACE_SLIP_WEAKENING_TPV28 = 2 };
 
 /* Interface between the x-high face of block L and the x-low face of block
  * R (same ny, nz).  Face arrays [k][NF], fidx = (y-1) + ny*(z-1).
@@ -115,6 +115,10 @@ enum { ORC_IFACE_LOCKED = 0, ORC_IFACE_SLIP_WEAKENING_TPV5 = 1 };
 typedef struct orc_iface {
   int ny, nz;
   int coupling;
+  /* TPV28 initial stress (xx, yy, zz, xy, xz, yz): the source's values are
+   * (-60, 0, 60, 0, 29.38, 0), initial_stress_condition.f90:78-84; the
+   * archived golden files test_problems/truth/2 were made with zz = -60 */
+  double s0[6];
   double *yf, *zf;
   double *S, *DS, *W, *DW;
   double *Uhat, *Vhat, *Svel, *trup;

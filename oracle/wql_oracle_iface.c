@@ -15,8 +15,9 @@
  *     SAT forcing u_bc / v_bc;
  *   Interface_RHS (Interface_Condition.f90:12-513): hats by Welded_Interface
  *     (:515-590) or Slip_Weakening_Friction (:745-916) with solve (:1095-1168),
- *     Tau_strength TPV5 (:1283-1299), prestress TPV5 (:1802-1830), boxcar
- *     (:2438-2457); DS(1:3) += Q - P, DS(4) += |slip rate|;
+ *     Tau_strength TPV5 / TPV28 (:1283-1299, 1459-1475), prestress TPV5 /
+ *     TPV28 (:1802-1830, 2049-2079), boxcar (:2438-2457); DS(1:3) += Q - P,
+ *     DS(4) += |slip rate|;
  *   rupture_front (CouplingForcing.f90:580-600);
  *   scale_rates_iface / update_fields_iface (iface.f90:58-85).
  */
@@ -44,20 +45,62 @@ static void prestress_tpv5(double y, double z, double *T0n, double *T0m, double 
   *T0l = l;
 }
 
-/* Tau_strength, problem TPV5 (Interface_Condition.f90:1283-1299) */
-static double tau_strength_tpv5(double sigma_n, double D, double y, double z) {
-  double fs = 0.677;
-  const double fd = 0.525, Dc = 0.4;
-  fs = fs + 1.0e10 * (1.0 - boxcar(y, 15.0) * boxcar(z, 15.0));
+/* prestress, problem TPV28 (Interface_Condition.f90:2049-2079): the traction
+ * of the uniform initial stress tensor s0 = (xx, yy, zz, xy, xz, yz)
+ * (initial_stress_condition.f90:78-84) on the face, in (n, m, l), plus the
+ * nucleation stress on T0_l */
+static void prestress_tpv28(const double *s0, const double *n, const double *m, const double *l,
+                            double y, double z, double *T0n, double *T0m, double *T0l) {
+  const double pi = 3.141592653589793;
+  const double sxx = s0[0], syy = s0[1], szz = s0[2], sxy = s0[3], sxz = s0[4], syz = s0[5];
+  const double Tx = n[0] * sxx + n[1] * sxy + n[2] * sxz;
+  const double Ty = n[0] * sxy + n[1] * syy + n[2] * syz;
+  const double Tz = n[0] * sxz + n[1] * syz + n[2] * szz;
+  *T0n = n[0] * Tx + n[1] * Ty + n[2] * Tz;
+  *T0m = m[0] * Tx + m[1] * Ty + m[2] * Tz;
+  *T0l = l[0] * Tx + l[1] * Ty + l[2] * Tz;
+  const double r = sqrt((y - 7.5) * (y - 7.5) + z * z);
+  double tau_nuke;
+  if (r <= 1.4)
+    tau_nuke = 11.60;
+  else if (r >= 1.4 && r <= 2.0)
+    tau_nuke = 5.8 * (1.0 + cos(pi * (r - 1.4) / 0.6));
+  else
+    tau_nuke = 0.0;
+  *T0l = *T0l + tau_nuke;
+}
+
+/* prestress of the slip-weakening problems in (n, m, l) */
+static void prestress(const orc_iface *I, const double *n, const double *m, const double *l,
+                      double y, double z, double T0[3]) {
+  if (I->coupling == ORC_IFACE_SLIP_WEAKENING_TPV28)
+    prestress_tpv28(I->s0, n, m, l, y, z, &T0[0], &T0[1], &T0[2]);
+  else
+    prestress_tpv5(y, z, &T0[0], &T0[1], &T0[2]);
+}
+
+/* Tau_strength, problems TPV5 (Interface_Condition.f90:1283-1299) and TPV28
+ * (:1459-1475): linear slip weakening with a high-strength barrier outside
+ * the rupture area */
+static double tau_strength(int coupling, double sigma_n, double D, double y, double z) {
+  double fs = 0.677, fd, Dc = 0.4;
+  if (coupling == ORC_IFACE_SLIP_WEAKENING_TPV28) {
+    fd = 0.373;
+    fs = fs + 1.0e10 * (1.0 - boxcar(y, 15.0) * boxcar(z, 18.0));
+  } else {
+    fd = 0.525;
+    fs = fs + 1.0e10 * (1.0 - boxcar(y, 15.0) * boxcar(z, 15.0));
+  }
   const double f = fs - (fs - fd) * fmin(D, Dc) / Dc;
   return sigma_n * f;
 }
 
 /* hats P (left) and Q (right) in local (n, m, l) coordinates from the rotated
  * grid values U, V; returns the slip-rate magnitude of the friction solve */
-static void hats(int coupling, const double U[6], const double V[6], double Z1s, double Z2s,
-                 double Z1p, double Z2p, double slip, double y, double z, double P[6],
-                 double Q[6]) {
+static void hats(const orc_iface *I, const double *n, const double *m, const double *l,
+                 const double U[6], const double V[6], double Z1s, double Z2s, double Z1p,
+                 double Z2p, double slip, double y, double z, double P[6], double Q[6]) {
+  const int coupling = I->coupling;
   const double p_n = Z1p * U[0] - U[3], p_m = Z1s * U[1] - U[4], p_l = Z1s * U[2] - U[5];
   const double q_n = Z2p * V[0] + V[3], q_m = Z2s * V[1] + V[4], q_l = Z2s * V[2] + V[5];
   if (coupling == ORC_IFACE_LOCKED) {
@@ -78,8 +121,9 @@ static void hats(int coupling, const double U[6], const double V[6], double Z1s,
   /* Slip_Weakening_Friction (Interface_Condition.f90:745-916) */
   const double eta_s = 1.0 / (1.0 / Z1s + 1.0 / Z2s);
   const double eta_p = 1.0 / (1.0 / Z1p + 1.0 / Z2p);
-  double T0n, T0m, T0l;
-  prestress_tpv5(y, z, &T0n, &T0m, &T0l);
+  double T0[3];
+  prestress(I, n, m, l, y, z, T0);
+  const double T0n = T0[0], T0m = T0[1], T0l = T0[2];
   const double phi_n = eta_p * (q_n / Z2p - p_n / Z1p);
   const double phi_m = eta_s * (q_m / Z2s - p_m / Z1s);
   const double phi_l = eta_s * (q_l / Z2s - p_l / Z1s);
@@ -87,7 +131,7 @@ static void hats(int coupling, const double U[6], const double V[6], double Z1s,
   const double v_n_hat = (p_n + phi_n) / Z1p, Tv_n_hat = phi_n;
   const double tau_lock = sqrt((T0l + phi_l) * (T0l + phi_l) + (T0m + phi_m) * (T0m + phi_m));
   const double sigma_n = fmax(0.0, -(T0n + phi_n));
-  const double tau_str = tau_strength_tpv5(sigma_n, slip, y, z);
+  const double tau_str = tau_strength(coupling, sigma_n, slip, y, z);
   double Tu_m_hat, Tu_l_hat, Tv_m_hat, Tv_l_hat, vv_m, vv_l;
   if (tau_lock > tau_str) {
     /* solve, slip-weakening branch (Interface_Condition.f90:1143-1160) */
@@ -196,8 +240,8 @@ static void eval_side(orc_iface *I, orc_block *L, orc_block *R, int ib, double t
       const double c2p = sqrt((2.0 * mu2 + lam2) / rho2), c2s = sqrt(mu2 / rho2);
       const double Z1s = rho1 * c1s, Z2s = rho2 * c2s, Z1p = rho1 * c1p, Z2p = rho2 * c2p;
       double P[6], Q[6];
-      hats(I->coupling, U, V, Z1s, Z2s, Z1p, Z2p, I->S[3 * NF + fi], I->yf[fi], I->zf[fi], P,
-           Q);
+      hats(I, n, m, l, U, V, Z1s, Z2s, Z1p, Z2p, I->S[3 * NF + fi], I->yf[fi],
+           I->zf[fi], P, Q);
       const double slip_rate = sqrt((Q[1] - P[1]) * (Q[1] - P[1]) + (Q[2] - P[2]) * (Q[2] - P[2]));
       /* the interface state belongs to block L's side: one DS increment per
        * stage, as when each block keeps its own interface copy */
@@ -206,9 +250,8 @@ static void eval_side(orc_iface *I, orc_block *L, orc_block *R, int ib, double t
         I->DS[3 * NF + fi] = I->DS[3 * NF + fi] + slip_rate;
       }
       if (stage == 1) {
-        double T0n = 0.0, T0m = 0.0, T0l = 0.0;
-        if (I->coupling != ORC_IFACE_LOCKED) prestress_tpv5(I->yf[fi], I->zf[fi], &T0n, &T0m, &T0l);
-        const double T0[3] = {T0n, T0m, T0l};
+        double T0[3] = {0.0, 0.0, 0.0};
+        if (I->coupling != ORC_IFACE_LOCKED) prestress(I, n, m, l, I->yf[fi], I->zf[fi], T0);
         for (int c = 0; c < 3; ++c) {
           I->Uhat[(3 + c) * NF + fi] = P[3 + c] + T0[c];
           I->Vhat[(3 + c) * NF + fi] = Q[3 + c] + T0[c];

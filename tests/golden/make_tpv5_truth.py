@@ -1,12 +1,14 @@
-"""Writes tpv5_cart_truth.npz: the reference's archived TPV5 interface
-outputs test_problems/truth/1/test_rup_cart_fric_interface.{S,Svel,Uface,
-Vface,Uhat,Vhat,trup} (float32, 46 frames of a 21 x 21 face; layout and
-checker python/read_binary.py:7-37, writer fault_output.f90:73-123), and the
-station files test_rup_cart_fric_{21_11_11,1_11_11}.dat of the same run.
+"""Writes tpv5_cart_truth.npz and tpv28_curv_truth.npz: the reference's
+archived two-block rupture outputs test_problems/truth/{1,2}/
+test_rup_{cart,curv}_fric_interface.{S,Svel,Uface,Vface,Uhat,Vhat,trup}
+(float32, 46 frames of a 21 x 21 face; layout and checker
+python/read_binary.py:7-37, writer fault_output.f90:73-123), and the station
+files test_rup_{cart,curv}_fric_{21_11_11,1_11_11}.dat of the same runs.
 
-The files are data of the reference's own test (test_problems/
-test_rup_cart_fric.in, serial_test1 / mpi_test1 in src/CMakeLists.txt);
-they are stored unchanged, reshaped to (frame, component, z, y).
+The files are data of the reference's own tests (test_problems/
+test_rup_{cart,curv}_fric.in; serial_test1/2, mpi_test1/2 in
+src/CMakeLists.txt); they are stored unchanged, the fault outputs reshaped to
+(frame, component, z, y).
 
     python tests/golden/make_tpv5_truth.py /root/reference
 """
@@ -17,24 +19,26 @@ import numpy as np
 
 N, NT = 21, 46
 KEYS = (("S", 4), ("Svel", 3), ("Uface", 9), ("Vface", 9), ("Uhat", 6), ("Vhat", 6), ("trup", 1))
+SETS = (("1", "test_rup_cart_fric", "tpv5_cart_truth.npz"),
+        ("2", "test_rup_curv_fric", "tpv28_curv_truth.npz"))
 
 
 def main(ref):
-    base = os.path.join(ref, "test_problems", "truth", "1", "test_rup_cart_fric_interface.")
-    out = {}
-    for key, nc in KEYS:
-        a = np.fromfile(base + key, dtype="<f4")
-        if a.size != NT * nc * N * N:
-            raise SystemExit(f"{key}: {a.size} values, expected {NT * nc * N * N}")
-        out[key] = a.reshape(NT, nc, N, N)
-    # station files of the same run: block 1 node (21, 11, 11) on the fault
-    # and block 2 node (1, 11, 11), one line of t, vx, vy, vz per step
-    for label in ("21_11_11", "1_11_11"):
-        path = os.path.join(ref, "test_problems", "truth", "1", f"test_rup_cart_fric_{label}.dat")
-        out["station_" + label] = np.loadtxt(path)
-    dst = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tpv5_cart_truth.npz")
-    np.savez_compressed(dst, **out)
-    print("wrote", dst, os.path.getsize(dst), "bytes")
+    for set_id, prefix, out_name in SETS:
+        tdir = os.path.join(ref, "test_problems", "truth", set_id)
+        out = {}
+        for key, nc in KEYS:
+            a = np.fromfile(os.path.join(tdir, f"{prefix}_interface.{key}"), dtype="<f4")
+            if a.size != NT * nc * N * N:
+                raise SystemExit(f"{key}: {a.size} values, expected {NT * nc * N * N}")
+            out[key] = a.reshape(NT, nc, N, N)
+        # station files: block 1 node (21, 11, 11) on the fault and block 2
+        # node (1, 11, 11), one line of t, vx, vy, vz per step
+        for label in ("21_11_11", "1_11_11"):
+            out["station_" + label] = np.loadtxt(os.path.join(tdir, f"{prefix}_{label}.dat"))
+        dst = os.path.join(os.path.dirname(os.path.abspath(__file__)), out_name)
+        np.savez_compressed(dst, **out)
+        print("wrote", dst, os.path.getsize(dst), "bytes")
 
 
 if __name__ == "__main__":

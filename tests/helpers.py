@@ -62,7 +62,7 @@ def oracle_two_block(O, tp):
     L_ = oracle_block(O, tp.left, O.metrics(tp.left.X))
     R_ = oracle_block(O, tp.right, O.metrics(tp.right.X))
     nz, ny = tp.yf.shape
-    I = O.OracleIface(ny, nz, tp.yf, tp.zf, coupling=tp.coupling)
+    I = O.OracleIface(ny, nz, tp.yf, tp.zf, coupling=tp.coupling, s0=tp.s0)
     return L_, R_, I
 
 
@@ -70,9 +70,11 @@ TPV5_KEYS = (("S", 4), ("Svel", 3), ("Uface", 9), ("Vface", 9), ("Uhat", 6), ("V
              ("trup", 1))
 
 
-def tpv5_truth():
+def tpv5_truth(name="tpv5_cart_truth.npz"):
+    """The reference's archived two-block outputs (tests/golden/make_tpv5_truth.py):
+    tpv5_cart_truth.npz (truth/1) or tpv28_curv_truth.npz (truth/2)."""
     import os
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "tpv5_cart_truth.npz")
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", name)
     with np.load(path, allow_pickle=False) as z:
         out = {k: z[k].astype(np.float64) for k, _ in TPV5_KEYS}
         for label in ("21_11_11", "1_11_11"):

@@ -32,7 +32,7 @@ def _runtime():
 def _gpu_pair(oracle, tp, precision=L.WQL_FP64):
     bl = gpu_block(tp.left, oracle.metrics(tp.left.X), precision=precision)
     br = gpu_block(tp.right, oracle.metrics(tp.right.X), precision=precision)
-    return bl, br, Interface(bl, br, tp.yf, tp.zf, coupling=tp.coupling)
+    return bl, br, Interface(bl, br, tp.yf, tp.zf, coupling=tp.coupling, s0=tp.s0)
 
 
 def _frame_errors(got, ref):
@@ -194,3 +194,74 @@ def test_tpv5_output_files_match_reference_files(oracle, tmp_path):
         print(f"station {label}: max|diff| {d:.2e}", flush=True)
         assert d <= 6e-7, (label, d)
     itf.close()
+
+
+TPV28_TRUTH_STRESS = (-60.0, 0.0, -60.0, 0.0, 29.38, 0.0)
+
+
+@pytest.mark.parametrize("s0", [None, TPV28_TRUTH_STRESS])
+def test_tpv28_curvilinear_fp64_matches_oracle(oracle, s0):
+    """Curvilinear TPV28 (prestress from the face frames of each side): the
+    source's initial stress and the one of the golden files."""
+    tp = PB.tpv28_curvilinear(21, sat_form=L.WQL_SAT_ALPHA_THETA, s0=s0)
+    print(f"case tpv28 curvilinear s0={s0}", flush=True)
+    Lo, Ro, Io = oracle_two_block(oracle, tp)
+    _, fr_o = oracle.two_block_time_step(Lo, Ro, Io, tp.dt, tp.nsteps, 0.0)
+    bl, br, itf = _gpu_pair(oracle, tp)
+    itf.set_frames(tp.nsteps)
+    itf.time_step(tp.dt, tp.nsteps, 0.0)
+    fr = itf.frames()
+    errs = _frame_errors(fr, fr_o)
+    print("frame errors", errs, flush=True)
+    assert max(v for k, v in errs.items() if k != "trup") <= TOL64, errs
+    assert errs["trup"] <= 1e-12, errs
+    bl.download(L.WQL_F)
+    br.download(L.WQL_F)
+    assert max(field_errors(bl.F, Lo.F)) <= TOL64 and max(field_errors(br.F, Ro.F)) <= TOL64
+    itf.close()
+
+
+def test_tpv28_matches_golden_files(oracle):
+    tp = PB.tpv28_curvilinear(21, sat_form=L.WQL_SAT_ALPHA_THETA, s0=TPV28_TRUTH_STRESS)
+    bl, br, itf = _gpu_pair(oracle, tp)
+    itf.set_frames(tp.nsteps)
+    itf.time_step(tp.dt, tp.nsteps, 0.0)
+    fr = itf.frames()
+    truth = tpv5_truth("tpv28_curv_truth.npz")
+    for key, _ in TPV5_KEYS:
+        d = float(np.abs(fr[key] - truth[key]).max())
+        scale = max(1.0, float(np.abs(truth[key][truth[key] < 1e8]).max()))
+        print(f"golden tpv28 {key}: max|diff| {d:.3e}", flush=True)
+        assert d <= 1.0e-3, (key, d)
+        assert key == "trup" or d <= 4e-7 * scale + 1e-7, (key, d)
+    itf.close()
+
+
+def test_slip_weakening_tables_from_caller(oracle):
+    """WQL_IFACE_SLIP_WEAKENING with caller tables equal to the TPV5
+    problem's gives the built-in TPV5 run bit for bit."""
+    tp = PB.tpv5_cartesian(21, sat_form=L.WQL_SAT_ALPHA_THETA)
+    y, z = tp.yf, tp.zf
+
+    def box(x, w):
+        tol = 100.0 * 2.220446049250313e-16
+        return np.where((-w + tol < x) & (x < w - tol), 1.0,
+                        np.where((np.abs(-w - x) <= tol) | (np.abs(x - w) <= tol), 0.5, 0.0))
+    T0 = np.zeros((3,) + y.shape)
+    T0[0] = -120.0
+    T0[2] = (70.0 + 8.0 * box(y - 7.5, 1.5) * box(z + 7.5, 1.5)
+             + 11.6 * box(y - 7.5, 1.5) * box(z, 1.5) - 8.0 * box(y - 7.5, 1.5) * box(z - 7.5, 1.5))
+    tables = {"T0L": T0, "T0R": T0, "fs": 0.677 + 1.0e10 * (1.0 - box(y, 15.0) * box(z, 15.0)),
+              "fd": np.full(y.shape, 0.525), "Dc": np.full(y.shape, 0.4)}
+    runs = []
+    for kw in ({"coupling": L.WQL_IFACE_SLIP_WEAKENING_TPV5},
+               {"coupling": L.WQL_IFACE_SLIP_WEAKENING, "tables": tables}):
+        bl = gpu_block(tp.left, oracle.metrics(tp.left.X))
+        br = gpu_block(tp.right, oracle.metrics(tp.right.X))
+        itf = Interface(bl, br, y, z, **kw)
+        itf.set_frames(10)
+        itf.time_step(tp.dt, 10, 0.0)
+        runs.append(itf.frames())
+        itf.close()
+    for key, _ in TPV5_KEYS:
+        assert np.array_equal(runs[0][key], runs[1][key]), key

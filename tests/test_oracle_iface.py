@@ -125,3 +125,51 @@ def test_tpv5_station_files(oracle):
         assert np.abs(st[:, 0] - t).max() <= 6e-7
         v = fr[key][:, 0:3, 10, 10]
         assert np.abs(v - st[:, 1:4]).max() <= 6e-7, (label, np.abs(v - st[:, 1:4]).max())
+
+
+# TPV28 initial stress of the archived golden files test_problems/truth/2
+# (xx, yy, zz, xy, xz, yz): zz = -60 where the current source has +60
+# (initial_stress_condition.f90:78-84)
+TPV28_TRUTH_STRESS = (-60.0, 0.0, -60.0, 0.0, 29.38, 0.0)
+
+
+def test_tpv28_curvilinear_golden_files(oracle):
+    """test_problems/truth/2 (test_rup_curv_fric.in): TPV28 on two curvilinear
+    21^3 blocks meshed by Curve_Grid_3D3 ('analytical_test_problem'), with the
+    same historical choices as truth/1 (alpha/theta SAT, one slip-rate
+    increment per stage) and the initial stress with zz = -60.  All seven
+    outputs match to float32 rounding; the station files to their digits."""
+    tp = PB.tpv28_curvilinear(21, sat_form=1, s0=TPV28_TRUTH_STRESS)
+    assert tp.nsteps == 46
+    Lb, Rb, I = oracle_two_block(oracle, tp)
+    _, fr = oracle.two_block_time_step(Lb, Rb, I, tp.dt, tp.nsteps, 0.0)
+    truth = tpv5_truth("tpv28_curv_truth.npz")
+    for key, _ in TPV5_KEYS:
+        d = float(np.abs(fr[key] - truth[key]).max())
+        assert d <= 1.0e-3, (key, d)
+        scale = max(1.0, float(np.abs(truth[key][truth[key] < 1e8]).max()))
+        assert d <= 4e-7 * scale + 1e-7 or key == "trup", (key, d)
+    for key, label in (("Uface", "21_11_11"), ("Vface", "1_11_11")):
+        st = truth["station_" + label]
+        assert np.abs(fr[key][:, 0:3, 10, 10] - st[:, 1:4]).max() <= 6e-7
+
+
+def test_tpv28_source_stress_differs_from_golden_files(oracle):
+    """With the source's initial stress (zz = +60) the prestress at the first
+    frame already differs from the files by far more than their tolerance."""
+    tp = PB.tpv28_curvilinear(21, sat_form=1)
+    Lb, Rb, I = oracle_two_block(oracle, tp)
+    _, fr = oracle.two_block_time_step(Lb, Rb, I, tp.dt, 1, 0.0)
+    truth = tpv5_truth("tpv28_curv_truth.npz")
+    assert float(np.abs(fr["Uhat"][0] - truth["Uhat"][0]).max()) > 1.0
+
+
+def test_transfinite_grid_reduces_to_cartesian():
+    """Curve_Grid_3D3 with flat x faces (lc = rc = 0) is the uniform grid, and
+    the two TPV28 blocks share their fault nodes (to rounding)."""
+    a, b = (0.0, 6.0, -1.5), (3.0, 9.0, 1.5)
+    flat = PB.curve_grid_analytical_test_problem((21, 17, 13), a, b, 0.0, 0.0)
+    assert np.abs(flat - PB.cartesian_grid((21, 17, 13), a, b)).max() <= 1e-13
+    tp = PB.tpv28_curvilinear(21)
+    assert np.abs(tp.left.X[:, :, :, -1] - tp.right.X[:, :, :, 0]).max() <= 1e-14
+    assert np.abs(tp.left.X[0][:, :, -1]).max() > 0.05      # the fault face is curved

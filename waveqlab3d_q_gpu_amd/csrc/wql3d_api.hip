@@ -140,8 +140,10 @@ struct wql_iface {
   int ny, nz;
   long long NF;
   int coupling;
-  double* buf;      // one allocation of 29 planes: S 4, DS 4, trup, y, z, satL 9, satR 9
-  double *S, *DS, *trup, *yf, *zf, *satL, *satR;
+  // one allocation of 36 planes: S 4, DS 4, trup, prestress of L 3 and of R 3,
+  // fs, fd, Dc, SAT forcing of L 9 and of R 9
+  double* buf;
+  double *S, *DS, *trup, *T0L, *T0R, *fs, *fd, *Dc, *satL, *satR;
   double* frames;   // nslots * kFramePlanes planes of fault output
   int nslots, slot;
   hipEvent_t ev_iface;  // on L's stream after k_iface: block R's stage waits for it
@@ -1071,14 +1073,130 @@ static int iface_ok(wql_iface* I) {
   return WQL_OK;
 }
 
+static double boxcar(double x, double w) {
+  const double tol = 100.0 * 2.220446049250313e-16;  // Interface_Condition.f90:2438-2457
+  if (-w + tol < x && x < w - tol) return 1.0;
+  if (fabs(-w - x) <= tol || fabs(x - w) <= tol) return 0.5;
+  return 0.0;
+}
+
+// copy NF doubles (host or device source) to host
+static int to_host(std::vector<double>& dst, const double* src, long long n, bool on_device) {
+  dst.resize((size_t)n);
+  const hipMemcpyKind kind = on_device ? hipMemcpyDeviceToHost : hipMemcpyHostToHost;
+  HIP_TRY(hipMemcpy(dst.data(), src, (size_t)n * sizeof(double), kind));
+  return WQL_OK;
+}
+
+// the (l, m, n) frame planes of a block face, as doubles on the host
+template <typename T>
+static int face_frame_host(wql_block* b, int f, std::vector<double>& fr) {
+  const long long NF = b->NF[f];
+  std::vector<T> h((size_t)(9 * NF));
+  HIP_TRY(hipMemcpy(h.data(), b->face[f], h.size() * sizeof(T), hipMemcpyDeviceToHost));
+  fr.assign(h.begin(), h.end());
+  return WQL_OK;
+}
+
+// Slip-weakening tables of the interface: prestress in each side's (n, m, l)
+// frame and fs, fd, Dc per node (see wql3d.h).
+static int iface_tables(wql_iface* I, wql_block* L, wql_block* R, const wql_iface_desc* d) {
+  const long long NF = I->NF;
+  if (d->coupling == WQL_IFACE_LOCKED) return WQL_OK;
+  const bool dev = d->arrays_on_device != 0;
+  std::vector<double> T0L(3 * NF), T0R(3 * NF), fs(NF), fd(NF), Dc(NF);
+  if (d->coupling == WQL_IFACE_SLIP_WEAKENING) {
+    if (int rc = to_host(T0L, d->T0L, 3 * NF, dev)) return rc;
+    if (int rc = to_host(T0R, d->T0R, 3 * NF, dev)) return rc;
+    if (int rc = to_host(fs, d->fs, NF, dev)) return rc;
+    if (int rc = to_host(fd, d->fd, NF, dev)) return rc;
+    if (int rc = to_host(Dc, d->Dc, NF, dev)) return rc;
+  } else {
+    std::vector<double> y, z;
+    if (int rc = to_host(y, d->y, NF, dev)) return rc;
+    if (int rc = to_host(z, d->z, NF, dev)) return rc;
+    if (d->coupling == WQL_IFACE_SLIP_WEAKENING_TPV5) {
+      // prestress TPV5 (Interface_Condition.f90:1821-1830): T0 = (-120, 0, 70 + patches)
+      for (long long k = 0; k < NF; ++k) {
+        double l = 70.0;
+        l = l + 8.0 * boxcar(y[k] - 7.5, 1.5) * boxcar(z[k] + 7.5, 1.5);
+        l = l + 11.6 * boxcar(y[k] - 7.5, 1.5) * boxcar(z[k], 1.5);
+        l = l - 8.0 * boxcar(y[k] - 7.5, 1.5) * boxcar(z[k] - 7.5, 1.5);
+        for (std::vector<double>* t : {&T0L, &T0R}) {
+          (*t)[k] = -120.0;
+          (*t)[NF + k] = 0.0;
+          (*t)[2 * NF + k] = l;
+        }
+        // Tau_strength TPV5 (:1283-1299)
+        fs[k] = 0.677 + 1.0e10 * (1.0 - boxcar(y[k], 15.0) * boxcar(z[k], 15.0));
+        fd[k] = 0.525;
+        Dc[k] = 0.4;
+      }
+    } else {
+      // prestress TPV28 (:2049-2079): traction of the initial stress on the
+      // face in each side's frame, plus the nucleation stress on T0_l
+      static const double kS0[6] = {-60.0, 0.0, 60.0, 0.0, 29.38, 0.0};
+      const double* s0 = d->s0 ? d->s0 : kS0;
+      const double pi = 3.141592653589793;
+      std::vector<double> frL, frR;
+      int rc = L->d.precision == WQL_FP64 ? face_frame_host<double>(L, 1, frL)
+                                          : face_frame_host<float>(L, 1, frL);
+      if (!rc) rc = R->d.precision == WQL_FP64 ? face_frame_host<double>(R, 0, frR)
+                                               : face_frame_host<float>(R, 0, frR);
+      if (rc) return rc;
+      for (long long k = 0; k < NF; ++k) {
+        const double r = sqrt((y[k] - 7.5) * (y[k] - 7.5) + z[k] * z[k]);
+        double tau_nuke;
+        if (r <= 1.4) tau_nuke = 11.60;
+        else if (r >= 1.4 && r <= 2.0) tau_nuke = 5.8 * (1.0 + cos(pi * (r - 1.4) / 0.6));
+        else tau_nuke = 0.0;
+        const std::vector<double>* frs[2] = {&frL, &frR};
+        std::vector<double>* outs[2] = {&T0L, &T0R};
+        for (int side = 0; side < 2; ++side) {
+          const std::vector<double>& fr = *frs[side];
+          double l[3], m[3], n[3];
+          for (int c = 0; c < 3; ++c) {
+            l[c] = fr[c * NF + k];
+            m[c] = fr[(3 + c) * NF + k];
+            n[c] = fr[(6 + c) * NF + k];
+          }
+          const double Tx = n[0] * s0[0] + n[1] * s0[3] + n[2] * s0[4];
+          const double Ty = n[0] * s0[3] + n[1] * s0[1] + n[2] * s0[5];
+          const double Tz = n[0] * s0[4] + n[1] * s0[5] + n[2] * s0[2];
+          std::vector<double>& t = *outs[side];
+          t[k] = n[0] * Tx + n[1] * Ty + n[2] * Tz;
+          t[NF + k] = m[0] * Tx + m[1] * Ty + m[2] * Tz;
+          t[2 * NF + k] = l[0] * Tx + l[1] * Ty + l[2] * Tz + tau_nuke;
+        }
+        // Tau_strength TPV28 (:1459-1475)
+        fs[k] = 0.677 + 1.0e10 * (1.0 - boxcar(y[k], 15.0) * boxcar(z[k], 18.0));
+        fd[k] = 0.373;
+        Dc[k] = 0.4;
+      }
+    }
+  }
+  const size_t nb = (size_t)NF * sizeof(double);
+  HIP_TRY(hipMemcpy(I->T0L, T0L.data(), 3 * nb, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(I->T0R, T0R.data(), 3 * nb, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(I->fs, fs.data(), nb, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(I->fd, fd.data(), nb, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(I->Dc, Dc.data(), nb, hipMemcpyHostToDevice));
+  return WQL_OK;
+}
+
 extern "C" int wql_iface_create(wql_block* L, wql_block* R, const wql_iface_desc* d,
                                 wql_iface** out) {
   if (!out) return fail(WQL_EINVAL, "null output pointer");
   *out = nullptr;
   if (!L || !R || !d || L == R) return fail(WQL_EINVAL, "need two distinct blocks and a descriptor");
-  if (d->coupling != WQL_IFACE_LOCKED && d->coupling != WQL_IFACE_SLIP_WEAKENING_TPV5)
+  if (d->coupling < WQL_IFACE_LOCKED || d->coupling > WQL_IFACE_SLIP_WEAKENING)
     return fail(WQL_EINVAL, "invalid coupling %d", d->coupling);
-  if (!d->y || !d->z) return fail(WQL_EINVAL, "face coordinates y, z are null");
+  if ((d->coupling == WQL_IFACE_SLIP_WEAKENING_TPV5 || d->coupling == WQL_IFACE_SLIP_WEAKENING_TPV28) &&
+      (!d->y || !d->z))
+    return fail(WQL_EINVAL, "face coordinates y, z are null");
+  if (d->coupling == WQL_IFACE_SLIP_WEAKENING &&
+      (!d->T0L || !d->T0R || !d->fs || !d->fd || !d->Dc))
+    return fail(WQL_EINVAL, "slip-weakening tables T0L, T0R, fs, fd, Dc are required");
   if (L->iface || R->iface) return fail(WQL_ESTATE, "a block is already coupled to an interface");
   if (L->d.precision != R->d.precision) return fail(WQL_EINVAL, "blocks differ in precision");
   if (L->d.precision == WQL_FP32_KAHAN)
@@ -1098,13 +1216,14 @@ extern "C" int wql_iface_create(wql_block* L, wql_block* R, const wql_iface_desc
   I->NF = (long long)I->ny * I->nz;
   I->coupling = d->coupling;
   const long long NF = I->NF;
-  hipError_t e = hipMalloc((void**)&I->buf, 29 * NF * sizeof(double));
+  hipError_t e = hipMalloc((void**)&I->buf, 36 * NF * sizeof(double));
   if (e != hipSuccess) {
     free(I);
     return fail(WQL_ENOMEM, "hipMalloc(interface) failed: %s", hipGetErrorString(e));
   }
-  I->S = I->buf; I->DS = I->S + 4 * NF; I->trup = I->DS + 4 * NF; I->yf = I->trup + NF;
-  I->zf = I->yf + NF; I->satL = I->zf + NF; I->satR = I->satL + 9 * NF;
+  I->S = I->buf; I->DS = I->S + 4 * NF; I->trup = I->DS + 4 * NF; I->T0L = I->trup + NF;
+  I->T0R = I->T0L + 3 * NF; I->fs = I->T0R + 3 * NF; I->fd = I->fs + NF; I->Dc = I->fd + NF;
+  I->satL = I->Dc + NF; I->satR = I->satL + 9 * NF;
   auto cleanup = [&](int rc) {
     (void)hipFree(I->buf);
     if (I->ev_iface) (void)hipEventDestroy(I->ev_iface);
@@ -1112,10 +1231,7 @@ extern "C" int wql_iface_create(wql_block* L, wql_block* R, const wql_iface_desc
     free(I);
     return rc;
   };
-  const hipMemcpyKind kind = d->arrays_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
-  if ((e = hipMemcpy(I->yf, d->y, NF * sizeof(double), kind)) != hipSuccess ||
-      (e = hipMemcpy(I->zf, d->z, NF * sizeof(double), kind)) != hipSuccess)
-    return cleanup(fail(WQL_EHIP, "face coordinates: %s", hipGetErrorString(e)));
+  if (int rc = iface_tables(I, L, R, d)) return cleanup(rc);
   // init_iface: S = 0, DS = 1e40, trup = 1e9; the SAT planes start at zero
   if ((e = launch_fill<double>(I->S, 4 * NF, 0.0, L->stream)) != hipSuccess ||
       (e = launch_fill<double>(I->DS, 4 * NF, 1.0e40, L->stream)) != hipSuccess ||
@@ -1173,7 +1289,8 @@ static int iface_stage_t(wql_iface* I, double A, double Bdt, double t, int stage
   a.frL = (const T*)L->face[1]; a.frR = (const T*)R->face[0];
   a.penL = L->d.tau0 / L->d.hq;
   a.penR = R->d.tau0 / R->d.hq;
-  a.S = I->S; a.DS = I->DS; a.trup = I->trup; a.yf = I->yf; a.zf = I->zf;
+  a.S = I->S; a.DS = I->DS; a.trup = I->trup;
+  a.T0L = I->T0L; a.T0R = I->T0R; a.fs = I->fs; a.fd = I->fd; a.Dc = I->Dc;
   a.satL = I->satL; a.satR = I->satR;
   a.A = A; a.Bdt = Bdt; a.t = t;
   a.stage1 = stage1 ? 1 : 0;

@@ -10,9 +10,10 @@
 //     :466-522), hats, stage-1 outputs (:200-226), rotation back (:525-578),
 //     SAT forcing u_bc / v_bc with S^T Z S and S^T Z^-1 S (:228-457);
 //   Interface_RHS (Interface_Condition.f90:12-513): Welded_Interface (:515-590)
-//     or Slip_Weakening_Friction (:745-916) with solve (:1095-1168),
-//     Tau_strength TPV5 (:1283-1299), prestress TPV5 (:1802-1830), boxcar
-//     (:2438-2457); DS(1:3) += Q - P, DS(4) += |slip rate|;
+//     or Slip_Weakening_Friction (:745-916) with solve (:1095-1168), the
+//     prestress and the linear slip-weakening strength of the problem taken
+//     from per-node tables (built at wql_iface_create); DS(1:3) += Q - P,
+//     DS(4) += |slip rate|;
 //   rupture_front (CouplingForcing.f90:580-600);
 //   scale_rates_iface / update_fields_iface (iface.f90:58-85).
 // The interface state is block L's copy (one slip-rate increment per stage,
@@ -25,40 +26,13 @@
 namespace wql {
 namespace {
 
-__device__ __forceinline__ double boxcar(double x, double w) {
-  const double tol = 100.0 * 2.220446049250313e-16;
-  if (-w + tol < x && x < w - tol) return 1.0;
-  if (fabs(-w - x) <= tol || fabs(x - w) <= tol) return 0.5;
-  return 0.0;
-}
-
-// prestress, problem TPV5: normal -120, strike 0, dip 70 plus the nucleation
-// and the two asperity patches (MPa)
-__device__ __forceinline__ void prestress_tpv5(double y, double z, double T0[3]) {
-  T0[0] = -120.0;
-  T0[1] = 0.0;
-  double l = 70.0;
-  l = l + 8.0 * boxcar(y - 7.5, 1.5) * boxcar(z + 7.5, 1.5);
-  l = l + 11.6 * boxcar(y - 7.5, 1.5) * boxcar(z, 1.5);
-  l = l - 8.0 * boxcar(y - 7.5, 1.5) * boxcar(z - 7.5, 1.5);
-  T0[2] = l;
-}
-
-// linear slip-weakening strength: fs = 0.677 inside |y|, |z| < 15 km (1e10
-// outside), fd = 0.525, Dc = 0.4 m
-__device__ __forceinline__ double tau_strength_tpv5(double sigma_n, double D, double y, double z) {
-  double fs = 0.677;
-  const double fd = 0.525, Dc = 0.4;
-  fs = fs + 1.0e10 * (1.0 - boxcar(y, 15.0) * boxcar(z, 15.0));
-  const double f = fs - (fs - fd) * fmin(D, Dc) / Dc;
-  return sigma_n * f;
-}
-
 // hats P (minus side) and Q (plus side) in (n, m, l) from the rotated grid
 // values U (block L) and V (block R): velocities [0..2], tractions [3..5]
+// T0: prestress (n, m, l); fs, fd, Dc: strength parameters of the node
 __device__ __forceinline__ void hats(int coupling, const double U[6], const double V[6],
                                      double Z1s, double Z2s, double Z1p, double Z2p, double slip,
-                                     double y, double z, double P[6], double Q[6]) {
+                                     const double T0[3], double fs, double fd, double Dc,
+                                     double P[6], double Q[6]) {
   const double p_n = Z1p * U[0] - U[3], p_m = Z1s * U[1] - U[4], p_l = Z1s * U[2] - U[5];
   const double q_n = Z2p * V[0] + V[3], q_m = Z2s * V[1] + V[4], q_l = Z2s * V[2] + V[5];
   const double eta_p = 1.0 / (1.0 / Z1p + 1.0 / Z2p);
@@ -75,14 +49,13 @@ __device__ __forceinline__ void hats(int coupling, const double U[6], const doub
     Q[1] = (p_m + phi_m) / Z1s; Q[4] = phi_m;
     return;
   }
-  double T0[3];
-  prestress_tpv5(y, z, T0);
   const double u_n_hat = (q_n - phi_n) / Z2p, Tu_n_hat = phi_n;
   const double v_n_hat = (p_n + phi_n) / Z1p, Tv_n_hat = phi_n;
   const double tau_lock =
       sqrt((T0[2] + phi_l) * (T0[2] + phi_l) + (T0[1] + phi_m) * (T0[1] + phi_m));
   const double sigma_n = fmax(0.0, -(T0[0] + phi_n));
-  const double tau_str = tau_strength_tpv5(sigma_n, slip, y, z);
+  // linear slip weakening (Tau_strength)
+  const double tau_str = sigma_n * (fs - (fs - fd) * fmin(slip, Dc) / Dc);
   double Tu_m, Tu_l, Tv_m, Tv_l, vv_m, vv_l;
   if (tau_lock > tau_str) {
     // sliding: |tau| = strength, slip rate V = (|phi| - strength) / eta_s
@@ -143,9 +116,9 @@ struct Side {
 template <typename T>
 __device__ __forceinline__ void evaluate(const IfaceArgs<T>& a, int ib, const T* fr, long long fi,
                                          long long NF, const double u[9], const double v[9],
-                                         const Side& s1, const Side& s2, double slip, double y,
-                                         double z, double P[6], double Q[6], double* slip_rate,
-                                         double bc[9]) {
+                                         const Side& s1, const Side& s2, double slip,
+                                         const double T0[3], double P[6], double Q[6],
+                                         double* slip_rate, double bc[9]) {
   double l[3], m[3], n[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
@@ -172,7 +145,13 @@ __device__ __forceinline__ void evaluate(const IfaceArgs<T>& a, int ib, const T*
   const double c1p = sqrt((2.0 * s1.mu + s1.lam) / s1.rho), c1s = sqrt(s1.mu / s1.rho);
   const double c2p = sqrt((2.0 * s2.mu + s2.lam) / s2.rho), c2s = sqrt(s2.mu / s2.rho);
   const double Z1s = s1.rho * c1s, Z2s = s2.rho * c2s, Z1p = s1.rho * c1p, Z2p = s2.rho * c2p;
-  hats(a.coupling, U, V, Z1s, Z2s, Z1p, Z2p, slip, y, z, P, Q);
+  double fs = 0.0, fd = 0.0, Dc = 1.0;
+  if (a.coupling != WQL_IFACE_LOCKED) {
+    fs = a.fs[fi];
+    fd = a.fd[fi];
+    Dc = a.Dc[fi];
+  }
+  hats(a.coupling, U, V, Z1s, Z2s, Z1p, Z2p, slip, T0, fs, fd, Dc, P, Q);
   *slip_rate = sqrt((Q[1] - P[1]) * (Q[1] - P[1]) + (Q[2] - P[2]) * (Q[2] - P[2]));
   double Pb[6], Qb[6];
 #pragma unroll
@@ -257,10 +236,9 @@ __device__ __forceinline__ void evaluate(const IfaceArgs<T>& a, int ib, const T*
 // stage-1 outputs of one evaluation (CouplingForcing.f90:200-226,
 // Interface_Condition.f90:494-503, rupture_front)
 template <typename T>
-__device__ __forceinline__ void stage1_output(const IfaceArgs<T>& a, const double P[6], const double Q[6], double y,
-                                              double z, double out[15], double* trup) {
-  double T0[3] = {0.0, 0.0, 0.0};
-  if (a.coupling != WQL_IFACE_LOCKED) prestress_tpv5(y, z, T0);
+__device__ __forceinline__ void stage1_output(const IfaceArgs<T>& a, const double P[6],
+                                              const double Q[6], const double T0[3],
+                                              double out[15], double* trup) {
   // out: Svel[3], Uhat[6], Vhat[6]
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
@@ -290,23 +268,32 @@ __global__ __launch_bounds__(256) void k_iface(const IfaceArgs<T> a) {
   Side s1, s2;
   s1.lam = (double)a.ML[iL]; s1.mu = (double)a.ML[a.PL + iL]; s1.rho = (double)a.ML[2 * a.PL + iL];
   s2.lam = (double)a.MR[iR]; s2.mu = (double)a.MR[a.PR + iR]; s2.rho = (double)a.MR[2 * a.PR + iR];
-  const double yc = a.yf[fi], zc = a.zf[fi];
   const double slip = a.S[3 * NF + fi];
+  // prestress of each side (zero for a welded interface, whose output
+  // carries no prestress, Interface_Condition.f90:482-486)
+  double T0L[3] = {0.0, 0.0, 0.0}, T0R[3] = {0.0, 0.0, 0.0};
+  if (a.coupling != WQL_IFACE_LOCKED) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      T0L[c] = a.T0L[c * NF + fi];
+      T0R[c] = a.T0R[c * NF + fi];
+    }
+  }
   double P[6], Q[6], srate, bc[9];
   double trup = a.trup[fi];
   double out[15];
   // block L, side 2: SAT into L, the slip-rate increment
-  evaluate(a, 2, a.frL, fi, NF, u, v, s1, s2, slip, yc, zc, P, Q, &srate, bc);
+  evaluate(a, 2, a.frL, fi, NF, u, v, s1, s2, slip, T0L, P, Q, &srate, bc);
   double ds[4];
 #pragma unroll
   for (int c = 0; c < 3; ++c) ds[c] = a.A * a.DS[c * NF + fi] + (Q[c] - P[c]);
   ds[3] = a.A * a.DS[3 * NF + fi] + srate;
-  if (a.stage1) stage1_output(a, P, Q, yc, zc, out, &trup);
+  if (a.stage1) stage1_output(a, P, Q, T0L, out, &trup);
 #pragma unroll
   for (int c = 0; c < 9; ++c) a.satL[c * NF + fi] = a.penL * bc[c];
   // block R, side 1: SAT into R (its hats are the same up to rounding)
-  evaluate(a, 1, a.frR, fi, NF, u, v, s1, s2, slip, yc, zc, P, Q, &srate, bc);
-  if (a.stage1) stage1_output(a, P, Q, yc, zc, out, &trup);
+  evaluate(a, 1, a.frR, fi, NF, u, v, s1, s2, slip, T0R, P, Q, &srate, bc);
+  if (a.stage1) stage1_output(a, P, Q, T0R, out, &trup);
 #pragma unroll
   for (int c = 0; c < 9; ++c) a.satR[c * NF + fi] = a.penR * bc[c];
   if (a.stage1) {

@@ -169,8 +169,13 @@ struct IfaceArgs {
   double* S;               // 4 planes: slip (3) and slip-path length
   double* DS;              // 4 planes: rates of S
   double* trup;            // rupture time
-  const double* yf;        // face node coordinates
-  const double* zf;
+  // slip-weakening tables: prestress (n, m, l) in the frame of each side,
+  // static / dynamic friction coefficient and slip-weakening distance
+  const double* T0L;
+  const double* T0R;
+  const double* fs;
+  const double* fd;
+  const double* Dc;
   double* satL;            // 9 planes each: tau0/hq * SAT forcing
   double* satR;
   double A, Bdt, t;

@@ -27,11 +27,17 @@ class Interface:
     """Couples the x-high face (bc 0) of ``left`` to the x-low face (bc 0) of
     ``right``.  ``y``, ``z``: coordinates of the interface nodes, arrays of
     shape (nz, ny) (host numpy or device torch float64).  ``coupling``:
-    ``WQL_IFACE_LOCKED`` or ``WQL_IFACE_SLIP_WEAKENING_TPV5``.
+    ``WQL_IFACE_LOCKED``, ``WQL_IFACE_SLIP_WEAKENING_TPV5``,
+    ``WQL_IFACE_SLIP_WEAKENING_TPV28`` (``s0``: its initial stress
+    (xx, yy, zz, xy, xz, yz), default the source's), or
+    ``WQL_IFACE_SLIP_WEAKENING`` with ``tables`` = dict of host arrays T0L,
+    T0R (3, nz, ny) (prestress in each side's (n, m, l) face frame) and fs,
+    fd, Dc (nz, ny).
 
     The blocks must outlive the interface; call :meth:`close` first."""
 
-    def __init__(self, left: Block, right: Block, y, z, coupling=L.WQL_IFACE_SLIP_WEAKENING_TPV5):
+    def __init__(self, left: Block, right: Block, y, z, coupling=L.WQL_IFACE_SLIP_WEAKENING_TPV5,
+                 s0=None, tables=None):
         self.left, self.right = left, right
         self.ny, self.nz = left.nr, left.ns
         d = L.IfaceDesc()
@@ -42,9 +48,22 @@ class Interface:
             z = np.ascontiguousarray(z, dtype=np.float64)
         if tuple(y.shape) != (self.nz, self.ny) or tuple(z.shape) != (self.nz, self.ny):
             raise ValueError(f"face coordinates must be ({self.nz}, {self.ny})")
-        self._yz = (y, z)          # keep the arrays alive during create
+        keep = [y, z]              # keep the arrays alive during create
         d.arrays_on_device = 1 if on_dev else 0
         d.y, d.z = _ptr(y), _ptr(z)
+        if s0 is not None:
+            s0a = np.ascontiguousarray(s0, dtype=np.float64).reshape(6)
+            keep.append(s0a)
+            d.s0 = s0a.ctypes.data_as(C.POINTER(C.c_double))
+        if tables is not None:
+            if on_dev:
+                raise ValueError("tables are host arrays; pass host y, z with them")
+            for name, shape in (("T0L", (3,)), ("T0R", (3,)), ("fs", ()), ("fd", ()), ("Dc", ())):
+                a = np.ascontiguousarray(tables[name], dtype=np.float64).reshape(
+                    shape + (self.nz, self.ny))
+                keep.append(a)
+                setattr(d, name, _ptr(a))
+        self._keep = keep
         h = C.c_void_p()
         check(lib().wql_iface_create(left._h, right._h, C.byref(d), C.byref(h)))
         self._h = h

@@ -64,6 +64,120 @@ def cartesian_grid(n, a, b):
     return X
 
 
+def gen_surf_mesh(XL, XR, XB, XT, YL, YR, YB, YT):
+    """Gen_Surf_Mesh (grid.f90:2420-2457): the bilinear Coons patch of two
+    coordinate functions from their four boundary curves; XB, XT have mx
+    points, XL, XR my points; returns X, Y as (mx, my) arrays (Fortran
+    X(i, j))."""
+    mx, my = len(XB), len(XL)
+    q = (np.arange(mx) * (1.0 / (mx - 1)))[:, None]
+    r = (np.arange(my) * (1.0 / (my - 1)))[None, :]
+
+    def patch(L, R, B, T):
+        L, R, B, T = (np.asarray(v, dtype=np.float64) for v in (L, R, B, T))
+        return ((1.0 - q) * L[None, :] + q * R[None, :] + (1.0 - r) * B[:, None] + r * T[:, None]
+                - (1.0 - q) * (1.0 - r) * L[0] - q * (1.0 - r) * R[0] - r * (1.0 - q) * T[0]
+                - (r * q) * T[mx - 1])
+
+    return patch(XL, XR, XB, XT), patch(YL, YR, YB, YT)
+
+
+def gen3_curve_mesh(left, right, top, bottom, front, back):
+    """Gen3_Curve_Mesh (grid.f90:2292-2418): transfinite interpolation of one
+    coordinate from its six boundary surfaces, left/right (nr, ns), top/bottom
+    (nq, ns), front/back (nq, nr) (Fortran index order); returns (ns, nr, nq)."""
+    nr, ns = left.shape
+    nq = top.shape[0]
+    q = (np.arange(nq) * (1.0 / (nq - 1)))[None, None, :]
+    r = (np.arange(nr) * (1.0 / (nr - 1)))[None, :, None]
+    s = (np.arange(ns) * (1.0 / (ns - 1)))[:, None, None]
+    Lf, Rf = left.T[:, :, None], right.T[:, :, None]            # (ns, nr, 1)
+    Tp, Bt = top.T[:, None, :], bottom.T[:, None, :]            # (ns, 1, nq)
+    Fr, Bk = front.T[None, :, :], back.T[None, :, :]            # (1, nr, nq)
+    U = (1.0 - q) * Lf + q * Rf
+    V = (1.0 - r) * Tp + r * Bt
+    W = (1.0 - s) * Fr + s * Bk
+    UW = ((1.0 - q) * ((1.0 - r) * left[0, :][:, None, None] + r * left[nr - 1, :][:, None, None])
+          + q * ((1.0 - r) * right[0, :][:, None, None] + r * right[nr - 1, :][:, None, None]))
+    UV = ((1.0 - r) * ((1.0 - s) * top[:, 0][None, None, :] + s * top[:, ns - 1][None, None, :])
+          + r * ((1.0 - s) * bottom[:, 0][None, None, :] + s * bottom[:, ns - 1][None, None, :]))
+    VW = ((1.0 - s) * ((1.0 - q) * front[0, :][None, :, None] + q * front[nq - 1, :][None, :, None])
+          + s * ((1.0 - q) * back[0, :][None, :, None] + q * back[nq - 1, :][None, :, None]))
+    UVW1 = (1.0 - q) * ((1.0 - r) * ((1.0 - s) * top[0, 0] + s * top[0, ns - 1])
+                        + r * ((1.0 - s) * bottom[0, 0] + s * bottom[0, ns - 1]))
+    UVW2 = q * ((1.0 - r) * ((1.0 - s) * top[nq - 1, 0] + s * top[nq - 1, ns - 1])
+                + r * ((1.0 - s) * bottom[nq - 1, 0] + s * bottom[nq - 1, ns - 1]))
+    return U + V + W - UV - UW - VW + UVW1 + UVW2
+
+
+def curve_grid_analytical_test_problem(n, a, b, lc, rc):
+    """Curve_Grid_3D3 (grid.f90:473-1909) with profile 'analytical_test_problem'
+    and no topography: the x faces carry a Gaussian bump (width^2 0.025) plus
+    a sine pattern of amplitude 0.1, scaled by rc (x-low face) and lc (x-high
+    face) (:777-833); the other four faces are Coons patches of the x faces'
+    edges (:929-1180); the volume is the transfinite interpolation of the six
+    faces (:1874-1876).  Returns X (3, ns, nr, nq)."""
+    mx, my, mz = n
+    ax, ay, az = a
+    bx, by, bz = b
+    pi = 3.141592653589793
+    hy = (by - ay) / float(my - 1)
+    hz = (bz - az) / float(mz - 1)
+    yline = ay + np.arange(my) * hy
+    zline = az + np.arange(mz) * hz
+    # x faces: coordinates from their (straight) boundary curves
+    Yl, Zl = gen_surf_mesh(np.full(mz, ay), np.full(mz, by), yline, yline,
+                           zline, zline, np.full(my, az), np.full(my, bz))
+    Yr, Zr = Yl.copy(), Zl.copy()
+
+    def bump(Y, Z, c):
+        return (0.1 * c * np.exp(-((1.0 * Y - 0.5 * (ay + by)) ** 2
+                                   + 1.0 * (Z - 0.5 * (az + bz)) ** 2) / 0.025)
+                + 0.1 * c * np.sin(2.0 / (by - ay) * pi * Y) * np.sin(2.0 / (bz - az) * pi * Z))
+
+    Xl = ax + bump(Yl, Zl, rc)
+    Xr = bx + bump(Yr, Zr, lc)
+    # front (k = 1) and back (k = mz) faces
+    def xface(Xf_l, Xf_r, Yf_l, Yf_r, jlo, jhi):
+        hxb = (Xf_r[jlo] - Xf_l[jlo]) / float(mx - 1)
+        hxt = (Xf_r[jhi] - Xf_l[jhi]) / float(mx - 1)
+        hyb = (Yf_r[jlo] - Yf_l[jlo]) / float(mx - 1)
+        hyt = (Yf_r[jhi] - Yf_l[jhi]) / float(mx - 1)
+        i = np.arange(mx)
+        XB = Xf_l[jlo] + i * hxb
+        XT = Xf_l[jhi] + i * hxt
+        YB = Yf_l[jlo] + i * hyb
+        YT = Yf_l[jhi] + i * hyt
+        return XB, XT, YB, YT
+
+    XB, XT, YB, YT = xface(Xl[:, 0], Xr[:, 0], Yl[:, 0], Yr[:, 0], 0, my - 1)
+    Xfront, Yfront = gen_surf_mesh(Xl[:, 0], Xr[:, 0], XB, XT, Yl[:, 0], Yr[:, 0], YB, YT)
+    Zfront = np.full((mx, my), az)
+    XB, XT, YB, YT = xface(Xl[:, mz - 1], Xr[:, mz - 1], Yl[:, mz - 1], Yr[:, mz - 1], 0, my - 1)
+    Xback, Yback = gen_surf_mesh(Xl[:, mz - 1], Xr[:, mz - 1], XB, XT, Yl[:, mz - 1],
+                                 Yr[:, mz - 1], YB, YT)
+    Zback = np.full((mx, my), bz)
+    # top (j = 1) and bottom (j = my) faces
+    i = np.arange(mx)
+    for jj, name in ((0, "top"), (my - 1, "bottom")):
+        ZT = Zback[:, jj]
+        ZB = Zfront[:, jj]
+        XBy = Xl[jj, 0] + i * ((Xr[jj, 0] - Xl[jj, 0]) / float(mx - 1))
+        XTy = Xl[jj, mz - 1] + i * ((Xr[jj, mz - 1] - Xl[jj, mz - 1]) / float(mx - 1))
+        Xf, Zf = gen_surf_mesh(Xl[jj, :], Xr[jj, :], XBy, XTy, Zl[jj, :], Zr[jj, :], ZB, ZT)
+        if name == "top":
+            Xtop, Ztop = Xf, Zf
+        else:
+            Xbottom, Zbottom = Xf, Zf
+    Ytop = np.full((mx, mz), ay)
+    Ybottom = np.full((mx, mz), by)
+    X = np.empty((3, mz, my, mx))
+    X[0] = gen3_curve_mesh(Xl, Xr, Xtop, Xbottom, Xfront, Xback)
+    X[1] = gen3_curve_mesh(Yl, Yr, Ytop, Ybottom, Yfront, Yback)
+    X[2] = gen3_curve_mesh(Zl, Zr, Ztop, Zbottom, Zfront, Zback)
+    return X
+
+
 def gaussian_hills(xs, zs, seed, nhills=12, width=(500.0, 2000.0), height=(-300.0, 300.0)):
     """Surface elevation e(x, z) = sum of seeded Gaussian hills."""
     rng = np.random.default_rng(seed)
@@ -277,6 +391,7 @@ class TwoBlockProblem:
     dt: float
     nsteps: int
     name: str = ""
+    s0: tuple = None               # TPV28 initial stress (None: the source's values)
 
 
 def tpv5_cartesian(n=21, sat_form=1, coupling=None):
@@ -303,6 +418,33 @@ def tpv5_cartesian(n=21, sat_form=1, coupling=None):
                            yf=np.ascontiguousarray(XL[1][:, :, -1]),
                            zf=np.ascontiguousarray(XL[2][:, :, -1]), dt=dt, nsteps=nsteps,
                            name=f"tpv5_cartesian_{n}")
+
+
+def tpv28_curvilinear(n=21, sat_form=1, s0=None):
+    """test_problems/test_rup_curv_fric.in: problem TPV28 on two n^3
+    curvilinear blocks over the boxes of the Cartesian TPV5 test, meshed by
+    Curve_Grid_3D3 with profile 'analytical_test_problem' (block 1: lc = 1,
+    rc = 0; block 2: lc = 0, rc = 1, so both carry the same curved fault
+    face), rho_s_p = (2.67, 3.464, 6), slip-weakening friction, CFL 0.5,
+    t_final 0.5.  sat_form 1 selects the SAT form of the archived golden
+    files (test_problems/truth/2); s0 overrides the TPV28 initial stress
+    (xx, yy, zz, xy, xz, yz), whose source values are (-60, 0, 60, 0, 29.38,
+    0) (initial_stress_condition.f90:78-84) while the golden files were made
+    with zz = -60."""
+    nn = (n, n, n)
+    XL = curve_grid_analytical_test_problem(nn, (-3.0, 6.0, -1.5), (0.0, 9.0, 1.5), 1.0, 0.0)
+    XR = curve_grid_analytical_test_problem(nn, (0.0, 6.0, -1.5), (3.0, 9.0, 1.5), 0.0, 1.0)
+    M = material_from_rho_s_p(nn, 2.67, 3.464, 6.0)
+    dt = block_time_step(3.0 / float(n - 1), 0.5, 3.464, 6.0)
+    nsteps = int(math.floor(0.5 / dt))
+    left = Problem(n=nn, mesh=L.WQL_CURVILINEAR, X=XL, M=M.copy(), bc=(1, 0, 2, 1, 1, 1), dt=dt,
+                   nsteps=nsteps, name="tpv28_block1", sat_form=sat_form)
+    right = Problem(n=nn, mesh=L.WQL_CURVILINEAR, X=XR, M=M.copy(), bc=(0, 1, 2, 1, 1, 1), dt=dt,
+                    nsteps=nsteps, name="tpv28_block2", sat_form=sat_form)
+    return TwoBlockProblem(left=left, right=right, coupling=L.WQL_IFACE_SLIP_WEAKENING_TPV28,
+                           yf=np.ascontiguousarray(XL[1][:, :, -1]),
+                           zf=np.ascontiguousarray(XL[2][:, :, -1]), dt=dt, nsteps=nsteps,
+                           name=f"tpv28_curvilinear_{n}", s0=None if s0 is None else tuple(s0))
 
 
 def locked_interface_41():
