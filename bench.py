@@ -73,10 +73,12 @@ def byte_model(prob, precision):
     PML Q and DQ read and write (36 values per slab node).
 
     Per kernel (what each kernel of the stage must move, its W hand-off
-    included): interior stencil = F read (9 x 8), geometry, material, W write
-    (15); shell = the same per node plus the PML streams; memory-variable
-    kernel = W read (15), lambda, mu, Q^-1 (4), eta/Deta read and write, DF
-    read and write, F read and write."""
+    included): interior stencil = F read (9), geometry, material, the
+    velocity components' F write and DF read and write (3), W write (12: six
+    stress rates, six strain rates); shell = the same per node plus the PML
+    streams; memory-variable kernel = W read (12), lambda, mu, Q^-1 (4),
+    eta/Deta read and write, and the six stress components' F read and write
+    and DF read and write."""
     nq, nr, ns = prob.n
     P = nq * nr * ns
     es = 8 if precision == 0 else 4
@@ -93,11 +95,14 @@ def byte_model(prob, precision):
     lo, hi = interior_box(prob)
     n_int = int(np.prod([max(0, hi[a] - lo[a] + 1) for a in range(3)]))
     # elastic stages have no hand-off: the node kernels write F and DF
-    stencil_node = 9 * ss + (ngeo + 3) * es + (15 * es if prob.nmech else 9 * (state - ss))
+    # velocity update in the stencil kernels: F write, DF read + write (and
+    # the Kahan compensation), the F read being the stencil's own centre value
+    vel = 3 * (state - ss)
+    stencil_node = 9 * ss + (ngeo + 3) * es + (12 * es + vel if prob.nmech else 9 * (state - ss))
     kernels = {
         "interior": n_int * stencil_node,
         "shell": (P - n_int) * stencil_node + slab * 36 * es,
-        "memory_update": P * (15 * es + 4 * es + 24 * na * es + 9 * state) if prob.nmech else 0,
+        "memory_update": P * (12 * es + 4 * es + 24 * na * es + 6 * state) if prob.nmech else 0,
     }
     return {"P": P, "interior_nodes": n_int, "active_mechanisms": na,
             "end_to_end": end_to_end, "kernels": kernels}

@@ -90,7 +90,7 @@ struct wql_block {
   int tune_zm, tune_zchunk;                      // z-marching kernel, z levels per chunk
   int tune_mem_npt;                              // nodes per thread of k_memory_update
   void* arena;    // one allocation holding F[2], DF, G, M, eta, Deta, Qs, Qp, W
-  void* W;        // 15 planes of T: split anelastic stage hand-off
+  void* W;        // 12 planes of T: split anelastic stage hand-off
   int mech_zero;  // mechanisms whose eta and Deta are known to be all zero
   // kernel timing of fused stages (wql_block_set_timing): 4 events per stage
   // around the interior, shell and memory-variable kernels
@@ -346,7 +346,7 @@ extern "C" int wql_block_create(const wql_block_desc* d, wql_block** out) {
         {&b->G, ngeo * P * (long long)b->esz}, {&b->M, 3 * P * (long long)b->esz},
         {&b->eta, 6 * nm * P * (long long)b->esz}, {&b->Deta, 6 * nm * P * (long long)b->esz},
         {&b->Qs, nm ? P * (long long)b->esz : 0}, {&b->Qp, nm ? P * (long long)b->esz : 0},
-        {&b->W, nm ? 15 * P * (long long)b->esz : 0}};
+        {&b->W, nm ? 12 * P * (long long)b->esz : 0}};
     const long long kPage = 2LL << 20;
     long long stagger = 0;
     if (const char* sv = getenv("WQL3D_ARENA_STAGGER")) stagger = atoll(sv);

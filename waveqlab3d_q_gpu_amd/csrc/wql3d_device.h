@@ -482,4 +482,24 @@ __device__ __forceinline__ void rk_finish(const StepArgs<T, S>& a, int c, unsign
   }
 }
 
+// The same update from values loaded ahead of time: the state fin, the old
+// rate dfo and (FP32 state) the compensation comp of component c.
+template <bool NT, typename T, typename S>
+__device__ __forceinline__ void rk_finish_pre(const StepArgs<T, S>& a, int c, unsigned idu,
+                                              double df, S fin, S comp) {
+  const long long P = a.P;
+  if constexpr (std::is_same<S, double>::value) {
+    (void)comp;
+    st_stream<NT>(atp(a.DF + c * P, idu), df);
+    st_stream<NT>(atp(a.Fout + c * P, idu), fin + a.Bdt * df);
+  } else {
+    const float d = (float)df;
+    st_stream<NT>(atp(a.DF + c * P, idu), d);
+    const float y = (float)a.Bdt * d - comp;
+    const float t = fin + y;
+    st_stream<NT>(atp(a.Fc + c * P, idu), (t - fin) - y);
+    st_stream<NT>(atp(a.Fout + c * P, idu), t);
+  }
+}
+
 }  // namespace wql

@@ -81,7 +81,8 @@ struct StepArgs {
   // node-kernel launch shape (see map_node): y-band height, 256-node segments
   // per band slab, XCD-contiguous remap, non-temporal streams
   int band_rows, segs, xcd_remap, nontemporal;
-  // split anelastic stage: k_node MODE 3 writes W (15 planes), then
+  // split anelastic stage: the MODE 3 stencil kernels finish the velocity
+  // components and write W (12 planes: 6 stress rates, 6 strain rates), then
   // k_memory_update; mech_skip = mechanisms with zero weight whose eta and
   // Deta are identically zero (they stay zero, so they are not streamed)
   T* W;

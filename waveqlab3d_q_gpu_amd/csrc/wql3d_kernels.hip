@@ -315,8 +315,9 @@ __device__ __forceinline__ void node_epilogue(const StepArgs<T, S>& a, int x, in
       rk_finish<NT>(a, c, idu, df);
     }
   } else if (MODE == 3) {
-    // hand the complete non-memory-variable rates and the (PML-modified)
-    // strain rates to k_memory_update through the W planes
+    // the velocity rates carry no memory-variable term: finish their RK
+    // update here; hand the stress rates without memory-variable terms and
+    // the (PML-modified) strain rates to k_memory_update through W
     double srcv[6];
     const bool has_src = a.nsrc > 0 && source_at<T, MESH>(a, x, y, z, id, srcv);
 #pragma unroll
@@ -324,10 +325,14 @@ __device__ __forceinline__ void node_epilogue(const StepArgs<T, S>& a, int x, in
       T r = DU[c];
       if (has_src && c >= 3) r -= (T)srcv[c - 3];
       if (has_sat) r -= (T)satDF[c];
-      st_stream<NT>(atp(a.W + c * P, idu), r);
+      if (c < 3) {
+        rk_finish<NT>(a, c, idu, scale_rate(a.A, ld_stream<NT>(atp(a.DF + c * P, idu))) + (double)r);
+      } else {
+        st_stream<NT>(atp(a.W + (c - 3) * P, idu), r);
+      }
     }
 #pragma unroll
-    for (int c = 0; c < 6; ++c) st_stream<NT>(atp(a.W + (9 + c) * P, idu), e[c]);
+    for (int c = 0; c < 6; ++c) st_stream<NT>(atp(a.W + (6 + c) * P, idu), e[c]);
   } else {
 #pragma unroll
     for (int c = 0; c < 9; ++c) a.DF[c * P + id] = a.DF[c * P + id] + (double)DU[c];
@@ -335,11 +340,12 @@ __device__ __forceinline__ void node_epilogue(const StepArgs<T, S>& a, int x, in
 }
 
 // Memory-variable part of a fused anelastic stage, one thread per node over
-// the flat node index (pure streaming, unit stride): W holds the rates
-// without the memory-variable terms (planes 0..8) and the strain rates used
-// by the relaxation equations (planes 9..14), both from k_node MODE 3.
+// the flat node index (pure streaming, unit stride): W holds the stress rates
+// without the memory-variable terms (planes 0..5) and the strain rates used
+// by the relaxation equations (planes 6..11), both from the MODE 3 stencil
+// kernels, which also finish the velocity components.
 //   DU(4..9) -= sum_i eta_i;  Deta_i = A*Deta_i + (src_i(eps) - eta_i)/tau_i;
-//   eta_i += Bdt*Deta_i;  DF = A*DF + DU;  F_out = F_in + Bdt*DF
+//   eta_i += Bdt*Deta_i;  DF(4..9) = A*DF + DU;  F_out(4..9) = F_in + Bdt*DF
 // Memory variables of one node in a fused stage: DU(4..9) -= sum_i eta_i and
 // the relaxation update of eta_i / Deta_i from the strain rates e
 // (RHS_Interior.f90:2599-2906).  Mechanisms in mech_skip are not streamed.
@@ -400,14 +406,16 @@ __global__ __launch_bounds__(256) void k_memory_update(const StepArgs<T, S> a) {
   if (id >= (unsigned long long)a.node1) return;
   T DU[9], e[6];
 #pragma unroll
-  for (int c = 0; c < 9; ++c) DU[c] = ld_stream<NT>(atp(a.W + c * P, id));
+  for (int c = 0; c < 3; ++c) DU[c] = T(0);
 #pragma unroll
-  for (int c = 0; c < 6; ++c) e[c] = ld_stream<NT>(atp(a.W + (9 + c) * P, id));
+  for (int c = 3; c < 9; ++c) DU[c] = ld_stream<NT>(atp(a.W + (c - 3) * P, id));
+#pragma unroll
+  for (int c = 0; c < 6; ++c) e[c] = ld_stream<NT>(atp(a.W + (6 + c) * P, id));
   const T lam = ld_stream<NT>(atp(a.M, id)), mu = ld_stream<NT>(atp(a.M + P, id));
   const T l2m = lam + T(2) * mu;
   memory_update_node<T, NMECH, NT>(a, id, mu, l2m, e, DU);
 #pragma unroll
-  for (int c = 0; c < 9; ++c) {
+  for (int c = 3; c < 9; ++c) {
     const double df = scale_rate(a.A, ld_stream<NT>(atp(a.DF + c * P, id))) + (double)DU[c];
     rk_finish<NT>(a, c, id, df);
   }
@@ -510,16 +518,20 @@ __global__ __launch_bounds__(256) void k_memory_update_n(const StepArgs<T, S> a)
   if (id >= (unsigned long long)a.node1) return;
   T DU[N][9], e[N][6];
 #pragma unroll
-  for (int c = 0; c < 9; ++c) {
+  for (int n = 0; n < N; ++n)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) DU[n][c] = T(0);
+#pragma unroll
+  for (int c = 3; c < 9; ++c) {
     T t[N];
-    ldn<N>(a.W + c * P, id, t);
+    ldn<N>(a.W + (c - 3) * P, id, t);
 #pragma unroll
     for (int n = 0; n < N; ++n) DU[n][c] = t[n];
   }
 #pragma unroll
   for (int c = 0; c < 6; ++c) {
     T t[N];
-    ldn<N>(a.W + (9 + c) * P, id, t);
+    ldn<N>(a.W + (6 + c) * P, id, t);
 #pragma unroll
     for (int n = 0; n < N; ++n) e[n][c] = t[n];
   }
@@ -531,7 +543,7 @@ __global__ __launch_bounds__(256) void k_memory_update_n(const StepArgs<T, S> a)
   memory_update_planes_n<T, NMECH, N>(a.eta, a.Deta, a.Qs, a.Qp, P, a.mech_skip, a.wgt,
                                       a.tau_inv, a.A, a.Bdt, id, mu, l2m, e, DU);
 #pragma unroll
-  for (int c = 0; c < 9; ++c) {
+  for (int c = 3; c < 9; ++c) {
     S dfo[N], fin[N];
     ldn<N>(a.DF + c * P, id, dfo);
     ldn<N>(a.Fin + c * P, id, fin);
@@ -704,6 +716,21 @@ void k_interior(const StepArgs<T, S> a) {
         for (int c = 0; c < 3; ++c) { qv[k][c] = qv[k + 1][c]; qf[k][c] = qf[k + 1][c]; }
       continue;
     }
+    // MODE 3 finishes the velocity components of its node: their state, old
+    // rate (and compensation) are loaded here, ahead of the LDS staging and
+    // the differences that hide their latency
+    S pre_f[3], pre_df[3], pre_c[3];
+    if constexpr (MODE == 3) {
+      if (active) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          pre_f[c] = ldp(a.Fin + c * P, p);
+          pre_df[c] = ld_stream<NT>(atp(a.DF + c * P, p));
+          if constexpr (std::is_same<S, float>::value) pre_c[c] = ld_stream<NT>(atp(a.Fc + c * P, p));
+          else pre_c[c] = S(0);
+        }
+      }
+    }
     // own node of level z: v from the queue, fl_q and fl_r
     if (inblk) {
       T vq[3], fq[3], fr[3];
@@ -825,17 +852,22 @@ void k_interior(const StepArgs<T, S> a) {
       double srcv[6];
       const bool has_src = a.nsrc > 0 && source_at<T, MESH>(a, x, y, z, (long long)idu, srcv);
       if constexpr (MODE == 3) {
-        // rates and strain rates to W for k_memory_update (as k_node MODE 3)
+        // velocity RK update here, stress rates and strain rates to W for
+        // k_memory_update (as k_node MODE 3)
         const T e[6] = {gr[0][0], gr[1][1], gr[2][2], gr[1][0] + gr[0][1], gr[2][0] + gr[0][2],
                         gr[2][1] + gr[1][2]};
 #pragma unroll
-        for (int c = 0; c < 9; ++c) {
+        for (int c = 0; c < 3; ++c)
+          rk_finish_pre<NT>(a, c, idu, scale_rate(a.A, pre_df[c]) + (double)DU[c], pre_f[c],
+                            pre_c[c]);
+#pragma unroll
+        for (int c = 3; c < 9; ++c) {
           T r = DU[c];
-          if (has_src && c >= 3) r -= (T)srcv[c - 3];
-          st_stream<NT>(atp(a.W + c * P, idu), r);
+          if (has_src) r -= (T)srcv[c - 3];
+          st_stream<NT>(atp(a.W + (c - 3) * P, idu), r);
         }
 #pragma unroll
-        for (int c = 0; c < 6; ++c) st_stream<NT>(atp(a.W + (9 + c) * P, idu), e[c]);
+        for (int c = 0; c < 6; ++c) st_stream<NT>(atp(a.W + (6 + c) * P, idu), e[c]);
       } else {
         if constexpr (NMECH > 0) {
           const T e[6] = {gr[0][0], gr[1][1], gr[2][2], gr[1][0] + gr[0][1],
