@@ -1305,6 +1305,9 @@ static int iface_stage_t(wql_iface* I, double A, double Bdt, double t, int stage
   rc = stage_fused_t<T>(R, A, Bdt, t);
   if (rc) return rc;
   HIP_TRY(hipEventRecord(I->ev_right, R->stream));
+  // block L's stream then orders everything of the stage (callers time and
+  // synchronise on it)
+  HIP_TRY(hipStreamWaitEvent(L->stream, I->ev_right, 0));
   L->cur ^= 1;
   R->cur ^= 1;
   return WQL_OK;

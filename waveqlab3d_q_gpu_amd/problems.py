@@ -447,6 +447,42 @@ def tpv28_curvilinear(n=21, sat_form=1, s0=None):
                            name=f"tpv28_curvilinear_{n}", s0=None if s0 is None else tuple(s0))
 
 
+def fault_two_block(n=(201, 401, 201), h=0.1, nmech=8, npml=20, cfl=0.5, nsteps=2000):
+    """Two Cartesian blocks of n = (nq, nr, ns) nodes each, block L over
+    x in [-(nq-1)h, 0] and block R over [0, (nq-1)h], y (depth, r) in
+    [0, (nr-1)h], z in [-(ns-1)h/2, (ns-1)h/2] (km), coupled by a planar TPV5
+    slip-weakening fault (mu_s 0.677, mu_d 0.525, Dc 0.4 m, sigma_n -120 MPa,
+    tau 70 MPa, 81.6 MPa in the 3 km nucleation patch at y = 7.5 km, z = 0);
+    rho/cs/cp = 2.67/3.464/6, anelastic Q (Qs = 50 cs[km/s], Qp = 2 Qs), free
+    surface on y-min, PML of npml nodes on the other outer faces."""
+    nq, nr, ns = n
+    Lz = (ns - 1) * h
+    XL = cartesian_grid(n, (-(nq - 1) * h, 0.0, -0.5 * Lz), (0.0, (nr - 1) * h, 0.5 * Lz))
+    XR = cartesian_grid(n, (0.0, 0.0, -0.5 * Lz), ((nq - 1) * h, (nr - 1) * h, 0.5 * Lz))
+    M = material_from_rho_s_p(n, 2.67, 3.464, 6.0)
+    tau = weight = Qs_inv = Qp_inv = None
+    if nmech:
+        tau, weight = q_coefficients(nmech)
+        Qs_inv = np.full(XL.shape[1:], 1.0 / (50.0 * 3.464))
+        Qp_inv = 0.5 * Qs_inv
+        M = modulus_correction(M, Qs_inv, Qp_inv, tau, weight)
+    dt = block_time_step(h, cfl, 3.464, 6.0)
+    if nmech:
+        dt = min(dt, relaxation_dt_limit(tau))
+    pmlL = (1, 0, 0, 1, 1, 1) if npml else (0,) * 6
+    pmlR = (0, 1, 0, 1, 1, 1) if npml else (0,) * 6
+    kw = dict(mesh=L.WQL_CARTESIAN, dt=dt, nsteps=nsteps, nmech=nmech, tau=tau, weight=weight,
+              Qs_inv=Qs_inv, Qp_inv=Qp_inv)
+    left = Problem(n=n, X=XL, M=M, bc=(1, 0, 2, 1, 1, 1), pml=pmlL,
+                   npml=tuple(npml * p for p in pmlL), name="fault_block1", **kw)
+    right = Problem(n=n, X=XR, M=M.copy(), bc=(0, 1, 2, 1, 1, 1), pml=pmlR,
+                    npml=tuple(npml * p for p in pmlR), name="fault_block2", **kw)
+    return TwoBlockProblem(left=left, right=right, coupling=L.WQL_IFACE_SLIP_WEAKENING_TPV5,
+                           yf=np.ascontiguousarray(XL[1][:, :, -1]),
+                           zf=np.ascontiguousarray(XL[2][:, :, -1]), dt=dt, nsteps=nsteps,
+                           name=f"fault_two_block_{nq}x{nr}x{ns}")
+
+
 def locked_interface_41():
     """inputfile/test_elastic_locked_interface_o6.in: two 41^3 Cartesian blocks,
     [-2,0] x [-1,1]^2 and [0,2] x [-1,1]^2 (km), rho_s_p = (2.7, 3.464, 6),
