@@ -103,6 +103,7 @@ struct StepArgs {
   // node1 are even)
   long long node0, node1;
   int mem_npt;
+  int mem_nt;       // non-temporal accesses in k_memory_update only
 };
 
 // Launchers (wql3d_kernels.hip).  All return hipError_t of the launch.

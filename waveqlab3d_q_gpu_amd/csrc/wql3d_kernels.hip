@@ -425,44 +425,47 @@ __global__ __launch_bounds__(256) void k_memory_update(const StepArgs<T, S> a) {
 // access is one N-wide vector load or store, so a wavefront moves twice the
 // bytes per memory instruction; the arithmetic per node is unchanged (same
 // operations in the same order as memory_update_planes / rk_finish).
+typedef float f2v __attribute__((ext_vector_type(2)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef double d2v __attribute__((ext_vector_type(2)));
+typedef double d4v __attribute__((ext_vector_type(4)));
 template <int N, typename V> struct VecN;
-template <> struct VecN<2, float> { using type = float2; };
-template <> struct VecN<2, double> { using type = double2; };
-template <> struct VecN<4, float> { using type = float4; };
-template <> struct VecN<4, double> { using type = double4; };
+template <> struct VecN<2, float> { using type = f2v; };
+template <> struct VecN<2, double> { using type = d2v; };
+template <> struct VecN<4, float> { using type = f4v; };
+template <> struct VecN<4, double> { using type = d4v; };
 
-template <int N, typename V>
+// N-wide plane access at node id (id a multiple of N); NT = non-temporal
+template <int N, bool NT = false, typename V>
 __device__ __forceinline__ void ldn(const V* base, unsigned id, V (&o)[N]) {
   using VV = typename VecN<N, V>::type;
-  const VV t = ldp(reinterpret_cast<const VV*>(base), id / N);
-  o[0] = t.x;
-  o[1] = t.y;
-  if constexpr (N == 4) {
-    o[2] = t.z;
-    o[3] = t.w;
-  }
+  const VV* p = reinterpret_cast<const VV*>(reinterpret_cast<const char*>(base) +
+                                            id * (unsigned)sizeof(V));
+  VV t;
+  if constexpr (NT) t = __builtin_nontemporal_load(p);
+  else t = *p;
+#pragma unroll
+  for (int k = 0; k < N; ++k) o[k] = t[k];
 }
-template <int N, typename V>
+template <int N, bool NT = false, typename V>
 __device__ __forceinline__ void stn(V* base, unsigned id, const V (&v)[N]) {
   using VV = typename VecN<N, V>::type;
+  VV* p = reinterpret_cast<VV*>(reinterpret_cast<char*>(base) + id * (unsigned)sizeof(V));
   VV t;
-  t.x = v[0];
-  t.y = v[1];
-  if constexpr (N == 4) {
-    t.z = v[2];
-    t.w = v[3];
-  }
-  *atp(reinterpret_cast<VV*>(base), id / N) = t;
+#pragma unroll
+  for (int k = 0; k < N; ++k) t[k] = v[k];
+  if constexpr (NT) __builtin_nontemporal_store(t, p);
+  else *p = t;
 }
 
-template <typename T, int NMECH, int N>
+template <typename T, int NMECH, int N, bool NT>
 __device__ __forceinline__ void memory_update_planes_n(
     T* __restrict__ eta, T* __restrict__ Deta, const T* __restrict__ Qsp, const T* __restrict__ Qpp,
     long long P, int mech_skip, const double* wgt, const double* tau_inv, double A, double Bdt,
     unsigned id, const T (&mu)[N], const T (&l2m)[N], const T (&e)[N][6], T (&DU)[N][9]) {
   T Qs[N], Qp[N];
-  ldn<N>(Qsp, id, Qs);
-  ldn<N>(Qpp, id, Qp);
+  ldn<N, NT>(Qsp, id, Qs);
+  ldn<N, NT>(Qpp, id, Qp);
   T tr[N], twomuQs[N], pfac[N], muQs[N];
 #pragma unroll
   for (int n = 0; n < N; ++n) {
@@ -484,8 +487,8 @@ __device__ __forceinline__ void memory_update_planes_n(
         for (int n = 0; n < N; ++n) { ev[i][n] = T(0); dv[i][n] = T(0); }
         continue;
       }
-      ldn<N>(eta + (long long)(c * NMECH + i) * P, id, ev[i]);
-      ldn<N>(Deta + (long long)(c * NMECH + i) * P, id, dv[i]);
+      ldn<N, NT>(eta + (long long)(c * NMECH + i) * P, id, ev[i]);
+      ldn<N, NT>(Deta + (long long)(c * NMECH + i) * P, id, dv[i]);
 #pragma unroll
       for (int n = 0; n < N; ++n) sum[n] += ev[i][n];
     }
@@ -505,13 +508,13 @@ __device__ __forceinline__ void memory_update_planes_n(
         dn[n] = scale_rate(A, dv[i][n]) + r;
         en[n] = ev[i][n] + (T)Bdt * dn[n];
       }
-      stn<N>(Deta + (long long)(c * NMECH + i) * P, id, dn);
-      stn<N>(eta + (long long)(c * NMECH + i) * P, id, en);
+      stn<N, NT>(Deta + (long long)(c * NMECH + i) * P, id, dn);
+      stn<N, NT>(eta + (long long)(c * NMECH + i) * P, id, en);
     }
   }
 }
 
-template <typename T, int NMECH, int N, typename S>
+template <typename T, int NMECH, int N, bool NT, typename S>
 __global__ __launch_bounds__(256) void k_memory_update_n(const StepArgs<T, S> a) {
   const unsigned id = (unsigned)a.node0 + (blockIdx.x * 256u + threadIdx.x) * (unsigned)N;
   const long long P = a.P;
@@ -524,29 +527,29 @@ __global__ __launch_bounds__(256) void k_memory_update_n(const StepArgs<T, S> a)
 #pragma unroll
   for (int c = 3; c < 9; ++c) {
     T t[N];
-    ldn<N>(a.W + (c - 3) * P, id, t);
+    ldn<N, NT>(a.W + (c - 3) * P, id, t);
 #pragma unroll
     for (int n = 0; n < N; ++n) DU[n][c] = t[n];
   }
 #pragma unroll
   for (int c = 0; c < 6; ++c) {
     T t[N];
-    ldn<N>(a.W + (6 + c) * P, id, t);
+    ldn<N, NT>(a.W + (6 + c) * P, id, t);
 #pragma unroll
     for (int n = 0; n < N; ++n) e[n][c] = t[n];
   }
   T lam[N], mu[N], l2m[N];
-  ldn<N>(a.M, id, lam);
-  ldn<N>(a.M + P, id, mu);
+  ldn<N, NT>(a.M, id, lam);
+  ldn<N, NT>(a.M + P, id, mu);
 #pragma unroll
   for (int n = 0; n < N; ++n) l2m[n] = lam[n] + T(2) * mu[n];
-  memory_update_planes_n<T, NMECH, N>(a.eta, a.Deta, a.Qs, a.Qp, P, a.mech_skip, a.wgt,
+  memory_update_planes_n<T, NMECH, N, NT>(a.eta, a.Deta, a.Qs, a.Qp, P, a.mech_skip, a.wgt,
                                       a.tau_inv, a.A, a.Bdt, id, mu, l2m, e, DU);
 #pragma unroll
   for (int c = 3; c < 9; ++c) {
     S dfo[N], fin[N];
-    ldn<N>(a.DF + c * P, id, dfo);
-    ldn<N>(a.Fin + c * P, id, fin);
+    ldn<N, NT>(a.DF + c * P, id, dfo);
+    ldn<N, NT>(a.Fin + c * P, id, fin);
     if constexpr (std::is_same<S, double>::value) {
       double df[N], fo[N];
 #pragma unroll
@@ -554,11 +557,11 @@ __global__ __launch_bounds__(256) void k_memory_update_n(const StepArgs<T, S> a)
         df[n] = scale_rate(a.A, dfo[n]) + (double)DU[n][c];
         fo[n] = fin[n] + a.Bdt * df[n];
       }
-      stn<N>(a.DF + c * P, id, df);
-      stn<N>(a.Fout + c * P, id, fo);
+      stn<N, NT>(a.DF + c * P, id, df);
+      stn<N, NT>(a.Fout + c * P, id, fo);
     } else {
       float d[N], cn[N], fo[N], cpv[N];
-      ldn<N>(a.Fc + c * P, id, cpv);
+      ldn<N, NT>(a.Fc + c * P, id, cpv);
 #pragma unroll
       for (int n = 0; n < N; ++n) {
         d[n] = (float)(scale_rate(a.A, dfo[n]) + (double)DU[n][c]);
@@ -567,9 +570,9 @@ __global__ __launch_bounds__(256) void k_memory_update_n(const StepArgs<T, S> a)
         cn[n] = (t - fin[n]) - y;
         fo[n] = t;
       }
-      stn<N>(a.DF + c * P, id, d);
-      stn<N>(a.Fc + c * P, id, cn);
-      stn<N>(a.Fout + c * P, id, fo);
+      stn<N, NT>(a.DF + c * P, id, d);
+      stn<N, NT>(a.Fc + c * P, id, cn);
+      stn<N, NT>(a.Fout + c * P, id, fo);
     }
   }
 }
@@ -1088,17 +1091,17 @@ static hipError_t launch_node(const StepArgs<T, S>& a, int mesh, hipStream_t s) 
 template <typename T, bool NT, typename S>
 static hipError_t launch_memory_update(const StepArgs<T, S>& a, hipStream_t s) {
   if (a.node1 <= a.node0) return hipSuccess;
-  if (!NT && a.mem_npt == 4 && a.node0 % 4 == 0 && a.node1 % 4 == 0) {
+  if (a.mem_npt == 4 && a.node0 % 4 == 0 && a.node1 % 4 == 0) {
     const dim3 g((unsigned)((a.node1 - a.node0 + 1023) / 1024));
-    if (a.nmech == 4) hipLaunchKernelGGL((k_memory_update_n<T, 4, 4, S>), g, dim3(256), 0, s, a);
-    else if (a.nmech == 8) hipLaunchKernelGGL((k_memory_update_n<T, 8, 4, S>), g, dim3(256), 0, s, a);
+    if (a.nmech == 4) hipLaunchKernelGGL((k_memory_update_n<T, 4, 4, NT, S>), g, dim3(256), 0, s, a);
+    else if (a.nmech == 8) hipLaunchKernelGGL((k_memory_update_n<T, 8, 4, NT, S>), g, dim3(256), 0, s, a);
     else return hipErrorInvalidValue;
     return hipGetLastError();
   }
-  if (!NT && a.mem_npt >= 2 && a.node0 % 2 == 0 && a.node1 % 2 == 0) {
+  if (a.mem_npt >= 2 && a.node0 % 2 == 0 && a.node1 % 2 == 0) {
     const dim3 g((unsigned)((a.node1 - a.node0 + 511) / 512));
-    if (a.nmech == 4) hipLaunchKernelGGL((k_memory_update_n<T, 4, 2, S>), g, dim3(256), 0, s, a);
-    else if (a.nmech == 8) hipLaunchKernelGGL((k_memory_update_n<T, 8, 2, S>), g, dim3(256), 0, s, a);
+    if (a.nmech == 4) hipLaunchKernelGGL((k_memory_update_n<T, 4, 2, NT, S>), g, dim3(256), 0, s, a);
+    else if (a.nmech == 8) hipLaunchKernelGGL((k_memory_update_n<T, 8, 2, NT, S>), g, dim3(256), 0, s, a);
     else return hipErrorInvalidValue;
     return hipGetLastError();
   }
@@ -1184,7 +1187,7 @@ hipError_t launch_stage_part(const StepArgs<T, S>& a, int mesh, StagePart part, 
     }
     default:
       if (!split_stage(a)) return hipSuccess;
-      return a.nontemporal ? launch_memory_update<T, true>(a, s)
+      return (a.nontemporal || a.mem_nt) ? launch_memory_update<T, true>(a, s)
                            : launch_memory_update<T, false>(a, s);
   }
 }
@@ -1246,7 +1249,7 @@ hipError_t launch_stage_part_z(const StepArgs<T, S>& a, int mesh, StagePart part
       const long long plane = (long long)a.nx * a.ny;
       b.node0 = z0 * plane;
       b.node1 = z1 >= a.nz ? a.P : z1 * plane;
-      return a.nontemporal ? launch_memory_update<T, true>(b, s)
+      return (a.nontemporal || a.mem_nt) ? launch_memory_update<T, true>(b, s)
                            : launch_memory_update<T, false>(b, s);
     }
   }
