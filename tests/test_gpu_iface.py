@@ -141,9 +141,25 @@ def test_interface_argument_errors(oracle):
         Interface(bl, br, tp.yf, tp.zf)
     with pytest.raises(L.WqlError):          # no adjoint through an interface
         L.check(L.lib().wql_adjoint_create(bl._h, 2))
+    with pytest.raises(L.WqlError):          # caller tables are required for generic friction
+        Interface(gpu_block(tp.left, oracle.metrics(tp.left.X)),
+                  gpu_block(tp.right, oracle.metrics(tp.right.X)), tp.yf, tp.zf,
+                  coupling=L.WQL_IFACE_SLIP_WEAKENING)
+    itf.close()
+    # a block destroyed under a live interface: the interface refuses to step
+    itf = Interface(bl, br, tp.yf, tp.zf)
+    br.close()
+    with pytest.raises(L.WqlError):
+        itf.stage(0.0, 1e-3, 0.0)
     itf.close()
     bl.close()
-    br.close()
+    # FP32-Kahan blocks (FP32 RK state) are refused
+    kl = gpu_block(tp.left, oracle.metrics(tp.left.X), precision=L.WQL_FP32_KAHAN)
+    kr = gpu_block(tp.right, oracle.metrics(tp.right.X), precision=L.WQL_FP32_KAHAN)
+    with pytest.raises(L.WqlError):
+        Interface(kl, kr, tp.yf, tp.zf)
+    kl.close()
+    kr.close()
 
 
 def test_locked_interface_kat_41(oracle):
