@@ -105,14 +105,20 @@ def test_curvilinear_q_pml(oracle, nmech, fused):
 # variables in their own kernel (split 1) or in the node kernel (split 0).
 # WQL3D_STAGE_CHUNKS: z chunks of the two-stream pipeline of anelastic stages
 # (1 = one stream; 13 chunks of 3 z levels = the halo width; 5 uneven chunks).
-@pytest.mark.parametrize("tuning,chunks", [("0,1,0,1,0,64", "4"), ("0,1,0,1,1,7", "4"),
-                                           ("0,1,0,1,2,5", "5"), ("0,1,0,1,3,64", "4"),
-                                           ("0,1,0,0,1,16", "4"), ("0,0,0,1,1,9", "4"),
-                                           ("0,1,0,1,1,32", "1"), ("0,1,0,1,1,32", "13"),
-                                           ("0,1,0,1,0,32", "13")])
-def test_fused_stage_launch_shapes(oracle, monkeypatch, tuning, chunks):
+# WQL3D_MEM_NPT / WQL3D_MEM_NT: nodes per thread (1, 2, 4) and non-temporal
+# streams of the memory-variable kernel.
+@pytest.mark.parametrize("tuning,chunks,npt,nt", [
+    ("0,1,0,1,0,64", "4", "2", "1"), ("0,1,0,1,1,7", "4", "2", "1"),
+    ("0,1,0,1,2,5", "5", "2", "1"), ("0,1,0,1,3,64", "4", "2", "1"),
+    ("0,1,0,0,1,16", "4", "2", "1"), ("0,0,0,1,1,9", "4", "2", "1"),
+    ("0,1,0,1,1,32", "1", "2", "1"), ("0,1,0,1,1,32", "13", "2", "1"),
+    ("0,1,0,1,0,32", "13", "2", "1"), ("0,0,0,1,1,32", "1", "1", "0"),
+    ("0,0,0,1,1,32", "1", "4", "1"), ("0,0,0,1,1,32", "1", "2", "0")])
+def test_fused_stage_launch_shapes(oracle, monkeypatch, tuning, chunks, npt, nt):
     monkeypatch.setenv("WQL3D_NODE_TUNING", tuning)
     monkeypatch.setenv("WQL3D_STAGE_CHUNKS", chunks)
+    monkeypatch.setenv("WQL3D_MEM_NPT", npt)
+    monkeypatch.setenv("WQL3D_MEM_NT", nt)
     prob = small_curvilinear(n=40, nmech=8, nsteps=3, seed=11)
     met = oracle.metrics(prob.X)
     rng = np.random.default_rng(7)
@@ -136,7 +142,8 @@ def test_fused_stage_launch_shapes(oracle, monkeypatch, tuning, chunks):
     time_step_RK(blk, prob.dt, prob.nsteps, fused=True)
     _download_all(blk)
     errs = field_errors(blk.F, orc.F)
-    print(f"tuning={tuning} chunks={chunks} field rel L2:", ["%.2e" % e for e in errs], flush=True)
+    print(f"tuning={tuning} chunks={chunks} npt={npt} nt={nt} field rel L2:",
+          ["%.2e" % e for e in errs], flush=True)
     assert max(errs) <= TOL64
     assert rel_l2(blk.DF, orc.DF) <= TOL64
     assert rel_l2(blk.eta, orc.eta) <= TOL64
