@@ -78,7 +78,9 @@ def byte_model(prob, precision):
     stress rates, six strain rates); shell = the same per node plus the PML
     streams; memory-variable kernel = W read (12), lambda, mu, Q^-1 (4),
     eta/Deta read and write, and the six stress components' F read and write
-    and DF read and write."""
+    and DF read and write.  FP32-mixed: the stencils read F rounded to float
+    (9 x 4 B) plus the velocity components' FP64 F (3 x 8 B), and every F
+    write also writes the float copy (4 B per component)."""
     nq, nr, ns = prob.n
     P = nq * nr * ns
     es = 8 if precision == 0 else 4
@@ -98,11 +100,19 @@ def byte_model(prob, precision):
     # velocity update in the stencil kernels: F write, DF read + write (and
     # the Kahan compensation), the F read being the stencil's own centre value
     vel = 3 * (state - ss)
-    stencil_node = 9 * ss + (ngeo + 3) * es + (12 * es + vel if prob.nmech else 9 * (state - ss))
+    opcopy = precision == 1
+    if opcopy:
+        # float operands; the updated components' FP64 F read and float copy written
+        nupd = 3 if prob.nmech else 9
+        operands = 9 * 4 + nupd * (8 + 4)
+    else:
+        operands = 9 * ss
+    stencil_node = operands + (ngeo + 3) * es + (12 * es + vel if prob.nmech else 9 * (state - ss))
     kernels = {
         "interior": n_int * stencil_node,
         "shell": (P - n_int) * stencil_node + slab * 36 * es,
-        "memory_update": P * (12 * es + 4 * es + 24 * na * es + 6 * state) if prob.nmech else 0,
+        "memory_update": (P * (12 * es + 4 * es + 24 * na * es + 6 * state + (6 * 4 if opcopy else 0))
+                          if prob.nmech else 0),
     }
     return {"P": P, "interior_nodes": n_int, "active_mechanisms": na,
             "end_to_end": end_to_end, "kernels": kernels}

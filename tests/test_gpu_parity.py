@@ -257,3 +257,49 @@ def test_fp32_error_budget(oracle, precision):
     print(f"{name} seismogram rel L2:", ["%.2e" % e for e in serr], flush=True)
     assert max(errs) <= 1e-4
     assert max(serr) <= 1e-5
+
+
+def _state_arrays(blk):
+    arrs = [blk.F, blk.DF]
+    if blk.nmech:
+        arrs += [blk.eta, blk.Deta]
+    arrs += [q for q in blk.pml_Q if q is not None] + [q for q in blk.pml_DQ if q is not None]
+    return arrs
+
+
+@pytest.mark.parametrize("precision", [L.WQL_FP64, L.WQL_FP32_MIXED, L.WQL_FP32_KAHAN])
+def test_state_changes_between_fused_stages(oracle, precision):
+    """Fused steps after the RK state changed through the API (F uploaded,
+    the reference-order RK update, init_state) equal, bit for bit, fused steps
+    on a fresh block given the same state: the FP32-mixed stencils' float
+    copy of F follows every change of F."""
+    prob = small_curvilinear(n=30, nmech=8, nsteps=2, seed=3)
+    met = oracle.metrics(prob.X)
+    blk = gpu_block(prob, met, precision=precision)
+    state = L.WQL_F | L.WQL_DF | L.WQL_MEMORY | L.WQL_PML
+
+    def continues_like_fresh(t, label):
+        blk.download(state)
+        fresh = gpu_block(prob, met, precision=precision)
+        for dst, src in zip(_state_arrays(fresh), _state_arrays(blk)):
+            dst[...] = src
+        fresh.upload(state)
+        t_next = blk.time_step(prob.dt, 2, t)
+        fresh.time_step(prob.dt, 2, t)
+        blk.download(state)
+        fresh.download(state)
+        for k, (a, b) in enumerate(zip(_state_arrays(blk), _state_arrays(fresh))):
+            assert np.array_equal(a, b), (label, k, np.abs(a - b).max())
+        print(f"precision={precision} {label}: identical", flush=True)
+        return t_next
+
+    t = blk.time_step(prob.dt, 3)
+    blk.download(state)
+    blk.F[...] *= 1.25
+    blk.upload(L.WQL_F)
+    t = continues_like_fresh(t, "after F upload")
+    if precision != L.WQL_FP32_KAHAN:
+        blk.update_fields(0.37 * prob.dt)
+        t = continues_like_fresh(t, "after RK update")
+    blk.init_state()
+    continues_like_fresh(0.0, "after init_state")

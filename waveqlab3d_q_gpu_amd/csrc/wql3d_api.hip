@@ -64,6 +64,9 @@ struct wql_block {
   void* F[2];                // S planes
   void* DF;
   void* Fc;                  // FP32-Kahan: compensation of F (float planes)
+  void* Fo[2];               // FP32-mixed: F[k] rounded to float (stencil operands)
+  bool fo_valid;             // Fo[cur] holds F[cur] rounded (cleared wherever F changes
+                             // outside the fused stage)
   void* G;
   void* M;
   void* eta;
@@ -349,6 +352,8 @@ extern "C" int wql_block_create(const wql_block_desc* d, wql_block** out) {
     const long long nm = d->nmech;
     const Part parts[] = {
         {&b->F[0], 9 * P * (long long)b->ssz}, {&b->F[1], 9 * P * (long long)b->ssz},
+        {&b->Fo[0], d->precision == WQL_FP32_MIXED ? 9 * P * 4 : 0},
+        {&b->Fo[1], d->precision == WQL_FP32_MIXED ? 9 * P * 4 : 0},
         {&b->DF, 9 * P * (long long)b->ssz},
         {&b->Fc, d->precision == WQL_FP32_KAHAN ? 9 * P * 4 : 0},
         {&b->G, ngeo * P * (long long)b->esz}, {&b->M, 3 * P * (long long)b->esz},
@@ -358,7 +363,7 @@ extern "C" int wql_block_create(const wql_block_desc* d, wql_block** out) {
     const long long kPage = 2LL << 20;
     long long stagger = 0;
     if (const char* sv = getenv("WQL3D_ARENA_STAGGER")) stagger = atoll(sv);
-    long long off[11], end = 0;
+    long long off[sizeof parts / sizeof parts[0]], end = 0;
     int k = 0;
     for (const Part& p : parts) {
       off[k] = (end + kPage - 1) / kPage * kPage + ((k * stagger) % kPage + kPage) % kPage / 256 * 256;
@@ -591,6 +596,7 @@ static int transfer(wql_block* b, uint32_t mask, bool up) {
     return up ? put_comp<double>(b, dev, v, c, lo1, n, nm) : get_comp<double>(b, dev, v, c, lo1, n, nm);
   };
   if (mask & WQL_F) {
+    if (up) b->fo_valid = false;
     for (int c = 0; c < 9 && !rc; ++c) rc = xfer_state(b->F[b->cur], d.F, c, "F");
     if (!rc && up && kahan) HIP_TRY(launch_fill<float>((float*)b->Fc, 9 * P, 0.0, b->stream));
   }
@@ -692,6 +698,7 @@ static int init_state_t(wql_block* b) {
 extern "C" int wql_block_init_state(wql_block* b) {
   if (!b) return fail(WQL_EINVAL, "null block");
   b->cur = 0;
+  b->fo_valid = false;
   b->mech_zero = (1 << b->d.nmech) - 1;  // eta = Deta = 0 after the fill below
   b->slot = 0;
   return b->d.precision == WQL_FP64 ? init_state_t<double>(b) : init_state_t<float>(b);
@@ -775,6 +782,8 @@ static StepArgs<T, S> make_args(wql_block* b, double t) {
   for (int k = 0; k < 3; ++k) { a.ih[k] = 1.0 / a.h[k]; a.pen[k] = d.tau0 / a.h[k]; }
   a.Fin = (const S*)b->F[b->cur];
   a.Fout = (S*)b->F[b->cur ^ 1];
+  a.Fo_in = (const T*)b->Fo[b->cur];
+  a.Fo_out = (T*)b->Fo[b->cur ^ 1];
   a.DF = (S*)b->DF;
   a.Fc = (S*)b->Fc;
   a.G = (const T*)b->G;
@@ -844,6 +853,7 @@ static int scale_t(wql_block* b, double A) {
 template <typename T>
 static int update_t(wql_block* b, double inc) {
   const long long P = b->P;
+  b->fo_valid = false;
   HIP_TRY(launch_update<double>((double*)b->F[b->cur], (const double*)b->DF, 9 * P, inc, b->stream));
   if (b->d.nmech)
     HIP_TRY(launch_update<T>((T*)b->eta, (const T*)b->Deta, 6LL * b->d.nmech * P, inc, b->stream));
@@ -989,6 +999,16 @@ template <typename T, typename S = double>
 static int stage_fused_t(wql_block* b, double A, double Bdt, double t, bool keep_pending = false) {
   StepArgs<T, S> a = make_args<T, S>(b, t);
   a.A = A; a.Bdt = Bdt;
+  if constexpr (kOpCopy<T, S>) {
+    // the stencils' float operands: refreshed here after F changed outside
+    // the fused stage; every fused stage writes Fo_out beside F_out
+    if (!b->fo_valid) {
+      if (int rc = join(b)) return rc;
+      HIP_TRY(launch_round_float((float*)b->Fo[b->cur], (const double*)b->F[b->cur], 9 * b->P,
+                                 b->stream));
+      b->fo_valid = true;
+    }
+  }
   if (!b->timing) {
     if (b->nchunks > 1 && stage_has_memory_part(a)) {
       if (int rc = stage_pipelined<T, S>(b, a)) return rc;
@@ -1488,6 +1508,7 @@ extern "C" int wql_state_load(wql_block* b, int32_t slot) {
   int rc = check_slot(b, slot);
   if (rc) return rc;
   const long long P = b->P, ne = 6LL * b->d.nmech * P;
+  b->fo_valid = false;
   HIP_TRY(hipMemcpyAsync((double*)b->F[b->cur], b->slot_F + (size_t)slot * 9 * P, 9 * P * sizeof(double),
                          hipMemcpyDeviceToDevice, b->stream));
   if (ne)

@@ -97,6 +97,13 @@ __device__ __forceinline__ V* atp(V* base, unsigned idx) {
   using Byte = typename std::conditional<std::is_const<V>::value, const char, char>::type;
   return reinterpret_cast<V*>(reinterpret_cast<Byte*>(base) + idx * (unsigned)sizeof(V));
 }
+// the planes the stencils difference: F_in, or its float copy (FP32-mixed)
+template <typename T, typename S>
+__device__ __forceinline__ const OpType<T, S>* op_planes(const StepArgs<T, S>& a) {
+  if constexpr (kOpCopy<T, S>) return a.Fo_in;
+  else return a.Fin;
+}
+
 template <typename T, typename S>
 __device__ __forceinline__ void derivs_curv(const StepArgs<T, S>& a, const int pos[3],
                                             long long id64, T U[3][9]) {
@@ -104,9 +111,9 @@ __device__ __forceinline__ void derivs_curv(const StepArgs<T, S>& a, const int p
   const unsigned id = (unsigned)id64;
   const unsigned st[3] = {1u, (unsigned)a.nx, (unsigned)a.nx * (unsigned)a.ny};
   const int n[3] = {a.nx, a.ny, a.nz};
-  const S* Fc[9];
+  const OpType<T, S>* Fc[9];
 #pragma unroll
-  for (int c = 0; c < 9; ++c) Fc[c] = a.Fin + c * P;
+  for (int c = 0; c < 9; ++c) Fc[c] = op_planes(a) + c * P;
   const T* Gk[10];
 #pragma unroll
   for (int k = 0; k < 10; ++k) Gk[k] = a.G + k * P;
@@ -162,9 +169,9 @@ __device__ __forceinline__ void derivs_cart(const StepArgs<T, S>& a, const int p
   const unsigned id = (unsigned)id64;
   const unsigned st[3] = {1u, (unsigned)a.nx, (unsigned)a.nx * (unsigned)a.ny};
   const int n[3] = {a.nx, a.ny, a.nz};
-  const S* Fc[9];
+  const OpType<T, S>* Fc[9];
 #pragma unroll
-  for (int c = 0; c < 9; ++c) Fc[c] = a.Fin + c * P;
+  for (int c = 0; c < 9; ++c) Fc[c] = op_planes(a) + c * P;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     T acc[9];
@@ -469,7 +476,9 @@ __device__ __forceinline__ void rk_finish(const StepArgs<T, S>& a, int c, unsign
   const long long P = a.P;
   if constexpr (std::is_same<S, double>::value) {
     st_stream<NT>(atp(a.DF + c * P, idu), df);
-    st_stream<NT>(atp(a.Fout + c * P, idu), ldp(a.Fin + c * P, idu) + a.Bdt * df);
+    const double fo = ldp(a.Fin + c * P, idu) + a.Bdt * df;
+    st_stream<NT>(atp(a.Fout + c * P, idu), fo);
+    if constexpr (kOpCopy<T, S>) st_stream<NT>(atp(a.Fo_out + c * P, idu), (T)fo);
   } else {
     const float d = (float)df;
     st_stream<NT>(atp(a.DF + c * P, idu), d);
@@ -491,7 +500,9 @@ __device__ __forceinline__ void rk_finish_pre(const StepArgs<T, S>& a, int c, un
   if constexpr (std::is_same<S, double>::value) {
     (void)comp;
     st_stream<NT>(atp(a.DF + c * P, idu), df);
-    st_stream<NT>(atp(a.Fout + c * P, idu), fin + a.Bdt * df);
+    const double fo = fin + a.Bdt * df;
+    st_stream<NT>(atp(a.Fout + c * P, idu), fo);
+    if constexpr (kOpCopy<T, S>) st_stream<NT>(atp(a.Fo_out + c * P, idu), (T)fo);
   } else {
     const float d = (float)df;
     st_stream<NT>(atp(a.DF + c * P, idu), d);

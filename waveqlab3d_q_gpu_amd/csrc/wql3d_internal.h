@@ -6,6 +6,8 @@
 //   multiple of kPlaneAlign nodes (the tail of each plane is padding that
 //   streaming kernels may read and write but no node reads)
 //   F[2]  : 9 planes of double each (ping-pong for the fused stage)
+//   Fo[2] : FP32-mixed only: 9 float planes each, F[k] rounded to float (the
+//           stencil operands of the fused stage)
 //   DF    : 9 planes of double
 //   G     : geometry planes of type T: cartesian 3 (q_x, r_y, s_z);
 //           curvilinear 10 (q_x,r_x,s_x, q_y,r_y,s_y, q_z,r_z,s_z, J)
@@ -22,6 +24,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "../../include/wql3d.h"
 
 namespace wql {
@@ -32,6 +36,12 @@ constexpr int kMaxChunks = 16;
 constexpr int kDefaultChunks = 1;
 // plane stride granularity in nodes (see the device layout above)
 constexpr long long kPlaneAlign = 256;
+
+// the stencil operands come from the float copy of F (FP32-mixed)
+template <typename T, typename S>
+constexpr bool kOpCopy = std::is_same<T, float>::value && std::is_same<S, double>::value;
+template <typename T, typename S>
+using OpType = typename std::conditional<kOpCopy<T, S>, T, S>::type;
 
 struct DevSource {
   int near[3];      // 0-based nearest node
@@ -55,6 +65,12 @@ struct StepArgs {
   S* Fout;
   S* DF;
   S* Fc;            // Kahan compensation of F (S = float only), updated in place
+  // FP32-mixed (T float, S double): F rounded to float, the stencils'
+  // operands (each stencil kernel reads F of a node up to 7 times through the
+  // caches; the float copy halves those bytes).  Fo_in = (float)F_in exactly,
+  // Fo_out is written wherever F_out is.  Null in the other precisions.
+  const T* Fo_in;
+  T* Fo_out;
   const T* G;
   const T* M;
   T* eta;
@@ -134,6 +150,7 @@ template <typename T>
 hipError_t launch_update(T* x, const T* r, long long n, double inc, hipStream_t s);
 template <typename T>
 hipError_t launch_fill(T* x, long long n, double v, hipStream_t s);
+hipError_t launch_round_float(float* dst, const double* src, long long n, hipStream_t s);
 // copy a strided (Fortran-bounded) double array into a dense T plane set and back
 template <typename T>
 hipError_t launch_gather(T* dst, const double* src, int nx, int ny, int nz, int ncomp,

@@ -559,6 +559,12 @@ __global__ __launch_bounds__(256) void k_memory_update_n(const StepArgs<T, S> a)
       }
       stn<N, NT>(a.DF + c * P, id, df);
       stn<N, NT>(a.Fout + c * P, id, fo);
+      if constexpr (kOpCopy<T, S>) {
+        T fo32[N];
+#pragma unroll
+        for (int n = 0; n < N; ++n) fo32[n] = (T)fo[n];
+        stn<N, NT>(a.Fo_out + c * P, id, fo32);
+      }
     } else {
       float d[N], cn[N], fo[N], cpv[N];
       ldn<N, NT>(a.Fc + c * P, id, cpv);
@@ -607,7 +613,7 @@ __host__ __device__ constexpr int sig_comp(int m, int c) {
 // operands, so the two can be placed apart to hide the load latency.
 template <typename T, int MESH, typename S = double>
 struct RawOps {
-  S f[9];
+  OpType<T, S> f[9];
   T g[MESH ? 4 : 1];
 };
 
@@ -616,7 +622,7 @@ __device__ __forceinline__ void load_ops(const StepArgs<T, S>& a, unsigned p,
                                          RawOps<T, MESH, S>& r) {
   const long long P = a.P;
 #pragma unroll
-  for (int c = 0; c < 9; ++c) r.f[c] = ldp(a.Fin + c * P, p);
+  for (int c = 0; c < 9; ++c) r.f[c] = ldp(op_planes(a) + c * P, p);
   if (MESH) {
     r.g[0] = ldp(a.G + 9 * P, p);
 #pragma unroll
@@ -1333,6 +1339,18 @@ template hipError_t launch_update<double>(double*, const double*, long long, dou
 template hipError_t launch_update<float>(float*, const float*, long long, double, hipStream_t);
 template hipError_t launch_fill<double>(double*, long long, double, hipStream_t);
 template hipError_t launch_fill<float>(float*, long long, double, hipStream_t);
+
+// dst = (float)src, round to nearest: the float copy of the RK state
+__global__ __launch_bounds__(256) void k_round_float(float* dst, const double* src, long long n) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  dst[i] = (float)src[i];
+}
+hipError_t launch_round_float(float* dst, const double* src, long long n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_round_float, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dst, src, n);
+  return hipGetLastError();
+}
 
 template <typename T>
 hipError_t launch_gather(T* dst, const double* src, int nx, int ny, int nz, int ncomp,
