@@ -196,8 +196,9 @@ def rel_l2(a, b):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=2)
+    # >= 50 steady-state steps after warm-up, timed with HIP events
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=385)
     ap.add_argument("--nmech", type=int, default=8)
     ap.add_argument("--npml", type=int, default=20)
