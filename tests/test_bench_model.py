@@ -1,0 +1,63 @@
+"""The byte model behind bench.py's roofline figures (host logic, CPU only):
+per-node bytes of each kernel of the 385^3 curvilinear Q8 + PML20 stage,
+counted by hand from the kernels' streams."""
+from types import SimpleNamespace
+
+import numpy as np
+
+import bench
+from waveqlab3d_q_gpu_amd import problems as PB
+
+N = 385
+NODES = N ** 3
+
+
+def _c3():
+    _, weight = PB.q_coefficients(8)
+    return SimpleNamespace(n=(N, N, N), mesh=1, nmech=8, pml=(1, 1, 0, 1, 1, 1),
+                           npml=(20, 20, 0, 20, 20, 20), weight=weight)
+
+
+def test_active_mechanisms_and_boxes():
+    prob = _c3()
+    assert bench.active_mechanisms(prob) == 6          # Q8 weights 2 and 7 are zero
+    lo, hi = bench.interior_box(prob)
+    assert lo == [20, 6, 20] and hi == [364, 364, 364]
+    # five slabs of 20 nodes: x low/high and z low/high over the full faces, y high
+    assert bench.slab_nodes(prob) == 5 * 20 * N * N
+
+
+def test_memory_kernel_bytes_per_node():
+    prob = _c3()
+    # FP32-mixed: W 12 planes + lambda, mu, Qs^-1, Qp^-1 (4 B each), 6 active
+    # mechanisms x 6 components x (eta, Deta) read and written (4 B), the six
+    # stress components' F and DF read and written (8 B) and their float copy
+    # written (4 B)
+    per_node = 12 * 4 + 4 * 4 + 6 * 6 * 4 * 4 + 6 * 4 * 8 + 6 * 4
+    assert per_node == 856
+    assert bench.byte_model(prob, 1)["kernels"]["memory_update"] == NODES * per_node
+    # FP64: everything in 8 B, no float copy
+    assert bench.byte_model(prob, 0)["kernels"]["memory_update"] == NODES * (
+        12 * 8 + 4 * 8 + 6 * 6 * 4 * 8 + 6 * 4 * 8)
+    # FP32-Kahan: F, DF and the compensation in 4 B (F, DF, Fc read + write)
+    assert bench.byte_model(prob, 2)["kernels"]["memory_update"] == NODES * (
+        12 * 4 + 4 * 4 + 6 * 6 * 4 * 4 + 6 * (4 * 4 + 2 * 4))
+
+
+def test_stencil_and_end_to_end_bytes():
+    prob = _c3()
+    m = bench.byte_model(prob, 1)
+    n_int = 345 * 359 * 345
+    assert m["interior_nodes"] == n_int
+    # FP32-mixed stencil node: F as float (36) + velocity FP64 F read (24) and its
+    # float copy written (12) + geometry 10 and material 3 (4 B) + W written (48)
+    # + velocity DF read and written and F written (3 x 24)
+    stencil = 36 + 3 * (8 + 4) + 13 * 4 + 12 * 4 + 3 * 24
+    assert stencil == 244
+    assert m["kernels"]["interior"] == n_int * stencil
+    assert m["kernels"]["shell"] == (NODES - n_int) * stencil + 5 * 20 * N * N * 36 * 4
+    # end to end: F, DF read + write in FP64, geometry + material, eta/Deta
+    # + Q^-1, PML auxiliaries
+    e2e = NODES * (36 * 8 + 13 * 4 + (24 * 6 + 2) * 4) + 5 * 20 * N * N * 36 * 4
+    assert m["end_to_end"] == e2e
+    assert np.isclose(e2e / NODES, 961.4, atol=0.1)
