@@ -12,7 +12,9 @@ and against closed forms / invariants of the discretisation.
     free for r=1, zero incoming characteristic for r=0), and the discrete
     energy never grows with free-surface + characteristic SATs.
   * Q4/Q8: uniform strain rate has the closed-form relaxation solution.
-  * PML: the damping profile including the z-low |z - npml| quirk.
+  * PML: the damping profile including the z-low |z - npml| quirk, and the
+    feedback terms' closed form with F = 0 on a curvilinear mesh (the
+    Jacobian cancels).
   * RK: 4th-order convergence of the KC(5,4) scheme on the relaxation ODE.
   * singular_source: the discrete delta has unit mass and zero first moment.
 """
@@ -318,3 +320,60 @@ def test_singular_source_moments(oracle):
         for ax in range(3):
             first = (g * (X[ax] - loc[ax])).sum() * h ** 3
             assert abs(first) < 1e-12, (order, ax, first)
+
+
+def test_pml_feedback_closed_form_curvilinear(oracle):
+    """With F = 0 every derivative vanishes and set_rates leaves only the PML
+    feedback (RHS_Interior.f90:1846-2265): per slab f containing the node,
+    DU(1:3) -= d_f J Q(1:3) / (rho J) = d_f Q / rho, the normal stresses
+    -= d_f (lambda+2mu, lambda, lambda) . Q(4:6) (and permutations), the
+    shear stresses -= mu d_f Q(7:9), and DQ_f = -(d_f + 1) Q_f.  On a
+    curvilinear mesh with topography the Jacobian must cancel; a dropped or
+    doubled J, or rho in place of rho J, fails."""
+    prob = PB.curvilinear_q(n=24, h=120.0, nsteps=1, nmech=0, npml=9, seed=4, nrec=2,
+                            vk=False)
+    prob.sources = []
+    n = prob.n
+    b = oracle_block(oracle, prob)
+    assert np.ptp(b.J) > 0.01 * np.abs(b.J).mean()     # a genuinely curved mesh
+    rng = np.random.default_rng(5)
+    b.F[...] = 0.0
+    b.DF[...] = 0.0
+    q = {}
+    for f in range(6):
+        if prob.pml[f]:
+            b.pQ[f][...] = rng.uniform(-1, 1, b.pQ[f].shape)
+            b.pDQ[f][...] = 0.0
+            q[f] = b.pQ[f].copy()
+    b.set_rates()
+    lam, mu, rho = prob.M
+    l2m = lam + 2 * mu
+    d0 = 3.0 * math.log(1000.0)
+    idx = [np.arange(1, k + 1, dtype=float) for k in n]
+    expect = np.zeros_like(b.DF)
+    for f, Q in q.items():
+        ax, npml = f // 2, prob.npml[f]
+        i = idx[ax]
+        if f % 2 == 0:
+            dline = d0 * np.abs(i - (npml if f == 4 else 1 + npml)) / npml
+            dline[i > npml] = 0.0
+        else:
+            dline = d0 * np.abs(i - (n[ax] - npml)) / npml
+            dline[i < n[ax] - npml + 1] = 0.0
+        shape = [1, 1, 1]
+        shape[2 - ax] = n[ax]
+        d = dline.reshape(shape)
+        for c in range(3):
+            expect[c] -= d * Q[c] / rho
+        expect[3] -= d * (l2m * Q[3] + lam * Q[4] + lam * Q[5])
+        expect[4] -= d * (lam * Q[3] + l2m * Q[4] + lam * Q[5])
+        expect[5] -= d * (lam * Q[3] + lam * Q[4] + l2m * Q[5])
+        for c in range(6, 9):
+            expect[c] -= mu * d * Q[c]
+        dfull = np.broadcast_to(d, expect.shape[1:])
+        inslab = np.broadcast_to((dline > 0).reshape(shape), expect.shape[1:])
+        assert np.allclose(b.pDQ[f][:, inslab], -(dfull[inslab] + 1.0) * Q[:, inslab],
+                           rtol=1e-13, atol=1e-13), f
+    scale = [np.abs(expect[c]).max() for c in range(9)]
+    for c in range(9):
+        assert np.allclose(b.DF[c], expect[c], rtol=1e-13, atol=1e-13 * scale[c]), c
