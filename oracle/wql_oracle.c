@@ -22,10 +22,9 @@
  *   Q4/Q8                     pinned: closed-form relaxation, modulus formula
  *   PML                       pinned: d=0 reduces to elastic; slab/profile
  *                             quirks and the feedback terms (F = 0, curved
- *                             mesh) checked against closed forms; the
- *                             auxiliary source terms under curvilinear
- *                             metrics are parity unpinned (shared formula
- *                             with the Cartesian case only)
+ *                             mesh) checked against closed forms; auxiliary
+ *                             source terms exact for cubic fields on a
+ *                             non-diagonal affine curvilinear map
  *   orc_source_time           pinned: unit integrals (Gaussian, cosine bell);
  *                             Ricker = -g''/(2 pi^2 f0^2), zero mean, spectral
  *                             peak at f0

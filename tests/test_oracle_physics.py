@@ -14,7 +14,8 @@ and against closed forms / invariants of the discretisation.
   * Q4/Q8: uniform strain rate has the closed-form relaxation solution.
   * PML: the damping profile including the z-low |z - npml| quirk, and the
     feedback terms' closed form with F = 0 on a curvilinear mesh (the
-    Jacobian cancels).
+    Jacobian cancels), and the auxiliary source terms exact for cubic fields
+    on a non-diagonal affine map.
   * RK: 4th-order convergence of the KC(5,4) scheme on the relaxation ODE.
   * singular_source: the discrete delta has unit mass and zero first moment.
 """
@@ -377,3 +378,50 @@ def test_pml_feedback_closed_form_curvilinear(oracle):
     scale = [np.abs(expect[c]).max() for c in range(9)]
     for c in range(9):
         assert np.allclose(b.DF[c], expect[c], rtol=1e-13, atol=1e-13 * scale[c]), c
+
+
+def test_pml_auxiliary_sources_exact_on_cubics_curvilinear(oracle):
+    """With Q = 0 the PML auxiliary rates of slab f are its source terms
+    alone: the derivatives of the fields along the slab's physical axis
+    (RHS_Interior.f90:1846-2265), stresses differentiated in conservative
+    form and divided by J.  On a non-diagonal affine map, for cubic fields,
+    they equal the exact derivatives; a wrong J factor, metric column or
+    component fails."""
+    rng = np.random.default_rng(13)
+    n = (16, 14, 15)
+    A = np.array([[1.2, 0.3, -0.1], [0.2, 0.9, 0.25], [-0.15, 0.1, 1.1]])
+    Xc = PB.cartesian_grid(n, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
+    X = np.einsum("ij,jzyx->izyx", A, Xc) + np.array([0.5, -0.3, 0.2])[:, None, None, None]
+    M = np.empty((3,) + X.shape[1:])
+    M[0] = 2.0 + 0.1 * X[0]
+    M[1] = 1.5 + 0.05 * X[1]
+    M[2] = 2.5 - 0.1 * X[2]
+    prob = PB.Problem(n=n, mesh=1, X=X, M=M, bc=(0,) * 6, dt=0.0, nsteps=0,
+                      pml=(1,) * 6, npml=(5,) * 6)
+    b = oracle_block(oracle, prob)
+    F, G = _cubic_fields(X, rng)
+    b.F[...] = F
+    b.DF[...] = 0.0
+    for f in range(6):
+        b.pQ[f][...] = 0.0
+        b.pDQ[f][...] = 0.0
+    b.set_rates()
+    # G[c, d] = dF_c/dx_d; F = (vx, vy, vz, sxx, syy, szz, sxy, sxz, syz)
+    zero = np.zeros_like(G[0, 0])
+    src = {
+        0: [G[3, 0], G[6, 0], G[7, 0], G[0, 0], zero, zero, G[1, 0], G[2, 0], zero],
+        1: [G[6, 1], G[4, 1], G[8, 1], zero, G[1, 1], zero, G[0, 1], zero, G[2, 1]],
+        2: [G[7, 2], G[8, 2], G[5, 2], zero, zero, G[2, 2], zero, G[0, 2], G[1, 2]],
+    }
+    for f in range(6):
+        ax = f // 2
+        sl = [slice(None)] * 3
+        sl[2 - ax] = slice(0, 5) if f % 2 == 0 else slice(n[ax] - 5, n[ax])
+        sl = tuple(sl)
+        expect = np.stack([s[sl] for s in src[ax]])
+        got = b.pDQ[f][(slice(None),) + sl]
+        err = np.abs(got - expect).max() / np.abs(expect).max()
+        assert err < 1e-11, (f, err)
+    # the elastic rates are those of the exact gradients (Q = 0: no feedback)
+    R = _exact_rates(F, G, M)
+    assert np.abs(b.DF - R).max() / np.abs(R).max() < 1e-11
