@@ -11,7 +11,9 @@ and against closed forms / invariants of the discretisation.
   * SAT: the penalty vanishes when the boundary condition holds (traction
     free for r=1, zero incoming characteristic for r=0), and the discrete
     energy never grows with free-surface + characteristic SATs.
-  * Q4/Q8: uniform strain rate has the closed-form relaxation solution.
+  * Q4/Q8: uniform strain rate has the closed-form relaxation solution;
+    inside PML slabs the memory variables relax towards the PML-modified
+    strains (closed form with zero fields, edges and corners included).
   * PML: the damping profile including the z-low |z - npml| quirk, and the
     feedback terms' closed form with F = 0 on a curvilinear mesh (the
     Jacobian cancels), and the auxiliary source terms exact for cubic fields
@@ -425,3 +427,61 @@ def test_pml_auxiliary_sources_exact_on_cubics_curvilinear(oracle):
     # the elastic rates are those of the exact gradients (Q = 0: no feedback)
     R = _exact_rates(F, G, M)
     assert np.abs(b.DF - R).max() / np.abs(R).max() < 1e-11
+
+
+def test_anelastic_strains_in_pml_closed_form(oracle):
+    """With F = 0 and eta = 0, the memory-variable rates inside the PML slabs
+    come from the PML-modified strains alone (RHS_Interior.f90:2857-2864):
+    eps_aa = -d_a Q_a(4/5/6), gamma_xy = -d_x Q_x(7) - d_y Q_y(7),
+    gamma_xz = -d_x Q_x(8) - d_z Q_z(8), gamma_yz = -d_y Q_y(9) - d_z Q_z(9),
+    then Deta_i = src_i(eps) / tau_i; edges and corners take two or three
+    slabs."""
+    nmech = 8
+    prob = _uniform_strain_problem(nmech, n=20)
+    prob.pml, prob.npml = (1,) * 6, (6,) * 6
+    n = prob.n
+    b = oracle_block(oracle, prob)
+    rng = np.random.default_rng(17)
+    b.F[...] = 0.0
+    b.DF[...] = 0.0
+    b.eta[...] = 0.0
+    b.Deta[...] = 0.0
+    Qf = {}
+    for f in range(6):
+        b.pQ[f][...] = rng.uniform(-1, 1, b.pQ[f].shape)
+        b.pDQ[f][...] = 0.0
+        Qf[f] = b.pQ[f].copy()
+    b.set_rates()
+    d0 = 3.0 * math.log(1000.0)
+    idx = [np.arange(1, k + 1, dtype=float) for k in n]
+    dax = [np.zeros((n[2], n[1], n[0])) for _ in range(3)]
+    qax = [np.zeros((9, n[2], n[1], n[0])) for _ in range(3)]
+    for f in range(6):
+        ax, npml = f // 2, 6
+        i = idx[ax]
+        if f % 2 == 0:
+            dl = d0 * np.abs(i - (npml if f == 4 else 1 + npml)) / npml
+            ins = i <= npml
+        else:
+            dl = d0 * np.abs(i - (n[ax] - npml)) / npml
+            ins = i >= n[ax] - npml + 1
+        shape = [1, 1, 1]
+        shape[2 - ax] = n[ax]
+        m = np.broadcast_to(ins.reshape(shape), dax[ax].shape)
+        dax[ax] = np.where(m, np.broadcast_to(dl.reshape(shape), m.shape), dax[ax])
+        qax[ax] = np.where(m, Qf[f], qax[ax])
+    dx, dy, dz = dax
+    qx, qy, qz = qax
+    e = [-dx * qx[3], -dy * qy[4], -dz * qz[5], -dx * qx[6] - dy * qy[6],
+         -dx * qx[7] - dz * qz[7], -dy * qy[8] - dz * qz[8]]
+    lam, mu = prob.M[0], prob.M[1]
+    Qs, Qp = prob.Qs_inv, prob.Qp_inv
+    tr = e[0] + e[1] + e[2]
+    for i in range(nmech):
+        w, ta = prob.weight[i], prob.tau[i]
+        for c in range(6):
+            src = (w * 2 * mu * Qs * e[c] + w * ((lam + 2 * mu) * Qp - 2 * mu * Qs) * tr
+                   if c < 3 else w * mu * Qs * e[c])
+            expect = src / ta
+            scale = max(np.abs(expect).max(), 1e-300)
+            assert np.allclose(b.Deta[c, i], expect, rtol=1e-12, atol=1e-13 * scale), (c, i)
