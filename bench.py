@@ -144,34 +144,47 @@ def timed_run(blk, prob, steps, warmup, t0):
     return e0.elapsed_time(e1), wall, t
 
 
-def cpu_baseline(n_sample, nmech, npml, seed):
-    """Oracle (scalar C, 1 thread) on a sub-block of the same set-up, 1 step."""
+def cpu_baseline(n_sample, n_single, nmech, npml, seed):
+    """Oracle (C, OpenMP over its node loops) on sub-blocks of the same set-up,
+    1 step each: all host threads (OMP_NUM_THREADS) on n_sample^3, and one
+    thread on n_single^3."""
     from oracle import oracle as O
     from waveqlab3d_q_gpu_amd import problems as PB
-    prob = PB.curvilinear_q(n=n_sample, h=50.0, nsteps=1, nmech=nmech, npml=npml, seed=seed,
-                            nrec=4)
-    met = O.metrics(prob.X)
-    nq, nr, ns = prob.n
-    b = O.OracleBlock(nq, nr, ns, curvilinear=1)
-    b.metx, b.mety, b.metz, b.J = met
-    b.M[...] = prob.M
-    b.bc = list(prob.bc)
-    for f in range(6):
-        if prob.pml[f]:
-            b.set_pml(f, prob.npml[f])
-    if nmech:
-        b.set_anelastic(nmech, prob.Qs_inv, prob.Qp_inv, correct_modulus=False)
-    b.normals()
-    t0 = time.perf_counter()
-    b.time_step(prob.dt, 1, 0.0)
-    el = time.perf_counter() - t0
-    P = nq * nr * ns
-    return {"value": round(P * 5 / el / 1e6, 4), "unit": "Mpt-RK-stage/s", "cores": 1,
+
+    def run(n, threads):
+        prob = PB.curvilinear_q(n=n, h=50.0, nsteps=1, nmech=nmech, npml=npml, seed=seed,
+                                nrec=4)
+        met = O.metrics(prob.X)
+        nq, nr, ns = prob.n
+        b = O.OracleBlock(nq, nr, ns, curvilinear=1)
+        b.metx, b.mety, b.metz, b.J = met
+        b.M[...] = prob.M
+        b.bc = list(prob.bc)
+        for f in range(6):
+            if prob.pml[f]:
+                b.set_pml(f, prob.npml[f])
+        if nmech:
+            b.set_anelastic(nmech, prob.Qs_inv, prob.Qp_inv, correct_modulus=False)
+        b.normals()
+        O.set_num_threads(threads)
+        t0 = time.perf_counter()
+        b.time_step(prob.dt, 1, 0.0)
+        el = time.perf_counter() - t0
+        return nq * nr * ns * 5 / el / 1e6, el
+
+    threads = O.num_threads()
+    value, el = run(n_sample, threads)
+    v1, el1 = run(n_single, 1)
+    O.set_num_threads(threads)
+    return {"value": round(value, 4), "unit": "Mpt-RK-stage/s", "cores": threads,
             "kind": "port",
             "sample": f"{n_sample}^3 sub-block of the same curvilinear Q{nmech} + PML{npml} "
-                      f"set-up, 1 step (5 stages), oracle/wql_oracle.c (gcc -O2, 1 thread, "
-                      f"no MPI/OpenMP)",
-            "seconds": round(el, 3)}
+                      f"set-up, 1 step (5 stages), oracle/wql_oracle.c (gcc -O2, OpenMP "
+                      f"{threads} threads over the node loops, no MPI)",
+            "seconds": round(el, 3),
+            "single_thread": {"value": round(v1, 4), "cores": 1,
+                              "sample": f"{n_single}^3 sub-block, 1 step, 1 thread",
+                              "seconds": round(el1, 3)}}
 
 
 def read_traffic(name):
@@ -210,7 +223,10 @@ def main():
                     help="skip the two-block fault workload (2 x 201x401x201, Q8, PML 20)")
     ap.add_argument("--no-accuracy", action="store_true",
                     help="skip the FP32-mixed vs FP64 field comparison (host copies)")
-    ap.add_argument("--cpu-sample", type=int, default=97)
+    # oracle sub-blocks: n^3 on all host threads (~10-30 s of CPU work), and a
+    # smaller one on one thread
+    ap.add_argument("--cpu-sample", type=int, default=161)
+    ap.add_argument("--cpu-sample-single", type=int, default=97)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run a device copy of this many MiB 5 times (scale factor "
                          "for FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -343,7 +359,7 @@ def main():
             "stage_end_to_end": {"achieved": round(r32["stage_GBs"], 1),
                                  "frac": round(r32["stage_GBs"] / HBM_PEAK_GBS, 4),
                                  "bytes_per_stage": r32["stage_bytes"]}}
-    cpu = None if args.no_cpu else cpu_baseline(args.cpu_sample, args.nmech, args.npml, 2024)
+    cpu = None if args.no_cpu else cpu_baseline(args.cpu_sample, args.cpu_sample_single, args.nmech, args.npml, 2024)
     out = {
         "metric": METRIC, "value": round(r32["value"], 2), "unit": "Mpt-RK-stage/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,

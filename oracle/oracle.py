@@ -92,6 +92,8 @@ def lib():
                                     C.c_int, C.POINTER(C.c_long), _P]
         L.orc_source_time.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double]
         L.orc_source_time.restype = C.c_double
+        L.orc_num_threads.restype = C.c_int
+        L.orc_set_num_threads.argtypes = [C.c_int]
         L.orc_iface_enforce.argtypes = [C.POINTER(_Iface), C.POINTER(_Block), C.POINTER(_Block),
                                         C.c_double, C.c_int, C.c_int]
         L.orc_iface_scale.argtypes = [C.POINTER(_Iface), C.c_double]
@@ -145,6 +147,15 @@ def q_coefficients(nmech, fmin=0.05, fmax=20.0):
 
 def source_time(kind, t, t0, w):
     return lib().orc_source_time(kind, t, t0, w)
+
+
+def num_threads():
+    """OpenMP threads of the node loops (OMP_NUM_THREADS, or set_num_threads)."""
+    return lib().orc_num_threads()
+
+
+def set_num_threads(n):
+    lib().orc_set_num_threads(int(n))
 
 
 class OracleBlock:

@@ -33,6 +33,28 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* Node loops whose iterations write only their own node (the RHS, the RK
+ * vector updates) run on OpenMP threads; each node's arithmetic is the same
+ * on any thread, so the results do not depend on the thread count. */
+int orc_num_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+void orc_set_num_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
 
 #define IDX(b, x, y, z) ((long)((x) - 1) + (long)(b)->nx * ((long)((y) - 1) + (long)(b)->ny * (long)((z) - 1)))
 
@@ -290,10 +312,15 @@ static long pml_slab_contains(const orc_block *b, int f, int x, int y, int z) {
 
 void orc_scale_rates(orc_block *b, double A) {
   long N = (long)b->nx * b->ny * b->nz;
+#pragma omp parallel for schedule(static)
   for (long i = 0; i < 9 * N; ++i) b->DF[i] = A * b->DF[i];
-  if (b->nmech) for (long i = 0; i < 6L * b->nmech * N; ++i) b->Deta[i] = A * b->Deta[i];
+  if (b->nmech) {
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < 6L * b->nmech * N; ++i) b->Deta[i] = A * b->Deta[i];
+  }
   for (int f = 0; f < 6; ++f) {
     if (!b->pml[f]) continue;
+#pragma omp parallel for schedule(static)
     for (int z = 1; z <= b->nz; ++z)
       for (int y = 1; y <= b->ny; ++y)
         for (int x = 1; x <= b->nx; ++x)
@@ -307,10 +334,15 @@ void orc_scale_rates(orc_block *b, double A) {
 
 void orc_update_fields(orc_block *b, double inc) {
   long N = (long)b->nx * b->ny * b->nz;
+#pragma omp parallel for schedule(static)
   for (long i = 0; i < 9 * N; ++i) b->F[i] = b->F[i] + inc * b->DF[i];
-  if (b->nmech) for (long i = 0; i < 6L * b->nmech * N; ++i) b->eta[i] = b->eta[i] + inc * b->Deta[i];
+  if (b->nmech) {
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < 6L * b->nmech * N; ++i) b->eta[i] = b->eta[i] + inc * b->Deta[i];
+  }
   for (int f = 0; f < 6; ++f) {
     if (!b->pml[f]) continue;
+#pragma omp parallel for schedule(static)
     for (int z = 1; z <= b->nz; ++z)
       for (int y = 1; y <= b->ny; ++y)
         for (int x = 1; x <= b->nx; ++x)
@@ -410,6 +442,7 @@ static void rhs_center(orc_block *b) {
   for (int s = 0; s < 3; ++s) { cx[s] = c3[s] / b->hq; cy[s] = c3[s] / b->hr; cz[s] = c3[s] / b->hs; }
   const double *F = b->F, *J = b->J;
   const double *met[3] = {b->metx, b->mety, b->metz};
+#pragma omp parallel for collapse(2) schedule(static)
   for (int z = iz + 1; z <= b->nz - fz; ++z)
     for (int y = iy + 1; y <= b->ny - fy; ++y)
       for (int x = ix + 1; x <= b->nx - fx; ++x) {
@@ -567,6 +600,7 @@ static void rhs_near_boundaries(orc_block *b) {
   long N = (long)b->nx * b->ny * b->nz;
   int ix, fx, iy, fy, iz, fz;
   interior_extents(b, &ix, &fx, &iy, &fy, &iz, &fz);
+#pragma omp parallel for collapse(2) schedule(static)
   for (int z = 1; z <= b->nz; ++z)
     for (int y = 1; y <= b->ny; ++y)
       for (int x = 1; x <= b->nx; ++x) {

@@ -99,6 +99,10 @@ void orc_update_fields(orc_block *b, double inc);           /* fields.f90:158-26
 void orc_time_step(orc_block *b, double dt, int nsteps, double *t,
                    int nrec, const long *rec_idx, double *seis);
 
+/* OpenMP threads the node loops use (1 without OpenMP) */
+int orc_num_threads(void);
+void orc_set_num_threads(int n);
+
 /* source_time (moment_tensor.f90:813-872) + Ricker (new) */
 double orc_source_time(int type, double t, double t0, double w);
 

@@ -485,3 +485,27 @@ def test_anelastic_strains_in_pml_closed_form(oracle):
             expect = src / ta
             scale = max(np.abs(expect).max(), 1e-300)
             assert np.allclose(b.Deta[c, i], expect, rtol=1e-12, atol=1e-13 * scale), (c, i)
+
+
+def test_thread_count_does_not_change_results(oracle):
+    """The oracle's node loops run on OpenMP threads; every threaded loop
+    writes only its own node, so the state after a step is bit-identical for
+    any thread count."""
+    prob = PB.curvilinear_q(n=22, h=120.0, nsteps=2, nmech=8, npml=7, seed=9, nrec=2, f0=4.0,
+                            vk=False)
+    rng = np.random.default_rng(23)
+    F0 = rng.uniform(-1, 1, (9,) + prob.X.shape[1:])
+    F0[3:] *= 1e6
+    prob.F0 = F0
+    met = oracle.metrics(prob.X)
+    threads = oracle.num_threads()
+    out = []
+    for nt in (1, max(2, threads)):
+        oracle.set_num_threads(nt)
+        b = oracle_block(oracle, prob, met)
+        b.time_step(prob.dt, prob.nsteps, 0.5 / 4.0)
+        out.append((b.F.copy(), b.eta.copy(), [q.copy() for q in b.pQ if q is not None]))
+    oracle.set_num_threads(threads)
+    a, c = out
+    assert np.array_equal(a[0], c[0]) and np.array_equal(a[1], c[1])
+    assert all(np.array_equal(x, y) for x, y in zip(a[2], c[2]))
