@@ -44,8 +44,9 @@ enum {
 
 enum { WQL_CARTESIAN = 0, WQL_CURVILINEAR = 1 };       /* type_of_mesh */
 /* arithmetic mode: FP64; FP32 stencil/metric/memory arithmetic with the RK
- * state F, DF in FP64 (FP32_MIXED); or all FP32 with a Kahan-compensated F
- * update (FP32_KAHAN, fused stage only) */
+ * state F, DF in FP64 (FP32_MIXED; the block also keeps F rounded to float as
+ * the stencils' operands, 72 B per node); or all FP32 with a
+ * Kahan-compensated F update (FP32_KAHAN, fused stage only) */
 enum { WQL_FP64 = 0, WQL_FP32_MIXED = 1, WQL_FP32_KAHAN = 2 };
 enum { WQL_BC_NONE = 0, WQL_BC_CHARACTERISTIC = 1,     /* BoundaryConditions.f90:28-30 */
        WQL_BC_FREE_SURFACE = 2, WQL_BC_CLAMPED = 3 };
@@ -82,7 +83,7 @@ typedef struct wql_box {
 typedef struct wql_block_desc {
   int32_t nq, nr, ns;        /* nodes of the block, nodes are 1..n per axis */
   int32_t mesh;              /* WQL_CARTESIAN / WQL_CURVILINEAR */
-  int32_t precision;         /* WQL_FP64 / WQL_FP32_MIXED */
+  int32_t precision;         /* WQL_FP64 / WQL_FP32_MIXED / WQL_FP32_KAHAN */
   int32_t nmech;             /* 0 elastic, 4 anelastic-Q4, 8 anelastic-Q8 */
   int32_t bc[6];             /* Lx,Rx,Ly,Ry,Lz,Rz (boundary_type Lx..Rz) */
   int32_t pml[6];            /* PMLB(1..6)%pml */
