@@ -120,3 +120,45 @@ def test_adjoint_gradient_matches_central_differences():
         rel = abs(fd - adj) / abs(fd)
         print(f"direction {k}: fd {fd!r} adjoint {adj!r} rel {rel:.2e}", flush=True)
         assert rel <= 1e-6
+
+
+def test_adjoint_fp32_mixed_matches_fp64():
+    """The FP32-mixed block (float stencil and memory arithmetic, FP64 RK
+    state) through the same adjoint runner: checkpoint reloads and the
+    forward recomputation between them (which rebuild the stencils' float
+    copy of F) give the FP64 misfit and gradient to FP32 accuracy."""
+    import torch
+    from waveqlab3d_q_gpu_amd.adjoint import AdjointRunner, receiver_traces
+    N = 40
+    prob = _problem("curv_q4", n=28, nsteps=N, f0=20.0)
+    prob.sources[0].t_init = 0.06
+    near = np.asarray(prob.sources[0].near)
+    rec = np.asarray([near + o for o in ((5, 0, 0), (-5, 0, 0), (0, -5, 0), (0, 5, 0))])
+    out = {}
+    for prec in (L.WQL_FP64, L.WQL_FP32_MIXED):
+        blk = PB.make_device_block(prob, precision=prec, state_mirror=True)
+        run = AdjointRunner(blk, prob.dt, N, rec, every=7)
+        dev = blk.M.device
+        M0 = blk.M.clone()
+        lam0, mu0, rho0 = M0[0].clone(), M0[1].clone(), M0[2].clone()
+        rng = np.random.default_rng(5)
+        pert = [torch.from_numpy(1.0 + 0.02 * rng.uniform(-1, 1, lam0.shape)).to(dev)
+                for _ in range(3)]
+        with torch.no_grad():
+            obs = receiver_traces(rho0 * pert[0], lam0 * pert[1], mu0 * pert[2], run).clone()
+        rho = rho0.clone().requires_grad_(True)
+        lam = lam0.clone().requires_grad_(True)
+        mu = mu0.clone().requires_grad_(True)
+        tr = receiver_traces(rho, lam, mu, run)
+        J = 0.5 * ((tr[..., :3] - obs[..., :3]) ** 2).sum()
+        J.backward()
+        d = [torch.from_numpy(rng.uniform(-1, 1, lam0.shape)).to(dev) * m
+             for m in (rho0, lam0, mu0)]
+        dJ = sum(float((g * dk).sum()) for g, dk in zip((rho.grad, lam.grad, mu.grad), d))
+        out[prec] = (float(J), dJ)
+        blk.close()
+    (J64, g64), (J32, g32) = out[L.WQL_FP64], out[L.WQL_FP32_MIXED]
+    print(f"misfit FP64 {J64!r} FP32-mixed {J32!r}; directional gradient {g64!r} {g32!r}",
+          flush=True)
+    assert J64 > 0 and abs(J32 - J64) / J64 <= 1e-4
+    assert abs(g32 - g64) / abs(g64) <= 1e-3

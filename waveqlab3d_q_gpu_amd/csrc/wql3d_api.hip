@@ -995,20 +995,24 @@ static int stage_pipelined(wql_block* b, const StepArgs<T, S>& a) {
   return WQL_OK;
 }
 
+// FP32-mixed: the float copy of F[cur] that every kernel differencing F reads
+// (the fused stage's stencils, the adjoint's material gradient), rebuilt here
+// after F changed outside the fused stage; every fused stage writes Fo_out
+// beside F_out, so the copy of the new state is current afterwards
+static int ensure_operand_copy(wql_block* b) {
+  if (!b->Fo[0] || b->fo_valid) return WQL_OK;
+  if (int rc = join(b)) return rc;
+  HIP_TRY(launch_round_float((float*)b->Fo[b->cur], (const double*)b->F[b->cur], 9 * b->P,
+                             b->stream));
+  b->fo_valid = true;
+  return WQL_OK;
+}
+
 template <typename T, typename S = double>
 static int stage_fused_t(wql_block* b, double A, double Bdt, double t, bool keep_pending = false) {
   StepArgs<T, S> a = make_args<T, S>(b, t);
   a.A = A; a.Bdt = Bdt;
-  if constexpr (kOpCopy<T, S>) {
-    // the stencils' float operands: refreshed here after F changed outside
-    // the fused stage; every fused stage writes Fo_out beside F_out
-    if (!b->fo_valid) {
-      if (int rc = join(b)) return rc;
-      HIP_TRY(launch_round_float((float*)b->Fo[b->cur], (const double*)b->F[b->cur], 9 * b->P,
-                                 b->stream));
-      b->fo_valid = true;
-    }
-  }
+  if (int rc = ensure_operand_copy(b)) return rc;
   if (!b->timing) {
     if (b->nchunks > 1 && stage_has_memory_part(a)) {
       if (int rc = stage_pipelined<T, S>(b, a)) return rc;
@@ -1522,6 +1526,7 @@ extern "C" int wql_state_load(wql_block* b, int32_t slot) {
 
 template <typename T>
 static int adjoint_stage_t(wql_block* b, double A, double Bdt) {
+  if (int rc = ensure_operand_copy(b)) return rc;   // the gradient differences F
   StepArgs<T> a = make_args<T>(b, 0.0);
   AdjArgs<T> j;
   j.Fb = b->adj_Fb;
