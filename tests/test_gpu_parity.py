@@ -303,3 +303,46 @@ def test_state_changes_between_fused_stages(oracle, precision):
         t = continues_like_fresh(t, "after RK update")
     blk.init_state()
     continues_like_fresh(0.0, "after init_state")
+
+
+@pytest.mark.parametrize("precision", [L.WQL_FP32_MIXED, L.WQL_FP32_KAHAN, L.WQL_FP64])
+def test_launch_shapes_consistent_all_precisions(oracle, monkeypatch, precision):
+    """Every precision (the FP32-mixed float copy of F, the Kahan
+    compensation, FP64) under launch shapes that cut a stage differently.
+    The two-stream z-chunk pipeline runs the same kernels on the same nodes,
+    so it must reproduce the one-stream stage bit for bit (its event order
+    covers F, the float copy and the memory variables).  Other tiles, and the
+    shell kernel over the whole block (JJU form instead of the flux form),
+    evaluate the same rates with another operation order: equal to rounding
+    (FP64 1e-13, FP32 1e-5 relative)."""
+    prob = small_curvilinear(n=40, nmech=8, nsteps=3, seed=11)
+    met = oracle.metrics(prob.X)
+    rng = np.random.default_rng(7)
+    F0 = rng.uniform(-1, 1, (9,) + prob.X.shape[1:])
+    F0[3:] *= 1e6
+
+    def run(tuning, chunks):
+        monkeypatch.setenv("WQL3D_NODE_TUNING", tuning)
+        monkeypatch.setenv("WQL3D_STAGE_CHUNKS", chunks)
+        prob.F0 = F0.copy()          # the block's host F aliases prob.F0
+        blk = gpu_block(prob, met, precision=precision)
+        blk.set_receivers(prob.receivers, prob.nsteps)
+        time_step_RK(blk, prob.dt, prob.nsteps, fused=True)
+        _download_all(blk)
+        out = (blk.F.copy(), blk.DF.copy(), blk.eta.copy(), blk.seismograms())
+        blk.close()
+        return out
+
+    tol = 1e-13 if precision == L.WQL_FP64 else 1e-5
+    ref = run("0,0,0,1,1,32", "1")
+    failures = []
+    for tuning, chunks, exact in (("0,0,0,1,1,32", "5", True), ("0,0,0,1,1,32", "13", True),
+                                  ("0,1,0,1,2,5", "1", False), ("0,0,0,1,0,32", "4", False)):
+        got = run(tuning, chunks)
+        diff = [float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+                for a, b in zip(got, ref)]
+        print(f"precision={precision} tuning={tuning} chunks={chunks}: max rel diff "
+              f"{['%.1e' % d for d in diff]}", flush=True)
+        if (max(diff) > 0.0) if exact else (max(diff) > tol):
+            failures.append((tuning, chunks, diff))
+    assert not failures, failures
