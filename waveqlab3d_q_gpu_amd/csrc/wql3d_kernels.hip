@@ -1,14 +1,18 @@
 // HIP kernels for gfx950 (MI355X) of the per-block WaveQLab3D-Q time step.
 //
-// One thread per node.  The workgroup (256 threads = 4 wavefronts) covers a
-// contiguous run of the (x,y) plane of one z level; x is the fastest index,
-// so every plane load of a wavefront is one unit-stride 512 B (fp64) or
-// 256 B (fp32) transaction.  Stencil neighbours are re-read through the L1 /
-// L2 caches; the per-node streams (DF, memory variables, PML auxiliaries)
-// are read and written exactly once per stage.
+// A fused RK stage is three kernels: k_interior (the strict interior, a
+// z-marching 64x8 tile with its x/y operands staged in LDS and a register
+// queue along z), k_node (the shell: closure rows, PML slabs, faces; one
+// thread per node) and k_memory_update_n (memory variables and the stress
+// components' RK update, pure streaming, two nodes per thread).
 //
-// Closure rows differ only per (y,z) line for the y/z axes, so wavefronts
-// stay convergent there.  Device helpers: wql3d_device.h.
+// k_node: the workgroup (256 threads = 4 wavefronts) covers a contiguous run
+// of one box row by row; x is the fastest index, so every plane load of a
+// wavefront is unit stride.  Stencil neighbours are re-read through the L1 /
+// L2 caches; the per-node streams (DF, memory variables, PML auxiliaries)
+// are read and written exactly once per stage.  Closure rows differ only per
+// (y,z) line for the y/z axes, so wavefronts stay convergent there.  Device
+// helpers: wql3d_device.h.
 #include "wql3d_device.h"
 
 namespace wql {
