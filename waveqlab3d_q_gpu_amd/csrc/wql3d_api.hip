@@ -7,6 +7,8 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
+#include <thread>
 #include <vector>
 
 #include "wql3d_adjoint.h"
@@ -131,6 +133,13 @@ struct wql_block {
   hipEvent_t ev_sten[kMaxChunks];
   hipEvent_t ev_mem[kMaxChunks];
   int pending;
+  // pinned host staging of host-array transfers (allocated on the first
+  // one, one plane of doubles each): a component is packed into one buffer
+  // while the other's copy runs; stage_ev[k] marks buffer k free again
+  void* stage_buf[2];
+  size_t stage_bytes;
+  hipEvent_t stage_ev[2];
+  int stage_k;
 };
 
 // Two-block q-interface (wql_iface_create): block L's x-high face against
@@ -257,6 +266,10 @@ extern "C" void wql_block_destroy(wql_block* b) {
                   b->adj_g, b->adj_Z, b->slot_F, b->slot_E};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+  for (int k = 0; k < 2; ++k) {
+    if (b->stage_buf[k]) (void)hipHostFree(b->stage_buf[k]);
+    if (b->stage_ev[k]) (void)hipEventDestroy(b->stage_ev[k]);
+  }
   for (int f = 0; f < 6; ++f) {
     if (b->pQ[f]) (void)hipFree(b->pQ[f]);
     if (b->pDQ[f]) (void)hipFree(b->pDQ[f]);
@@ -447,6 +460,46 @@ static int check_view(const View& v, const int lo1[3], const int n[3], const cha
   return WQL_OK;
 }
 
+// Run body(z0, z1) over the z levels [0, nz) on up to 16 host threads.
+template <typename Fn>
+static void host_parallel_z(long long nz, Fn&& body) {
+  const unsigned hw = std::thread::hardware_concurrency();
+  const long long nt = std::min<long long>(nz, std::max(1u, std::min(hw, 16u)));
+  if (nt <= 1) { body(0LL, nz); return; }
+  std::vector<std::thread> th;
+  th.reserve((size_t)nt);
+  for (long long t = 0; t < nt; ++t)
+    th.emplace_back([&, t] { body(nz * t / nt, nz * (t + 1) / nt); });
+  for (auto& x : th) x.join();
+}
+
+// pinned staging buffer for one plane of `bytes` (bytes <= one plane of
+// doubles), waiting until its previous copy has finished
+static int staging(wql_block* b, size_t bytes, void** out) {
+  if (!b->stage_buf[0]) {
+    const size_t plane = (size_t)b->nx * b->ny * b->nz * sizeof(double);
+    for (int k = 0; k < 2; ++k) {
+      HIP_TRY(hipHostMalloc(&b->stage_buf[k], plane, hipHostMallocDefault));
+      HIP_TRY(hipEventCreateWithFlags(&b->stage_ev[k], hipEventDisableTiming));
+      HIP_TRY(hipEventRecord(b->stage_ev[k], b->stream));
+    }
+    b->stage_bytes = plane;
+    b->stage_k = 0;
+  }
+  if (bytes > b->stage_bytes) return fail(WQL_EINVAL, "transfer larger than a plane");
+  const int k = b->stage_k;
+  HIP_TRY(hipEventSynchronize(b->stage_ev[k]));
+  *out = b->stage_buf[k];
+  return WQL_OK;
+}
+
+// the copy out of (or into) the current staging buffer has been issued
+static int staging_issued(wql_block* b) {
+  HIP_TRY(hipEventRecord(b->stage_ev[b->stage_k], b->stream));
+  b->stage_k ^= 1;
+  return WQL_OK;
+}
+
 template <typename T>
 static int put_comp(wql_block* b, T* dst, const View& v, int c, const int lo1[3], const int n[3],
                     const char* name) {
@@ -461,16 +514,21 @@ static int put_comp(wql_block* b, T* dst, const View& v, int c, const int lo1[3]
     HIP_TRY(hipStreamSynchronize(b->stream));
     return WQL_OK;
   }
-  std::vector<T> h((size_t)cnt);
-  for (long long z = 0; z < n[2]; ++z)
-    for (long long y = 0; y < n[1]; ++y) {
-      const double* src = v.base + off0 + y * v.sy + z * v.sz;
-      T* o = h.data() + (size_t)(n[0] * (y + (long long)n[1] * z));
-      for (long long x = 0; x < n[0]; ++x) o[x] = (T)src[x];
-    }
-  HIP_TRY(hipMemcpyAsync(dst, h.data(), cnt * sizeof(T), hipMemcpyHostToDevice, b->stream));
-  HIP_TRY(hipStreamSynchronize(b->stream));
-  return WQL_OK;
+  // pack (and convert) into pinned memory on host threads, then copy; the
+  // copy overlaps the packing of the next component
+  void* buf = nullptr;
+  if ((rc = staging(b, (size_t)cnt * sizeof(T), &buf))) return rc;
+  T* h = (T*)buf;
+  host_parallel_z(n[2], [&](long long z0, long long z1) {
+    for (long long z = z0; z < z1; ++z)
+      for (long long y = 0; y < n[1]; ++y) {
+        const double* src = v.base + off0 + y * v.sy + z * v.sz;
+        T* o = h + (size_t)(n[0] * (y + (long long)n[1] * z));
+        for (long long x = 0; x < n[0]; ++x) o[x] = (T)src[x];
+      }
+  });
+  HIP_TRY(hipMemcpyAsync(dst, h, cnt * sizeof(T), hipMemcpyHostToDevice, b->stream));
+  return staging_issued(b);
 }
 
 template <typename T>
@@ -488,15 +546,20 @@ static int get_comp(wql_block* b, const T* src, const View& v, int c, const int 
     HIP_TRY(hipStreamSynchronize(b->stream));
     return WQL_OK;
   }
-  std::vector<T> h((size_t)cnt);
-  HIP_TRY(hipMemcpyAsync(h.data(), src, cnt * sizeof(T), hipMemcpyDeviceToHost, b->stream));
+  void* buf = nullptr;
+  if ((rc = staging(b, (size_t)cnt * sizeof(T), &buf))) return rc;
+  T* h = (T*)buf;
+  HIP_TRY(hipMemcpyAsync(h, src, cnt * sizeof(T), hipMemcpyDeviceToHost, b->stream));
+  if ((rc = staging_issued(b))) return rc;
   HIP_TRY(hipStreamSynchronize(b->stream));
-  for (long long z = 0; z < n[2]; ++z)
-    for (long long y = 0; y < n[1]; ++y) {
-      double* o = base + off0 + y * v.sy + z * v.sz;
-      const T* s = h.data() + (size_t)(n[0] * (y + (long long)n[1] * z));
-      for (long long x = 0; x < n[0]; ++x) o[x] = (double)s[x];
-    }
+  host_parallel_z(n[2], [&](long long z0, long long z1) {
+    for (long long z = z0; z < z1; ++z)
+      for (long long y = 0; y < n[1]; ++y) {
+        double* o = base + off0 + y * v.sy + z * v.sz;
+        const T* s = h + (size_t)(n[0] * (y + (long long)n[1] * z));
+        for (long long x = 0; x < n[0]; ++x) o[x] = (double)s[x];
+      }
+  });
   return WQL_OK;
 }
 
@@ -658,7 +721,12 @@ static int transfer(wql_block* b, uint32_t mask, bool up) {
 
 extern "C" int wql_block_upload(wql_block* b, uint32_t mask) {
   if (!b) return fail(WQL_EINVAL, "null block");
-  return b->d.precision == WQL_FP64 ? transfer<double>(b, mask, true) : transfer<float>(b, mask, true);
+  const int rc = b->d.precision == WQL_FP64 ? transfer<double>(b, mask, true)
+                                            : transfer<float>(b, mask, true);
+  // the data is on the device when the call returns (other streams, e.g. an
+  // interface's, may read it next)
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  return rc;
 }
 
 extern "C" int wql_block_download(wql_block* b, uint32_t mask) {
