@@ -246,7 +246,9 @@ typedef struct wql_iface_desc {
 } wql_iface_desc;
 typedef struct wql_iface wql_iface;
 /* init_iface (iface.f90:13-55): S = 0, DS = 1e40, rupture time 1e9; for
- * TPV28 the blocks' face frames must be uploaded (WQL_FACES) before */
+ * TPV28 the blocks' face frames must be uploaded (WQL_FACES) before.  Both
+ * blocks share one precision (any WQL_FP*); the interface arithmetic and its
+ * slip state are FP64 in all three */
 int wql_iface_create(wql_block *L, wql_block *R, const wql_iface_desc *desc, wql_iface **out);
 void wql_iface_destroy(wql_iface *I);
 /* one RK stage of both blocks and the interface; stage1 != 0 records the fault

@@ -110,21 +110,25 @@ def test_curvilinear_locked_interface_matches_oracle(oracle, nmech):
     itf.close()
 
 
-def test_tpv5_fp32_mixed_against_fp64(oracle):
-    """FP32 stencil/material/memory arithmetic with the FP64 RK state and FP64
-    interface arithmetic: fault output within 1e-5 relative L2 of the FP64
-    oracle over the 46 steps."""
+@pytest.mark.parametrize("precision", [L.WQL_FP32_MIXED, L.WQL_FP32_KAHAN])
+def test_tpv5_fp32_against_fp64(oracle, precision):
+    """FP32 stencil/material/memory arithmetic with the RK state in FP64
+    (FP32-mixed) or in FP32 with the Kahan-compensated update (FP32-Kahan), the
+    interface arithmetic and its slip state in FP64: fault output within 1e-5
+    relative L2 of the FP64 oracle over the 46 steps."""
+    name = "fp32-mixed" if precision == L.WQL_FP32_MIXED else "fp32-kahan"
     tp = PB.tpv5_cartesian(21, sat_form=L.WQL_SAT_ALPHA_THETA)
     Lo, Ro, Io = oracle_two_block(oracle, tp)
     _, fr_o = oracle.two_block_time_step(Lo, Ro, Io, tp.dt, tp.nsteps, 0.0)
-    bl, br, itf = _gpu_pair(oracle, tp, precision=L.WQL_FP32_MIXED)
+    bl, br, itf = _gpu_pair(oracle, tp, precision=precision)
     itf.set_frames(tp.nsteps)
     itf.time_step(tp.dt, tp.nsteps, 0.0)
     fr = itf.frames()
-    for key in ("S", "Svel", "Uface", "Vface", "Uhat", "Vhat"):
-        e = rel_l2(fr[key], fr_o[key])
-        print(f"fp32-mixed {key} {e:.2e}", flush=True)
-        assert e <= 1e-5, (key, e)
+    errs = {key: rel_l2(fr[key], fr_o[key])
+            for key in ("S", "Svel", "Uface", "Vface", "Uhat", "Vhat")}
+    print(name, {k: "%.2e" % e for k, e in errs.items()}, flush=True)
+    assert float(np.abs(fr_o["S"]).max()) > 0       # the fault slipped
+    assert max(errs.values()) <= 1e-5, errs
     itf.close()
 
 
@@ -153,9 +157,9 @@ def test_interface_argument_errors(oracle):
         itf.stage(0.0, 1e-3, 0.0)
     itf.close()
     bl.close()
-    # FP32-Kahan blocks (FP32 RK state) are refused
+    # blocks of different precisions are refused
     kl = gpu_block(tp.left, oracle.metrics(tp.left.X), precision=L.WQL_FP32_KAHAN)
-    kr = gpu_block(tp.right, oracle.metrics(tp.right.X), precision=L.WQL_FP32_KAHAN)
+    kr = gpu_block(tp.right, oracle.metrics(tp.right.X), precision=L.WQL_FP32_MIXED)
     with pytest.raises(L.WqlError):
         Interface(kl, kr, tp.yf, tp.zf)
     kl.close()

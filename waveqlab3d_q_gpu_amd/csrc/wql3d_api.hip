@@ -1302,8 +1302,6 @@ extern "C" int wql_iface_create(wql_block* L, wql_block* R, const wql_iface_desc
     return fail(WQL_EINVAL, "slip-weakening tables T0L, T0R, fs, fd, Dc are required");
   if (L->iface || R->iface) return fail(WQL_ESTATE, "a block is already coupled to an interface");
   if (L->d.precision != R->d.precision) return fail(WQL_EINVAL, "blocks differ in precision");
-  if (L->d.precision == WQL_FP32_KAHAN)
-    return fail(WQL_EINVAL, "the interface needs an FP64 RK state (FP64 or FP32_MIXED blocks)");
   if (L->d.bc[1] != 0 || R->d.bc[0] != 0)
     return fail(WQL_EINVAL, "the coupled faces (Rx of L, Lx of R) need bc 0 (got %d, %d)",
                 L->d.bc[1], R->d.bc[0]);
@@ -1375,18 +1373,18 @@ extern "C" void wql_iface_destroy(wql_iface* I) {
 // one stage of the two-block domain: k_iface on L's stream (after R's
 // previous stage), block L's fused stage behind it on L's stream, block R's on
 // R's stream after k_iface
-template <typename T>
+template <typename T, typename S = double>
 static int iface_stage_t(wql_iface* I, double A, double Bdt, double t, int stage1) {
   wql_block* L = I->L;
   wql_block* R = I->R;
   HIP_TRY(hipStreamWaitEvent(L->stream, I->ev_right, 0));
-  IfaceArgs<T> a;
+  IfaceArgs<T, S> a;
   memset(&a, 0, sizeof a);
   a.ny = I->ny; a.nz = I->nz;
   a.coupling = I->coupling;
   a.nxL = L->nx; a.nxR = R->nx;
-  a.FL = (const double*)L->F[L->cur];
-  a.FR = (const double*)R->F[R->cur];
+  a.FL = (const S*)L->F[L->cur];
+  a.FR = (const S*)R->F[R->cur];
   a.PL = L->P; a.PR = R->P;
   a.ML = (const T*)L->M; a.MR = (const T*)R->M;
   a.frL = (const T*)L->face[1]; a.frR = (const T*)R->face[0];
@@ -1399,13 +1397,13 @@ static int iface_stage_t(wql_iface* I, double A, double Bdt, double t, int stage
   a.stage1 = stage1 ? 1 : 0;
   a.frame = (stage1 && I->frames && I->slot < I->nslots)
                 ? I->frames + (size_t)I->slot * kFramePlanes * I->NF : nullptr;
-  HIP_TRY(launch_iface<T>(a, L->stream));
+  HIP_TRY((launch_iface<T, S>(a, L->stream)));
   if (a.frame) ++I->slot;
   HIP_TRY(hipEventRecord(I->ev_iface, L->stream));
-  int rc = stage_fused_t<T>(L, A, Bdt, t);
+  int rc = stage_fused_t<T, S>(L, A, Bdt, t);
   if (rc) return rc;
   HIP_TRY(hipStreamWaitEvent(R->stream, I->ev_iface, 0));
-  rc = stage_fused_t<T>(R, A, Bdt, t);
+  rc = stage_fused_t<T, S>(R, A, Bdt, t);
   if (rc) return rc;
   HIP_TRY(hipEventRecord(I->ev_right, R->stream));
   // block L's stream then orders everything of the stage (callers time and
@@ -1418,8 +1416,11 @@ static int iface_stage_t(wql_iface* I, double A, double Bdt, double t, int stage
 
 extern "C" int wql_iface_stage(wql_iface* I, double A, double Bdt, double t, int32_t stage1) {
   if (int rc = iface_ok(I)) return rc;
-  return I->L->d.precision == WQL_FP64 ? iface_stage_t<double>(I, A, Bdt, t, stage1)
-                                       : iface_stage_t<float>(I, A, Bdt, t, stage1);
+  switch (I->L->d.precision) {
+    case WQL_FP64: return iface_stage_t<double>(I, A, Bdt, t, stage1);
+    case WQL_FP32_MIXED: return iface_stage_t<float>(I, A, Bdt, t, stage1);
+    default: return iface_stage_t<float, float>(I, A, Bdt, t, stage1);
+  }
 }
 
 extern "C" int wql_iface_time_step(wql_iface* I, double dt, int32_t nsteps, double* t) {

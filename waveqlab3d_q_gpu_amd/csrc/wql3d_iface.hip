@@ -113,8 +113,8 @@ struct Side {
 
 // one evaluation (the calling block's frame and metric column): hats,
 // slip-rate magnitude and the calling block's SAT forcing bc[9]
-template <typename T>
-__device__ __forceinline__ void evaluate(const IfaceArgs<T>& a, int ib, const T* fr, long long fi,
+template <typename T, typename S>
+__device__ __forceinline__ void evaluate(const IfaceArgs<T, S>& a, int ib, const T* fr, long long fi,
                                          long long NF, const double u[9], const double v[9],
                                          const Side& s1, const Side& s2, double slip,
                                          const double T0[3], double P[6], double Q[6],
@@ -235,8 +235,8 @@ __device__ __forceinline__ void evaluate(const IfaceArgs<T>& a, int ib, const T*
 
 // stage-1 outputs of one evaluation (CouplingForcing.f90:200-226,
 // Interface_Condition.f90:494-503, rupture_front)
-template <typename T>
-__device__ __forceinline__ void stage1_output(const IfaceArgs<T>& a, const double P[6],
+template <typename T, typename S>
+__device__ __forceinline__ void stage1_output(const IfaceArgs<T, S>& a, const double P[6],
                                               const double Q[6], const double T0[3],
                                               double out[15], double* trup) {
   // out: Svel[3], Uhat[6], Vhat[6]
@@ -251,8 +251,8 @@ __device__ __forceinline__ void stage1_output(const IfaceArgs<T>& a, const doubl
   if (sqrt(out[1] * out[1] + out[2] * out[2]) > 1e-3) *trup = fmin(*trup, a.t);
 }
 
-template <typename T>
-__global__ __launch_bounds__(256) void k_iface(const IfaceArgs<T> a) {
+template <typename T, typename S>
+__global__ __launch_bounds__(256) void k_iface(const IfaceArgs<T, S> a) {
   const long long NF = (long long)a.ny * a.nz;
   const long long fi = (long long)blockIdx.x * 256 + threadIdx.x;
   if (fi >= NF) return;
@@ -262,8 +262,8 @@ __global__ __launch_bounds__(256) void k_iface(const IfaceArgs<T> a) {
   double u[9], v[9];
 #pragma unroll
   for (int c = 0; c < 9; ++c) {
-    u[c] = a.FL[c * a.PL + iL];
-    v[c] = a.FR[c * a.PR + iR];
+    u[c] = (double)a.FL[c * a.PL + iL];
+    v[c] = (double)a.FR[c * a.PR + iR];
   }
   Side s1, s2;
   s1.lam = (double)a.ML[iL]; s1.mu = (double)a.ML[a.PL + iL]; s1.rho = (double)a.ML[2 * a.PL + iL];
@@ -322,14 +322,15 @@ __global__ __launch_bounds__(256) void k_iface(const IfaceArgs<T> a) {
 
 }  // namespace
 
-template <typename T>
-hipError_t launch_iface(const IfaceArgs<T>& a, hipStream_t s) {
+template <typename T, typename S>
+hipError_t launch_iface(const IfaceArgs<T, S>& a, hipStream_t s) {
   const long long NF = (long long)a.ny * a.nz;
-  hipLaunchKernelGGL(k_iface<T>, dim3((unsigned)((NF + 255) / 256)), dim3(256), 0, s, a);
+  hipLaunchKernelGGL((k_iface<T, S>), dim3((unsigned)((NF + 255) / 256)), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 
-template hipError_t launch_iface<double>(const IfaceArgs<double>&, hipStream_t);
-template hipError_t launch_iface<float>(const IfaceArgs<float>&, hipStream_t);
+template hipError_t launch_iface<double, double>(const IfaceArgs<double, double>&, hipStream_t);
+template hipError_t launch_iface<float, double>(const IfaceArgs<float, double>&, hipStream_t);
+template hipError_t launch_iface<float, float>(const IfaceArgs<float, float>&, hipStream_t);
 
 }  // namespace wql

@@ -172,13 +172,13 @@ hipError_t launch_metrics(int nx, int ny, int nz, const double* X, double* mx, d
 // SAT forcing of each block, updates the interface state in place and, at
 // stage 1 with a slot, the fault output.
 constexpr int kFramePlanes = 38;  // Uface 9, Vface 9, S 4, Svel 3, Uhat 6, Vhat 6, trup 1
-template <typename T>
+template <typename T, typename St = double>
 struct IfaceArgs {
   int ny, nz;
   int coupling;
   int nxL, nxR;            // x extents (L's face is x = nxL - 1, R's is x = 0)
-  const double* FL;        // block L / R fields at the start of the stage
-  const double* FR;
+  const St* FL;            // block L / R fields at the start of the stage (the
+  const St* FR;            // RK state: double, or float under FP32-Kahan)
   long long PL, PR;        // plane strides
   const T* ML;             // lambda, mu, rho planes
   const T* MR;
@@ -201,7 +201,7 @@ struct IfaceArgs {
   int stage1;
   double* frame;           // null, or kFramePlanes planes of the current output slot
 };
-template <typename T>
-hipError_t launch_iface(const IfaceArgs<T>& a, hipStream_t s);
+template <typename T, typename S>
+hipError_t launch_iface(const IfaceArgs<T, S>& a, hipStream_t s);
 
 }  // namespace wql
