@@ -278,13 +278,14 @@ int wql_iface_get_state(wql_iface *I, double *S, double *DS, double *trup);
  *   rb += Bdt ub;  g += (dL/dm)^T rb;  ub += K^T rb;  rb = A rb
  * with ub, rb the adjoints of u, r, K = dL/du and g the gradient planes.
  * PML slabs, interfaces and the alpha/theta SAT form are not differentiated
- * (wql_adjoint_create fails with any of them).
+ * (wql_adjoint_create fails with any of them); all three precisions are.
  * Dense arrays below are DEVICE pointers, node order x fastest, no ghosts. */
 /* adjoint state, rate adjoints, gradients (zeroed) and nslots state slots */
 int wql_adjoint_create(wql_block *b, int32_t nslots);
 /* ub = rb = 0, g = 0 */
 int wql_adjoint_reset(wql_block *b);
-/* copy the current state (F, eta) into slot / back from slot (checkpoints) */
+/* copy the current state (F, eta; under FP32-Kahan also F's compensation)
+ * into slot / back from slot (checkpoints) */
 int wql_state_save(wql_block *b, int32_t slot);
 int wql_state_load(wql_block *b, int32_t slot);
 /* reverse one stage with RK coefficients A (a_s) and Bdt (b_s * dt) */

@@ -122,10 +122,12 @@ def test_adjoint_gradient_matches_central_differences():
         assert rel <= 1e-6
 
 
-def test_adjoint_fp32_mixed_matches_fp64():
+@pytest.mark.parametrize("precision", [L.WQL_FP32_MIXED, L.WQL_FP32_KAHAN])
+def test_adjoint_fp32_matches_fp64(precision):
     """The FP32-mixed block (float stencil and memory arithmetic, FP64 RK
-    state) through the same adjoint runner: checkpoint reloads and the
-    forward recomputation between them (which rebuild the stencils' float
+    state) and the FP32-Kahan block (FP32 RK state with its compensation,
+    both checkpointed) through the same adjoint runner: checkpoint reloads and
+    the forward recomputation between them (which rebuild the stencils' float
     copy of F) give the FP64 misfit and gradient to FP32 accuracy."""
     import torch
     from waveqlab3d_q_gpu_amd.adjoint import AdjointRunner, receiver_traces
@@ -135,7 +137,7 @@ def test_adjoint_fp32_mixed_matches_fp64():
     near = np.asarray(prob.sources[0].near)
     rec = np.asarray([near + o for o in ((5, 0, 0), (-5, 0, 0), (0, -5, 0), (0, 5, 0))])
     out = {}
-    for prec in (L.WQL_FP64, L.WQL_FP32_MIXED):
+    for prec in (L.WQL_FP64, precision):
         blk = PB.make_device_block(prob, precision=prec, state_mirror=True)
         run = AdjointRunner(blk, prob.dt, N, rec, every=7)
         dev = blk.M.device
@@ -157,8 +159,37 @@ def test_adjoint_fp32_mixed_matches_fp64():
         dJ = sum(float((g * dk).sum()) for g, dk in zip((rho.grad, lam.grad, mu.grad), d))
         out[prec] = (float(J), dJ)
         blk.close()
-    (J64, g64), (J32, g32) = out[L.WQL_FP64], out[L.WQL_FP32_MIXED]
-    print(f"misfit FP64 {J64!r} FP32-mixed {J32!r}; directional gradient {g64!r} {g32!r}",
-          flush=True)
+    (J64, g64), (J32, g32) = out[L.WQL_FP64], out[precision]
+    print(f"misfit FP64 {J64!r} FP32 (precision {precision}) {J32!r}; "
+          f"directional gradient {g64!r} {g32!r}", flush=True)
     assert J64 > 0 and abs(J32 - J64) / J64 <= 1e-4
     assert abs(g32 - g64) / abs(g64) <= 1e-3
+
+
+@pytest.mark.parametrize("precision", [L.WQL_FP64, L.WQL_FP32_MIXED, L.WQL_FP32_KAHAN])
+def test_checkpoint_reload_steps_bit_identically(precision):
+    """wql_state_save / wql_state_load hold everything the next steps read:
+    two steps from a reloaded checkpoint repeat the two steps taken after the
+    save bit for bit (FP64, FP32-mixed with its float copy of F, FP32-Kahan
+    with the compensation of its FP32 state)."""
+    import torch
+    prob = _problem("curv_q8", n=24, nsteps=5)
+    blk = PB.make_device_block(prob, precision=precision, state_mirror=True)
+    L.check(L.lib().wql_adjoint_create(blk._h, 1))
+    blk.time_step(prob.dt, 3, 0.0)
+    L.check(L.lib().wql_state_save(blk._h, 0))
+    t0 = 3 * prob.dt
+    blk.time_step(prob.dt, 2, t0)
+    blk.download(L.WQL_F | L.WQL_MEMORY)
+    torch.cuda.synchronize()
+    a = (blk.F.clone(), blk.eta.clone())
+    L.check(L.lib().wql_state_load(blk._h, 0))
+    blk.time_step(prob.dt, 2, t0)
+    blk.download(L.WQL_F | L.WQL_MEMORY)
+    torch.cuda.synchronize()
+    same = [bool(torch.equal(a[0], blk.F)), bool(torch.equal(a[1], blk.eta))]
+    moved = float(torch.abs(a[0]).max())
+    print(f"precision {precision}: reload repeat identical (F, eta) = {same}, max|F| {moved:.3e}",
+          flush=True)
+    blk.close()
+    assert moved > 0 and all(same)

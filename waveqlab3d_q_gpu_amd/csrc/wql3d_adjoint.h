@@ -19,8 +19,8 @@ struct AdjArgs {
 };
 
 // reverse of one fused stage whose input state is the block's current F / eta
-template <typename T>
-hipError_t launch_adjoint_stage(const StepArgs<T>& a, const AdjArgs<T>& j, int mesh, double A,
+template <typename T, typename St>
+hipError_t launch_adjoint_stage(const StepArgs<T, St>& a, const AdjArgs<T>& j, int mesh, double A,
                                 double Bdt, hipStream_t s);
 // Fb at receiver nodes idx[r] += g[r][0..8]
 hipError_t launch_adjoint_inject(double* Fb, long long P, const long long* idx, int n,

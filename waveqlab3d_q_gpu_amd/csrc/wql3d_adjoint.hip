@@ -105,8 +105,8 @@ __device__ __forceinline__ D3 sqrt(const D3& a) {
 }
 
 // node (x, y, z) of flat index id < nx*ny*nz
-template <typename T>
-__device__ __forceinline__ bool flat_node(const StepArgs<T>& a, long long id, int& x, int& y, int& z) {
+template <typename T, typename St>
+__device__ __forceinline__ bool flat_node(const StepArgs<T, St>& a, long long id, int& x, int& y, int& z) {
   const long long nodes = (long long)a.nx * a.ny * a.nz;
   if (id >= nodes) return false;
   const long long plane = (long long)a.nx * a.ny;
@@ -118,8 +118,8 @@ __device__ __forceinline__ bool flat_node(const StepArgs<T>& a, long long id, in
 }
 
 // SAT rate vector of face f at a node for state u: -pen_f * structure(ubc(u))
-template <typename S, typename T>
-__device__ __forceinline__ void sat_rate(const StepArgs<T>& a, int f, long long fidx, long long NF,
+template <typename S, typename T, typename St>
+__device__ __forceinline__ void sat_rate(const StepArgs<T, St>& a, int f, long long fidx, long long NF,
                                          const S u[9], S lam, S mu, S rho, S out[9]) {
   T fr[10];
   face_frame(a, f, fidx, NF, fr);
@@ -134,8 +134,8 @@ __device__ __forceinline__ void sat_rate(const StepArgs<T>& a, int f, long long 
 // ---------------------------------------------------------------------------
 // gradient: g += (dL/dm)^T rb at the stage input state (a.Fin, a.eta)
 // ---------------------------------------------------------------------------
-template <typename T, int MESH, int NMECH>
-__global__ __launch_bounds__(256) void k_adj_grad(const StepArgs<T> a, const AdjArgs<T> j) {
+template <typename T, typename St, int MESH, int NMECH>
+__global__ __launch_bounds__(256) void k_adj_grad(const StepArgs<T, St> a, const AdjArgs<T> j) {
   int x, y, z;
   const long long id = (long long)blockIdx.x * 256 + threadIdx.x;
   if (!flat_node(a, id, x, y, z)) return;
@@ -220,8 +220,8 @@ __global__ __launch_bounds__(256) void k_adj_grad(const StepArgs<T> a, const Adj
 //              the adjoint of the velocity gradient (Hooke + memory variables)
 //   etab += -rb_sigma - rb_eta/tau ;  Fb += S^T rb_F at face nodes (SAT)
 // ---------------------------------------------------------------------------
-template <typename T, int MESH, int NMECH>
-__global__ __launch_bounds__(256) void k_adj_pre(const StepArgs<T> a, const AdjArgs<T> j) {
+template <typename T, typename St, int MESH, int NMECH>
+__global__ __launch_bounds__(256) void k_adj_pre(const StepArgs<T, St> a, const AdjArgs<T> j) {
   int x, y, z;
   const long long id = (long long)blockIdx.x * 256 + threadIdx.x;
   if (!flat_node(a, id, x, y, z)) return;
@@ -326,8 +326,8 @@ __global__ __launch_bounds__(256) void k_adj_pre(const StepArgs<T> a, const AdjA
 //   curvilinear: sigma_b(m,c) += J X_m(d) / h_d (D_d^T Z[c]);  vb_c += 1/h_d D_d^T Z[9+3d+c]
 //   Cartesian:   sigma_b(d,c) += 1/h_d D_d^T Z[3d+c];         vb_c as above
 // ---------------------------------------------------------------------------
-template <typename T, int MESH>
-__global__ __launch_bounds__(256) void k_adj_stencil(const StepArgs<T> a, const AdjArgs<T> j) {
+template <typename T, typename St, int MESH>
+__global__ __launch_bounds__(256) void k_adj_stencil(const StepArgs<T, St> a, const AdjArgs<T> j) {
   int x, y, z;
   const long long id = (long long)blockIdx.x * 256 + threadIdx.x;
   if (!flat_node(a, id, x, y, z)) return;
@@ -386,8 +386,8 @@ __global__ void k_adj_inject(double* Fb, long long P, const long long* idx, int 
 }
 
 // ---------------------------------------------------------------------------
-template <typename T>
-hipError_t launch_adjoint_stage(const StepArgs<T>& a, const AdjArgs<T>& j, int mesh, double A,
+template <typename T, typename St>
+hipError_t launch_adjoint_stage(const StepArgs<T, St>& a, const AdjArgs<T>& j, int mesh, double A,
                                 double Bdt, hipStream_t s) {
   const long long nodes = (long long)a.nx * a.ny * a.nz;
   const long long ne = 6LL * a.nmech * a.P;
@@ -400,9 +400,9 @@ hipError_t launch_adjoint_stage(const StepArgs<T>& a, const AdjArgs<T>& j, int m
   const dim3 g((unsigned)((nodes + 255) / 256)), b(256);
 #define WQL_CASE(MS, NM)                                                       \
   if (mesh == MS && a.nmech == NM) {                                           \
-    hipLaunchKernelGGL((k_adj_grad<T, MS, NM>), g, b, 0, s, a, j);             \
-    hipLaunchKernelGGL((k_adj_pre<T, MS, NM>), g, b, 0, s, a, j);              \
-    hipLaunchKernelGGL((k_adj_stencil<T, MS>), g, b, 0, s, a, j);              \
+    hipLaunchKernelGGL((k_adj_grad<T, St, MS, NM>), g, b, 0, s, a, j);         \
+    hipLaunchKernelGGL((k_adj_pre<T, St, MS, NM>), g, b, 0, s, a, j);          \
+    hipLaunchKernelGGL((k_adj_stencil<T, St, MS>), g, b, 0, s, a, j);          \
     e = hipGetLastError();                                                     \
     if (e != hipSuccess) return e;                                             \
   } else
@@ -414,10 +414,15 @@ hipError_t launch_adjoint_stage(const StepArgs<T>& a, const AdjArgs<T>& j, int m
   if (ne) e = launch_scale<T>(j.DEb, ne, A, s);
   return e;
 }
-template hipError_t launch_adjoint_stage<double>(const StepArgs<double>&, const AdjArgs<double>&,
-                                                 int, double, double, hipStream_t);
-template hipError_t launch_adjoint_stage<float>(const StepArgs<float>&, const AdjArgs<float>&,
-                                                int, double, double, hipStream_t);
+template hipError_t launch_adjoint_stage<double, double>(const StepArgs<double, double>&,
+                                                         const AdjArgs<double>&, int, double,
+                                                         double, hipStream_t);
+template hipError_t launch_adjoint_stage<float, double>(const StepArgs<float, double>&,
+                                                        const AdjArgs<float>&, int, double,
+                                                        double, hipStream_t);
+template hipError_t launch_adjoint_stage<float, float>(const StepArgs<float, float>&,
+                                                       const AdjArgs<float>&, int, double,
+                                                       double, hipStream_t);
 
 hipError_t launch_adjoint_inject(double* Fb, long long P, const long long* idx, int n,
                                  const double* g, hipStream_t s) {

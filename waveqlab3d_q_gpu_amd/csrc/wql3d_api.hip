@@ -1510,10 +1510,16 @@ static void adjoint_free(wql_block* b) {
   b->adj_nslots = 0;
 }
 
+// bytes of one checkpoint of F: the RK state, and under FP32-Kahan its
+// compensation as well (a reloaded state then steps bit-identically)
+static size_t slot_F_bytes(const wql_block* b) {
+  const size_t n = 9 * (size_t)b->P;
+  return n * b->ssz + (b->d.precision == WQL_FP32_KAHAN ? n * sizeof(float) : 0);
+}
+
 extern "C" int wql_adjoint_create(wql_block* b, int32_t nslots) {
   if (!b) return fail(WQL_EINVAL, "null block");
   if (nslots < 0) return fail(WQL_EINVAL, "nslots must be >= 0");
-  if (int rc = no_kahan(b, "the adjoint")) return rc;
   for (int f = 0; f < 6; ++f)
     if (b->d.pml[f]) return fail(WQL_EINVAL, "the adjoint does not differentiate PML slabs (face %d)", f);
   if (b->iface) return fail(WQL_EINVAL, "the adjoint does not differentiate interfaces");
@@ -1530,7 +1536,7 @@ extern "C" int wql_adjoint_create(wql_block* b, int32_t nslots) {
   if (!rc) rc = dalloc(b, &b->adj_DEb, ne * es);
   if (!rc) rc = dalloc(b, (void**)&b->adj_g, 3 * P * sizeof(double));
   if (!rc) rc = dalloc(b, &b->adj_Z, 18 * P * es);
-  if (!rc) rc = dalloc(b, (void**)&b->slot_F, (size_t)nslots * 9 * P * sizeof(double));
+  if (!rc) rc = dalloc(b, (void**)&b->slot_F, (size_t)nslots * slot_F_bytes(b));
   if (!rc) rc = dalloc(b, &b->slot_E, (size_t)nslots * ne * es);
   if (!rc) {
     b->slot_mz = (int*)calloc(nslots > 0 ? nslots : 1, sizeof(int));
@@ -1570,8 +1576,11 @@ extern "C" int wql_state_save(wql_block* b, int32_t slot) {
   int rc = check_slot(b, slot);
   if (rc) return rc;
   const long long P = b->P, ne = 6LL * b->d.nmech * P;
-  HIP_TRY(hipMemcpyAsync(b->slot_F + (size_t)slot * 9 * P, (const double*)b->F[b->cur], 9 * P * sizeof(double),
-                         hipMemcpyDeviceToDevice, b->stream));
+  char* sf = (char*)b->slot_F + (size_t)slot * slot_F_bytes(b);
+  HIP_TRY(hipMemcpyAsync(sf, b->F[b->cur], 9 * P * b->ssz, hipMemcpyDeviceToDevice, b->stream));
+  if (b->d.precision == WQL_FP32_KAHAN)
+    HIP_TRY(hipMemcpyAsync(sf + 9 * P * b->ssz, b->Fc, 9 * P * sizeof(float),
+                           hipMemcpyDeviceToDevice, b->stream));
   if (ne)
     HIP_TRY(hipMemcpyAsync((char*)b->slot_E + (size_t)slot * ne * b->esz, b->eta, ne * b->esz,
                            hipMemcpyDeviceToDevice, b->stream));
@@ -1584,8 +1593,11 @@ extern "C" int wql_state_load(wql_block* b, int32_t slot) {
   if (rc) return rc;
   const long long P = b->P, ne = 6LL * b->d.nmech * P;
   b->fo_valid = false;
-  HIP_TRY(hipMemcpyAsync((double*)b->F[b->cur], b->slot_F + (size_t)slot * 9 * P, 9 * P * sizeof(double),
-                         hipMemcpyDeviceToDevice, b->stream));
+  const char* sf = (const char*)b->slot_F + (size_t)slot * slot_F_bytes(b);
+  HIP_TRY(hipMemcpyAsync(b->F[b->cur], sf, 9 * P * b->ssz, hipMemcpyDeviceToDevice, b->stream));
+  if (b->d.precision == WQL_FP32_KAHAN)
+    HIP_TRY(hipMemcpyAsync(b->Fc, sf + 9 * P * b->ssz, 9 * P * sizeof(float),
+                           hipMemcpyDeviceToDevice, b->stream));
   if (ne)
     HIP_TRY(hipMemcpyAsync(b->eta, (char*)b->slot_E + (size_t)slot * ne * b->esz, ne * b->esz,
                            hipMemcpyDeviceToDevice, b->stream));
@@ -1595,10 +1607,10 @@ extern "C" int wql_state_load(wql_block* b, int32_t slot) {
   return WQL_OK;
 }
 
-template <typename T>
+template <typename T, typename St = double>
 static int adjoint_stage_t(wql_block* b, double A, double Bdt) {
   if (int rc = ensure_operand_copy(b)) return rc;   // the gradient differences F
-  StepArgs<T> a = make_args<T>(b, 0.0);
+  StepArgs<T, St> a = make_args<T, St>(b, 0.0);
   AdjArgs<T> j;
   j.Fb = b->adj_Fb;
   j.Eb = (T*)b->adj_Eb;
@@ -1606,14 +1618,17 @@ static int adjoint_stage_t(wql_block* b, double A, double Bdt) {
   j.DEb = (T*)b->adj_DEb;
   for (int k = 0; k < 3; ++k) j.g[k] = b->adj_g + k * b->P;
   j.Z = (T*)b->adj_Z;
-  HIP_TRY(launch_adjoint_stage<T>(a, j, b->d.mesh, A, Bdt, b->stream));
+  HIP_TRY((launch_adjoint_stage<T, St>(a, j, b->d.mesh, A, Bdt, b->stream)));
   return WQL_OK;
 }
 
 extern "C" int wql_adjoint_stage(wql_block* b, double A, double Bdt) {
   if (!b || !b->adj_Fb) return fail(WQL_ESTATE, "no adjoint context (wql_adjoint_create)");
-  return b->d.precision == WQL_FP64 ? adjoint_stage_t<double>(b, A, Bdt)
-                                    : adjoint_stage_t<float>(b, A, Bdt);
+  switch (b->d.precision) {
+    case WQL_FP64: return adjoint_stage_t<double>(b, A, Bdt);
+    case WQL_FP32_MIXED: return adjoint_stage_t<float>(b, A, Bdt);
+    default: return adjoint_stage_t<float, float>(b, A, Bdt);
+  }
 }
 
 extern "C" int wql_adjoint_add_receivers(wql_block* b, const double* g) {
