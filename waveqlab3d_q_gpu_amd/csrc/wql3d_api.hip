@@ -467,9 +467,18 @@ static void host_parallel_z(long long nz, Fn&& body) {
   const long long nt = std::min<long long>(nz, std::max(1u, std::min(hw, 16u)));
   if (nt <= 1) { body(0LL, nz); return; }
   std::vector<std::thread> th;
-  th.reserve((size_t)nt);
-  for (long long t = 0; t < nt; ++t)
-    th.emplace_back([&, t] { body(nz * t / nt, nz * (t + 1) / nt); });
+  long long started = 0;
+  try {
+    th.reserve((size_t)nt);
+    for (; started < nt; ++started) {
+      const long long t = started;
+      th.emplace_back([&, t] { body(nz * t / nt, nz * (t + 1) / nt); });
+    }
+  } catch (...) {
+    // thread creation failed (std::system_error / bad_alloc): the threads
+    // already running keep their ranges, this thread does the rest
+  }
+  if (started < nt) body(nz * started / nt, nz);
   for (auto& x : th) x.join();
 }
 
@@ -1039,21 +1048,63 @@ static int join(wql_block* b) {
   return WQL_OK;
 }
 
+// z levels [z0, z1) of chunk k of K over nz levels
+static void chunk_range(int nz, int K, int k, int* z0, int* z1) {
+  *z0 = (int)((long long)nz * k / K);
+  *z1 = (int)((long long)nz * (k + 1) / K);
+}
+
+// The chunk whose previous-stage memory kernel the stencils of chunk k must
+// wait for: the chunk holding the highest z level they read.  The central
+// stencil reads z1 + 2 (3-node halo); the SBP closure rows z = 0..5 read
+// taps up to z = 8 (row 6 of the closure has 9 taps), so a chunk with a
+// bottom closure row reads up to max(z1 + 2, 8).  Levels below z0 belong to
+// lower chunks, which stream2 has finished before (it runs in order).
+static int chunk_wait_index(int nz, int K, int k) {
+  int z0, z1;
+  chunk_range(nz, K, k, &z0, &z1);
+  int zmax = z1 + 2;
+  if (z0 < 6 && zmax < 8) zmax = 8;
+  if (zmax > nz - 1) zmax = nz - 1;
+  int j = k;
+  for (;;) {
+    int c0, c1;
+    chunk_range(nz, K, j, &c0, &c1);
+    if (zmax < c1 || j == K - 1) break;
+    ++j;
+  }
+  return j;
+}
+
 // One stage as a z-chunk pipeline: the stencil kernels (latency bound) of
 // chunk k on the block's stream overlap the memory-variable kernel (HBM
 // bound) of chunk k - 1 on stream2.  Hazards: the stencil of chunk k reads F
-// of levels z0-3 .. z1+2, written by the previous stage's memory kernels of
-// chunks k-1, k, k+1 (it waits for ev_mem[k+1], stream2 runs in order); it
+// of levels z0-3 .. max(z1+2, 8 for a chunk with bottom closure rows),
+// written by the previous stage's memory kernels of the chunks holding them
+// (it waits for the event of the chunk holding the highest of these levels,
+// chunk_wait_index; stream2 runs in order, so the lower chunks are done); it
 // writes W and the PML fields of chunk k only, which the previous stage's
 // memory kernel of chunk k has finished reading (same wait).  The memory
 // kernel of chunk k writes F_out, DF, eta of chunk k after the stencil of
 // chunk k (ev_sten[k]); F_out is the buffer this stage's stencils do not read.
+// The stencils of all chunks run in order on the block's stream, so those of
+// the previous stage (whose top closure rows read down to z = nz - 9) have
+// finished before this stage's stencils write the velocity part of F_out.
 template <typename T, typename S>
 static int stage_pipelined(wql_block* b, const StepArgs<T, S>& a) {
   const int K = b->nchunks, nz = b->nz;
   for (int k = 0; k < K; ++k) {
-    const int z0 = (int)((long long)nz * k / K), z1 = (int)((long long)nz * (k + 1) / K);
-    HIP_TRY(hipStreamWaitEvent(b->stream, b->ev_mem[k + 1 < K ? k + 1 : K - 1], 0));
+    int z0, z1;
+    chunk_range(nz, K, k, &z0, &z1);
+    const int kw = chunk_wait_index(nz, K, k);
+    {
+      // the waited-for chunk ends above every level this chunk's stencils read
+      int w0, w1;
+      chunk_range(nz, K, kw, &w0, &w1);
+      const int need = std::min(nz - 1, std::max(z1 + 2, z0 < 6 ? 8 : 0));
+      if (need >= w1) return fail(WQL_ESTATE, "z-chunk wait rule: chunk %d reads level %d", k, need);
+    }
+    HIP_TRY(hipStreamWaitEvent(b->stream, b->ev_mem[kw], 0));
     HIP_TRY(launch_stage_part_z<T>(a, b->d.mesh, StagePart::kInterior, z0, z1, b->stream));
     HIP_TRY(launch_stage_part_z<T>(a, b->d.mesh, StagePart::kShell, z0, z1, b->stream));
     HIP_TRY(hipEventRecord(b->ev_sten[k], b->stream));
