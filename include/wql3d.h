@@ -191,6 +191,24 @@ int wql_grid_metrics(int32_t nq, int32_t nr, int32_t ns, const double *X,
 
 /* ---- diagnostics --------------------------------------------------------- */
 const char *wql_last_error(void);
+/* close_domain's shutdown check (domain.f90:520-545): out[0] = max|F| over
+ * the block's nine field components, out[1] = max|eta| over its memory
+ * variables (0 for an elastic block), out[2] = the number of non-finite
+ * values among them (the reference stops with RUN-ELASTIC-001 / RUN-Q4-001 /
+ * RUN-Q8-001 when there is one).  out is a host array of 3; the call
+ * allocates its partial results on the device for the call and synchronises
+ * the block's stream. */
+int wql_block_max_state(wql_block *b, double *out);
+/* No counterpart in the reference: wql_block_max_state for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  The same kernels, with their partial
+ * results in a fixed 48 KiB device array of the library (2048 partials; one
+ * copy per GPU, held until the process ends; the first pass never uses more
+ * partials, whatever the block size) instead of an allocation; the three
+ * results go to out_dev, a DEVICE array of 3 doubles, stream-ordered on the
+ * block's stream (no synchronisation).  Calls that overlap on one GPU share
+ * that array and race. */
+int wql_block_max_state_noalloc(wql_block *b, double *out_dev);
 /* synchronise the block's stream */
 int wql_block_sync(wql_block *b);
 /* the block's HIP stream (hipStream_t) for event timing by the caller */

@@ -90,6 +90,8 @@ SIGNATURES = {
     "wql_time_step": (C.c_int, [_P, C.c_double, C.c_int32, _D]),
     "wql_grid_metrics": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, _P, _P, _P, _P, _P]),
     "wql_last_error": (C.c_char_p, []),
+    "wql_block_max_state": (C.c_int, [_P, _D]),
+    "wql_block_max_state_noalloc": (C.c_int, [_P, _P]),
     "wql_block_sync": (C.c_int, [_P]),
     "wql_block_stream": (_P, [_P]),
     "wql_block_set_timing": (C.c_int, [_P, C.c_int32]),

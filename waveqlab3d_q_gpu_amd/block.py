@@ -267,6 +267,21 @@ class Block:
         """Record per-stage kernel durations (HIP events on the block stream)."""
         check(lib().wql_block_set_timing(self._h, 1 if enable else 0))
 
+    def max_state(self):
+        """close_domain's shutdown check (domain.f90:520-545) on the device:
+        (max|F|, max|eta|, number of non-finite values)."""
+        out = (C.c_double * 3)()
+        check(lib().wql_block_max_state(self._h, out))
+        return out[0], out[1], int(out[2])
+
+    def max_state_noalloc(self, out_dev):
+        """The same check without a device allocation: the three results go
+        to ``out_dev`` (a float64 CUDA tensor of 3), ordered on the block's
+        stream (call :meth:`sync` before reading it)."""
+        if str(out_dev.dtype) != "torch.float64" or out_dev.numel() < 3 or not out_dev.is_cuda:
+            raise ValueError("out_dev must be a float64 CUDA tensor of at least 3 values")
+        check(lib().wql_block_max_state_noalloc(self._h, C.c_void_p(out_dev.data_ptr())))
+
     def kernel_times(self):
         """([interior_ms, shell_ms, memory_ms], stages) summed since the last call."""
         ms, n = (C.c_double * 3)(), C.c_int32()

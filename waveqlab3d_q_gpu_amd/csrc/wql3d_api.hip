@@ -8,6 +8,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -55,6 +56,7 @@ long long box_extent(const wql_box& b, int d) { return (long long)b.hi[d] - b.lo
 
 struct wql_block {
   wql_block_desc d;
+  int device;                // HIP device current at wql_block_create
   int nx, ny, nz;
   // plane stride: nodes rounded up to a multiple of kPlaneAlign, so every
   // plane starts on a 1 KiB (fp32) / 2 KiB (fp64) boundary and the streaming
@@ -163,6 +165,161 @@ struct wql_iface {
   hipEvent_t ev_right;  // on R's stream after block R's stage: the next k_iface waits
 };
 
+// ---------------------------------------------------------------------------
+// close_domain's shutdown check (domain.f90:520-545) on the device: the
+// largest |F| and |eta| of a block and the count of non-finite values.  Two
+// passes: each workgroup of k_max_state_part reduces a grid-stride share of
+// every plane to one partial (3 doubles), then one workgroup of
+// k_max_state_final reduces the partials.  Only the nodes of each plane are
+// read (not the padding tail of the plane stride).
+// ---------------------------------------------------------------------------
+constexpr int kMaxStateThreads = 256;
+constexpr int kMaxStateParts = 2048;  // partials per call (3 doubles each)
+
+// the allocation-free variant's partials: one copy per GPU, 48 KiB
+static __device__ __align__(256) double scratch[3 * kMaxStateParts];
+
+__device__ inline void max_state_wg_reduce(double& mf, double& me, double& bad) {
+  for (int o = 32; o > 0; o >>= 1) {
+    mf = fmax(mf, __shfl_xor(mf, o));
+    me = fmax(me, __shfl_xor(me, o));
+    bad += __shfl_xor(bad, o);
+  }
+  __shared__ double red[3][kMaxStateThreads / 64];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 0) { red[0][w] = mf; red[1][w] = me; red[2][w] = bad; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < kMaxStateThreads / 64; ++k) {
+      mf = fmax(mf, red[0][k]);
+      me = fmax(me, red[1][k]);
+      bad += red[2][k];
+    }
+  }
+}
+
+template <typename V>
+__device__ inline void max_state_plane(const V* p, long long nodes, double& m, double& bad) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nodes; i += stride) {
+    const double v = (double)p[i];
+    if (isfinite(v)) m = fmax(m, fabs(v));
+    else bad += 1.0;
+  }
+}
+
+// F: 9 planes of S; eta: neta planes of T; plane stride P, nodes per plane
+template <typename S, typename T>
+__global__ __launch_bounds__(kMaxStateThreads) void k_max_state_part(
+    const S* F, const T* eta, long long P, long long nodes, int neta, double* part) {
+  double mf = 0.0, me = 0.0, bad = 0.0;
+  for (int c = 0; c < 9; ++c) max_state_plane(F + c * P, nodes, mf, bad);
+  for (int c = 0; c < neta; ++c) max_state_plane(eta + c * P, nodes, me, bad);
+  max_state_wg_reduce(mf, me, bad);
+  if (threadIdx.x == 0) {
+    part[3 * blockIdx.x] = mf;
+    part[3 * blockIdx.x + 1] = me;
+    part[3 * blockIdx.x + 2] = bad;
+  }
+}
+
+__global__ __launch_bounds__(kMaxStateThreads) void k_max_state_final(const double* part,
+                                                                      int nparts, double* out) {
+  double mf = 0.0, me = 0.0, bad = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
+    mf = fmax(mf, part[3 * i]);
+    me = fmax(me, part[3 * i + 1]);
+    bad += part[3 * i + 2];
+  }
+  max_state_wg_reduce(mf, me, bad);
+  if (threadIdx.x == 0) { out[0] = mf; out[1] = me; out[2] = bad; }
+}
+
+// workgroups of the first pass: one per 8 * 256 nodes of a plane, at most
+// kMaxStateParts (the grid-stride loop covers any block size)
+static int max_state_parts(const wql_block* b) {
+  const long long nodes = (long long)b->nx * b->ny * b->nz;
+  const long long g = (nodes + 8LL * kMaxStateThreads - 1) / (8LL * kMaxStateThreads);
+  return (int)std::max(1LL, std::min<long long>(g, kMaxStateParts));
+}
+
+// both passes on the block's stream with partials `part` (3 * nparts
+// doubles) and the result in `out` (3 doubles, device)
+static hipError_t launch_max_state(const wql_block* b, double* part, int nparts, double* out) {
+  const long long nodes = (long long)b->nx * b->ny * b->nz;
+  const int neta = 6 * b->d.nmech;
+  const dim3 grid((unsigned)nparts), block(kMaxStateThreads);
+  const void* F = b->F[b->cur];
+  if (b->d.precision == WQL_FP64)
+    hipLaunchKernelGGL((k_max_state_part<double, double>), grid, block, 0, b->stream,
+                       (const double*)F, (const double*)b->eta, b->P, nodes, neta, part);
+  else if (b->d.precision == WQL_FP32_MIXED)
+    hipLaunchKernelGGL((k_max_state_part<double, float>), grid, block, 0, b->stream,
+                       (const double*)F, (const float*)b->eta, b->P, nodes, neta, part);
+  else
+    hipLaunchKernelGGL((k_max_state_part<float, float>), grid, block, 0, b->stream,
+                       (const float*)F, (const float*)b->eta, b->P, nodes, neta, part);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_max_state_final, dim3(1), block, 0, b->stream, (const double*)part, nparts,
+                     out);
+  return hipGetLastError();
+}
+
+// device address of `scratch` on each GPU (each holds its own copy), looked
+// up once per device with that device current: the array's name is a
+// host-side stand-in, never a device address
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[kMaxDevices];
+
+static int scratch_for_device(int dev, double** out) {
+  if (dev < 0 || dev >= kMaxDevices) return fail(WQL_EINVAL, "device %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    int cur = -1;
+    HIP_TRY(hipGetDevice(&cur));
+    if (cur != dev) HIP_TRY(hipSetDevice(dev));
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != dev) HIP_TRY(hipSetDevice(cur));
+    if (e != hipSuccess || !p)
+      return fail(WQL_EHIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                  hipGetErrorString(e));
+    g_scratch_ptr[dev] = (double*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return WQL_OK;
+}
+
+extern "C" int wql_block_max_state(wql_block* b, double* out) {
+  if (!b || !out) return fail(WQL_EINVAL, "null argument");
+  if (int rc = join(b)) return rc;
+  const int nparts = max_state_parts(b);
+  double* buf = nullptr;
+  HIP_TRY(hipMalloc((void**)&buf, (size_t)(3 * nparts + 3) * sizeof(double)));
+  hipError_t e = launch_max_state(b, buf, nparts, buf + 3 * nparts);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(out, buf + 3 * nparts, 3 * sizeof(double), hipMemcpyDeviceToHost, b->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+  (void)hipFree(buf);
+  if (e != hipSuccess) return fail(WQL_EHIP, "max state: %s", hipGetErrorString(e));
+  return WQL_OK;
+}
+
+extern "C" int wql_block_max_state_noalloc(wql_block* b, double* out_dev) {
+  if (!b || !out_dev) return fail(WQL_EINVAL, "null argument");
+  double* part = nullptr;
+  if (int rc = scratch_for_device(b->device, &part)) return rc;
+  const int nparts = max_state_parts(b);
+  if (nparts > kMaxStateParts)
+    return fail(WQL_EINVAL, "max state needs %d partials; the scratch array holds %d", nparts,
+                kMaxStateParts);
+  if (int rc = join(b)) return rc;
+  HIP_TRY(launch_max_state(b, part, nparts, out_dev));
+  return WQL_OK;
+}
+
 static int g_device = -1;
 
 extern "C" const char* wql_last_error(void) { return g_err; }
@@ -175,6 +332,10 @@ extern "C" int wql_runtime_init(int32_t local_rank, int32_t* device) {
   const int dev = (local_rank < 0 ? 0 : local_rank) % n;
   HIP_TRY(hipSetDevice(dev));
   g_device = dev;
+  // device address of the allocation-free entry points' scratch array on
+  // this device (blocks created on another device look theirs up on first use)
+  double* scratch_ptr = nullptr;
+  if (int rc = scratch_for_device(dev, &scratch_ptr)) return rc;
   if (device) *device = dev;
   return WQL_OK;
 }
@@ -295,6 +456,7 @@ extern "C" int wql_block_create(const wql_block_desc* d, wql_block** out) {
   wql_block* b = (wql_block*)calloc(1, sizeof(wql_block));
   if (!b) return fail(WQL_ENOMEM, "host allocation failed");
   b->d = *d;
+  if (hipGetDevice(&b->device) != hipSuccess) b->device = 0;
   b->nx = d->nq; b->ny = d->nr; b->nz = d->ns;
   b->P = ((long long)b->nx * b->ny * b->nz + kPlaneAlign - 1) / kPlaneAlign * kPlaneAlign;
   b->esz = d->precision == WQL_FP64 ? sizeof(double) : sizeof(float);
