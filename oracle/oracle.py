@@ -10,6 +10,7 @@ Arrays use the layout documented in ``wql_oracle.h``: numpy arrays of shape
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 import os
 import subprocess
@@ -70,36 +71,66 @@ IFACE_LOCKED, IFACE_SLIP_WEAKENING_TPV5, IFACE_SLIP_WEAKENING_TPV28 = 0, 1, 2
 _lib = None
 
 
+def build_planted(plant: int, outdir: str) -> str:
+    """Compile the restatement with the planted mistake ``ORC_PLANT == plant``
+    (wql_oracle.c: 1 lambda for lambda + 2 mu in the Qp term, 2 the PML
+    feedback sign flipped, 3 the PML auxiliary source sign flipped, 4 2 mu for
+    mu in the shear memory source) into ``outdir``: the tests that must catch
+    each mistake run against it."""
+    out = os.path.join(outdir, f"libwql_oracle_plant{plant}.so")
+    src = [os.path.join(_HERE, f) for f in ("wql_oracle.c", "wql_oracle_iface.c")]
+    subprocess.run(["gcc", "-O2", "-fPIC", "-fopenmp", "-ffp-contract=off", "-fno-fast-math",
+                    f"-DORC_PLANT={int(plant)}", "-shared", "-o", out, *src, "-lm"], check=True)
+    return out
+
+
+@contextlib.contextmanager
+def using_library(path: str):
+    """Route every oracle call to the library at ``path`` (a planted build)
+    inside the block."""
+    global _lib
+    saved = _lib
+    _lib = _load(path)
+    try:
+        yield
+    finally:
+        _lib = saved
+
+
 def lib():
     global _lib
     if _lib is None:
         build()
-        L = C.CDLL(_LIB_PATH)
-        L.orc_rk_coefficients.argtypes = [_P, _P, _P]
-        L.orc_sbp_closure.argtypes = [_P]
-        L.orc_apply_d1.argtypes = [C.c_int, C.c_double, _P, _P]
-        L.orc_metrics.argtypes = [C.c_int, C.c_int, C.c_int, _P, _P, _P, _P, _P]
-        L.orc_normals.argtypes = [C.POINTER(_Block)]
-        L.orc_q_coefficients.argtypes = [C.c_int, C.c_double, C.c_double, _P, _P]
-        L.orc_q_modulus_correction.argtypes = [C.POINTER(_Block), C.c_double]
-        L.orc_q_modulus_correction.restype = C.c_int
-        for name in ("orc_set_rates", "orc_impose_boundary_condition"):
-            getattr(L, name).argtypes = [C.POINTER(_Block)]
-        L.orc_scale_rates.argtypes = [C.POINTER(_Block), C.c_double]
-        L.orc_update_fields.argtypes = [C.POINTER(_Block), C.c_double]
-        L.orc_set_moment_tensor.argtypes = [C.POINTER(_Block), C.c_double]
-        L.orc_time_step.argtypes = [C.POINTER(_Block), C.c_double, C.c_int, _P,
-                                    C.c_int, C.POINTER(C.c_long), _P]
-        L.orc_source_time.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double]
-        L.orc_source_time.restype = C.c_double
-        L.orc_num_threads.restype = C.c_int
-        L.orc_set_num_threads.argtypes = [C.c_int]
-        L.orc_iface_enforce.argtypes = [C.POINTER(_Iface), C.POINTER(_Block), C.POINTER(_Block),
-                                        C.c_double, C.c_int, C.c_int]
-        L.orc_iface_scale.argtypes = [C.POINTER(_Iface), C.c_double]
-        L.orc_iface_update.argtypes = [C.POINTER(_Iface), C.c_double]
-        _lib = L
+        _lib = _load(_LIB_PATH)
     return _lib
+
+
+def _load(path):
+    L = C.CDLL(path)
+    L.orc_rk_coefficients.argtypes = [_P, _P, _P]
+    L.orc_sbp_closure.argtypes = [_P]
+    L.orc_apply_d1.argtypes = [C.c_int, C.c_double, _P, _P]
+    L.orc_metrics.argtypes = [C.c_int, C.c_int, C.c_int, _P, _P, _P, _P, _P]
+    L.orc_normals.argtypes = [C.POINTER(_Block)]
+    L.orc_q_coefficients.argtypes = [C.c_int, C.c_double, C.c_double, _P, _P]
+    L.orc_q_modulus_correction.argtypes = [C.POINTER(_Block), C.c_double]
+    L.orc_q_modulus_correction.restype = C.c_int
+    for name in ("orc_set_rates", "orc_impose_boundary_condition"):
+        getattr(L, name).argtypes = [C.POINTER(_Block)]
+    L.orc_scale_rates.argtypes = [C.POINTER(_Block), C.c_double]
+    L.orc_update_fields.argtypes = [C.POINTER(_Block), C.c_double]
+    L.orc_set_moment_tensor.argtypes = [C.POINTER(_Block), C.c_double]
+    L.orc_time_step.argtypes = [C.POINTER(_Block), C.c_double, C.c_int, _P,
+                                C.c_int, C.POINTER(C.c_long), _P]
+    L.orc_source_time.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double]
+    L.orc_source_time.restype = C.c_double
+    L.orc_num_threads.restype = C.c_int
+    L.orc_set_num_threads.argtypes = [C.c_int]
+    L.orc_iface_enforce.argtypes = [C.POINTER(_Iface), C.POINTER(_Block), C.POINTER(_Block),
+                                    C.c_double, C.c_int, C.c_int]
+    L.orc_iface_scale.argtypes = [C.POINTER(_Iface), C.c_double]
+    L.orc_iface_update.argtypes = [C.POINTER(_Iface), C.c_double]
+    return L
 
 
 def _ptr(a):
