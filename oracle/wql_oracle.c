@@ -422,12 +422,24 @@ static void q_rates(orc_block *b, long id, double exx, double eyy, double ezz, d
     double w = b->weight[i], ta = b->tau[i];
     for (int c = 0; c < 3; ++c) {
       long k = ((long)c * b->nmech + i) * N + id;
+#if ORC_PLANT == 1
+      /* planted mistake for the tests that must catch it: lambda in place of
+       * lambda + 2 mu in the Qp term */
+      b->Deta[k] = b->Deta[k] + ((((w * 2.0 * mu * Qs) * e[c]
+                   + (w * (lam * Qp - 2.0 * mu * Qs)) * tr)) - b->eta[k]) / ta;
+#else
       b->Deta[k] = b->Deta[k] + ((((w * 2.0 * mu * Qs) * e[c]
                    + (w * ((lam + 2.0 * mu) * Qp - 2.0 * mu * Qs)) * tr)) - b->eta[k]) / ta;
+#endif
     }
     for (int c = 0; c < 3; ++c) {
       long k = ((long)(3 + c) * b->nmech + i) * N + id;
+#if ORC_PLANT == 4
+      /* planted mistake: 2 mu in place of mu in the shear source */
+      b->Deta[k] = b->Deta[k] + ((w * 2.0 * mu * Qs * g[c] - b->eta[k]) / ta);
+#else
       b->Deta[k] = b->Deta[k] + ((w * mu * Qs * g[c] - b->eta[k]) / ta);
+#endif
     }
   }
 }
@@ -528,6 +540,10 @@ static void pml_point(orc_block *b, double DU[9], const double Ux[9], const doub
     double q[9];
     for (int c = 0; c < 9; ++c) q[c] = Q[c * N + id];
     double Jp = b->J[id];
+#if ORC_PLANT == 2
+    /* planted mistake: the PML feedback with the opposite sign */
+    d = -d;
+#endif
     DU[0] = DU[0] - d * Jp * q[0] * rhoJ_inv;
     DU[1] = DU[1] - d * Jp * q[1] * rhoJ_inv;
     DU[2] = DU[2] - d * Jp * q[2] * rhoJ_inv;
@@ -549,6 +565,13 @@ static void pml_point(orc_block *b, double DU[9], const double Ux[9], const doub
       double s[9] = {Uz[7] / Jp, Uz[8] / Jp, Uz[5] / Jp, 0.0, 0.0, Uz[2], 0.0, Uz[0], Uz[1]};
       memcpy(src, s, sizeof s);
     }
+#if ORC_PLANT == 2
+    d = -d;
+#endif
+#if ORC_PLANT == 3
+    /* planted mistake: the auxiliary source with the opposite sign */
+    for (int c = 0; c < 9; ++c) src[c] = -src[c];
+#endif
     for (int c = 0; c < 9; ++c) DQ[c * N + id] = DQ[c * N + id] + src[c] - (d + cfs) * q[c];
   }
 }
