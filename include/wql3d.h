@@ -164,6 +164,28 @@ int wql_rk_update(wql_block *b, double Bdt);
 int wql_scale_rate_array(double *rate, int64_t n, double coefficient);
 int wql_update_state_array(double *state, const double *rate, int64_t n, double increment);
 
+/* ---- device mirrors of single host arrays: the present table that the
+ * reference's OpenACC backend keeps (acc enter data copyin / acc_is_present /
+ * update device / update self / exit data; rk_vector_openacc.f90:8-55,
+ * tests/rk_vector_test_data_openacc.f90:5-12), for the per-array rk_vector
+ * seam.  A host range is present when it lies inside one entered array. */
+int wql_array_enter(const void *host, int64_t bytes);      /* device mirror + copy in */
+int wql_array_exit(const void *host);                      /* drop the mirror (no copy out) */
+int wql_array_present(const void *host, int64_t bytes);    /* 1 present, 0 not */
+int wql_array_update_device(const void *host, int64_t bytes);
+int wql_array_update_self(void *host, int64_t bytes);
+/* scale_rate_array / update_state_array (rk_vector_cpu.f90:7-19) on one
+ * present host array of n doubles (the whole array, ghost nodes included),
+ * as the OpenACC backend runs them (rk_vector_openacc.f90:8-55): update the
+ * mirror from the host, the kernel on the mirror (bit-exact with the CPU
+ * expressions), the result back to the host.  WQL_ESTATE when an array is
+ * not present (the OpenACC backend's 'not present on device' stop). */
+int wql_scale_rate_present(double *rate, int64_t n, double coefficient);
+int wql_update_state_present(double *state, const double *rate, int64_t n, double increment);
+/* the block created with host array desc->F == host_F (the swap-in seams that
+ * receive a block_type find its device mirror through F%F's address) */
+int wql_block_find(const double *host_F, wql_block **out);
+
 /* ---- set_rates_elastic seam (elastic.f90:13-29): RHS_Center +
  *      RHS_near_boundaries incl. Q4/Q8 and PML: DF += L(F) ------------------ */
 int wql_rhs(wql_block *b);

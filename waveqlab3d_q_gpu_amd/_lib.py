@@ -82,6 +82,14 @@ SIGNATURES = {
     "wql_rk_update": (C.c_int, [_P, C.c_double]),
     "wql_scale_rate_array": (C.c_int, [_P, C.c_int64, C.c_double]),
     "wql_update_state_array": (C.c_int, [_P, _P, C.c_int64, C.c_double]),
+    "wql_array_enter": (C.c_int, [_P, C.c_int64]),
+    "wql_array_exit": (C.c_int, [_P]),
+    "wql_array_present": (C.c_int, [_P, C.c_int64]),
+    "wql_array_update_device": (C.c_int, [_P, C.c_int64]),
+    "wql_array_update_self": (C.c_int, [_P, C.c_int64]),
+    "wql_scale_rate_present": (C.c_int, [_P, C.c_int64, C.c_double]),
+    "wql_update_state_present": (C.c_int, [_P, _P, C.c_int64, C.c_double]),
+    "wql_block_find": (C.c_int, [_P, C.POINTER(_P)]),
     "wql_rhs": (C.c_int, [_P]),
     "wql_rhs_interior": (C.c_int, [_P, C.POINTER(C.c_int32)]),
     "wql_set_moment_tensor": (C.c_int, [_P, C.c_double]),

@@ -415,8 +415,33 @@ static int dalloc(wql_block* b, void** p, size_t bytes) {
   return WQL_OK;
 }
 
+// live blocks, for wql_block_find (host F pointer -> block)
+static std::mutex g_blocks_mu;
+static std::vector<wql_block*> g_blocks;
+
+static void register_block(wql_block* b) {
+  std::lock_guard<std::mutex> lock(g_blocks_mu);
+  g_blocks.push_back(b);
+}
+
+extern "C" int wql_block_find(const double* host_F, wql_block** out) {
+  if (!out) return fail(WQL_EINVAL, "null output pointer");
+  *out = nullptr;
+  std::lock_guard<std::mutex> lock(g_blocks_mu);
+  for (wql_block* b : g_blocks)
+    if (host_F && b->d.F == host_F) {
+      *out = b;
+      return WQL_OK;
+    }
+  return fail(WQL_EINVAL, "no block was created with F at %p", (const void*)host_F);
+}
+
 extern "C" void wql_block_destroy(wql_block* b) {
   if (!b) return;
+  {
+    std::lock_guard<std::mutex> lock(g_blocks_mu);
+    g_blocks.erase(std::remove(g_blocks.begin(), g_blocks.end(), b), g_blocks.end());
+  }
   if (b->stream2) (void)hipStreamSynchronize(b->stream2);
   (void)hipStreamSynchronize(b->stream);
   if (b->iface) {  // the interface outlives the block: it refuses further stages
@@ -573,6 +598,7 @@ extern "C" int wql_block_create(const wql_block_desc* d, wql_block** out) {
   }
 #undef ALLOC
   *out = b;
+  register_block(b);
   return wql_block_init_state(b);
 }
 
@@ -622,11 +648,14 @@ static int check_view(const View& v, const int lo1[3], const int n[3], const cha
   return WQL_OK;
 }
 
-// Run body(z0, z1) over the z levels [0, nz) on up to 16 host threads.
+// Run body(z0, z1) over the z levels [0, nz) on up to 16 host threads, one
+// per 256 Ki values at least (a small component is packed on this thread:
+// starting threads would cost more than the copy).
 template <typename Fn>
-static void host_parallel_z(long long nz, Fn&& body) {
+static void host_parallel_z(long long nz, long long values, Fn&& body) {
   const unsigned hw = std::thread::hardware_concurrency();
-  const long long nt = std::min<long long>(nz, std::max(1u, std::min(hw, 16u)));
+  long long nt = std::min<long long>(nz, std::max(1u, std::min(hw, 16u)));
+  nt = std::min<long long>(nt, std::max(1LL, values >> 18));
   if (nt <= 1) { body(0LL, nz); return; }
   std::vector<std::thread> th;
   long long started = 0;
@@ -690,7 +719,7 @@ static int put_comp(wql_block* b, T* dst, const View& v, int c, const int lo1[3]
   void* buf = nullptr;
   if ((rc = staging(b, (size_t)cnt * sizeof(T), &buf))) return rc;
   T* h = (T*)buf;
-  host_parallel_z(n[2], [&](long long z0, long long z1) {
+  host_parallel_z(n[2], cnt, [&](long long z0, long long z1) {
     for (long long z = z0; z < z1; ++z)
       for (long long y = 0; y < n[1]; ++y) {
         const double* src = v.base + off0 + y * v.sy + z * v.sz;
@@ -723,7 +752,7 @@ static int get_comp(wql_block* b, const T* src, const View& v, int c, const int 
   HIP_TRY(hipMemcpyAsync(h, src, cnt * sizeof(T), hipMemcpyDeviceToHost, b->stream));
   if ((rc = staging_issued(b))) return rc;
   HIP_TRY(hipStreamSynchronize(b->stream));
-  host_parallel_z(n[2], [&](long long z0, long long z1) {
+  host_parallel_z(n[2], cnt, [&](long long z0, long long z1) {
     for (long long z = z0; z < z1; ++z)
       for (long long y = 0; y < n[1]; ++y) {
         double* o = base + off0 + y * v.sy + z * v.sz;
@@ -1132,6 +1161,119 @@ extern "C" int wql_update_state_array(double* state, const double* rate, int64_t
   if (n < 0 || (n > 0 && (!state || !rate))) return fail(WQL_EINVAL, "bad array");
   HIP_TRY(launch_update<double>(state, rate, n, inc, 0));
   HIP_TRY(hipStreamSynchronize(0));
+  return WQL_OK;
+}
+
+// ---------------------------------------------------------------------------
+// device mirrors of single host arrays: the present table of the reference's
+// OpenACC backend (acc enter data / acc_is_present / update device / update
+// self / exit data, rk_vector_openacc.f90, tests/rk_vector_test_data_openacc.f90)
+// ---------------------------------------------------------------------------
+struct Mirror {
+  uintptr_t host;
+  size_t bytes;
+  void* dev;
+};
+static std::mutex g_mirror_mu;
+static std::vector<Mirror> g_mirrors;
+
+// the mirror holding [host, host + bytes), or null (caller holds the lock)
+static Mirror* find_mirror(const void* host, size_t bytes) {
+  const uintptr_t h = (uintptr_t)host;
+  for (Mirror& m : g_mirrors)
+    if (h >= m.host && h + bytes <= m.host + m.bytes) return &m;
+  return nullptr;
+}
+
+extern "C" int wql_array_enter(const void* host, int64_t bytes) {
+  if (!host || bytes <= 0) return fail(WQL_EINVAL, "bad host array");
+  std::lock_guard<std::mutex> lock(g_mirror_mu);
+  if (find_mirror(host, (size_t)bytes)) return WQL_OK;  // already present
+  for (const Mirror& m : g_mirrors)
+    if ((uintptr_t)host < m.host + m.bytes && m.host < (uintptr_t)host + (size_t)bytes)
+      return fail(WQL_EINVAL, "host range overlaps a present array only in part");
+  Mirror m{(uintptr_t)host, (size_t)bytes, nullptr};
+  hipError_t e = hipMalloc(&m.dev, m.bytes);
+  if (e != hipSuccess) return fail(WQL_ENOMEM, "hipMalloc(%zu): %s", m.bytes, hipGetErrorString(e));
+  e = hipMemcpy(m.dev, host, m.bytes, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    (void)hipFree(m.dev);
+    return fail(WQL_EHIP, "copy in: %s", hipGetErrorString(e));
+  }
+  g_mirrors.push_back(m);
+  return WQL_OK;
+}
+
+extern "C" int wql_array_exit(const void* host) {
+  std::lock_guard<std::mutex> lock(g_mirror_mu);
+  for (size_t k = 0; k < g_mirrors.size(); ++k)
+    if (g_mirrors[k].host == (uintptr_t)host) {
+      HIP_TRY(hipDeviceSynchronize());
+      (void)hipFree(g_mirrors[k].dev);
+      g_mirrors.erase(g_mirrors.begin() + (long)k);
+      return WQL_OK;
+    }
+  return fail(WQL_EINVAL, "array at %p is not present", host);
+}
+
+extern "C" int wql_array_present(const void* host, int64_t bytes) {
+  if (!host || bytes < 0) return 0;
+  std::lock_guard<std::mutex> lock(g_mirror_mu);
+  return find_mirror(host, (size_t)bytes) ? 1 : 0;
+}
+
+static int mirror_copy(void* host, int64_t bytes, bool to_device) {
+  if (!host || bytes < 0) return fail(WQL_EINVAL, "bad host array");
+  std::lock_guard<std::mutex> lock(g_mirror_mu);
+  Mirror* m = find_mirror(host, (size_t)bytes);
+  if (!m) return fail(WQL_ESTATE, "array at %p (%lld bytes) is not present on the device", host,
+                      (long long)bytes);
+  char* d = (char*)m->dev + ((uintptr_t)host - m->host);
+  if (to_device) HIP_TRY(hipMemcpy(d, host, (size_t)bytes, hipMemcpyHostToDevice));
+  else HIP_TRY(hipMemcpy(host, d, (size_t)bytes, hipMemcpyDeviceToHost));
+  return WQL_OK;
+}
+
+extern "C" int wql_array_update_device(const void* host, int64_t bytes) {
+  return mirror_copy((void*)host, bytes, true);
+}
+
+extern "C" int wql_array_update_self(void* host, int64_t bytes) {
+  return mirror_copy(host, bytes, false);
+}
+
+// the device address of a present host range
+static int mirror_of(const void* host, size_t bytes, double** dev) {
+  std::lock_guard<std::mutex> lock(g_mirror_mu);
+  Mirror* m = find_mirror(host, bytes);
+  if (!m) return fail(WQL_ESTATE, "RK array at %p (%zu bytes) is not present on the device",
+                      host, bytes);
+  *dev = (double*)((char*)m->dev + ((uintptr_t)host - m->host));
+  return WQL_OK;
+}
+
+extern "C" int wql_scale_rate_present(double* rate, int64_t n, double coefficient) {
+  if (!rate || n < 0) return fail(WQL_EINVAL, "bad array");
+  double* d = nullptr;
+  const size_t bytes = (size_t)n * sizeof(double);
+  if (int rc = mirror_of(rate, bytes, &d)) return rc;
+  HIP_TRY(hipMemcpy(d, rate, bytes, hipMemcpyHostToDevice));   // update device(rate)
+  HIP_TRY(launch_scale<double>(d, n, coefficient, 0));
+  HIP_TRY(hipMemcpy(rate, d, bytes, hipMemcpyDeviceToHost));   // update self(rate)
+  return WQL_OK;
+}
+
+extern "C" int wql_update_state_present(double* state, const double* rate, int64_t n,
+                                        double increment) {
+  if (!state || !rate || n < 0) return fail(WQL_EINVAL, "bad array");
+  double *ds = nullptr, *dr = nullptr;
+  const size_t bytes = (size_t)n * sizeof(double);
+  if (int rc = mirror_of(state, bytes, &ds)) return rc;
+  if (int rc = mirror_of(rate, bytes, &dr)) return rc;
+  HIP_TRY(hipMemcpy(ds, state, bytes, hipMemcpyHostToDevice));  // update device(state)
+  HIP_TRY(hipMemcpy(dr, rate, bytes, hipMemcpyHostToDevice));   // update device(rate)
+  HIP_TRY(launch_update<double>(ds, dr, n, increment, 0));
+  HIP_TRY(hipMemcpy(state, ds, bytes, hipMemcpyDeviceToHost));  // update self(state)
   return WQL_OK;
 }
 
