@@ -447,40 +447,90 @@ def tpv28_curvilinear(n=21, sat_form=1, s0=None):
                            name=f"tpv28_curvilinear_{n}", s0=None if s0 is None else tuple(s0))
 
 
-def fault_two_block(n=(201, 401, 201), h=0.1, nmech=8, npml=20, cfl=0.5, nsteps=2000):
-    """Two Cartesian blocks of n = (nq, nr, ns) nodes each, block L over
-    x in [-(nq-1)h, 0] and block R over [0, (nq-1)h], y (depth, r) in
-    [0, (nr-1)h], z in [-(ns-1)h/2, (ns-1)h/2] (km), coupled by a planar TPV5
-    slip-weakening fault (mu_s 0.677, mu_d 0.525, Dc 0.4 m, sigma_n -120 MPa,
-    tau 70 MPa, 81.6 MPa in the 3 km nucleation patch at y = 7.5 km, z = 0);
-    rho/cs/cp = 2.67/3.464/6, anelastic Q (Qs = 50 cs[km/s], Qp = 2 Qs), free
-    surface on y-min, PML of npml nodes on the other outer faces."""
+def fault_two_block(n=(201, 401, 201), h=0.1, nmech=8, npml=20, cfl=0.5, nsteps=2000,
+                    curvilinear=True, seed=2025, qs=100.0, qp=200.0, nrec=32):
+    """Two blocks of n = (nq, nr, ns) nodes each, block L over x in
+    [-(nq-1)h, 0] and block R over [0, (nq-1)h], y (depth, r) from the
+    surface to (nr-1)h, z in [-(ns-1)h/2, (ns-1)h/2] (km), coupled by a planar
+    vertical TPV5 slip-weakening fault at x = 0 (mu_s 0.677, mu_d 0.525, Dc
+    0.4 m, sigma_n -120 MPa, tau 70 MPa, 81.6 MPa in the 3 km nucleation
+    patch at y = 7.5 km, z = 0); rho/cs/cp = 2.67/3.464/6, anelastic Q (Qs,
+    Qp), free surface on y-min, PML of npml nodes on the other outer faces.
+
+    curvilinear: both blocks are curvilinear, the surface carries 12 seeded
+    Gaussian hills (widths 0.5-2 km, heights -0.3..0.3 km) over the whole
+    domain and each column blends linearly from its surface point to the flat
+    bottom (the transfinite grid of topography_grid); x depends on q only, so
+    the fault stays the plane x = 0 and its nodes coincide on both sides.
+    Otherwise both blocks are Cartesian with a flat surface.
+
+    Receivers: nrec surface nodes, half in each block, 2-8 km from the fault
+    (off-fault stations)."""
     nq, nr, ns = n
     Lz = (ns - 1) * h
-    XL = cartesian_grid(n, (-(nq - 1) * h, 0.0, -0.5 * Lz), (0.0, (nr - 1) * h, 0.5 * Lz))
-    XR = cartesian_grid(n, (0.0, 0.0, -0.5 * Lz), ((nq - 1) * h, (nr - 1) * h, 0.5 * Lz))
+    Ly = (nr - 1) * h
+    mesh = L.WQL_CURVILINEAR if curvilinear else L.WQL_CARTESIAN
+
+    def grid(x0, x1):
+        X = cartesian_grid(n, (x0, 0.0, -0.5 * Lz), (x1, Ly, 0.5 * Lz))
+        if curvilinear:
+            xs_all = np.linspace(-(nq - 1) * h, (nq - 1) * h, 2 * nq - 1)
+            zs = X[2][:, 0, 0]
+            rng = np.random.default_rng(seed)
+            cx = rng.uniform(xs_all[0], xs_all[-1], 12)
+            cz = rng.uniform(zs[0], zs[-1], 12)
+            w = rng.uniform(0.5, 2.0, 12)
+            hgt = rng.uniform(-0.3, 0.3, 12)
+            xs = X[0][0, 0, :]
+            e = np.zeros((ns, nq))
+            for k in range(12):
+                e += hgt[k] * np.exp(-(((xs[None, :] - cx[k]) ** 2 + (zs[:, None] - cz[k]) ** 2)
+                                       / (2.0 * w[k] ** 2)))
+            r = np.arange(nr) / (nr - 1)
+            ytop = -e
+            X[1] = ytop[:, None, :] + r[None, :, None] * (Ly - ytop[:, None, :])
+        return X
+
+    XL = grid(-(nq - 1) * h, 0.0)
+    XR = grid(0.0, (nq - 1) * h)
     M = material_from_rho_s_p(n, 2.67, 3.464, 6.0)
     tau = weight = Qs_inv = Qp_inv = None
     if nmech:
         tau, weight = q_coefficients(nmech)
-        Qs_inv = np.full(XL.shape[1:], 1.0 / (50.0 * 3.464))
-        Qp_inv = 0.5 * Qs_inv
+        Qs_inv = np.full(XL.shape[1:], 1.0 / qs)
+        Qp_inv = np.full(XL.shape[1:], 1.0 / qp)
         M = modulus_correction(M, Qs_inv, Qp_inv, tau, weight)
     dt = block_time_step(h, cfl, 3.464, 6.0)
     if nmech:
         dt = min(dt, relaxation_dt_limit(tau))
     pmlL = (1, 0, 0, 1, 1, 1) if npml else (0,) * 6
     pmlR = (0, 1, 0, 1, 1, 1) if npml else (0,) * 6
-    kw = dict(mesh=L.WQL_CARTESIAN, dt=dt, nsteps=nsteps, nmech=nmech, tau=tau, weight=weight,
+    rng = np.random.default_rng(seed + 1)
+    half = nrec // 2
+    margin = max(npml, 6) + 1
+
+    def stations(side):
+        # surface nodes 2-8 km from the fault, clear of the PML slabs
+        dist = rng.uniform(2.0, 8.0, half)
+        ix = np.rint(dist / h).astype(int)
+        ix = np.clip(ix, 0, nq - 1 - margin)
+        ix = (nq - 1 - ix) if side == 0 else ix
+        iz = rng.integers(margin, ns - margin, half)
+        return np.stack([ix + 1, np.ones(half, dtype=int), iz + 1], axis=1)
+
+    kw = dict(mesh=mesh, dt=dt, nsteps=nsteps, nmech=nmech, tau=tau, weight=weight,
               Qs_inv=Qs_inv, Qp_inv=Qp_inv)
     left = Problem(n=n, X=XL, M=M, bc=(1, 0, 2, 1, 1, 1), pml=pmlL,
-                   npml=tuple(npml * p for p in pmlL), name="fault_block1", **kw)
+                   npml=tuple(npml * p for p in pmlL), receivers=stations(0),
+                   name="fault_block1", **kw)
     right = Problem(n=n, X=XR, M=M.copy(), bc=(0, 1, 2, 1, 1, 1), pml=pmlR,
-                    npml=tuple(npml * p for p in pmlR), name="fault_block2", **kw)
+                    npml=tuple(npml * p for p in pmlR), receivers=stations(1),
+                    name="fault_block2", **kw)
+    kind = "curvilinear" if curvilinear else "cartesian"
     return TwoBlockProblem(left=left, right=right, coupling=L.WQL_IFACE_SLIP_WEAKENING_TPV5,
                            yf=np.ascontiguousarray(XL[1][:, :, -1]),
                            zf=np.ascontiguousarray(XL[2][:, :, -1]), dt=dt, nsteps=nsteps,
-                           name=f"fault_two_block_{nq}x{nr}x{ns}")
+                           name=f"fault_two_block_{kind}_{nq}x{nr}x{ns}")
 
 
 def locked_interface_41():
@@ -728,7 +778,9 @@ def make_block(prob: Problem, metrics, precision=L.WQL_FP64, frames=None, upload
             shp = (9, hi[2] - lo[2] + 1, hi[1] - lo[1] + 1, hi[0] - lo[0] + 1)
             pq[f] = np.zeros(shp)
             pdq[f] = np.full(shp, 1.0e40)
-    F = np.ascontiguousarray(prob.F0) if prob.F0 is not None else np.zeros((9, ns, nr, nq))
+    # a copy: downloads write the block's F, which must not alias the problem's F0
+    F = (np.array(prob.F0, dtype=np.float64, order="C", copy=True) if prob.F0 is not None
+         else np.zeros((9, ns, nr, nq)))
     blk = Block(nq=nq, nr=nr, ns=ns, mesh=prob.mesh, precision=precision, nmech=prob.nmech,
                 bc=prob.bc, pml=prob.pml, npml=prob.npml,
                 tau=tuple(prob.tau) if prob.tau is not None else (),
