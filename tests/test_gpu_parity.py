@@ -99,6 +99,31 @@ def test_curvilinear_q_pml(oracle, nmech, fused):
         assert rel_l2(seis_g, seis_o) <= TOL64
 
 
+@pytest.mark.parametrize("fused", [True, False])
+def test_clamped_faces_match_oracle(oracle, fused):
+    """Clamped faces (bc 3, BoundaryConditions.f90:151-161) on a curvilinear
+    Q4 block: x-high, y-high and z-low clamped, free surface on y-low,
+    characteristic elsewhere; fields, memory variables and seismograms."""
+    prob = small_curvilinear(nmech=4, npml=0)
+    prob.bc = (1, 3, 2, 3, 3, 1)
+    # a random start, so every face carries velocity and stress from step 1
+    prob.F0 = np.random.default_rng(33).uniform(-1.0, 1.0, (9,) + prob.X.shape[1:])
+    prob.F0[3:] *= 1.0e7
+    met = oracle.metrics(prob.X)
+    orc = oracle_block(oracle, prob, met)
+    _, seis_o = orc.time_step(prob.dt, prob.nsteps, receivers=receiver_flat(prob))
+    blk = gpu_block(prob, met)
+    blk.set_receivers(prob.receivers, prob.nsteps)
+    time_step_RK(blk, prob.dt, prob.nsteps, fused=fused)
+    _download_all(blk)
+    errs = field_errors(blk.F, orc.F)
+    print(f"clamped fused={fused} field rel L2:", ["%.2e" % e for e in errs], flush=True)
+    assert max(errs) <= TOL64
+    assert rel_l2(blk.eta, orc.eta) <= TOL64
+    if fused:
+        assert rel_l2(blk.seismograms(), seis_o) <= TOL64
+
+
 # Launch shapes of the fused stage (WQL3D_NODE_TUNING = "nontemporal,xcd,band,
 # split,interior,zchunk"): strict-interior kernel off, or tiles 64x8 / 32x16 /
 # 64x4 with z chunks that cut the interior box unevenly; shell memory
