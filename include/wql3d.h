@@ -261,7 +261,9 @@ int wql_block_kernel_times(wql_block *b, double ms[3], int32_t *stages);
  * copy of the interface), S += Bdt*DS (scale_rates_iface / update_fields_iface,
  * iface.f90:58-85).  The stage-1 fault output (write_fault_output /
  * write_hat_output, domain.f90:782-847) is recorded into device slots.
- * FP64 and FP32_MIXED blocks (the RK state is FP64 in both). */
+ * The interface reads FP64 copies of the coupled faces' frames and material
+ * (taken at wql_iface_create and at every upload of WQL_FACES / WQL_MATERIAL
+ * of a coupled block), in every precision. */
 /* Slip-weakening friction is evaluated from per-node tables: the prestress
  * (T0_n, T0_m, T0_l) in each block's face frame (prestress(), Interface_Condition.f90:
  * 1802-2440) and the parameters of the linear slip-weakening strength
@@ -270,8 +272,21 @@ int wql_block_kernel_times(wql_block *b, double ms[3], int32_t *stages);
  * are built at wql_iface_create (TPV5 :1283-1299, 1821-1830; TPV28 :1459-1475,
  * 2049-2079 with initial_stress_condition.f90:78-84); WQL_IFACE_SLIP_WEAKENING
  * takes them from the caller (e.g. filled by the reference's own prestress()). */
+/* Rate-and-state friction, problems TPV101 / TPV102 (coupling
+ * 'rate-and-state_friction', Interface_Condition.f90:375-436): per node the
+ * slip rate V solves V + a sigma_n / eta asinh(V / (2 V0) e^(psi / a)) =
+ * Phi / eta by the reference's regula falsi (:1170-1265; FP64 in every
+ * precision), the hats follow rate_state_friction (:918-1067), and the
+ * state psi evolves as dpsi/dt = b V0 / L0 e^(-(psi - f0) / b) - V b / L0
+ * (:431-436) with L0 0.02, f0 0.6, b 0.012, V0 1e-6 and a(y, z) 0.008 inside
+ * the 15 km box, 0.016 outside (smooth_boxcar); the prestress is the traction
+ * of the stress (-120, 0, -120, 0, 75, 0) plus the nucleation 25 F(r) G(t)
+ * on T0_l (:2324-2378).  The initial state is desc->W0 ([nz][ny]) or 0 as
+ * init_iface sets it (iface.f90:38; the reference's Init_Vel_State,
+ * :2722-2850, which would set the steady state, is not called there). */
 enum { WQL_IFACE_LOCKED = 0, WQL_IFACE_SLIP_WEAKENING_TPV5 = 1,
-       WQL_IFACE_SLIP_WEAKENING_TPV28 = 2, WQL_IFACE_SLIP_WEAKENING = 3 };
+       WQL_IFACE_SLIP_WEAKENING_TPV28 = 2, WQL_IFACE_SLIP_WEAKENING = 3,
+       WQL_IFACE_RATE_STATE_TPV101 = 4, WQL_IFACE_RATE_STATE_TPV102 = 5 };
 typedef struct wql_iface_desc {
   int32_t coupling;          /* WQL_IFACE_*: coupling 'locked' / 'slip-weakening_friction'
                                 with problem 'TPV5' / 'TPV28' / caller tables */
@@ -283,6 +298,7 @@ typedef struct wql_iface_desc {
   const double *T0L, *T0R;   /* WQL_IFACE_SLIP_WEAKENING only: prestress [3][nz][ny] in the
                                 (n, m, l) frame of block L's face / block R's face */
   const double *fs, *fd, *Dc;  /* WQL_IFACE_SLIP_WEAKENING only: [nz][ny] each */
+  const double *W0;          /* WQL_IFACE_RATE_STATE_*: initial state [nz][ny]; null = 0 */
 } wql_iface_desc;
 typedef struct wql_iface wql_iface;
 /* init_iface (iface.f90:13-55): S = 0, DS = 1e40, rupture time 1e9; for
@@ -308,6 +324,8 @@ int wql_iface_set_frames(wql_iface *I, int32_t nslots);
 int wql_iface_get_frames(wql_iface *I, int32_t nslots, double *out);
 /* interface state S[4][nz][ny], DS[4][nz][ny], rupture time [nz][ny] (host) */
 int wql_iface_get_state(wql_iface *I, double *S, double *DS, double *trup);
+/* rate-and-state state variable psi [nz][ny] (host; I%W of iface_type) */
+int wql_iface_get_friction_state(wql_iface *I, double *W);
 
 /* ---- discrete adjoint of the fused stage --------------------------------
  * No reference counterpart (the reference has no adjoint); these serve the

@@ -67,6 +67,7 @@ class _Iface(C.Structure):
 
 
 IFACE_LOCKED, IFACE_SLIP_WEAKENING_TPV5, IFACE_SLIP_WEAKENING_TPV28 = 0, 1, 2
+IFACE_RATE_STATE_TPV101, IFACE_RATE_STATE_TPV102 = 4, 5   # the library's coupling codes
 
 _lib = None
 
@@ -130,7 +131,26 @@ def _load(path):
                                     C.c_double, C.c_int, C.c_int]
     L.orc_iface_scale.argtypes = [C.POINTER(_Iface), C.c_double]
     L.orc_iface_update.argtypes = [C.POINTER(_Iface), C.c_double]
+    L.orc_rs_regula_falsi.argtypes = [C.c_double] * 7
+    L.orc_rs_regula_falsi.restype = C.c_double
+    L.orc_rs_state_rate.argtypes = [C.c_double] * 6
+    L.orc_rs_state_rate.restype = C.c_double
+    L.orc_rs_a.argtypes = [C.c_double, C.c_double]
+    L.orc_rs_a.restype = C.c_double
     return L
+
+
+def rs_regula_falsi(V, Phi, eta, sigma_n, psi, V0, a):
+    """Regula_Falsi of the restatement: the slip rate from the guess V."""
+    return lib().orc_rs_regula_falsi(V, Phi, eta, sigma_n, psi, V0, a)
+
+
+def rs_state_rate(W, Vel, L0=0.02, f0=0.6, b=0.012, V0=1.0e-6):
+    return lib().orc_rs_state_rate(W, Vel, L0, f0, b, V0)
+
+
+def rs_a(y, z):
+    return lib().orc_rs_a(y, z)
 
 
 def _ptr(a):
@@ -322,7 +342,7 @@ class OracleIface:
     # TPV28 initial stress of the source (initial_stress_condition.f90:78-84)
     TPV28_STRESS = (-60.0, 0.0, 60.0, 0.0, 29.38, 0.0)
 
-    def __init__(self, ny, nz, yf, zf, coupling=IFACE_SLIP_WEAKENING_TPV5, s0=None):
+    def __init__(self, ny, nz, yf, zf, coupling=IFACE_SLIP_WEAKENING_TPV5, s0=None, W0=None):
         self.ny, self.nz, self.coupling = ny, nz, coupling
         self.s0 = tuple(self.TPV28_STRESS if s0 is None else s0)
         shp = (nz, ny)
@@ -330,7 +350,8 @@ class OracleIface:
         self.zf = np.ascontiguousarray(zf, dtype=np.float64).reshape(shp)
         self.S = np.zeros((4,) + shp)
         self.DS = np.full((4,) + shp, 1.0e40)       # iface.f90:38-39
-        self.W = np.zeros(shp)
+        # state variable: 0 (iface.f90:38) or the caller's initial state
+        self.W = np.zeros(shp) if W0 is None else np.array(W0, dtype=np.float64).reshape(shp)
         self.DW = np.full(shp, 1.0e40)
         self.Uhat = np.zeros((6,) + shp)
         self.Vhat = np.zeros((6,) + shp)

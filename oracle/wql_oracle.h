@@ -107,7 +107,21 @@ void orc_set_num_threads(int n);
 double orc_source_time(int type, double t, double t0, double w);
 
 /* ---- two-block q-interface (wql_oracle_iface.c) ------------------------ */
-enum { ORC_IFACE_LOCKED = 0, ORC_IFACE_SLIP_WEAKENING_TPV5 = 1, ORC_IFACE_SLIP_WEAKENING_TPV28 = 2 };
+enum { ORC_IFACE_LOCKED = 0, ORC_IFACE_SLIP_WEAKENING_TPV5 = 1, ORC_IFACE_SLIP_WEAKENING_TPV28 = 2,
+       /* coupling 'rate-and-state_friction' with problem 'TPV101' / 'TPV102'
+        * (the values of the library's WQL_IFACE_* codes; 3 is its caller-table
+        * slip weakening, which the oracle has no use for) */
+       ORC_IFACE_RATE_STATE_TPV101 = 4, ORC_IFACE_RATE_STATE_TPV102 = 5 };
+
+/* rate-and-state pieces (Interface_Condition.f90), exposed for their pins:
+ * Regula_Falsi (:1170-1239) on f (:1241-1265) from the guess V, returning the
+ * slip rate; and the state rate of TPV101 / TPV102 (:431-436) */
+double orc_rs_regula_falsi(double V, double Phi, double eta, double sigma_n, double psi,
+                           double V0, double a);
+double orc_rs_state_rate(double W, double Vel, double L0, double f0, double b, double V0);
+/* the friction parameter a of TPV101 / TPV102 at (y, z) (:386-400, with
+ * smooth_boxcar :2460-2476) */
+double orc_rs_a(double y, double z);
 
 /* Interface between the x-high face of block L and the x-low face of block
  * R (same ny, nz).  Face arrays [k][NF], fidx = (y-1) + ny*(z-1).
@@ -130,6 +144,9 @@ typedef struct orc_iface {
 
 /* enforce_iface_conditions for one side: side 2 = block L's call (SAT into
  * L), side 1 = block R's call (SAT into R); both add to DS */
+/* rate-and-state: I->W is the state psi (iface.f90:38, 0 there; or the
+ * caller's initial state), I->DW its rate; the evaluation of block L's side
+ * adds the state rate once per stage, like DS */
 void orc_iface_enforce(orc_iface *I, orc_block *L, orc_block *R, double t, int stage, int side);
 void orc_iface_scale(orc_iface *I, double A);     /* scale_rates_iface (iface.f90:63-73) */
 void orc_iface_update(orc_iface *I, double inc);  /* update_fields_iface (iface.f90:76-88) */

@@ -1,7 +1,8 @@
 /*
  * wql_oracle_iface.c -- CPU restatement (FP64) of the two-block interface of
  * WaveQLab3D-Q: the q-direction interface between the x-high face of block L
- * and the x-low face of block R, locked or TPV5 linear slip-weakening.
+ * and the x-low face of block R: locked, TPV5 / TPV28 linear slip-weakening,
+ * or TPV101 / TPV102 rate-and-state friction.
  * TEST INFRASTRUCTURE ONLY (see wql_oracle.h).
  *
  * Follows, step by step and in the same order of operations:
@@ -16,7 +17,10 @@
  *   Interface_RHS (Interface_Condition.f90:12-513): hats by Welded_Interface
  *     (:515-590) or Slip_Weakening_Friction (:745-916) with solve (:1095-1168),
  *     Tau_strength TPV5 / TPV28 (:1283-1299, 1459-1475), prestress TPV5 /
- *     TPV28 (:1802-1830, 2049-2079), boxcar (:2438-2457); DS(1:3) += Q - P,
+ *     TPV28 (:1802-1830, 2049-2079), boxcar (:2438-2457); or
+ *     rate_state_friction (:918-1067) with solve's rate-and-state branch
+ *     (:1120-1146), Regula_Falsi and f (:1170-1265), prestress TPV101 /
+ *     TPV102 (:2324-2378) and the state rate (:420-436); DS(1:3) += Q - P,
  *     DS(4) += |slip rate|;
  *   rupture_front (CouplingForcing.f90:580-600);
  *   scale_rates_iface / update_fields_iface (iface.f90:58-85).
@@ -70,13 +74,96 @@ static void prestress_tpv28(const double *s0, const double *n, const double *m, 
   *T0l = *T0l + tau_nuke;
 }
 
-/* prestress of the slip-weakening problems in (n, m, l) */
+/* prestress, problems TPV101 / TPV102 (Interface_Condition.f90:2324-2378):
+ * the traction of the initial stress tensor (-120, 0, -120, 0, 75, 0)
+ * (initial_stress_condition.f90:114-120) on the face, in (n, m, l), plus the
+ * nucleation stress 25 F(r) G(t) on T0_l, F(r) = exp(r^2 / (r^2 - 9)) for
+ * r < 3 km around (y, z) = (7.5, 0), G ramping from 0 to 1 over 1 s */
+static void prestress_tpv101(const double *n, const double *m, const double *l, double y,
+                             double z, double t, double *T0n, double *T0m, double *T0l) {
+  const double sxx = -120.0, syy = 0.0, szz = -120.0, sxy = 0.0, sxz = 75.0, syz = 0.0;
+  const double Tx = n[0] * sxx + n[1] * sxy + n[2] * sxz;
+  const double Ty = n[0] * sxy + n[1] * syy + n[2] * syz;
+  const double Tz = n[0] * sxz + n[1] * syz + n[2] * szz;
+  *T0n = n[0] * Tx + n[1] * Ty + n[2] * Tz;
+  *T0m = m[0] * Tx + m[1] * Ty + m[2] * Tz;
+  *T0l = l[0] * Tx + l[1] * Ty + l[2] * Tz;
+  const double r = sqrt((y - 7.5) * (y - 7.5) + z * z);
+  double F = 0.0;
+  if (r < 3.0) F = exp(r * r / (r * r - 9.0));
+  double G = 0.0;
+  if (t > 0.0 && t < 1.0) G = exp((t - 1.0) * (t - 1.0) / (t * (t - 2.0)));
+  if (t >= 1.0) G = 1.0;
+  *T0l = *T0l + 25.0 * F * G;
+}
+
+static int is_rate_state(int coupling) {
+  return coupling == ORC_IFACE_RATE_STATE_TPV101 || coupling == ORC_IFACE_RATE_STATE_TPV102;
+}
+
+/* prestress of the friction problems in (n, m, l) at time t */
 static void prestress(const orc_iface *I, const double *n, const double *m, const double *l,
-                      double y, double z, double T0[3]) {
+                      double y, double z, double t, double T0[3]) {
   if (I->coupling == ORC_IFACE_SLIP_WEAKENING_TPV28)
     prestress_tpv28(I->s0, n, m, l, y, z, &T0[0], &T0[1], &T0[2]);
+  else if (is_rate_state(I->coupling))
+    prestress_tpv101(n, m, l, y, z, t, &T0[0], &T0[1], &T0[2]);
   else
     prestress_tpv5(y, z, &T0[0], &T0[1], &T0[2]);
+}
+
+/* smooth_boxcar (Interface_Condition.f90:2460-2476) */
+static double smooth_boxcar(double x, double w, double wr) {
+  if (-w <= x && x <= w) return 1.0;
+  if (fabs(x) > w && fabs(x) < w + wr)
+    return 0.5 * (1.0 + tanh(wr / (fabs(x) - w - wr) + wr / (fabs(x) - w)));
+  return 0.0;
+}
+
+double orc_rs_a(double y, double z) {
+  return 0.008 + 0.008 * (1.0 - smooth_boxcar(y, 15.0, 3.0) * smooth_boxcar(z, 15.0, 3.0));
+}
+
+/* f (Interface_Condition.f90:1241-1265) */
+static double rs_f(double V, double Phi, double eta, double sigma_n, double psi, double V0,
+                   double a) {
+  return V + a * sigma_n / eta * asinh(0.5 * V / V0 * exp(psi / a)) - Phi / eta;
+}
+
+double orc_rs_regula_falsi(double V, double Phi, double eta, double sigma_n, double psi,
+                           double V0, double a) {
+  const double tol = 1.0e-12;
+  double err = 1.0;
+  double Vl = 0.0, Vr = Phi / eta;
+  int k = 1;
+  const int maxit = 5000;
+  double fv = rs_f(V, Phi, eta, sigma_n, psi, V0, a);
+  double fl = rs_f(Vl, Phi, eta, sigma_n, psi, V0, a);
+  double fr = rs_f(Vr, Phi, eta, sigma_n, psi, V0, a);
+  if (fabs(fv) <= tol) return V;
+  if (fabs(fl) <= tol) return Vl;
+  if (fabs(fr) <= tol) return Vr;
+  while (k <= maxit && err >= tol) {
+    const double V1 = V;
+    if (fv * fl > tol) {
+      Vl = V;
+      V = Vr - (Vr - Vl) * fr / (fr - fl);
+      fl = fv;
+    } else if (fv * fr > tol) {
+      Vr = V;
+      V = Vr - (Vr - Vl) * fr / (fr - fl);
+      fr = fv;
+    }
+    fv = rs_f(V, Phi, eta, sigma_n, psi, V0, a);
+    err = fabs(V - V1);
+    k = k + 1;
+  }
+  return V;
+}
+
+/* the state rate of TPV101 / TPV102 (Interface_Condition.f90:431-436) */
+double orc_rs_state_rate(double W, double Vel, double L0, double f0, double b, double V0) {
+  return b * V0 / L0 * exp(-(W - f0) / b) - Vel * b / L0;
 }
 
 /* Tau_strength, problems TPV5 (Interface_Condition.f90:1283-1299) and TPV28
@@ -99,8 +186,10 @@ static double tau_strength(int coupling, double sigma_n, double D, double y, dou
  * grid values U, V; returns the slip-rate magnitude of the friction solve */
 static void hats(const orc_iface *I, const double *n, const double *m, const double *l,
                  const double U[6], const double V[6], double Z1s, double Z2s, double Z1p,
-                 double Z2p, double slip, double y, double z, double P[6], double Q[6]) {
+                 double Z2p, double slip, double psi, double y, double z, double t, double P[6],
+                 double Q[6], double *Vel) {
   const int coupling = I->coupling;
+  *Vel = 0.0;
   const double p_n = Z1p * U[0] - U[3], p_m = Z1s * U[1] - U[4], p_l = Z1s * U[2] - U[5];
   const double q_n = Z2p * V[0] + V[3], q_m = Z2s * V[1] + V[4], q_l = Z2s * V[2] + V[5];
   if (coupling == ORC_IFACE_LOCKED) {
@@ -118,11 +207,49 @@ static void hats(const orc_iface *I, const double *n, const double *m, const dou
     Q[1] = (p_m + phi_m) / Z1s; Q[4] = phi_m;
     return;
   }
+  if (is_rate_state(coupling)) {
+    /* rate_state_friction (Interface_Condition.f90:918-1067) with solve's
+     * rate-and-state branch (:1120-1146); parameters of TPV101 / TPV102
+     * (:386-400) */
+    const double V0 = 1.0e-6, a = orc_rs_a(y, z);
+    const double eta_s = 1.0 / (1.0 / Z1s + 1.0 / Z2s);
+    const double eta_p = 1.0 / (1.0 / Z1p + 1.0 / Z2p);
+    double T0[3];
+    prestress(I, n, m, l, y, z, t, T0);
+    const double phi_n = eta_p * (q_n / Z2p - p_n / Z1p);
+    const double phi_m = eta_s * (q_m / Z2s - p_m / Z1s);
+    const double phi_l = eta_s * (q_l / Z2s - p_l / Z1s);
+    const double u_n_hat = (q_n - phi_n) / Z2p, Tu_n_hat = phi_n;
+    const double v_n_hat = (p_n + phi_n) / Z1p, Tv_n_hat = phi_n;
+    const double sigma_n = fmax(0.0, -(T0[0] + phi_n));
+    double vv_m = V[1] - U[1], vv_l = V[2] - U[2];
+    const double phi_1 = T0[1] + phi_m, phi_2 = T0[2] + phi_l;
+    const double Phi = sqrt(phi_1 * phi_1 + phi_2 * phi_2);
+    double Vs = sqrt(vv_m * vv_m + vv_l * vv_l);
+    if (Vs > Phi) Vs = 0.5 * Phi / eta_s;
+    Vs = orc_rs_regula_falsi(Vs, Phi, eta_s, sigma_n, psi, V0, a);
+    *Vel = Vs;
+    const double fv = sigma_n * a * asinh(0.5 * Vs / V0 * exp(psi / a));
+    vv_m = phi_1 / (eta_s + fv / Vs);
+    vv_l = phi_2 / (eta_s + fv / Vs);
+    const double tau_m = phi_1 - eta_s * vv_m, tau_l = phi_2 - eta_s * vv_l;
+    const double Tu_m_hat = tau_m - T0[1], Tu_l_hat = tau_l - T0[2];
+    const double Tv_m_hat = tau_m - T0[1], Tv_l_hat = tau_l - T0[2];
+    P[0] = u_n_hat;
+    P[1] = (q_m - Tv_m_hat) / Z2s - vv_m;
+    P[2] = (q_l - Tv_l_hat) / Z2s - vv_l;
+    P[3] = Tu_n_hat; P[4] = Tu_m_hat; P[5] = Tu_l_hat;
+    Q[0] = v_n_hat;
+    Q[1] = (Tu_m_hat + p_m) / Z1s + vv_m;
+    Q[2] = (Tu_l_hat + p_l) / Z1s + vv_l;
+    Q[3] = Tv_n_hat; Q[4] = Tv_m_hat; Q[5] = Tv_l_hat;
+    return;
+  }
   /* Slip_Weakening_Friction (Interface_Condition.f90:745-916) */
   const double eta_s = 1.0 / (1.0 / Z1s + 1.0 / Z2s);
   const double eta_p = 1.0 / (1.0 / Z1p + 1.0 / Z2p);
   double T0[3];
-  prestress(I, n, m, l, y, z, T0);
+  prestress(I, n, m, l, y, z, t, T0);
   const double T0n = T0[0], T0m = T0[1], T0l = T0[2];
   const double phi_n = eta_p * (q_n / Z2p - p_n / Z1p);
   const double phi_m = eta_s * (q_m / Z2s - p_m / Z1s);
@@ -239,19 +366,21 @@ static void eval_side(orc_iface *I, orc_block *L, orc_block *R, int ib, double t
       const double c1p = sqrt((2.0 * mu1 + lam1) / rho1), c1s = sqrt(mu1 / rho1);
       const double c2p = sqrt((2.0 * mu2 + lam2) / rho2), c2s = sqrt(mu2 / rho2);
       const double Z1s = rho1 * c1s, Z2s = rho2 * c2s, Z1p = rho1 * c1p, Z2p = rho2 * c2p;
-      double P[6], Q[6];
-      hats(I, n, m, l, U, V, Z1s, Z2s, Z1p, Z2p, I->S[3 * NF + fi], I->yf[fi],
-           I->zf[fi], P, Q);
+      double P[6], Q[6], Vel;
+      hats(I, n, m, l, U, V, Z1s, Z2s, Z1p, Z2p, I->S[3 * NF + fi], I->W[fi], I->yf[fi],
+           I->zf[fi], t, P, Q, &Vel);
       const double slip_rate = sqrt((Q[1] - P[1]) * (Q[1] - P[1]) + (Q[2] - P[2]) * (Q[2] - P[2]));
       /* the interface state belongs to block L's side: one DS increment per
        * stage, as when each block keeps its own interface copy */
       if (ib == 2) {
         for (int c = 0; c < 3; ++c) I->DS[c * NF + fi] = I->DS[c * NF + fi] + Q[c] - P[c];
         I->DS[3 * NF + fi] = I->DS[3 * NF + fi] + slip_rate;
+        if (is_rate_state(I->coupling))
+          I->DW[fi] = I->DW[fi] + orc_rs_state_rate(I->W[fi], Vel, 0.02, 0.6, 0.012, 1.0e-6);
       }
       if (stage == 1) {
         double T0[3] = {0.0, 0.0, 0.0};
-        if (I->coupling != ORC_IFACE_LOCKED) prestress(I, n, m, l, I->yf[fi], I->zf[fi], T0);
+        if (I->coupling != ORC_IFACE_LOCKED) prestress(I, n, m, l, I->yf[fi], I->zf[fi], t, T0);
         for (int c = 0; c < 3; ++c) {
           I->Uhat[(3 + c) * NF + fi] = P[3 + c] + T0[c];
           I->Vhat[(3 + c) * NF + fi] = Q[3 + c] + T0[c];

@@ -38,6 +38,7 @@ C_FIELDS = {
     "sizeof_iface_desc": "sizeof(wql_iface_desc)",
     "iface_y": "offsetof(wql_iface_desc, y)",
     "iface_Dc": "offsetof(wql_iface_desc, Dc)",
+    "iface_W0": "offsetof(wql_iface_desc, W0)",
 }
 
 

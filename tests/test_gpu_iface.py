@@ -285,3 +285,60 @@ def test_slip_weakening_tables_from_caller(oracle):
         itf.close()
     for key, _ in TPV5_KEYS:
         assert np.array_equal(runs[0][key], runs[1][key]), key
+
+
+@pytest.mark.parametrize("problem", [101, 102])
+def test_rate_state_fp64_matches_oracle(oracle, problem):
+    """Rate-and-state friction (TPV101 / TPV102) on two 21^3 blocks for
+    1.5 s, from steady sliding at 1e-12 m/s through nucleation to a rupture:
+    FP64 fault output and fields against the oracle.  The slip rate comes
+    from the reference's regula falsi, whose stopping rule (consecutive
+    iterates within 1e-12 m/s) can end one iteration apart when the last
+    bits of asinh / exp differ between the host's and the GPU's math
+    libraries; the tolerance is therefore stated on the outputs: 1e-9
+    relative L2 (measured below)."""
+    tp = PB.tpv101_cartesian(21, problem=problem)
+    print(f"case {tp.name}: {tp.nsteps} steps", flush=True)
+    Lo, Ro, Io = oracle_two_block(oracle, tp)
+    _, fr_o = oracle.two_block_time_step(Lo, Ro, Io, tp.dt, tp.nsteps, 0.0)
+    bl = gpu_block(tp.left, oracle.metrics(tp.left.X))
+    br = gpu_block(tp.right, oracle.metrics(tp.right.X))
+    itf = Interface(bl, br, tp.yf, tp.zf, coupling=tp.coupling, W0=tp.W0)
+    itf.set_frames(tp.nsteps)
+    itf.time_step(tp.dt, tp.nsteps, 0.0)
+    fr = itf.frames()
+    W = itf.friction_state()
+    errs = {k: rel_l2(fr[k], fr_o[k]) for k in ("S", "Svel", "Uface", "Vface", "Uhat", "Vhat")}
+    errs["W"] = rel_l2(W, Io.W)
+    print({k: "%.2e" % e for k, e in errs.items()}, flush=True)
+    assert max(errs.values()) <= 1e-9, errs
+    ok = fr_o["trup"][-1, 0] < 1e8
+    assert ok.sum() > 100
+    dt_rup = np.abs(fr["trup"][-1, 0][ok] - fr_o["trup"][-1, 0][ok]).max()
+    print(f"rupture times: {int(ok.sum())} nodes, max |diff| {dt_rup:.3e} s", flush=True)
+    assert dt_rup <= 1e-12 or dt_rup <= tp.dt    # equal, or one step apart at a threshold
+    bl.download(L.WQL_F)
+    br.download(L.WQL_F)
+    assert max(field_errors(bl.F, Lo.F) + field_errors(br.F, Ro.F)) <= 1e-9
+    itf.close()
+
+
+@pytest.mark.parametrize("precision", [L.WQL_FP32_MIXED, L.WQL_FP32_KAHAN])
+def test_rate_state_fp32_against_fp64(oracle, precision):
+    """FP32 blocks with the rate-and-state interface in FP64 (state, solve and
+    SAT on FP64 copies of the faces' frames and material): slip, slip rate
+    and state against the FP64 oracle over the whole rupture."""
+    tp = PB.tpv101_cartesian(21)
+    Lo, Ro, Io = oracle_two_block(oracle, tp)
+    _, fr_o = oracle.two_block_time_step(Lo, Ro, Io, tp.dt, tp.nsteps, 0.0)
+    bl = gpu_block(tp.left, oracle.metrics(tp.left.X), precision=precision)
+    br = gpu_block(tp.right, oracle.metrics(tp.right.X), precision=precision)
+    itf = Interface(bl, br, tp.yf, tp.zf, coupling=tp.coupling, W0=tp.W0)
+    itf.set_frames(tp.nsteps)
+    itf.time_step(tp.dt, tp.nsteps, 0.0)
+    fr = itf.frames()
+    errs = {k: rel_l2(fr[k], fr_o[k]) for k in ("S", "Svel", "Uhat", "Vhat")}
+    errs["W"] = rel_l2(itf.friction_state(), Io.W)
+    print(precision, {k: "%.2e" % e for k, e in errs.items()}, flush=True)
+    assert max(errs.values()) <= 1e-4, errs
+    itf.close()

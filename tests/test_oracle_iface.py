@@ -173,3 +173,112 @@ def test_transfinite_grid_reduces_to_cartesian():
     tp = PB.tpv28_curvilinear(21)
     assert np.abs(tp.left.X[:, :, :, -1] - tp.right.X[:, :, :, 0]).max() <= 1e-14
     assert np.abs(tp.left.X[0][:, :, -1]).max() > 0.05      # the fault face is curved
+
+
+# ---------------------------------------------------------------------------
+# rate-and-state friction (TPV101 / TPV102)
+# ---------------------------------------------------------------------------
+def _rs_root(Phi, eta, sigma, psi, V0, a):
+    """The slip rate of V + a sigma/eta asinh(V/(2 V0) e^(psi/a)) = Phi/eta by
+    bisection on [0, Phi/eta] (the function increases in V), 200 halvings."""
+    def f(V):
+        return V + a * sigma / eta * np.arcsinh(0.5 * V / V0 * np.exp(psi / a)) - Phi / eta
+    lo, hi = 0.0, Phi / eta
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        if f(mid) > 0:
+            hi = mid
+        else:
+            lo = mid
+    return 0.5 * (lo + hi)
+
+
+def test_regula_falsi_slip_rate_is_the_root(oracle):
+    """Regula_Falsi (Interface_Condition.f90:1170-1239) returns the root of
+    f (:1241-1265) within the accuracy its stopping rule allows, from any
+    guess, for slip rates from 1e-12 to 10 m/s: it stops when consecutive
+    iterates differ by less than 1e-12 m/s, which on its one-sided
+    convergence leaves up to ~1e-10 m/s (measured 9e-11) and, above 1e-6 m/s,
+    at most 1e-4 relative (measured 8e-5)."""
+    rng = np.random.default_rng(101)
+    worst = 0.0
+    for _ in range(200):
+        sigma = rng.uniform(20.0, 150.0)
+        a = rng.uniform(0.006, 0.02)
+        eta = rng.uniform(2.0, 6.0)
+        psi = rng.uniform(0.3, 0.9)
+        Vt = 10.0 ** rng.uniform(-12, 1)
+        # a stress transfer that makes Vt the exact root
+        Phi = eta * Vt + sigma * a * np.arcsinh(0.5 * Vt / 1e-6 * np.exp(psi / a))
+        V = oracle.rs_regula_falsi(rng.uniform(0.0, Phi / eta), Phi, eta, sigma, psi, 1e-6, a)
+        ref = _rs_root(Phi, eta, sigma, psi, 1e-6, a)
+        assert 0.0 <= V <= Phi / eta
+        assert abs(ref - Vt) <= 1e-9 * Vt + 1e-20
+        worst = max(worst, abs(V - Vt) / (1e-4 * Vt + 1e-10))
+    assert worst <= 1.0, worst
+
+
+def test_rate_state_steady_state(oracle):
+    """The state rate of TPV101 / TPV102 (:431-436) vanishes at psi_ss = f0 -
+    b ln(V/V0), is negative above it and positive below, and at psi_ss the
+    friction a asinh(V/(2 V0) e^(psi/a)) is the steady-state law f0 + (a - b)
+    ln(V/V0) (velocity weakening for a < b); the slip rate solved at the
+    steady-state stress is V itself."""
+    L0, f0, b, V0 = 0.02, 0.6, 0.012, 1e-6
+    for a in (0.008, 0.016):
+        for V in (1e-9, 1e-6, 1e-3, 0.1, 1.0, 5.0):
+            psi = f0 - b * np.log(V / V0)
+            rate = oracle.rs_state_rate(psi, V, L0, f0, b, V0)
+            assert abs(rate) <= 1e-12 * b * V / L0 + 1e-300
+            assert oracle.rs_state_rate(psi + 0.01, V, L0, f0, b, V0) < 0
+            assert oracle.rs_state_rate(psi - 0.01, V, L0, f0, b, V0) > 0
+            mu = a * np.arcsinh(0.5 * V / V0 * np.exp(psi / a))
+            assert abs(mu - (f0 + (a - b) * np.log(V / V0))) <= 1e-9
+            sigma, eta = 120.0, 5.0
+            Phi = eta * V + sigma * mu
+            got = oracle.rs_regula_falsi(0.5 * Phi / eta, Phi, eta, sigma, psi, V0, a)
+            assert abs(got - V) <= 1e-4 * V + 1e-10, (a, V, got)   # the solver's floor, above
+
+
+def test_rate_state_parameter_a():
+    """a(y, z) of TPV101 / TPV102: 0.008 inside the 15 km box, 0.016 beyond
+    the 3 km smooth_boxcar layer, monotone across it."""
+    from oracle import oracle as O
+    assert O.rs_a(7.5, 0.0) == 0.008 and O.rs_a(-15.0, 15.0) == 0.008
+    assert O.rs_a(18.0, 0.0) == 0.016 and O.rs_a(0.0, -18.5) == 0.016
+    vals = [O.rs_a(15.0 + 3.0 * k / 50.0, 0.0) for k in range(1, 50)]
+    assert all(0.008 <= v <= 0.016 for v in vals)
+    assert sum(0.009 < v < 0.015 for v in vals) >= 10
+    assert all(v2 >= v1 for v1, v2 in zip(vals, vals[1:]))
+
+
+def test_rate_state_initial_state_is_steady_sliding(oracle):
+    """The initial state of the TPV101 set-up (Init_Vel_State,
+    Interface_Condition.f90:2757-2787) makes the prestress the steady-state
+    friction at Vin = 1e-12 m/s: sigma_n a asinh(Vin/(2 V0) e^(psi/a)) = T0_l."""
+    tp = PB.tpv101_cartesian(21)
+    a = np.vectorize(oracle.rs_a)(tp.yf, tp.zf)
+    sigma, T0l = 120.0, 75.0
+    fric = sigma * a * np.arcsinh(0.5 * 1e-12 / 1e-6 * np.exp(tp.W0 / a))
+    assert np.allclose(fric, T0l, rtol=1e-10)
+
+
+def test_tpv101_nucleates_and_ruptures(oracle):
+    """TPV101 on the two 21^3 blocks for 1.5 s: the slip rate starts at Vin
+    (steady sliding), the nucleation stress ramps in, the rupture grows from
+    the nucleation zone (slip rate > 1 m/s, rupture times increasing with
+    distance from (7.5, 0))."""
+    from helpers import oracle_two_block
+    tp = PB.tpv101_cartesian(21)
+    Lo, Ro, Io = oracle_two_block(oracle, tp)
+    _, fr = oracle.two_block_time_step(Lo, Ro, Io, tp.dt, tp.nsteps, 0.0)
+    sv = np.sqrt((fr["Svel"][:, 1:] ** 2).sum(axis=1)).max(axis=(1, 2))
+    assert sv[0] < 1e-10 and sv[-1] > 1.0
+    tr = fr["trup"][-1, 0]
+    ok = tr < 1e8
+    assert ok.sum() > 100
+    r = np.sqrt((tp.yf - 7.5) ** 2 + tp.zf ** 2)
+    # later rupture farther out: rank correlation of rupture time with distance
+    from scipy.stats import spearmanr
+    rho = spearmanr(r[ok], tr[ok]).correlation
+    assert rho > 0.8, rho

@@ -83,6 +83,7 @@ struct wql_block {
   int slab_lo[6][3];         // slab node origin (0-based)
   int slab_n[6][3];          // slab extents
   void* face[6];
+  double* face64[6];         // host copy of face[f] in FP64 (the interface reads it)
   long long NF[6];
   hipStream_t stream;
   int nsrc;
@@ -155,10 +156,14 @@ struct wql_iface {
   int ny, nz;
   long long NF;
   int coupling;
-  // one allocation of 36 planes: S 4, DS 4, trup, prestress of L 3 and of R 3,
-  // fs, fd, Dc, SAT forcing of L 9 and of R 9
+  // one allocation of kIfacePlanes planes: S 4, DS 4, trup, prestress of L 3
+  // and of R 3, fs, fd, Dc, SAT forcing of L 9 and of R 9, the rate-and-state
+  // state W, its rate DW, the parameter a and the nucleation amplitude, and
+  // FP64 copies of the coupled faces' 13 frame planes and 3 material planes
+  // of each side
   double* buf;
   double *S, *DS, *trup, *T0L, *T0R, *fs, *fd, *Dc, *satL, *satR;
+  double *W, *DW, *rs_a, *rs_nuc, *frL, *frR, *matL, *matR;
   double* frames;   // nslots * kFramePlanes planes of fault output
   int nslots, slot;
   hipEvent_t ev_iface;  // on L's stream after k_iface: block R's stage waits for it
@@ -460,6 +465,7 @@ extern "C" void wql_block_destroy(wql_block* b) {
     if (b->pQ[f]) (void)hipFree(b->pQ[f]);
     if (b->pDQ[f]) (void)hipFree(b->pDQ[f]);
     if (b->face[f]) (void)hipFree(b->face[f]);
+    free(b->face64[f]);
   }
   for (int i = 0; i < b->nev_alloc; ++i) (void)hipEventDestroy(b->ev[i]);
   free(b->ev);
@@ -821,15 +827,21 @@ static int put_face(wql_block* b, int f) {
                 mets[m][off0 + x * body0.sx + y * body0.sy + z * body0.sz];
     }
   }
-  std::vector<T> out(13 * NF);
-  for (long long k = 0; k < 9 * NF; ++k) out[k] = (T)frame[k];
+  if (!b->face64[f]) {
+    b->face64[f] = (double*)malloc((size_t)(13 * NF) * sizeof(double));
+    if (!b->face64[f]) return fail(WQL_ENOMEM, "host allocation failed");
+  }
+  double* o64 = b->face64[f];
+  for (long long k = 0; k < 9 * NF; ++k) o64[k] = frame[k];
   for (long long k = 0; k < NF; ++k) {
     const double qx = met[k], qy = met[NF + k], qz = met[2 * NF + k];
-    out[9 * NF + k] = (T)sqrt(qx * qx + qy * qy + qz * qz);
-    out[10 * NF + k] = (T)qx;
-    out[11 * NF + k] = (T)qy;
-    out[12 * NF + k] = (T)qz;
+    o64[9 * NF + k] = sqrt(qx * qx + qy * qy + qz * qz);
+    o64[10 * NF + k] = qx;
+    o64[11 * NF + k] = qy;
+    o64[12 * NF + k] = qz;
   }
+  std::vector<T> out(13 * NF);
+  for (long long k = 0; k < 13 * NF; ++k) out[k] = (T)o64[k];
   HIP_TRY(hipMemcpy(b->face[f], out.data(), out.size() * sizeof(T), hipMemcpyHostToDevice));
   return WQL_OK;
 }
@@ -919,13 +931,18 @@ static int transfer(wql_block* b, uint32_t mask, bool up) {
   return rc;
 }
 
+static int iface_fill_faces(wql_iface* I);
+
 extern "C" int wql_block_upload(wql_block* b, uint32_t mask) {
   if (!b) return fail(WQL_EINVAL, "null block");
-  const int rc = b->d.precision == WQL_FP64 ? transfer<double>(b, mask, true)
-                                            : transfer<float>(b, mask, true);
+  int rc = b->d.precision == WQL_FP64 ? transfer<double>(b, mask, true)
+                                      : transfer<float>(b, mask, true);
   // the data is on the device when the call returns (other streams, e.g. an
   // interface's, may read it next)
   HIP_TRY(hipStreamSynchronize(b->stream));
+  // a coupled block's new frames or material reach the interface's FP64 copies
+  if (!rc && b->iface && b->iface->L && b->iface->R && (mask & (WQL_FACES | WQL_MATERIAL)))
+    rc = iface_fill_faces(b->iface);
   return rc;
 }
 
@@ -1546,14 +1563,66 @@ static int to_host(std::vector<double>& dst, const double* src, long long n, boo
   return WQL_OK;
 }
 
-// the (l, m, n) frame planes of a block face, as doubles on the host
-template <typename T>
-static int face_frame_host(wql_block* b, int f, std::vector<double>& fr) {
-  const long long NF = b->NF[f];
-  std::vector<T> h((size_t)(9 * NF));
-  HIP_TRY(hipMemcpy(h.data(), b->face[f], h.size() * sizeof(T), hipMemcpyDeviceToHost));
-  fr.assign(h.begin(), h.end());
+// the FP64 material (lambda, mu, rho) of the nodes of q face f (0: x = 0,
+// 1: x = nx - 1) from the descriptor's array (host or device), [c][y + ny z]
+static int face_material64(wql_block* b, int f, std::vector<double>& out) {
+  const wql_block_desc& d = b->d;
+  if (!d.M) return fail(WQL_EINVAL, "material array M is null");
+  const long long NF = (long long)b->ny * b->nz;
+  out.assign((size_t)(3 * NF), 0.0);
+  const View v = make_view(d.M, d.body);
+  const long long x1 = f == 1 ? b->nx : 1;
+  for (int c = 0; c < 3; ++c) {
+    const long long off0 = view_off(v, x1, 1, 1, c);
+    if (d.arrays_on_device) {
+      double* tmpd = nullptr;
+      HIP_TRY(hipMalloc((void**)&tmpd, NF * sizeof(double)));
+      hipError_t e = launch_gather<double>(tmpd, d.M + off0, 1, b->ny, b->nz, 1, v.sx, v.sy,
+                                           v.sz, 0, 0, b->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+      if (e == hipSuccess)
+        e = hipMemcpy(out.data() + c * NF, tmpd, NF * sizeof(double), hipMemcpyDeviceToHost);
+      (void)hipFree(tmpd);
+      if (e != hipSuccess) return fail(WQL_EHIP, "face material gather: %s", hipGetErrorString(e));
+    } else {
+      for (long long z = 0; z < b->nz; ++z)
+        for (long long y = 0; y < b->ny; ++y)
+          out[c * NF + y + b->ny * z] = d.M[off0 + y * v.sy + z * v.sz];
+    }
+  }
   return WQL_OK;
+}
+
+// FP64 copies of the coupled faces' frames and material in the interface
+// buffer (in every precision the interface arithmetic is FP64; the blocks'
+// FP32 planes are not its operands)
+static int iface_fill_faces(wql_iface* I) {
+  wql_block* L = I->L;
+  wql_block* R = I->R;
+  const long long NF = I->NF;
+  if (!L->face64[1] || !R->face64[0])
+    return fail(WQL_ESTATE, "the coupled faces' frames have not been uploaded (WQL_FACES)");
+  const size_t nb = (size_t)NF * sizeof(double);
+  HIP_TRY(hipMemcpy(I->frL, L->face64[1], 13 * nb, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(I->frR, R->face64[0], 13 * nb, hipMemcpyHostToDevice));
+  std::vector<double> mL, mR;
+  if (int rc = face_material64(L, 1, mL)) return rc;
+  if (int rc = face_material64(R, 0, mR)) return rc;
+  HIP_TRY(hipMemcpy(I->matL, mL.data(), 3 * nb, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(I->matR, mR.data(), 3 * nb, hipMemcpyHostToDevice));
+  return WQL_OK;
+}
+
+static bool is_rate_state(int coupling) {
+  return coupling == WQL_IFACE_RATE_STATE_TPV101 || coupling == WQL_IFACE_RATE_STATE_TPV102;
+}
+
+// smooth_boxcar (Interface_Condition.f90:2460-2476)
+static double smooth_boxcar(double x, double w, double wr) {
+  if (-w <= x && x <= w) return 1.0;
+  if (fabs(x) > w && fabs(x) < w + wr)
+    return 0.5 * (1.0 + tanh(wr / (fabs(x) - w - wr) + wr / (fabs(x) - w)));
+  return 0.0;
 }
 
 // Slip-weakening tables of the interface: prestress in each side's (n, m, l)
@@ -1563,6 +1632,50 @@ static int iface_tables(wql_iface* I, wql_block* L, wql_block* R, const wql_ifac
   if (d->coupling == WQL_IFACE_LOCKED) return WQL_OK;
   const bool dev = d->arrays_on_device != 0;
   std::vector<double> T0L(3 * NF), T0R(3 * NF), fs(NF), fd(NF), Dc(NF);
+  if (is_rate_state(d->coupling)) {
+    // TPV101 / TPV102 (Interface_Condition.f90:386-400, 2324-2378): the
+    // traction of the initial stress (-120, 0, -120, 0, 75, 0)
+    // (initial_stress_condition.f90:114-120) in each side's frame, the
+    // parameter a(y, z) and the nucleation amplitude 25 F(r) (the stage adds
+    // 25 F(r) G(t) to T0_l); the initial state from the caller or 0
+    // (iface.f90:38)
+    std::vector<double> y, z, a(NF), nuc(NF), W0(NF, 0.0);
+    if (int rc = to_host(y, d->y, NF, dev)) return rc;
+    if (int rc = to_host(z, d->z, NF, dev)) return rc;
+    if (d->W0)
+      if (int rc = to_host(W0, d->W0, NF, dev)) return rc;
+    static const double s0[6] = {-120.0, 0.0, -120.0, 0.0, 75.0, 0.0};
+    const double* frs[2] = {L->face64[1], R->face64[0]};
+    std::vector<double>* outs[2] = {&T0L, &T0R};
+    for (long long k = 0; k < NF; ++k) {
+      for (int side = 0; side < 2; ++side) {
+        const double* fr = frs[side];
+        double l[3], m[3], n[3];
+        for (int c = 0; c < 3; ++c) {
+          l[c] = fr[c * NF + k];
+          m[c] = fr[(3 + c) * NF + k];
+          n[c] = fr[(6 + c) * NF + k];
+        }
+        const double Tx = n[0] * s0[0] + n[1] * s0[3] + n[2] * s0[4];
+        const double Ty = n[0] * s0[3] + n[1] * s0[1] + n[2] * s0[5];
+        const double Tz = n[0] * s0[4] + n[1] * s0[5] + n[2] * s0[2];
+        std::vector<double>& t = *outs[side];
+        t[k] = n[0] * Tx + n[1] * Ty + n[2] * Tz;
+        t[NF + k] = m[0] * Tx + m[1] * Ty + m[2] * Tz;
+        t[2 * NF + k] = l[0] * Tx + l[1] * Ty + l[2] * Tz;
+      }
+      a[k] = 0.008 + 0.008 * (1.0 - smooth_boxcar(y[k], 15.0, 3.0) * smooth_boxcar(z[k], 15.0, 3.0));
+      const double r = sqrt((y[k] - 7.5) * (y[k] - 7.5) + z[k] * z[k]);
+      nuc[k] = r < 3.0 ? 25.0 * exp(r * r / (r * r - 9.0)) : 0.0;
+    }
+    const size_t nb = (size_t)NF * sizeof(double);
+    HIP_TRY(hipMemcpy(I->T0L, T0L.data(), 3 * nb, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(I->T0R, T0R.data(), 3 * nb, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(I->rs_a, a.data(), nb, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(I->rs_nuc, nuc.data(), nb, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(I->W, W0.data(), nb, hipMemcpyHostToDevice));
+    return WQL_OK;
+  }
   if (d->coupling == WQL_IFACE_SLIP_WEAKENING) {
     if (int rc = to_host(T0L, d->T0L, 3 * NF, dev)) return rc;
     if (int rc = to_host(T0R, d->T0R, 3 * NF, dev)) return rc;
@@ -1596,12 +1709,8 @@ static int iface_tables(wql_iface* I, wql_block* L, wql_block* R, const wql_ifac
       static const double kS0[6] = {-60.0, 0.0, 60.0, 0.0, 29.38, 0.0};
       const double* s0 = d->s0 ? d->s0 : kS0;
       const double pi = 3.141592653589793;
-      std::vector<double> frL, frR;
-      int rc = L->d.precision == WQL_FP64 ? face_frame_host<double>(L, 1, frL)
-                                          : face_frame_host<float>(L, 1, frL);
-      if (!rc) rc = R->d.precision == WQL_FP64 ? face_frame_host<double>(R, 0, frR)
-                                               : face_frame_host<float>(R, 0, frR);
-      if (rc) return rc;
+      std::vector<double> frL(L->face64[1], L->face64[1] + 9 * NF);
+      std::vector<double> frR(R->face64[0], R->face64[0] + 9 * NF);
       for (long long k = 0; k < NF; ++k) {
         const double r = sqrt((y[k] - 7.5) * (y[k] - 7.5) + z[k] * z[k]);
         double tau_nuke;
@@ -1647,9 +1756,10 @@ extern "C" int wql_iface_create(wql_block* L, wql_block* R, const wql_iface_desc
   if (!out) return fail(WQL_EINVAL, "null output pointer");
   *out = nullptr;
   if (!L || !R || !d || L == R) return fail(WQL_EINVAL, "need two distinct blocks and a descriptor");
-  if (d->coupling < WQL_IFACE_LOCKED || d->coupling > WQL_IFACE_SLIP_WEAKENING)
+  if (d->coupling < WQL_IFACE_LOCKED || d->coupling > WQL_IFACE_RATE_STATE_TPV102)
     return fail(WQL_EINVAL, "invalid coupling %d", d->coupling);
-  if ((d->coupling == WQL_IFACE_SLIP_WEAKENING_TPV5 || d->coupling == WQL_IFACE_SLIP_WEAKENING_TPV28) &&
+  if ((d->coupling == WQL_IFACE_SLIP_WEAKENING_TPV5 || d->coupling == WQL_IFACE_SLIP_WEAKENING_TPV28 ||
+       is_rate_state(d->coupling)) &&
       (!d->y || !d->z))
     return fail(WQL_EINVAL, "face coordinates y, z are null");
   if (d->coupling == WQL_IFACE_SLIP_WEAKENING &&
@@ -1672,7 +1782,8 @@ extern "C" int wql_iface_create(wql_block* L, wql_block* R, const wql_iface_desc
   I->NF = (long long)I->ny * I->nz;
   I->coupling = d->coupling;
   const long long NF = I->NF;
-  hipError_t e = hipMalloc((void**)&I->buf, 36 * NF * sizeof(double));
+  constexpr int kIfacePlanes = 36 + 4 + 26 + 6;
+  hipError_t e = hipMalloc((void**)&I->buf, kIfacePlanes * NF * sizeof(double));
   if (e != hipSuccess) {
     free(I);
     return fail(WQL_ENOMEM, "hipMalloc(interface) failed: %s", hipGetErrorString(e));
@@ -1680,6 +1791,9 @@ extern "C" int wql_iface_create(wql_block* L, wql_block* R, const wql_iface_desc
   I->S = I->buf; I->DS = I->S + 4 * NF; I->trup = I->DS + 4 * NF; I->T0L = I->trup + NF;
   I->T0R = I->T0L + 3 * NF; I->fs = I->T0R + 3 * NF; I->fd = I->fs + NF; I->Dc = I->fd + NF;
   I->satL = I->Dc + NF; I->satR = I->satL + 9 * NF;
+  I->W = I->satR + 9 * NF; I->DW = I->W + NF; I->rs_a = I->DW + NF; I->rs_nuc = I->rs_a + NF;
+  I->frL = I->rs_nuc + NF; I->frR = I->frL + 13 * NF; I->matL = I->frR + 13 * NF;
+  I->matR = I->matL + 3 * NF;
   auto cleanup = [&](int rc) {
     (void)hipFree(I->buf);
     if (I->ev_iface) (void)hipEventDestroy(I->ev_iface);
@@ -1687,10 +1801,17 @@ extern "C" int wql_iface_create(wql_block* L, wql_block* R, const wql_iface_desc
     free(I);
     return rc;
   };
+  if ((e = launch_fill<double>(I->W, NF, 0.0, L->stream)) != hipSuccess ||
+      (e = launch_fill<double>(I->rs_a, 2 * NF, 0.0, L->stream)) != hipSuccess ||
+      (e = hipStreamSynchronize(L->stream)) != hipSuccess)
+    return cleanup(fail(WQL_EHIP, "interface init: %s", hipGetErrorString(e)));
+  if (int rc = iface_fill_faces(I)) return cleanup(rc);
   if (int rc = iface_tables(I, L, R, d)) return cleanup(rc);
-  // init_iface: S = 0, DS = 1e40, trup = 1e9; the SAT planes start at zero
+  // init_iface: S = 0, DS = 1e40, W as set above, DW = 1e40, trup = 1e9;
+  // the SAT planes start at zero
   if ((e = launch_fill<double>(I->S, 4 * NF, 0.0, L->stream)) != hipSuccess ||
       (e = launch_fill<double>(I->DS, 4 * NF, 1.0e40, L->stream)) != hipSuccess ||
+      (e = launch_fill<double>(I->DW, NF, 1.0e40, L->stream)) != hipSuccess ||
       (e = launch_fill<double>(I->trup, NF, 1.0e9, L->stream)) != hipSuccess ||
       (e = launch_fill<double>(I->satL, 18 * NF, 0.0, L->stream)) != hipSuccess ||
       (e = hipStreamSynchronize(L->stream)) != hipSuccess)
@@ -1741,13 +1862,16 @@ static int iface_stage_t(wql_iface* I, double A, double Bdt, double t, int stage
   a.FL = (const S*)L->F[L->cur];
   a.FR = (const S*)R->F[R->cur];
   a.PL = L->P; a.PR = R->P;
-  a.ML = (const T*)L->M; a.MR = (const T*)R->M;
-  a.frL = (const T*)L->face[1]; a.frR = (const T*)R->face[0];
+  a.matL = I->matL; a.matR = I->matR;
+  a.frL = I->frL; a.frR = I->frR;
   a.penL = L->d.tau0 / L->d.hq;
   a.penR = R->d.tau0 / R->d.hq;
   a.S = I->S; a.DS = I->DS; a.trup = I->trup;
   a.T0L = I->T0L; a.T0R = I->T0R; a.fs = I->fs; a.fd = I->fd; a.Dc = I->Dc;
   a.satL = I->satL; a.satR = I->satR;
+  a.W = I->W; a.DW = I->DW; a.rs_a = I->rs_a; a.rs_nuc = I->rs_nuc;
+  // TPV101 / TPV102 friction constants (Interface_Condition.f90:386-400)
+  a.rs_L0 = 0.02; a.rs_f0 = 0.6; a.rs_b = 0.012; a.rs_V0 = 1.0e-6;
   a.A = A; a.Bdt = Bdt; a.t = t;
   a.stage1 = stage1 ? 1 : 0;
   a.frame = (stage1 && I->frames && I->slot < I->nslots)
@@ -1776,6 +1900,14 @@ extern "C" int wql_iface_stage(wql_iface* I, double A, double Bdt, double t, int
     case WQL_FP32_MIXED: return iface_stage_t<float>(I, A, Bdt, t, stage1);
     default: return iface_stage_t<float, float>(I, A, Bdt, t, stage1);
   }
+}
+
+extern "C" int wql_iface_get_friction_state(wql_iface* I, double* W) {
+  if (int rc = iface_ok(I)) return rc;
+  if (!W) return fail(WQL_EINVAL, "null output");
+  HIP_TRY(hipStreamSynchronize(I->L->stream));
+  HIP_TRY(hipMemcpy(W, I->W, (size_t)I->NF * sizeof(double), hipMemcpyDeviceToHost));
+  return WQL_OK;
 }
 
 extern "C" int wql_iface_time_step(wql_iface* I, double dt, int32_t nsteps, double* t) {

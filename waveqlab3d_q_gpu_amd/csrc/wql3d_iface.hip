@@ -12,13 +12,15 @@
 //   Interface_RHS (Interface_Condition.f90:12-513): Welded_Interface (:515-590)
 //     or Slip_Weakening_Friction (:745-916) with solve (:1095-1168), the
 //     prestress and the linear slip-weakening strength of the problem taken
-//     from per-node tables (built at wql_iface_create); DS(1:3) += Q - P,
+//     from per-node tables (built at wql_iface_create), or
+//     rate_state_friction (:918-1067) with the regula falsi slip-rate solve
+//     (:1120-1265) and the state rate (:420-436); DS(1:3) += Q - P,
 //     DS(4) += |slip rate|;
 //   rupture_front (CouplingForcing.f90:580-600);
 //   scale_rates_iface / update_fields_iface (iface.f90:58-85).
 // The interface state is block L's copy (one slip-rate increment per stage,
 // as each MPI rank holds its own interface).  All interface arithmetic is
-// FP64; material and face frames are read in the blocks' storage type T.
+// FP64, on FP64 copies of the coupled faces' frames and material.
 #include <math.h>
 
 #include "wql3d_internal.h"
@@ -85,6 +87,83 @@ __device__ __forceinline__ void hats(int coupling, const double U[6], const doub
   Q[3] = Tv_n_hat; Q[4] = Tv_m; Q[5] = Tv_l;
 }
 
+// f of the rate-and-state slip-rate equation (Interface_Condition.f90:1241-1265)
+__device__ __forceinline__ double rs_f(double V, double Phi, double eta, double sigma_n,
+                                       double psi, double V0, double a) {
+  return V + a * sigma_n / eta * asinh(0.5 * V / V0 * exp(psi / a)) - Phi / eta;
+}
+
+// Regula_Falsi (Interface_Condition.f90:1170-1239), step for step: bracket
+// [0, Phi/eta], the reference's update rule and stopping test (|V - V_prev|
+// < 1e-12 or 5000 iterations)
+__device__ double rs_regula_falsi(double V, double Phi, double eta, double sigma_n, double psi,
+                                  double V0, double a) {
+  const double tol = 1.0e-12;
+  double err = 1.0;
+  double Vl = 0.0, Vr = Phi / eta;
+  int k = 1;
+  const int maxit = 5000;
+  double fv = rs_f(V, Phi, eta, sigma_n, psi, V0, a);
+  double fl = rs_f(Vl, Phi, eta, sigma_n, psi, V0, a);
+  double fr = rs_f(Vr, Phi, eta, sigma_n, psi, V0, a);
+  if (fabs(fv) <= tol) return V;
+  if (fabs(fl) <= tol) return Vl;
+  if (fabs(fr) <= tol) return Vr;
+  while (k <= maxit && err >= tol) {
+    const double V1 = V;
+    if (fv * fl > tol) {
+      Vl = V;
+      V = Vr - (Vr - Vl) * fr / (fr - fl);
+      fl = fv;
+    } else if (fv * fr > tol) {
+      Vr = V;
+      V = Vr - (Vr - Vl) * fr / (fr - fl);
+      fr = fv;
+    }
+    fv = rs_f(V, Phi, eta, sigma_n, psi, V0, a);
+    err = fabs(V - V1);
+    k = k + 1;
+  }
+  return V;
+}
+
+// rate_state_friction (Interface_Condition.f90:918-1067) with solve's
+// rate-and-state branch (:1120-1146); returns the slip rate in *Vel
+__device__ __forceinline__ void hats_rate_state(const double U[6], const double V[6], double Z1s,
+                                                double Z2s, double Z1p, double Z2p,
+                                                const double T0[3], double psi, double a,
+                                                double V0, double P[6], double Q[6],
+                                                double* Vel) {
+  const double p_n = Z1p * U[0] - U[3], p_m = Z1s * U[1] - U[4], p_l = Z1s * U[2] - U[5];
+  const double q_n = Z2p * V[0] + V[3], q_m = Z2s * V[1] + V[4], q_l = Z2s * V[2] + V[5];
+  const double eta_s = 1.0 / (1.0 / Z1s + 1.0 / Z2s);
+  const double eta_p = 1.0 / (1.0 / Z1p + 1.0 / Z2p);
+  const double phi_n = eta_p * (q_n / Z2p - p_n / Z1p);
+  const double phi_m = eta_s * (q_m / Z2s - p_m / Z1s);
+  const double phi_l = eta_s * (q_l / Z2s - p_l / Z1s);
+  const double sigma_n = fmax(0.0, -(T0[0] + phi_n));
+  double vv_m = V[1] - U[1], vv_l = V[2] - U[2];
+  const double phi_1 = T0[1] + phi_m, phi_2 = T0[2] + phi_l;
+  const double Phi = sqrt(phi_1 * phi_1 + phi_2 * phi_2);
+  double Vs = sqrt(vv_m * vv_m + vv_l * vv_l);
+  if (Vs > Phi) Vs = 0.5 * Phi / eta_s;
+  Vs = rs_regula_falsi(Vs, Phi, eta_s, sigma_n, psi, V0, a);
+  *Vel = Vs;
+  const double fv = sigma_n * a * asinh(0.5 * Vs / V0 * exp(psi / a));
+  vv_m = phi_1 / (eta_s + fv / Vs);
+  vv_l = phi_2 / (eta_s + fv / Vs);
+  const double tau_m = phi_1 - eta_s * vv_m, tau_l = phi_2 - eta_s * vv_l;
+  const double Tm = tau_m - T0[1], Tl = tau_l - T0[2];
+  P[0] = (q_n - phi_n) / Z2p;
+  P[1] = (q_m - Tm) / Z2s - vv_m;
+  P[2] = (q_l - Tl) / Z2s - vv_l;
+  P[3] = phi_n; P[4] = Tm; P[5] = Tl;
+  Q[0] = (p_n + phi_n) / Z1p;
+  Q[1] = (Tm + p_m) / Z1s + vv_m;
+  Q[2] = (Tl + p_l) / Z1s + vv_l;
+  Q[3] = phi_n; Q[4] = Tm; Q[5] = Tl;
+}
+
 __device__ __forceinline__ void rot_in(const double n[3], const double m[3], const double l[3],
                                        double u[6]) {
   const double u1 = u[0], u2 = u[1], u3 = u[2], T1 = u[3], T2 = u[4], T3 = u[5];
@@ -114,20 +193,20 @@ struct Side {
 // one evaluation (the calling block's frame and metric column): hats,
 // slip-rate magnitude and the calling block's SAT forcing bc[9]
 template <typename T, typename S>
-__device__ __forceinline__ void evaluate(const IfaceArgs<T, S>& a, int ib, const T* fr, long long fi,
-                                         long long NF, const double u[9], const double v[9],
-                                         const Side& s1, const Side& s2, double slip,
-                                         const double T0[3], double P[6], double Q[6],
-                                         double* slip_rate, double bc[9]) {
+__device__ __forceinline__ void evaluate(const IfaceArgs<T, S>& a, int ib, const double* fr,
+                                         long long fi, long long NF, const double u[9],
+                                         const double v[9], const Side& s1, const Side& s2,
+                                         double slip, double psi, const double T0[3], double P[6],
+                                         double Q[6], double* slip_rate, double* Vel,
+                                         double bc[9]) {
   double l[3], m[3], n[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    l[k] = (double)fr[k * NF + fi];
-    m[k] = (double)fr[(3 + k) * NF + fi];
-    n[k] = (double)fr[(6 + k) * NF + fi];
+    l[k] = fr[k * NF + fi];
+    m[k] = fr[(3 + k) * NF + fi];
+    n[k] = fr[(6 + k) * NF + fi];
   }
-  const double qx = (double)fr[10 * NF + fi], qy = (double)fr[11 * NF + fi],
-               qz = (double)fr[12 * NF + fi];
+  const double qx = fr[10 * NF + fi], qy = fr[11 * NF + fi], qz = fr[12 * NF + fi];
   double U[6], V[6];
   U[3] = n[0] * u[3] + n[1] * u[6] + n[2] * u[7];
   U[4] = n[0] * u[6] + n[1] * u[4] + n[2] * u[8];
@@ -145,13 +224,18 @@ __device__ __forceinline__ void evaluate(const IfaceArgs<T, S>& a, int ib, const
   const double c1p = sqrt((2.0 * s1.mu + s1.lam) / s1.rho), c1s = sqrt(s1.mu / s1.rho);
   const double c2p = sqrt((2.0 * s2.mu + s2.lam) / s2.rho), c2s = sqrt(s2.mu / s2.rho);
   const double Z1s = s1.rho * c1s, Z2s = s2.rho * c2s, Z1p = s1.rho * c1p, Z2p = s2.rho * c2p;
-  double fs = 0.0, fd = 0.0, Dc = 1.0;
-  if (a.coupling != WQL_IFACE_LOCKED) {
-    fs = a.fs[fi];
-    fd = a.fd[fi];
-    Dc = a.Dc[fi];
+  *Vel = 0.0;
+  if (a.coupling == WQL_IFACE_RATE_STATE_TPV101 || a.coupling == WQL_IFACE_RATE_STATE_TPV102) {
+    hats_rate_state(U, V, Z1s, Z2s, Z1p, Z2p, T0, psi, a.rs_a[fi], a.rs_V0, P, Q, Vel);
+  } else {
+    double fs = 0.0, fd = 0.0, Dc = 1.0;
+    if (a.coupling != WQL_IFACE_LOCKED) {
+      fs = a.fs[fi];
+      fd = a.fd[fi];
+      Dc = a.Dc[fi];
+    }
+    hats(a.coupling, U, V, Z1s, Z2s, Z1p, Z2p, slip, T0, fs, fd, Dc, P, Q);
   }
-  hats(a.coupling, U, V, Z1s, Z2s, Z1p, Z2p, slip, T0, fs, fd, Dc, P, Q);
   *slip_rate = sqrt((Q[1] - P[1]) * (Q[1] - P[1]) + (Q[2] - P[2]) * (Q[2] - P[2]));
   double Pb[6], Qb[6];
 #pragma unroll
@@ -266,9 +350,12 @@ __global__ __launch_bounds__(256) void k_iface(const IfaceArgs<T, S> a) {
     v[c] = (double)a.FR[c * a.PR + iR];
   }
   Side s1, s2;
-  s1.lam = (double)a.ML[iL]; s1.mu = (double)a.ML[a.PL + iL]; s1.rho = (double)a.ML[2 * a.PL + iL];
-  s2.lam = (double)a.MR[iR]; s2.mu = (double)a.MR[a.PR + iR]; s2.rho = (double)a.MR[2 * a.PR + iR];
+  s1.lam = a.matL[fi]; s1.mu = a.matL[NF + fi]; s1.rho = a.matL[2 * NF + fi];
+  s2.lam = a.matR[fi]; s2.mu = a.matR[NF + fi]; s2.rho = a.matR[2 * NF + fi];
   const double slip = a.S[3 * NF + fi];
+  const bool rate_state =
+      a.coupling == WQL_IFACE_RATE_STATE_TPV101 || a.coupling == WQL_IFACE_RATE_STATE_TPV102;
+  const double psi = rate_state ? a.W[fi] : 0.0;
   // prestress of each side (zero for a welded interface, whose output
   // carries no prestress, Interface_Condition.f90:482-486)
   double T0L[3] = {0.0, 0.0, 0.0}, T0R[3] = {0.0, 0.0, 0.0};
@@ -278,12 +365,25 @@ __global__ __launch_bounds__(256) void k_iface(const IfaceArgs<T, S> a) {
       T0L[c] = a.T0L[c * NF + fi];
       T0R[c] = a.T0R[c * NF + fi];
     }
+    if (rate_state) {
+      // nucleation 25 F(r) G(t) on T0_l (Interface_Condition.f90:2340-2347)
+      const double t = a.t;
+      double G = 0.0;
+      if (t > 0.0 && t < 1.0) G = exp((t - 1.0) * (t - 1.0) / (t * (t - 2.0)));
+      if (t >= 1.0) G = 1.0;
+      T0L[2] = T0L[2] + a.rs_nuc[fi] * G;
+      T0R[2] = T0R[2] + a.rs_nuc[fi] * G;
+    }
   }
-  double P[6], Q[6], srate, bc[9];
+  double P[6], Q[6], srate, vel, bc[9];
   double trup = a.trup[fi];
   double out[15];
   // block L, side 2: SAT into L, the slip-rate increment
-  evaluate(a, 2, a.frL, fi, NF, u, v, s1, s2, slip, T0L, P, Q, &srate, bc);
+  evaluate(a, 2, a.frL, fi, NF, u, v, s1, s2, slip, psi, T0L, P, Q, &srate, &vel, bc);
+  double dw = 0.0;
+  if (rate_state)   // the state rate of TPV101 / TPV102 (Interface_Condition.f90:431-436)
+    dw = a.A * a.DW[fi] + (a.rs_b * a.rs_V0 / a.rs_L0 * exp(-(psi - a.rs_f0) / a.rs_b) -
+                           vel * a.rs_b / a.rs_L0);
   double ds[4];
 #pragma unroll
   for (int c = 0; c < 3; ++c) ds[c] = a.A * a.DS[c * NF + fi] + (Q[c] - P[c]);
@@ -292,7 +392,7 @@ __global__ __launch_bounds__(256) void k_iface(const IfaceArgs<T, S> a) {
 #pragma unroll
   for (int c = 0; c < 9; ++c) a.satL[c * NF + fi] = a.penL * bc[c];
   // block R, side 1: SAT into R (its hats are the same up to rounding)
-  evaluate(a, 1, a.frR, fi, NF, u, v, s1, s2, slip, T0R, P, Q, &srate, bc);
+  evaluate(a, 1, a.frR, fi, NF, u, v, s1, s2, slip, psi, T0R, P, Q, &srate, &vel, bc);
   if (a.stage1) stage1_output(a, P, Q, T0R, out, &trup);
 #pragma unroll
   for (int c = 0; c < 9; ++c) a.satR[c * NF + fi] = a.penR * bc[c];
@@ -317,6 +417,10 @@ __global__ __launch_bounds__(256) void k_iface(const IfaceArgs<T, S> a) {
   for (int c = 0; c < 4; ++c) {
     a.DS[c * NF + fi] = ds[c];
     a.S[c * NF + fi] = a.S[c * NF + fi] + a.Bdt * ds[c];
+  }
+  if (rate_state) {
+    a.DW[fi] = dw;
+    a.W[fi] = psi + a.Bdt * dw;
   }
 }
 

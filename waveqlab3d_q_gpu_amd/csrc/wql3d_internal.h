@@ -180,10 +180,10 @@ struct IfaceArgs {
   const St* FL;            // block L / R fields at the start of the stage (the
   const St* FR;            // RK state: double, or float under FP32-Kahan)
   long long PL, PR;        // plane strides
-  const T* ML;             // lambda, mu, rho planes
-  const T* MR;
-  const T* frL;            // 13 face planes of L's face 1 and R's face 0
-  const T* frR;
+  const double* matL;      // lambda, mu, rho of the coupled face nodes (FP64)
+  const double* matR;
+  const double* frL;       // 13 face planes of L's face 1 and R's face 0 (FP64)
+  const double* frR;
   double penL, penR;       // tau0/hq of each block
   double* S;               // 4 planes: slip (3) and slip-path length
   double* DS;              // 4 planes: rates of S
@@ -197,6 +197,14 @@ struct IfaceArgs {
   const double* Dc;
   double* satL;            // 9 planes each: tau0/hq * SAT forcing
   double* satR;
+  // rate-and-state friction (TPV101 / TPV102): state psi and its rate, the
+  // parameter a and the nucleation amplitude 25 F(r) per node, and the
+  // constants L0, f0, b, V0
+  double* W;
+  double* DW;
+  const double* rs_a;
+  const double* rs_nuc;
+  double rs_L0, rs_f0, rs_b, rs_V0;
   double A, Bdt, t;
   int stage1;
   double* frame;           // null, or kFramePlanes planes of the current output slot

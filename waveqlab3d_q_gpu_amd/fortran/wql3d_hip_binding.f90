@@ -20,7 +20,8 @@ module wql3d_hip_binding
   integer(c_int32_t), parameter :: WQL_F = 1, WQL_DF = 2, WQL_GEOMETRY = 4, WQL_MATERIAL = 8, &
        WQL_MEMORY = 16, WQL_QINV = 32, WQL_PML = 64, WQL_FACES = 128, WQL_ALL = 255
   integer(c_int32_t), parameter :: WQL_IFACE_LOCKED = 0, WQL_IFACE_SLIP_WEAKENING_TPV5 = 1, &
-       WQL_IFACE_SLIP_WEAKENING_TPV28 = 2, WQL_IFACE_SLIP_WEAKENING = 3
+       WQL_IFACE_SLIP_WEAKENING_TPV28 = 2, WQL_IFACE_SLIP_WEAKENING = 3, &
+       WQL_IFACE_RATE_STATE_TPV101 = 4, WQL_IFACE_RATE_STATE_TPV102 = 5
 
   type, bind(C) :: wql_box
      integer(c_int32_t) :: lo(3), hi(3)
@@ -49,7 +50,7 @@ module wql3d_hip_binding
 
   type, bind(C) :: wql_iface_desc
      integer(c_int32_t) :: coupling, arrays_on_device
-     type(c_ptr)        :: y, z, s0, T0L, T0R, fs, fd, Dc
+     type(c_ptr)        :: y, z, s0, T0L, T0R, fs, fd, Dc, W0
   end type wql_iface_desc
 
   interface
@@ -173,6 +174,9 @@ module wql3d_hip_binding
      end function
      integer(c_int) function wql_iface_set_frames(itf, nslots) bind(C)
        import; type(c_ptr), value :: itf; integer(c_int32_t), value :: nslots
+     end function
+     integer(c_int) function wql_iface_get_friction_state(itf, W) bind(C)
+       import; type(c_ptr), value :: itf; real(c_double) :: W(*)
      end function
      integer(c_int) function wql_iface_get_frames(itf, nslots, out) bind(C)
        import; type(c_ptr), value :: itf; integer(c_int32_t), value :: nslots

@@ -32,12 +32,13 @@ class Interface:
     (xx, yy, zz, xy, xz, yz), default the source's), or
     ``WQL_IFACE_SLIP_WEAKENING`` with ``tables`` = dict of host arrays T0L,
     T0R (3, nz, ny) (prestress in each side's (n, m, l) face frame) and fs,
-    fd, Dc (nz, ny).
+    fd, Dc (nz, ny), or ``WQL_IFACE_RATE_STATE_TPV101`` / ``_TPV102`` with
+    the initial state ``W0`` (nz, ny) (default 0, as init_iface sets it).
 
     The blocks must outlive the interface; call :meth:`close` first."""
 
     def __init__(self, left: Block, right: Block, y, z, coupling=L.WQL_IFACE_SLIP_WEAKENING_TPV5,
-                 s0=None, tables=None):
+                 s0=None, tables=None, W0=None):
         self.left, self.right = left, right
         self.ny, self.nz = left.nr, left.ns
         d = L.IfaceDesc()
@@ -63,6 +64,16 @@ class Interface:
                     shape + (self.nz, self.ny))
                 keep.append(a)
                 setattr(d, name, _ptr(a))
+        if W0 is not None:
+            # rate-and-state: initial state psi (nz, ny), same array kind as y, z
+            if on_dev:
+                if tuple(W0.shape) != (self.nz, self.ny):
+                    raise ValueError(f"W0 must be ({self.nz}, {self.ny})")
+                w = W0
+            else:
+                w = np.ascontiguousarray(W0, dtype=np.float64).reshape(self.nz, self.ny)
+            keep.append(w)
+            d.W0 = _ptr(w)
         self._keep = keep
         h = C.c_void_p()
         check(lib().wql_iface_create(left._h, right._h, C.byref(d), C.byref(h)))
@@ -117,6 +128,12 @@ class Interface:
         check(lib().wql_iface_get_state(self._h, S.ctypes.data_as(P), DS.ctypes.data_as(P),
                                         tr.ctypes.data_as(P)))
         return S, DS, tr
+
+    def friction_state(self):
+        """Rate-and-state state variable psi (nz, ny) (iface_type%W)."""
+        W = np.zeros((self.nz, self.ny))
+        check(lib().wql_iface_get_friction_state(self._h, W.ctypes.data_as(C.POINTER(C.c_double))))
+        return W
 
     def sync(self):
         self.left.sync()

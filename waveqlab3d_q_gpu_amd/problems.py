@@ -392,6 +392,7 @@ class TwoBlockProblem:
     nsteps: int
     name: str = ""
     s0: tuple = None               # TPV28 initial stress (None: the source's values)
+    W0: np.ndarray = None          # rate-and-state initial state (None: 0)
 
 
 def tpv5_cartesian(n=21, sat_form=1, coupling=None):
@@ -445,6 +446,74 @@ def tpv28_curvilinear(n=21, sat_form=1, s0=None):
                            yf=np.ascontiguousarray(XL[1][:, :, -1]),
                            zf=np.ascontiguousarray(XL[2][:, :, -1]), dt=dt, nsteps=nsteps,
                            name=f"tpv28_curvilinear_{n}", s0=None if s0 is None else tuple(s0))
+
+
+def rate_state_initial_state(yf, zf, T0, Vin=1.0e-12, L0=0.02, f0=0.6, b=0.012, V0=1.0e-6):
+    """Init_Vel_State for TPV101 / TPV102 (Interface_Condition.f90:2757-2787):
+    the state psi in which a fault loaded by the prestress T0 = (T0_n, T0_m,
+    T0_l) (3, nz, ny) slides steadily at Vin,
+    theta = L0/V0 exp((a ln(2 sinh(T0_l / (a sigma_n))) - f0 - a ln(Vin/V0)) / b),
+    psi = f0 + b ln(V0/L0 theta), sigma_n = -T0_n.  (The reference defines
+    this routine but does not call it: its interface starts at psi = 0.)"""
+    def sbox(x, w, wr):
+        ax = np.abs(x)
+        out = np.where(ax <= w, 1.0, 0.0)
+        mid = (ax > w) & (ax < w + wr)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            val = 0.5 * (1.0 + np.tanh(wr / (ax - w - wr) + wr / (ax - w)))
+        return np.where(mid, val, out)
+    a = 0.008 + 0.008 * (1.0 - sbox(yf, 15.0, 3.0) * sbox(zf, 15.0, 3.0))
+    sigma_n = -T0[0]
+    x = T0[2] / (a * sigma_n)
+    # ln(2 sinh x) without overflow: x + ln(1 - e^(-2x))
+    log2sinh = x + np.log1p(-np.exp(-2.0 * x))
+    log_theta = math.log(L0 / V0) + (a * log2sinh - f0 - a * math.log(Vin / V0)) / b
+    return f0 + b * (math.log(V0 / L0) + log_theta)
+
+
+def tpv101_cartesian(n=21, problem=101, nsteps=None, t_final=1.5):
+    """Rate-and-state friction, problem TPV101 or TPV102 (inputfile/tpv101.in,
+    tpv102.in; coupling 'rate-and-state_friction') on two n^3 Cartesian
+    blocks over the boxes of the TPV5 test ([-3, 0] / [0, 3] x [6, 9] x
+    [-1.5, 1.5] km), rho/cs/cp = 2.67/3.464/6, characteristic SAT outside and
+    a free surface on y-min, CFL 0.5.  Initial state: the steady state of the
+    prestress at Vin = 1e-12 m/s (rate_state_initial_state) and particle
+    velocities -+ Vin/2 in z (Init_Vel_State); the nucleation stress ramps in
+    over the first second."""
+    nn = (n, n, n)
+    XL = cartesian_grid(nn, (-3.0, 6.0, -1.5), (0.0, 9.0, 1.5))
+    XR = cartesian_grid(nn, (0.0, 6.0, -1.5), (3.0, 9.0, 1.5))
+    M = material_from_rho_s_p(nn, 2.67, 3.464, 6.0)
+    dt = block_time_step(3.0 / float(n - 1), 0.5, 3.464, 6.0)
+    if nsteps is None:
+        nsteps = int(math.floor(t_final / dt))
+    Vin = 1.0e-12
+    F0L = np.zeros((9,) + nn)
+    F0R = np.zeros((9,) + nn)
+    F0L[2] = -0.5 * Vin
+    F0R[2] = 0.5 * Vin
+    yf = np.ascontiguousarray(XL[1][:, :, -1])
+    zf = np.ascontiguousarray(XL[2][:, :, -1])
+    # block L's x-high face: n = (1, 0, 0); prestress = traction of
+    # (-120, 0, -120, 0, 75, 0) in its (n, m, l) frame
+    one = np.ones_like(yf)
+    l, m, nv = face_frames_from_gradient(one, 0.0 * one, 0.0 * one)
+    s = (-120.0, 0.0, -120.0, 0.0, 75.0, 0.0)
+    Tx = nv[0] * s[0] + nv[1] * s[3] + nv[2] * s[4]
+    Ty = nv[0] * s[3] + nv[1] * s[1] + nv[2] * s[5]
+    Tz = nv[0] * s[4] + nv[1] * s[5] + nv[2] * s[2]
+    T0 = np.stack([nv[0] * Tx + nv[1] * Ty + nv[2] * Tz, m[0] * Tx + m[1] * Ty + m[2] * Tz,
+                   l[0] * Tx + l[1] * Ty + l[2] * Tz])
+    W0 = rate_state_initial_state(yf, zf, T0, Vin)
+    coupling = L.WQL_IFACE_RATE_STATE_TPV101 if problem == 101 else L.WQL_IFACE_RATE_STATE_TPV102
+    left = Problem(n=nn, mesh=L.WQL_CARTESIAN, X=XL, M=M.copy(), bc=(1, 0, 2, 1, 1, 1), dt=dt,
+                   nsteps=nsteps, F0=F0L, name=f"tpv{problem}_block1")
+    right = Problem(n=nn, mesh=L.WQL_CARTESIAN, X=XR, M=M.copy(), bc=(0, 1, 2, 1, 1, 1), dt=dt,
+                    nsteps=nsteps, F0=F0R, name=f"tpv{problem}_block2")
+    tp = TwoBlockProblem(left=left, right=right, coupling=coupling, yf=yf, zf=zf, dt=dt,
+                         nsteps=nsteps, name=f"tpv{problem}_cartesian_{n}")
+    tp.W0 = W0
+    return tp
 
 
 def fault_two_block(n=(201, 401, 201), h=0.1, nmech=8, npml=20, cfl=0.5, nsteps=2000,
