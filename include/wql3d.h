@@ -315,12 +315,13 @@ int wql_iface_stage(wql_iface *I, double A, double Bdt, double t, int32_t stage1
  * recorded at stage 1 */
 int wql_iface_time_step(wql_iface *I, double dt, int32_t nsteps, double *t);
 /* device slots for nslots steps of fault output; each slot holds, per node
- * of the face ([nz][ny]), 38 planes: Uface (block L face fields, 9), Vface
+ * of the face ([nz][ny]), 39 planes: Uface (block L face fields, 9), Vface
  * (block R face fields, 9), S (4), Svel (3), Uhat (6), Vhat (6), rupture time (1),
- * in the order and meaning of the reference's .Uface/.Vface/.S/.Svel/.Uhat/.Vhat/
- * .trup files (fault_output.f90:73-123) */
+ * friction state psi (1; 0 for slip-weakening and locked coupling), in the order
+ * and meaning of the reference's .Uface/.Vface/.S/.Svel/.Uhat/.Vhat/.trup/.state
+ * files (fault_output.f90:73-123) */
 int wql_iface_set_frames(wql_iface *I, int32_t nslots);
-/* copy recorded slots 0..nslots-1 ([slot][38][nz][ny], host) */
+/* copy recorded slots 0..nslots-1 ([slot][39][nz][ny], host) */
 int wql_iface_get_frames(wql_iface *I, int32_t nslots, double *out);
 /* interface state S[4][nz][ny], DS[4][nz][ny], rupture time [nz][ny] (host) */
 int wql_iface_get_state(wql_iface *I, double *S, double *DS, double *trup);

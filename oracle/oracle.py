@@ -379,7 +379,7 @@ class OracleIface:
         lib().orc_iface_update(C.byref(self._sync()), inc)
 
 
-def two_block_time_step(L, R, I, dt, nsteps, t=0.0):
+def two_block_time_step(L, R, I, dt, nsteps, t=0.0, slices=False):
     """time_step_RK (time_step.f90:68-150) of a two-block domain: per stage
     scale_rates, set_rates with the moment tensor (domain.f90:951-979),
     boundary conditions (block L, then R) and the
@@ -387,10 +387,19 @@ def two_block_time_step(L, R, I, dt, nsteps, t=0.0):
     stage-1 output, update_fields (blocks, then interface).  Returns (t,
     frames) with the fault output of every step (write_fault_output /
     write_hat_output, domain.f90:782-847): Uface, Vface (block faces, 9),
-    S (4), Svel (3), Uhat, Vhat (6), trup, each (nsteps, ncomp, nz, ny)."""
+    S (4), Svel (3), Uhat, Vhat (6), trup, state (the friction state W), each
+    (nsteps, ncomp, nz, ny).  With slices, also block L's volume slices of
+    the same stage (write_slice, slice_output.f90:80-94): Hslice = F on the
+    plane r = 1, (nsteps, 9, nz, nx), and Vslice = F on s = 1,
+    (nsteps, 9, ny, nx)."""
     A, B, Cc = rk_coefficients()
-    keys = {"Uface": 9, "Vface": 9, "S": 4, "Svel": 3, "Uhat": 6, "Vhat": 6, "trup": 1}
+    keys = {"Uface": 9, "Vface": 9, "S": 4, "Svel": 3, "Uhat": 6, "Vhat": 6, "trup": 1,
+            "state": 1}
     frames = {k: np.zeros((nsteps, n, I.nz, I.ny)) for k, n in keys.items()}
+    if slices:
+        nz, ny, nx = L.F.shape[1:]
+        frames["Hslice"] = np.zeros((nsteps, 9, nz, nx))
+        frames["Vslice"] = np.zeros((nsteps, 9, ny, nx))
     for n in range(nsteps):
         t0 = t
         for s in range(5):
@@ -413,6 +422,10 @@ def two_block_time_step(L, R, I, dt, nsteps, t=0.0):
                 frames["Uhat"][n] = I.Uhat
                 frames["Vhat"][n] = I.Vhat
                 frames["trup"][n, 0] = I.trup
+                frames["state"][n, 0] = I.W
+                if slices:
+                    frames["Hslice"][n] = L.F[:, :, 0, :]
+                    frames["Vslice"][n] = L.F[:, 0, :, :]
             t = t0 + Cc[s + 1] * dt
             for b in (L, R):
                 b.update_fields(B[s] * dt)

@@ -80,4 +80,6 @@ def tpv5_truth(name="tpv5_cart_truth.npz"):
         out = {k: z[k].astype(np.float64) for k, _ in TPV5_KEYS}
         for label in ("21_11_11", "1_11_11"):
             out["station_" + label] = z["station_" + label].astype(np.float64)
+        for key in ("Hslice", "Vslice"):
+            out[key] = z[key].astype(np.float64)
         return out

@@ -69,13 +69,41 @@ def test_tpv5_fp64_matches_oracle(oracle):
     itf.close()
 
 
+def _steps_with_slices(bl, itf, dt, nsteps):
+    """Step one at a time, taking block L's volume slices (F on the planes
+    r = 1 and s = 1) at the start of every step, as write_slice does at stage
+    1 (slice_output.f90:80-94)."""
+    nz, ny, nx = bl.F.shape[1:]
+    H = np.zeros((nsteps, 9, nz, nx))
+    V = np.zeros((nsteps, 9, ny, nx))
+    t = 0.0
+    for n in range(nsteps):
+        bl.download(L.WQL_F)
+        H[n] = bl.F[:, :, 0, :]
+        V[n] = bl.F[:, 0, :, :]
+        t = itf.time_step(dt, 1, t)
+    return H, V
+
+
+def _check_golden_slices(H, V, truth, label):
+    for key, got in (("Hslice", H), ("Vslice", V)):
+        ref = truth[key]
+        d = float(np.abs(got - ref).max())
+        print(f"golden {label} {key}: max|diff| {d:.3e}", flush=True)
+        assert d <= 6e-8 * max(1.0, float(np.abs(ref).max())), (key, d)
+
+
 def test_tpv5_matches_golden_files(oracle):
+    """All seven fault outputs and block 1's volume slices (.Hslice /
+    .Vslice) of truth/1, stepping one step per call."""
     tp = PB.tpv5_cartesian(21, sat_form=L.WQL_SAT_ALPHA_THETA)
     bl, br, itf = _gpu_pair(oracle, tp)
     itf.set_frames(tp.nsteps)
-    itf.time_step(tp.dt, tp.nsteps, 0.0)
+    H, V = _steps_with_slices(bl, itf, tp.dt, tp.nsteps)
     fr = itf.frames()
     truth = tpv5_truth()
+    _check_golden_slices(H, V, truth, "tpv5")
+    assert not fr["state"].any()
     for key, _ in TPV5_KEYS:
         d = float(np.abs(fr[key] - truth[key]).max())
         scale = max(1.0, float(np.abs(truth[key][truth[key] < 1e8]).max()))
@@ -245,9 +273,10 @@ def test_tpv28_matches_golden_files(oracle):
     tp = PB.tpv28_curvilinear(21, sat_form=L.WQL_SAT_ALPHA_THETA, s0=TPV28_TRUTH_STRESS)
     bl, br, itf = _gpu_pair(oracle, tp)
     itf.set_frames(tp.nsteps)
-    itf.time_step(tp.dt, tp.nsteps, 0.0)
+    H, V = _steps_with_slices(bl, itf, tp.dt, tp.nsteps)
     fr = itf.frames()
     truth = tpv5_truth("tpv28_curv_truth.npz")
+    _check_golden_slices(H, V, truth, "tpv28")
     for key, _ in TPV5_KEYS:
         d = float(np.abs(fr[key] - truth[key]).max())
         scale = max(1.0, float(np.abs(truth[key][truth[key] < 1e8]).max()))
@@ -308,10 +337,13 @@ def test_rate_state_fp64_matches_oracle(oracle, problem):
     itf.time_step(tp.dt, tp.nsteps, 0.0)
     fr = itf.frames()
     W = itf.friction_state()
-    errs = {k: rel_l2(fr[k], fr_o[k]) for k in ("S", "Svel", "Uface", "Vface", "Uhat", "Vhat")}
+    errs = {k: rel_l2(fr[k], fr_o[k])
+            for k in ("S", "Svel", "Uface", "Vface", "Uhat", "Vhat", "state")}
     errs["W"] = rel_l2(W, Io.W)
     print({k: "%.2e" % e for k, e in errs.items()}, flush=True)
     assert max(errs.values()) <= 1e-9, errs
+    # the .state output: psi at the start of each step, steady at first
+    assert np.array_equal(fr["state"][0, 0], tp.W0.reshape(fr["state"].shape[2:]))
     ok = fr_o["trup"][-1, 0] < 1e8
     assert ok.sum() > 100
     dt_rup = np.abs(fr["trup"][-1, 0][ok] - fr_o["trup"][-1, 0][ok]).max()

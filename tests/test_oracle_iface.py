@@ -5,7 +5,7 @@ conditions fix.
 Golden files: test_problems/truth/1/test_rup_cart_fric_interface.* (stored
 as tests/golden/tpv5_cart_truth.npz), checked by python/read_binary.py:9-37
 with max |difference| <= 1e-3 over 46 frames of S, Svel, Uface, Vface, Uhat,
-Vhat.  The files predate two changes of the current source: they carry the
+Vhat; and the volume slices test_rup_cart_fric.{Hslice,Vslice}.  The files predate two changes of the current source: they carry the
 alpha/theta SAT form (commented out at BoundaryConditions.f90:833-841 and
 905-913) and one slip-rate increment per stage (each MPI rank owns its copy of
 the interface; on one rank both block evaluations of
@@ -20,12 +20,24 @@ from waveqlab3d_q_gpu_amd import _lib as L
 from waveqlab3d_q_gpu_amd import problems as PB
 
 
+def check_slices(fr, truth):
+    """Block 1's volume slices (the planes r = 1 and s = 1, all nine fields,
+    every frame) against the archived .Hslice / .Vslice to float32 rounding
+    (|values| <= 40; the nearest other planes differ by O(10))."""
+    for key in ("Hslice", "Vslice"):
+        ref = truth[key]
+        assert fr[key].shape == ref.shape, key
+        d = float(np.abs(fr[key] - ref).max())
+        assert d <= 6e-8 * max(1.0, float(np.abs(ref).max())), (key, d)
+
+
 def test_tpv5_cartesian_golden_files(oracle):
     tp = PB.tpv5_cartesian(21, sat_form=1)
     assert tp.nsteps == 46
     Lb, Rb, I = oracle_two_block(oracle, tp)
-    _, fr = oracle.two_block_time_step(Lb, Rb, I, tp.dt, tp.nsteps, 0.0)
+    _, fr = oracle.two_block_time_step(Lb, Rb, I, tp.dt, tp.nsteps, 0.0, slices=True)
     truth = tpv5_truth()
+    check_slices(fr, truth)
     for key, nc in TPV5_KEYS:
         mine, ref = fr[key], truth[key]
         assert mine.shape == ref.shape, key
@@ -142,8 +154,9 @@ def test_tpv28_curvilinear_golden_files(oracle):
     tp = PB.tpv28_curvilinear(21, sat_form=1, s0=TPV28_TRUTH_STRESS)
     assert tp.nsteps == 46
     Lb, Rb, I = oracle_two_block(oracle, tp)
-    _, fr = oracle.two_block_time_step(Lb, Rb, I, tp.dt, tp.nsteps, 0.0)
+    _, fr = oracle.two_block_time_step(Lb, Rb, I, tp.dt, tp.nsteps, 0.0, slices=True)
     truth = tpv5_truth("tpv28_curv_truth.npz")
+    check_slices(fr, truth)
     for key, _ in TPV5_KEYS:
         d = float(np.abs(fr[key] - truth[key]).max())
         assert d <= 1.0e-3, (key, d)
@@ -274,6 +287,12 @@ def test_tpv101_nucleates_and_ruptures(oracle):
     _, fr = oracle.two_block_time_step(Lo, Ro, Io, tp.dt, tp.nsteps, 0.0)
     sv = np.sqrt((fr["Svel"][:, 1:] ** 2).sum(axis=1)).max(axis=(1, 2))
     assert sv[0] < 1e-10 and sv[-1] > 1.0
+    # the .state output holds psi at the start of each step: the steady
+    # state first, then lower where the fault slips fast (state evolution
+    # towards f0 - b ln(V/V0))
+    st = fr["state"][:, 0]
+    assert np.array_equal(st[0], tp.W0.reshape(st.shape[1:]))
+    assert st[-1].min() < st[0].min() - 0.05
     tr = fr["trup"][-1, 0]
     ok = tr < 1e8
     assert ok.sum() > 100

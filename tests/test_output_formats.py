@@ -45,3 +45,35 @@ def test_fault_files_match_read_binary_layout(tmp_path):
     back = O.read_fault_file(str(tmp_path / "run_interface.Uface"), 9, ny, nz)
     assert np.allclose(back, frames["Uface"], rtol=1e-7, atol=0)
     assert not np.fromfile(tmp_path / "run_interface.state", dtype="<f4").any()
+
+
+def test_slice_files_reproduce_the_reference_files(tmp_path):
+    """The archived volume slices (test_problems/truth/1/test_rup_cart_fric.
+    {Hslice,Vslice}, stored as float32 in tests/golden/tpv5_cart_truth.npz)
+    written by write_slice_files are the reference's files byte for byte
+    (per step the Fortran array (nq, ns, 9) / (nq, nr, 9)), and read back
+    exactly."""
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                        "tpv5_cart_truth.npz")
+    with np.load(path, allow_pickle=False) as z:
+        H, V = z["Hslice"], z["Vslice"]
+    assert H.dtype == np.float32 and H.shape == (46, 9, 21, 21)
+    paths = O.write_slice_files(str(tmp_path), "run", H, V)
+    assert [p.rsplit("/", 1)[1] for p in paths] == ["run.Hslice", "run.Vslice"]
+    assert (tmp_path / "run.Hslice").read_bytes() == H.astype("<f4").tobytes()
+    assert os.path.getsize(tmp_path / "run.Vslice") == 730296   # the reference file's size
+    assert np.array_equal(O.read_slice_file(paths[1], 21, 21), V.astype(np.float64))
+    # element (q, s, c) of step n at offset q + nq*(s + ns*c) within the step
+    raw = np.fromfile(paths[0], dtype="<f4")
+    n, q, s_, c = 7, 20, 3, 5
+    assert raw[n * 9 * 441 + q + 21 * (s_ + 21 * c)] == H[n, c, s_, q]
+
+
+def test_fault_state_file_holds_the_state_frames(tmp_path):
+    ny, nz, ns = 4, 3, 2
+    rng = np.random.default_rng(3)
+    frames = {k: rng.normal(size=(ns, nc, nz, ny)) for k, nc in O.FAULT_FILES}
+    O.write_fault_files(str(tmp_path), "run", frames)
+    back = O.read_fault_file(str(tmp_path / "run_interface.state"), 1, ny, nz)
+    assert np.allclose(back, frames["state"], rtol=1e-7, atol=0)

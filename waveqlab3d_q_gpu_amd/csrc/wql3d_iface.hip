@@ -409,6 +409,7 @@ __global__ __launch_bounds__(256) void k_iface(const IfaceArgs<T, S> a) {
 #pragma unroll
       for (int c = 0; c < 15; ++c) f[(22 + c) * NF + fi] = out[c];
       f[37 * NF + fi] = trup;
+      f[38 * NF + fi] = psi;  // the state at the start of the step, as I%W at stage 1
     }
     a.trup[fi] = trup;
   }

@@ -112,6 +112,7 @@ struct StepArgs {
   // ([x0,x1,y0,y1,z0,z1] inclusive, 0-based) instead of the whole block;
   // box_start[b] = nodes of the boxes before b, box_start[nbox] = total
   int nbox;
+  int box_band;     // 0: box nodes x, y, z order; > 0: bands of box_band y-rows, z slower
   int box[6][6];
   long long box_start[7];
   // flat node range [node0, node1) of k_memory_update (0 and P: every node)
@@ -171,7 +172,7 @@ hipError_t launch_metrics(int nx, int ny, int nz, const double* X, double* mx, d
 // evaluates both sides (block L's side 2, then block R's side 1), writes the
 // SAT forcing of each block, updates the interface state in place and, at
 // stage 1 with a slot, the fault output.
-constexpr int kFramePlanes = 38;  // Uface 9, Vface 9, S 4, Svel 3, Uhat 6, Vhat 6, trup 1
+constexpr int kFramePlanes = 39;  // Uface 9, Vface 9, S 4, Svel 3, Uhat 6, Vhat 6, trup 1, state 1
 template <typename T, typename St = double>
 struct IfaceArgs {
   int ny, nz;

@@ -19,7 +19,7 @@ from .block import Block, _ptr
 # fault-output planes of one recorded step, in the order of the reference's
 # files (fault_output.f90:30-46): name -> number of components
 FRAME_KEYS = (("Uface", 9), ("Vface", 9), ("S", 4), ("Svel", 3), ("Uhat", 6), ("Vhat", 6),
-              ("trup", 1))
+              ("trup", 1), ("state", 1))
 FRAME_PLANES = sum(n for _, n in FRAME_KEYS)
 
 

@@ -7,6 +7,10 @@
   float32 record per step, the (ny, nz, ncomp) Fortran array, to
   ``<name>_interface.{Uface,Vface,S,Uhat,Vhat,Svel,trup,state}``
   (fault_output.f90:30-46); python/read_binary.py reads them back.
+* Volume slices: write_slice (slice_output.f90:80-94) appends block 1's nine
+  fields on the plane r = locationH to ``<name>.Hslice`` (the (nq, ns, 9)
+  Fortran array per step) and on s = locationV to ``<name>.Vslice`` ((nq, nr,
+  9)), float32.
 """
 from __future__ import annotations
 
@@ -68,8 +72,9 @@ def read_station_file(path: str) -> np.ndarray:
 
 def write_fault_files(directory: str, name: str, frames: dict) -> list:
     """Write the fault output of Interface.frames(): each key an array
-    (nsteps, ncomp, nz, ny); the state variable file holds zeros (no
-    rate-and-state coupling here).  Returns the paths."""
+    (nsteps, ncomp, nz, ny); a missing key (e.g. the state of a
+    slip-weakening fault, whose I%W the reference leaves at zero) is written
+    as zeros.  Returns the paths."""
     nsteps, _, nz, ny = frames["S"].shape
     paths = []
     for key, nc in FAULT_FILES:
@@ -89,3 +94,29 @@ def read_fault_file(path: str, ncomp: int, ny: int, nz: int) -> np.ndarray:
     """(nsteps, ncomp, nz, ny) float64 from one fault file."""
     a = np.fromfile(path, dtype="<f4")
     return a.reshape(-1, ncomp, nz, ny).astype(np.float64)
+
+
+def write_slice_files(directory: str, name: str, hslice=None, vslice=None) -> list:
+    """Write ``<name>.Hslice`` from ``hslice`` (nsteps, 9, nz, nx), block 1's
+    F on one r plane per step, and ``<name>.Vslice`` from ``vslice``
+    (nsteps, 9, ny, nx), F on one s plane; either may be None.  Returns the
+    paths."""
+    paths = []
+    for ext, a in (("Hslice", hslice), ("Vslice", vslice)):
+        if a is None:
+            continue
+        a = np.asarray(a)
+        if a.ndim != 4 or a.shape[1] != 9:
+            raise ValueError(f"{ext}: shape {a.shape}, expected (nsteps, 9, n2, nx)")
+        path = os.path.join(directory, f"{name}.{ext}")
+        # per step the Fortran array (nx, n2, 9) in column order == C (9, n2, nx)
+        np.ascontiguousarray(a, dtype="<f4").tofile(path)
+        paths.append(path)
+    return paths
+
+
+def read_slice_file(path: str, n2: int, nx: int) -> np.ndarray:
+    """(nsteps, 9, n2, nx) float64 from one .Hslice (n2 = nz) or .Vslice
+    (n2 = ny) file."""
+    a = np.fromfile(path, dtype="<f4")
+    return a.reshape(-1, 9, n2, nx).astype(np.float64)
