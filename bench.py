@@ -206,6 +206,149 @@ def rel_l2(a, b):
     return float(np.linalg.norm(((a - b) / s).ravel()) / np.linalg.norm((b / s).ravel()))
 
 
+ACCURACY_C3_FILE = "accuracy_config3_385_1000steps.json"
+ACCURACY_C5_FILE = "accuracy_config5_fault_2000steps.json"
+
+
+def _seismogram_summary(s_lo, s_hi):
+    """Errors of a receiver set: per receiver (with its peak relative to the
+    largest receiver peak), the worst over the receivers with peak >= 1e-2
+    and >= 1e-3 of the largest, and the relative L2 of all traces together."""
+    peaks = np.abs(s_hi).max(axis=(0, 2))
+    top = float(peaks.max())
+    per = {int(r): [float("%.3e" % rel_l2(s_lo[:, r], s_hi[:, r])),
+                    float("%.3e" % (peaks[r] / top))]
+           for r in range(s_hi.shape[1]) if peaks[r] > 0}
+
+    def worst(floor):
+        v = [e for e, p in per.values() if p >= floor]
+        return (float("%.3e" % max(v)) if v else None), len(v)
+    w2, n2 = worst(1e-2)
+    w3, n3 = worst(1e-3)
+    return {"seismogram_rel_l2_max_peak_ge_1e-2": w2, "receivers_peak_ge_1e-2": n2,
+            "seismogram_rel_l2_max_peak_ge_1e-3": w3, "receivers_peak_ge_1e-3": n3,
+            "seismogram_rel_l2_all_traces": float("%.3e" % rel_l2(s_lo, s_hi)),
+            "seismogram_rel_l2_and_relative_peak_per_receiver": per}
+
+
+def accuracy_config3(n=385, steps=1000, nmech=8, npml=20):
+    """The precision variant at length: the curvilinear Q8 + PML block run
+    from rest for ``steps`` steps in FP64, FP32-mixed and FP32-Kahan on this
+    GPU (the Ricker source starts at 0 and peaks at 0.6 s; 1000 steps are
+    3.0 s, in which P and S reach most of the 64 surface receivers); relative
+    L2 of every field at the end and of every receiver seismogram against
+    FP64."""
+    import torch
+    from waveqlab3d_q_gpu_amd import _lib as L
+    from waveqlab3d_q_gpu_amd import problems as PB
+    prob = PB.curvilinear_q(n=n, h=50.0, nsteps=steps, nmech=nmech, npml=npml, seed=2024, nrec=64)
+    res = {}
+    for prec, name in ((L.WQL_FP64, "fp64"), (L.WQL_FP32_MIXED, "fp32_mixed"),
+                       (L.WQL_FP32_KAHAN, "fp32_kahan")):
+        print(f"[accuracy c3] {name}: {n}^3, {steps} steps", file=sys.stderr, flush=True)
+        blk = PB.make_device_block(prob, precision=prec, state_mirror=True)
+        blk.set_receivers(prob.receivers, steps)
+        w0 = time.perf_counter()
+        blk.time_step(prob.dt, steps, 0.0)
+        blk.sync()
+        wall = time.perf_counter() - w0
+        blk.download(L.WQL_F)
+        res[name] = (blk.seismograms(), blk.F.cpu().numpy(), wall)
+        blk.close()
+        del blk
+        torch.cuda.empty_cache()
+    s64, F64, _ = res["fp64"]
+    out = {"workload": f"{n}^3 curvilinear block (seeded topography, layered + von Karman "
+                       f"medium), Q{nmech}, PML {npml} on 4 sides + bottom, free surface, "
+                       f"h = 50 m, Ricker f0 = 2 Hz from rest, {steps} steps (dt = "
+                       f"{prob.dt:.4e} s, {steps * prob.dt:.2f} s), 64 surface receivers",
+           "reference": "FP64 GPU path (parity with the FP64 oracle: <= 1e-12)",
+           "fp64_wall_s": round(res["fp64"][2], 2)}
+    for name in ("fp32_mixed", "fp32_kahan"):
+        s32, F32, wall = res[name]
+        out[name] = {
+            "field_rel_l2": {c: float("%.3e" % rel_l2(F32[k], F64[k]))
+                             for k, c in enumerate(("vx", "vy", "vz", "sxx", "syy", "szz",
+                                                    "sxy", "sxz", "syz"))},
+            **_seismogram_summary(s32, s64),
+            "wall_s": round(wall, 2)}
+    out["receiver_rule"] = ("per receiver: relative L2 of its 9 traces over the window; the "
+                            "worst is taken over the receivers whose FP64 peak is at least "
+                            "1e-2 (or 1e-3) of the largest receiver peak")
+    return out
+
+
+def accuracy_config5(steps=2000, every=10):
+    """The rupture variant at length: two curvilinear 201x401x201 blocks
+    (h = 100 m), Q8 (Qs 100, Qp 200), PML 20, TPV5 slip-weakening fault, in
+    FP64 and FP32-mixed (interface arithmetic FP64 in both) for ``steps``
+    steps: on-fault slip rate (the stage-1 fault output every ``every``
+    steps), rupture time at the end, and 32 off-fault surface receivers (16
+    per block, 2-8 km from the fault) against FP64."""
+    import torch
+    from waveqlab3d_q_gpu_amd import _lib as L
+    from waveqlab3d_q_gpu_amd import problems as PB
+    from waveqlab3d_q_gpu_amd.interface import Interface
+    tp = PB.fault_two_block(nsteps=steps)
+    res = {}
+    for prec, name in ((L.WQL_FP64, "fp64"), (L.WQL_FP32_MIXED, "fp32_mixed")):
+        print(f"[accuracy c5] {name}: {tp.name}, {steps} steps", file=sys.stderr, flush=True)
+        bl = PB.make_device_block(tp.left, precision=prec)
+        br = PB.make_device_block(tp.right, precision=prec)
+        bl.set_receivers(tp.left.receivers, steps)
+        br.set_receivers(tp.right.receivers, steps)
+        itf = Interface(bl, br, tp.yf, tp.zf, coupling=tp.coupling)
+        svel = []
+        t = 0.0
+        w0 = time.perf_counter()
+        for k in range(0, steps, every):
+            itf.set_frames(1)                 # record the first step of the chunk
+            t = itf.time_step(tp.dt, min(every, steps - k), t)
+            svel.append(itf.frames()["Svel"][0])
+        itf.sync()
+        wall = time.perf_counter() - w0
+        S, DS, trup = itf.state()
+        seis = np.concatenate([bl.seismograms(), br.seismograms()], axis=1)
+        res[name] = (np.array(svel), trup.copy(), S.copy(), seis, wall)
+        itf.close()
+        bl.close()
+        br.close()
+        del bl, br
+        torch.cuda.empty_cache()
+    v64, tr64, S64, s64, w64 = res["fp64"]
+    v32, tr32, S32, s32, w32 = res["fp32_mixed"]
+    rup64, rup32 = tr64 < 1e8, tr32 < 1e8
+    both = rup64 & rup32
+    vmag64 = np.sqrt((v64 ** 2).sum(axis=1))
+    vmag32 = np.sqrt((v32 ** 2).sum(axis=1))
+    return {
+        "workload": f"2 x 201x401x201 curvilinear blocks (seeded topography, planar fault x = "
+                    f"0), h = 100 m, Q8 (Qs 100, Qp 200), PML 20, TPV5 slip-weakening fault, "
+                    f"{steps} steps (dt = {tp.dt:.4e} s, {steps * tp.dt:.2f} s)",
+        "reference": "FP64 GPU path (parity with the FP64 oracle: <= 1e-12 at reduced size)",
+        "fault_nodes": int(tr64.size),
+        "ruptured_fp64": int(rup64.sum()), "ruptured_fp32_mixed": int(rup32.sum()),
+        "rupture_state_differs": int((rup64 != rup32).sum()),
+        "rupture_time_max_abs_diff_s": float("%.3e" % np.abs(tr32[both] - tr64[both]).max()),
+        "rupture_time_dt": tp.dt,
+        "slip_rate_rel_l2": float("%.3e" % rel_l2(v32, v64)),
+        "slip_rate_magnitude_rel_l2": float("%.3e" % rel_l2(vmag32, vmag64)),
+        "slip_rate_samples": f"stage-1 fault output every {every} steps ({v64.shape[0]} frames)",
+        "final_slip_rel_l2": float("%.3e" % rel_l2(S32[:3], S64[:3])),
+        "max_slip_m": float(np.abs(S64[:3]).max()),
+        **_seismogram_summary(s32, s64),
+        "wall_s": {"fp64": round(w64, 1), "fp32_mixed": round(w32, 1)}}
+
+
+def write_profile(name, data, command):
+    path = os.path.join(ROOT, "profiles", name)
+    data = dict(data)
+    data["command"] = command
+    with open(path, "w") as fh:
+        json.dump(data, fh, indent=1)
+    print(f"[bench] wrote {path}", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -227,6 +370,10 @@ def main():
     # smaller one on one thread
     ap.add_argument("--cpu-sample", type=int, default=161)
     ap.add_argument("--cpu-sample-single", type=int, default=97)
+    ap.add_argument("--long-accuracy", choices=("c3", "c5"), default=None,
+                    help="only run the precision variant (c3: 385^3, 1000 steps) or the "
+                         "rupture variant (c5: 2 x 201x401x201, 2000 steps) in FP32 against "
+                         "FP64 and write its summary under profiles/")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run a device copy of this many MiB 5 times (scale factor "
                          "for FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -265,6 +412,13 @@ def main():
 
     dev = ctypes.c_int32()
     L.check(L.lib().wql_runtime_init(0, ctypes.byref(dev)))
+    if args.long_accuracy:
+        cmd = f"python -u bench.py --long-accuracy {args.long_accuracy}"
+        if args.long_accuracy == "c3":
+            write_profile(ACCURACY_C3_FILE, accuracy_config3(), cmd)
+        else:
+            write_profile(ACCURACY_C5_FILE, accuracy_config5(), cmd)
+        return
     print(f"[bench] building {args.n}^3 curvilinear Q{args.nmech} PML{args.npml} set-up",
           file=sys.stderr, flush=True)
     prob = PB.curvilinear_q(n=args.n, h=50.0, nsteps=args.steps, nmech=args.nmech,
@@ -400,14 +554,35 @@ def main():
             out["fp32_kahan"]["speedup_vs_fp64"] = round(rk["value"] / results["fp64"]["value"], 3)
     if not args.no_two_block:
         out["two_block_fault"] = two_block_fault(args)
+    # the long-window accuracy runs (python bench.py --long-accuracy c3 / c5)
+    # take minutes each; their committed summaries are carried here
+    acc3 = read_traffic(ACCURACY_C3_FILE)
+    if acc3:
+        out["accuracy_config3_long"] = {
+            "file": "profiles/" + ACCURACY_C3_FILE, "workload": acc3.get("workload"),
+            **{k: {key: acc3[k].get(key) for key in (
+                "seismogram_rel_l2_max_peak_ge_1e-2", "receivers_peak_ge_1e-2",
+                "seismogram_rel_l2_max_peak_ge_1e-3", "receivers_peak_ge_1e-3",
+                "seismogram_rel_l2_all_traces")}
+               | {"field_rel_l2_max": max(acc3[k]["field_rel_l2"].values())}
+               for k in ("fp32_mixed", "fp32_kahan") if k in acc3}}
+    acc5 = read_traffic(ACCURACY_C5_FILE)
+    if acc5:
+        out["accuracy_config5_long"] = {
+            "file": "profiles/" + ACCURACY_C5_FILE,
+            **{k: acc5.get(k) for k in ("workload", "slip_rate_rel_l2",
+                                        "rupture_time_max_abs_diff_s", "rupture_state_differs",
+                                        "ruptured_fp64", "seismogram_rel_l2_max_peak_ge_1e-2",
+                                        "receivers_peak_ge_1e-2", "seismogram_rel_l2_all_traces")}}
     print(json.dumps(out), flush=True)
 
 
 def two_block_fault(args):
-    """The two-block fault workload (two 201 x 401 x 201 Cartesian blocks, Q8,
-    PML 20, TPV5 slip-weakening interface) in FP32-mixed: whole-domain
-    throughput with the interface kernel in every stage, and a check that
-    the rupture nucleated (slip > 0, finite fields)."""
+    """The two-block fault workload (two curvilinear 201 x 401 x 201 blocks
+    with topography and a planar fault, Q8, PML 20, TPV5 slip-weakening
+    interface) in FP32-mixed: whole-domain throughput with the interface
+    kernel in every stage, and a check that the rupture nucleated (slip > 0,
+    finite fields)."""
     import torch
     from waveqlab3d_q_gpu_amd import _lib as L
     from waveqlab3d_q_gpu_amd import problems as PB
@@ -431,8 +606,9 @@ def two_block_fault(args):
     ms = e0.elapsed_time(e1)
     S, _, _ = itf.state()
     nodes = 2 * int(np.prod(tp.left.n))
-    res = {"workload": "2 x 201x401x201 Cartesian blocks (h = 100 m), anelastic Q8, PML 20 on "
-                       "the outer faces, free surface, TPV5 slip-weakening fault between them; "
+    res = {"workload": "2 x 201x401x201 curvilinear blocks (h = 100 m, seeded topography, "
+                       "planar fault), anelastic Q8 (Qs 100, Qp 200), PML 20 on the outer "
+                       "faces, free surface, TPV5 slip-weakening fault between them; "
                        "FP32-mixed; one stage = interface kernel + both blocks' stages",
            "nodes": nodes, "steps": args.steps, "warmup": args.warmup,
            "value": round(nodes * 5 * args.steps / (ms / 1e3) / 1e6, 2),

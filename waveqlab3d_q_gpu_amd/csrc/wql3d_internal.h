@@ -112,7 +112,6 @@ struct StepArgs {
   // ([x0,x1,y0,y1,z0,z1] inclusive, 0-based) instead of the whole block;
   // box_start[b] = nodes of the boxes before b, box_start[nbox] = total
   int nbox;
-  int box_band;     // 0: box nodes x, y, z order; > 0: bands of box_band y-rows, z slower
   int box[6][6];
   long long box_start[7];
   // flat node range [node0, node1) of k_memory_update (0 and P: every node)
