@@ -114,8 +114,13 @@ def byte_model(prob, precision):
         "memory_update": (P * (12 * es + 4 * es + 24 * na * es + 6 * state + (6 * 4 if opcopy else 0))
                           if prob.nmech else 0),
     }
+    # the same count with every mechanism's eta and Deta streamed (the
+    # per-point figure quoted for the workload, 1116 B FP32-mixed Q8 before
+    # PML), whatever the mechanisms' weights
+    all_mech = (P * (9 * state + (ngeo + 3) * es + ((24 * prob.nmech + 2) * es if prob.nmech else 0))
+                + slab * 36 * es)
     return {"P": P, "interior_nodes": n_int, "active_mechanisms": na,
-            "end_to_end": end_to_end, "kernels": kernels}
+            "end_to_end": end_to_end, "end_to_end_all_mechanisms": all_mech, "kernels": kernels}
 
 
 def near_source_receivers(prob, offsets=((4, 0, 0), (-4, 0, 0), (0, 4, 0), (0, -4, 0),
@@ -362,6 +367,10 @@ def main():
     ap.add_argument("--no-kahan", action="store_true",
                     help="skip the FP32 state with Kahan-compensated RK update")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the discrete-adjoint cost measurement (129^3 Q4)")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="only run the discrete-adjoint cost measurement and print it")
     ap.add_argument("--no-two-block", action="store_true",
                     help="skip the two-block fault workload (2 x 201x401x201, Q8, PML 20)")
     ap.add_argument("--no-accuracy", action="store_true",
@@ -412,6 +421,9 @@ def main():
 
     dev = ctypes.c_int32()
     L.check(L.lib().wql_runtime_init(0, ctypes.byref(dev)))
+    if args.adjoint_only:
+        print(json.dumps({"adjoint_config4": adjoint_config4()}), flush=True)
+        return
     if args.long_accuracy:
         cmd = f"python -u bench.py --long-accuracy {args.long_accuracy}"
         if args.long_accuracy == "c3":
@@ -465,6 +477,7 @@ def main():
             "wall_ms_per_step": wall * 1e3 / args.steps,
             "stage_ms": stage_ms,
             "stage_bytes": bm["end_to_end"],
+            "stage_bytes_all_mechanisms": bm["end_to_end_all_mechanisms"],
             "stage_GBs": bm["end_to_end"] / (stage_ms / 1e3) / 1e9,
             "kernels": kernels,
             "byte_model": bm,
@@ -512,7 +525,12 @@ def main():
             "algorithmic_bytes_per_launch": k32[dom]["bytes"],
             "stage_end_to_end": {"achieved": round(r32["stage_GBs"], 1),
                                  "frac": round(r32["stage_GBs"] / HBM_PEAK_GBS, 4),
-                                 "bytes_per_stage": r32["stage_bytes"]}}
+                                 "bytes_per_stage": r32["stage_bytes"],
+                                 "bytes_per_node": round(r32["stage_bytes"] / P, 1),
+                                 "all_mechanisms_model": {
+                                     "bytes_per_node": round(r32["stage_bytes_all_mechanisms"] / P, 1),
+                                     "frac": round(r32["stage_bytes_all_mechanisms"]
+                                                   / (r32["stage_ms"] / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}}}
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_sample, args.cpu_sample_single, args.nmech, args.npml, 2024)
     out = {
         "metric": METRIC, "value": round(r32["value"], 2), "unit": "Mpt-RK-stage/s",
@@ -554,6 +572,8 @@ def main():
             out["fp32_kahan"]["speedup_vs_fp64"] = round(rk["value"] / results["fp64"]["value"], 3)
     if not args.no_two_block:
         out["two_block_fault"] = two_block_fault(args)
+    if not args.no_adjoint:
+        out["adjoint_config4"] = adjoint_config4()
     # the long-window accuracy runs (python bench.py --long-accuracy c3 / c5)
     # take minutes each; their committed summaries are carried here
     acc3 = read_traffic(ACCURACY_C3_FILE)
@@ -623,6 +643,97 @@ def two_block_fault(args):
     del bl, br
     torch.cuda.empty_cache()
     return res
+
+
+def adjoint_byte_model(prob, precision):
+    """Algorithmic HBM bytes of one reverse stage (wql_adjoint_stage) per
+    node, end to end: the stage input state read (F in the RK state's type,
+    eta), the state and rate adjoints read and written (F parts FP64, eta
+    parts in the storage type), geometry, material and Q^-1 read, the three
+    FP64 gradient planes read and written.  The 18 pointwise adjoint planes
+    between k_adj_pre and k_adj_stencil are a hand-off, not counted."""
+    es = 8 if precision == 0 else 4
+    ss = 4 if precision == 2 else 8
+    ne = 6 * active_mechanisms(prob) if prob.nmech else 0
+    ngeo = 10 if prob.mesh == 1 else 3
+    u = 9 * ss + ne * es
+    adj = 2 * (9 * 8 + ne * es)
+    return u + 2 * adj + (ngeo + 3 + (2 if prob.nmech else 0)) * es + 3 * 8 * 2
+
+
+def adjoint_config4(n=129, steps=40, every=7, reps=10):
+    """Cost of the discrete adjoint on the differentiation workload (129^3
+    curvilinear, Q4, 5 % von Karman, characteristic SAT), FP64 and
+    FP32-mixed: a forward step and a reverse step (five wql_adjoint_stage
+    calls, i.e. k_adj_grad + k_adj_pre + k_adj_stencil per stage) timed with
+    HIP events on the block's stream, the reverse stage against its
+    algorithmic bytes, and a whole gradient (forward with checkpoints every
+    ``every`` steps, segment and stage recomputation, reverse sweep; host
+    clock) over ``steps`` steps per step, against the forward step."""
+    import torch
+    from waveqlab3d_q_gpu_amd import _lib as L
+    from waveqlab3d_q_gpu_amd import problems as PB
+    from waveqlab3d_q_gpu_amd.adjoint import AdjointRunner, receiver_traces
+    from waveqlab3d_q_gpu_amd.block import RK_A, RK_B
+    prob = PB.differentiation_q4(n, nsteps=steps)
+    nodes = int(np.prod(prob.n))
+    out = {"workload": f"{n}^3 curvilinear block, h = 100 m, seeded topography, 5 % von "
+                       "Karman speeds and density, Q4, characteristic SAT, free surface, "
+                       f"Ricker 1.5 Hz, 64 surface receivers; gradient over {steps} steps, "
+                       f"checkpoints every {every}",
+           "nodes": nodes}
+    for prec, name in ((L.WQL_FP64, "fp64"), (L.WQL_FP32_MIXED, "fp32_mixed")):
+        print(f"[bench] adjoint {name}: {n}^3 Q4", file=sys.stderr, flush=True)
+        blk = PB.make_device_block(prob, precision=prec, state_mirror=True)
+        run = AdjointRunner(blk, prob.dt, steps, prob.receivers, every=every)
+        h = blk._h
+        st = torch.cuda.ExternalStream(blk.stream())
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        t = blk.time_step(prob.dt, 3, 0.3)
+        blk.sync()
+        e0.record(st)
+        blk.time_step(prob.dt, reps, t)
+        e1.record(st)
+        e1.synchronize()
+        fwd = e0.elapsed_time(e1) / reps
+        L.check(L.lib().wql_adjoint_reset(h))
+        for s in reversed(range(5)):
+            L.check(L.lib().wql_adjoint_stage(h, RK_A[s], RK_B[s] * prob.dt))
+        blk.sync()
+        e0.record(st)
+        for _ in range(reps):
+            for s in reversed(range(5)):
+                L.check(L.lib().wql_adjoint_stage(h, RK_A[s], RK_B[s] * prob.dt))
+        e1.record(st)
+        e1.synchronize()
+        rev = e0.elapsed_time(e1) / reps
+        M = blk.M
+        lam, mu, rho = (M[k].clone().requires_grad_(True) for k in range(3))
+        torch.cuda.synchronize()
+        w0 = time.perf_counter()
+        tr = receiver_traces(rho, lam, mu, run)
+        (0.5 * (tr[..., :3] ** 2).sum()).backward()
+        torch.cuda.synchronize()
+        grad = (time.perf_counter() - w0) * 1e3 / steps
+        fin = bool(torch.isfinite(rho.grad).all() and torch.isfinite(mu.grad).all())
+        bpn = adjoint_byte_model(prob, prec)
+        gbs = bpn * nodes / (rev / 5 / 1e3) / 1e9
+        out[name] = {"forward_ms_per_step": round(fwd, 4),
+                     "reverse_ms_per_step": round(rev, 4),
+                     "reverse_stage_bytes_per_node": bpn,
+                     "reverse_stage_GBs": round(gbs, 1),
+                     "reverse_stage_frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 4),
+                     "gradient_ms_per_step": round(grad, 3),
+                     "gradient_over_forward": round(grad / fwd, 2),
+                     "gradient_finite": fin}
+        print(f"[bench] adjoint {name}: forward {fwd:.3f} ms/step, reverse {rev:.3f} ms/step "
+              f"({gbs:.0f} GB/s), gradient {grad:.2f} ms/step", file=sys.stderr, flush=True)
+        del run, lam, mu, rho, tr
+        blk.close()
+        del blk
+        torch.cuda.empty_cache()
+    return out
 
 
 if __name__ == "__main__":

@@ -195,19 +195,6 @@ def test_checkpoint_reload_steps_bit_identically(precision):
     assert moved > 0 and all(same)
 
 
-def _config4_problem(n=129, h=100.0, nsteps=400, f0=1.5, seed=4):
-    """The differentiation workload: n^3 curvilinear block, h = 100 m,
-    seeded topography, 5 % von Karman perturbations of the wave speeds and an
-    independent 5 % one of rho (lambda and mu follow rho, so the speeds keep
-    theirs), Q4, characteristic SAT on the sides and bottom, free surface on
-    top, Ricker f0 = 1.5 Hz, 64 surface receivers."""
-    prob = PB.curvilinear_q(n=n, h=h, nsteps=nsteps, nmech=4, npml=0, seed=seed, nrec=64, f0=f0)
-    rho = prob.M[2] * (1.0 + PB.von_karman(prob.M.shape[1:], h, seed + 23))
-    ratio = rho / prob.M[2]
-    prob.M = np.stack([prob.M[0] * ratio, prob.M[1] * ratio, rho])
-    return prob
-
-
 def test_config4_adjoint_gradient_129():
     """The 129^3 Q4, 400-step workload with checkpoints every 20 steps: the
     loss is the L2 velocity misfit at 64 surface receivers against synthetic
@@ -223,7 +210,7 @@ def test_config4_adjoint_gradient_129():
 
     from waveqlab3d_q_gpu_amd.adjoint import AdjointRunner, receiver_traces
     n, h, steps, every, eps, seed = 129, 100.0, 400, 20, 1e-5, 4
-    prob = _config4_problem(n, h, steps, 1.5, seed)
+    prob = PB.differentiation_q4(n, h, steps, 1.5, seed)
     print(f"config 4: {n}^3 Q4, {steps} steps, checkpoints every {every}", flush=True)
 
     def runner_for(precision):

@@ -751,6 +751,20 @@ def curvilinear_q(n=257, h=50.0, nsteps=500, nmech=8, npml=20, seed=2024, cfl=0.
                    name=f"curvilinear_q{nmech}_{n}")
 
 
+def differentiation_q4(n=129, h=100.0, nsteps=400, f0=1.5, seed=4):
+    """The differentiation workload: n^3 curvilinear block, h = 100 m,
+    seeded topography, 5 % von Karman perturbations of the wave speeds and an
+    independent 5 % one of rho (lambda and mu follow rho, so the speeds keep
+    theirs), Q4, characteristic SAT on the sides and bottom, free surface on
+    top, Ricker f0 = 1.5 Hz, 64 surface receivers."""
+    prob = curvilinear_q(n=n, h=h, nsteps=nsteps, nmech=4, npml=0, seed=seed, nrec=64, f0=f0)
+    rho = prob.M[2] * (1.0 + von_karman(prob.M.shape[1:], h, seed + 23))
+    ratio = rho / prob.M[2]
+    prob.M = np.stack([prob.M[0] * ratio, prob.M[1] * ratio, rho])
+    prob.name = f"differentiation_q4_{n}"
+    return prob
+
+
 # ---------------------------------------------------------------------------
 def make_device_block(prob: Problem, precision=L.WQL_FP64, device="cuda", state_mirror=False):
     """Create a :class:`Block` whose set-up arrays are float64 torch tensors on
