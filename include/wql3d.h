@@ -347,6 +347,11 @@ int wql_adjoint_reset(wql_block *b);
  * into slot / back from slot (checkpoints) */
 int wql_state_save(wql_block *b, int32_t slot);
 int wql_state_load(wql_block *b, int32_t slot);
+/* the same for part of the state: mask WQL_F (F and, under FP32-Kahan, its
+ * compensation) and/or WQL_MEMORY (eta).  A reverse stage reads only F of its
+ * input state, so the stage inputs of a step need only F. */
+int wql_state_save_part(wql_block *b, int32_t slot, uint32_t mask);
+int wql_state_load_part(wql_block *b, int32_t slot, uint32_t mask);
 /* reverse one stage with RK coefficients A (a_s) and Bdt (b_s * dt) */
 int wql_adjoint_stage(wql_block *b, double A, double Bdt);
 /* ub(F) at the receiver nodes += g[r][0..8] (device array of nrec x 9): the

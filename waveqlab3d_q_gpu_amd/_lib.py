@@ -109,6 +109,8 @@ SIGNATURES = {
     "wql_adjoint_reset": (C.c_int, [_P]),
     "wql_state_save": (C.c_int, [_P, C.c_int32]),
     "wql_state_load": (C.c_int, [_P, C.c_int32]),
+    "wql_state_save_part": (C.c_int, [_P, C.c_int32, C.c_uint32]),
+    "wql_state_load_part": (C.c_int, [_P, C.c_int32, C.c_uint32]),
     "wql_adjoint_stage": (C.c_int, [_P, C.c_double, C.c_double]),
     "wql_adjoint_add_receivers": (C.c_int, [_P, _P]),
     "wql_adjoint_get_state": (C.c_int, [_P, _P, _P]),

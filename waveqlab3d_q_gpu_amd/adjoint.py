@@ -79,13 +79,15 @@ class AdjointRunner:
 
     # -- reverse sweep -------------------------------------------------------
     def _stage_adjoint(self, n):
-        """Reverse the five stages of step n (its start state is current)."""
+        """Reverse the five stages of step n (its start state is current).
+        A reverse stage reads F of its input state only, so the stage inputs
+        are kept without eta."""
         h, dt, t = self.blk._h, self.dt, self.ts[n]
         for s in range(5):
-            check(lib().wql_state_save(h, self.s_stage + s))
+            check(lib().wql_state_save_part(h, self.s_stage + s, L.WQL_F))
             check(lib().wql_stage_fused(h, RK_A[s], RK_B[s] * dt, t + RK_C[s] * dt))
         for s in reversed(range(5)):
-            check(lib().wql_state_load(h, self.s_stage + s))
+            check(lib().wql_state_load_part(h, self.s_stage + s, L.WQL_F))
             check(lib().wql_adjoint_stage(h, RK_A[s], RK_B[s] * dt))
 
     def backward(self, g_traces):

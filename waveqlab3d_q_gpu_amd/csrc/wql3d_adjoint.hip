@@ -11,12 +11,17 @@
 //   ub += K^T rb
 //   rb  = A_s rb
 // K acts pointwise except for the derivatives, and every material term of L
-// at a node uses only that node's material, so dL/dm is local: k_adj_grad
-// evaluates the forward derivatives at each node and contracts the local
-// material Jacobian with rb.  K^T = (pointwise)^T D^T (pointwise)^T:
-// k_adj_pre forms the pointwise adjoint fields Z (and the purely local parts
-// of K^T: memory-variable coupling and the SAT penalties), k_adj_stencil
-// applies the transposed SBP operators D_d^T to Z.
+// at a node uses only that node's material, so dL/dm is local.  A reverse
+// stage is four kernels: k_adj_grad evaluates the forward derivatives of the
+// stage input at each node and contracts the local material Jacobian with
+// rb + B dt ub (formed on the fly); k_adj_pre stores rb + B dt ub, forms the
+// pointwise adjoint fields Z of K^T = (pointwise)^T D^T (pointwise)^T and the
+// memory-variable coupling, and leaves A rb in the rate adjoints; k_adj_sat
+// adds the SAT penalties' transpose and material terms on the boundary
+// faces (their dual-number arithmetic would cost the per-node kernels half
+// their waves); k_adj_stencil applies the transposed SBP operators D_d^T to
+// Z.  The two streaming updates of rb thus ride on the node kernels' own
+// reads and writes of rb instead of four passes of their own.
 //
 // Flux form of the forward operator (as in the interior kernel):
 //   rho J dv_c/dt = sum_d D_d(fl_d[c]) / h_d,  fl_d[c] = J sum_m X_m(d) sigma(m,c)
@@ -131,11 +136,25 @@ __device__ __forceinline__ void sat_rate(const StepArgs<T, St>& a, int f, long l
   for (int c = 0; c < 9; ++c) out[c] = S(0.0) - pen * Ux[c];
 }
 
+// rb + Bdt ub in the order of k_update (the rk_vector update of the
+// reference order path), so the fused reverse stage equals the one built
+// from separate update / scale launches bit for bit
+__device__ __forceinline__ double adj_inc(double r, double inc, double u) {
+  return __dadd_rn(r, __dmul_rn(inc, u));
+}
+__device__ __forceinline__ float adj_inc(float r, double inc, float u) {
+  return __fadd_rn(r, __fmul_rn((float)inc, u));
+}
+
 // ---------------------------------------------------------------------------
-// gradient: g += (dL/dm)^T rb at the stage input state (a.Fin, a.eta)
+// gradient, interior part: g += (dL/dm)^T rb at the stage input state (its F,
+// a.Fin) for the terms that need the state's derivatives (velocity rates,
+// Hooke, memory variables); rb = DFb + Bdt Fb and rb_eta = DEb + Bdt Eb are
+// formed here without being stored (k_adj_pre stores them)
 // ---------------------------------------------------------------------------
 template <typename T, typename St, int MESH, int NMECH>
-__global__ __launch_bounds__(256) void k_adj_grad(const StepArgs<T, St> a, const AdjArgs<T> j) {
+__global__ __launch_bounds__(256) void k_adj_grad(const StepArgs<T, St> a, const AdjArgs<T> j,
+                                                  double Bdt) {
   int x, y, z;
   const long long id = (long long)blockIdx.x * 256 + threadIdx.x;
   if (!flat_node(a, id, x, y, z)) return;
@@ -144,11 +163,11 @@ __global__ __launch_bounds__(256) void k_adj_grad(const StepArgs<T, St> a, const
   T U[3][9];
   if (MESH) derivs_curv<T>(a, pos, id, U);
   else derivs_cart<T>(a, pos, id, U);
-  const double lam = a.M[id], mu = a.M[P + id], rho = a.M[2 * P + id];
+  const double rho = a.M[2 * P + id];
   const double Jp = MESH ? (double)a.G[9 * P + id] : 1.0;
   double rb[9];
 #pragma unroll
-  for (int c = 0; c < 9; ++c) rb[c] = j.DFb[c * P + id];
+  for (int c = 0; c < 9; ++c) rb[c] = adj_inc(j.DFb[c * P + id], Bdt, j.Fb[c * P + id]);
   double gr[3][3];
 #pragma unroll
   for (int m = 0; m < 3; ++m)
@@ -176,7 +195,8 @@ __global__ __launch_bounds__(256) void k_adj_grad(const StepArgs<T, St> a, const
       if (wt == 0.0) continue;
 #pragma unroll
       for (int c = 0; c < 6; ++c) {
-        const double re = (double)j.DEb[(long long)(c * NMECH + i) * P + id];
+        const long long pi = (long long)(c * NMECH + i) * P + id;
+        const double re = (double)adj_inc(j.DEb[pi], Bdt, j.Eb[pi]);
         if (c < 3) {
           g_lam += re * wt * Qp * tr;
           g_mu += re * wt * (2.0 * Qs * e[c] + (2.0 * Qp - 2.0 * Qs) * tr);
@@ -186,51 +206,53 @@ __global__ __launch_bounds__(256) void k_adj_grad(const StepArgs<T, St> a, const
       }
     }
   }
-  // SAT penalties at face nodes, differentiated with dual numbers
-#pragma unroll
-  for (int f = 0; f < 6; ++f) {
-    if (a.bc[f] <= 0) continue;
-    long long fidx, NF;
-    if (!on_face(a, f, x, y, z, &fidx, &NF)) continue;
-    D3 u[9], out[9];
-#pragma unroll
-    for (int c = 0; c < 9; ++c) u[c] = D3((double)(T)a.Fin[c * P + id]);
-    D3 dl(lam), dm(mu), dr(rho);
-    dr.d[0] = 1.0;
-    dl.d[1] = 1.0;
-    dm.d[2] = 1.0;
-    sat_rate<D3, T>(a, f, fidx, NF, u, dl, dm, dr, out);
-#pragma unroll
-    for (int c = 0; c < 9; ++c) {
-      g_rho += rb[c] * out[c].d[0];
-      g_lam += rb[c] * out[c].d[1];
-      g_mu += rb[c] * out[c].d[2];
-    }
-  }
   j.g[0][id] += g_rho;
   j.g[1][id] += g_lam;
   j.g[2][id] += g_mu;
 }
 
+// true when the node lies on a face with a boundary condition (SAT penalty)
+template <typename T, typename St>
+__device__ __forceinline__ bool on_sat_face(const StepArgs<T, St>& a, int x, int y, int z) {
+#pragma unroll
+  for (int f = 0; f < 6; ++f) {
+    long long fidx, NF;
+    if (a.bc[f] > 0 && on_face(a, f, x, y, z, &fidx, &NF)) return true;
+  }
+  return false;
+}
+
 // ---------------------------------------------------------------------------
-// pointwise part of K^T
-//   Z[0..8]  : curvilinear q_c = rb_v[c]/(rho J) in Z[c]; Cartesian
-//              q_{d,c} = G_d rb_v[c]/rho in Z[3d+c]
-//   Z[9..17] : h_{d,c} = sum_m X_m(d) gb[m][c] (Cartesian G_d gb[d][c]) with gb
-//              the adjoint of the velocity gradient (Hooke + memory variables)
-//   etab += -rb_sigma - rb_eta/tau ;  Fb += S^T rb_F at face nodes (SAT)
+// The node's own part of the reverse stage (no neighbour is read):
+//   rb  += Bdt ub                       (reverse of u <- u + Bdt r)
+//   pointwise part of K^T:
+//     Z[0..8]  : curvilinear q_c = rb_v[c]/(rho J) in Z[c]; Cartesian
+//                q_{d,c} = G_d rb_v[c]/rho in Z[3d+c]
+//     Z[9..17] : h_{d,c} = sum_m X_m(d) gb[m][c] (Cartesian G_d gb[d][c]) with
+//                gb the adjoint of the velocity gradient (Hooke + memory
+//                variables)
+//     etab += -rb_sigma - rb_eta/tau
+//   rb   = A rb                         (reverse of r <- A r + L)
+// At nodes of SAT faces rb is left unscaled for k_adj_sat, which adds the
+// penalties' terms and scales it.
 // ---------------------------------------------------------------------------
 template <typename T, typename St, int MESH, int NMECH>
-__global__ __launch_bounds__(256) void k_adj_pre(const StepArgs<T, St> a, const AdjArgs<T> j) {
+__global__ __launch_bounds__(256) void k_adj_pre(const StepArgs<T, St> a, const AdjArgs<T> j,
+                                                 double A, double Bdt) {
   int x, y, z;
   const long long id = (long long)blockIdx.x * 256 + threadIdx.x;
   if (!flat_node(a, id, x, y, z)) return;
   const long long P = a.P;
   const T lam = a.M[id], mu = a.M[P + id], rho = a.M[2 * P + id];
   const T l2m = lam + T(2) * mu;
+  const double As = on_sat_face(a, x, y, z) ? 1.0 : A;
   T rb[9];
 #pragma unroll
-  for (int c = 0; c < 9; ++c) rb[c] = (T)j.DFb[c * P + id];
+  for (int c = 0; c < 9; ++c) {
+    const double r = adj_inc(j.DFb[c * P + id], Bdt, j.Fb[c * P + id]);
+    j.DFb[c * P + id] = scale_rate(As, r);
+    rb[c] = (T)r;
+  }
   // adjoint of the velocity gradient gb[m][c] = d/d(dv_c/dx_m)
   T gb[3][3];
   gb[0][0] = l2m * rb[3] + lam * (rb[4] + rb[5]);
@@ -248,10 +270,12 @@ __global__ __launch_bounds__(256) void k_adj_pre(const StepArgs<T, St> a, const 
 #pragma unroll
       for (int i = 0; i < NMECH; ++i) {
         const long long pi = (long long)(c * NMECH + i) * P + id;
-        const T re = j.DEb[pi];
+        const T eu = j.Eb[pi];
+        const T re = adj_inc(j.DEb[pi], Bdt, eu);
         const T ti = (T)a.tau_inv[i], w = (T)a.wgt[i];
         // eta_{c,i} enters DU_sigma[c] with -1 and its own rate with -1/tau
-        j.Eb[pi] = j.Eb[pi] - rb[3 + c] - re * ti;
+        j.Eb[pi] = eu - rb[3 + c] - re * ti;
+        j.DEb[pi] = scale_rate(A, re);
         if (c < 3) {
           eb[c] += re * ti * w * twomuQs;
           trb += re * ti * w * pfac;
@@ -292,33 +316,109 @@ __global__ __launch_bounds__(256) void k_adj_pre(const StepArgs<T, St> a, const 
       }
     }
   }
-  // SAT: the penalty rate is linear in the node's own state, S u; its
-  // transpose is assembled column by column (S e_k)
-  bool any = false;
+}
+
+// node t of the block's boundary layer, every node once: the two x faces
+// whole, the y faces without their x edges, the z faces without x and y edges
+template <typename T, typename St>
+__device__ __forceinline__ bool boundary_node(const StepArgs<T, St>& a, long long t, int& x, int& y,
+                                              int& z) {
+  const long long nx = a.nx, ny = a.ny, nz = a.nz;
+  const long long nX = ny * nz, nY = (nx - 2) * nz, nZ = (nx - 2) * (ny - 2);
+  if (t < 2 * nX) {
+    const long long k = t >= nX ? t - nX : t;
+    x = t >= nX ? (int)nx - 1 : 0;
+    y = (int)(k % ny);
+    z = (int)(k / ny);
+    return true;
+  }
+  t -= 2 * nX;
+  if (t < 2 * nY) {
+    const long long k = t >= nY ? t - nY : t;
+    y = t >= nY ? (int)ny - 1 : 0;
+    x = 1 + (int)(k % (nx - 2));
+    z = (int)(k / (nx - 2));
+    return true;
+  }
+  t -= 2 * nY;
+  if (t < 2 * nZ) {
+    const long long k = t >= nZ ? t - nZ : t;
+    z = t >= nZ ? (int)nz - 1 : 0;
+    x = 1 + (int)(k % (nx - 2));
+    y = 1 + (int)(k / (nx - 2));
+    return true;
+  }
+  return false;
+}
+
+// ---------------------------------------------------------------------------
+// SAT penalties of the nodes on boundary-condition faces (after k_adj_pre,
+// which left rb = DFb + Bdt ub unscaled there):
+//   g  += (dL/dm)^T rb of the penalties at the stage input state (dual numbers)
+//   Fb += S^T rb (the penalty is linear in the node's own state, S u; its
+//         transpose is assembled column by column, S e_k)
+//   rb  = A rb
+// One thread per boundary node, so a node on several faces sums them all.
+// ---------------------------------------------------------------------------
+template <typename T, typename St>
+__global__ __launch_bounds__(256) void k_adj_sat(const StepArgs<T, St> a, const AdjArgs<T> j,
+                                                 double A) {
+  int x, y, z;
+  const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (!boundary_node(a, t, x, y, z)) return;
+  if (!on_sat_face(a, x, y, z)) return;
+  const long long P = a.P;
+  const long long id = x + (long long)a.nx * (y + (long long)a.ny * z);
+  const double lam = a.M[id], mu = a.M[P + id], rho = a.M[2 * P + id];
+  const T lamT = (T)lam, muT = (T)mu, rhoT = (T)rho;
+  double rb[9];
+  T rbT[9];
+#pragma unroll
+  for (int c = 0; c < 9; ++c) {
+    rb[c] = j.DFb[c * P + id];
+    rbT[c] = (T)rb[c];
+  }
   T sb[9];
 #pragma unroll
   for (int k = 0; k < 9; ++k) sb[k] = T(0);
-#pragma unroll
+  double g_rho = 0.0, g_lam = 0.0, g_mu = 0.0;
   for (int f = 0; f < 6; ++f) {
     if (a.bc[f] <= 0) continue;
     long long fidx, NF;
     if (!on_face(a, f, x, y, z, &fidx, &NF)) continue;
-    any = true;
     for (int k = 0; k < 9; ++k) {
       T u[9], out[9];
 #pragma unroll
       for (int c = 0; c < 9; ++c) u[c] = c == k ? T(1) : T(0);
-      sat_rate<T, T>(a, f, fidx, NF, u, lam, mu, rho, out);
+      sat_rate<T, T>(a, f, fidx, NF, u, lamT, muT, rhoT, out);
       T s = T(0);
 #pragma unroll
-      for (int c = 0; c < 9; ++c) s += out[c] * rb[c];
+      for (int c = 0; c < 9; ++c) s += out[c] * rbT[c];
       sb[k] += s;
     }
-  }
-  if (any) {
+    D3 u[9], out[9];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) j.Fb[k * P + id] += (double)sb[k];
+    for (int c = 0; c < 9; ++c) u[c] = D3((double)(T)a.Fin[c * P + id]);
+    D3 dl(lam), dm(mu), dr(rho);
+    dr.d[0] = 1.0;
+    dl.d[1] = 1.0;
+    dm.d[2] = 1.0;
+    sat_rate<D3, T>(a, f, fidx, NF, u, dl, dm, dr, out);
+#pragma unroll
+    for (int c = 0; c < 9; ++c) {
+      g_rho += rb[c] * out[c].d[0];
+      g_lam += rb[c] * out[c].d[1];
+      g_mu += rb[c] * out[c].d[2];
+    }
   }
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    j.Fb[k * P + id] += (double)sb[k];
+    j.DFb[k * P + id] = scale_rate(A, rb[k]);
+  }
+  j.g[0][id] += g_rho;
+  j.g[1][id] += g_lam;
+  j.g[2][id] += g_mu;
 }
 
 // ---------------------------------------------------------------------------
@@ -390,28 +490,23 @@ template <typename T, typename St>
 hipError_t launch_adjoint_stage(const StepArgs<T, St>& a, const AdjArgs<T>& j, int mesh, double A,
                                 double Bdt, hipStream_t s) {
   const long long nodes = (long long)a.nx * a.ny * a.nz;
-  const long long ne = 6LL * a.nmech * a.P;
-  hipError_t e = launch_update<double>(j.DFb, j.Fb, 9 * a.P, Bdt, s);
-  if (e != hipSuccess) return e;
-  if (ne) {
-    e = launch_update<T>(j.DEb, j.Eb, ne, Bdt, s);
-    if (e != hipSuccess) return e;
-  }
   const dim3 g((unsigned)((nodes + 255) / 256)), b(256);
+  // boundary layer: the x faces whole, y faces without x edges, z faces
+  // without x and y edges (block sizes >= 2 per axis)
+  const long long nb = 2LL * a.ny * a.nz + 2LL * (a.nx - 2) * a.nz + 2LL * (a.nx - 2) * (a.ny - 2);
+  const dim3 gb((unsigned)((nb + 255) / 256));
+  hipError_t e = hipSuccess;
 #define WQL_CASE(MS, NM)                                                       \
   if (mesh == MS && a.nmech == NM) {                                           \
-    hipLaunchKernelGGL((k_adj_grad<T, St, MS, NM>), g, b, 0, s, a, j);         \
-    hipLaunchKernelGGL((k_adj_pre<T, St, MS, NM>), g, b, 0, s, a, j);          \
+    hipLaunchKernelGGL((k_adj_grad<T, St, MS, NM>), g, b, 0, s, a, j, Bdt);    \
+    hipLaunchKernelGGL((k_adj_pre<T, St, MS, NM>), g, b, 0, s, a, j, A, Bdt);  \
+    hipLaunchKernelGGL((k_adj_sat<T, St>), gb, b, 0, s, a, j, A);              \
     hipLaunchKernelGGL((k_adj_stencil<T, St, MS>), g, b, 0, s, a, j);          \
     e = hipGetLastError();                                                     \
-    if (e != hipSuccess) return e;                                             \
   } else
   WQL_CASE(0, 0) WQL_CASE(0, 4) WQL_CASE(0, 8) WQL_CASE(1, 0) WQL_CASE(1, 4) WQL_CASE(1, 8)
   return hipErrorInvalidValue;
 #undef WQL_CASE
-  e = launch_scale<double>(j.DFb, 9 * a.P, A, s);
-  if (e != hipSuccess) return e;
-  if (ne) e = launch_scale<T>(j.DEb, ne, A, s);
   return e;
 }
 template hipError_t launch_adjoint_stage<double, double>(const StepArgs<double, double>&,

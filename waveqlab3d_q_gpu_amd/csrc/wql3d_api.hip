@@ -2059,39 +2059,57 @@ static int check_slot(wql_block* b, int slot) {
   return WQL_OK;
 }
 
-extern "C" int wql_state_save(wql_block* b, int32_t slot) {
+extern "C" int wql_state_save_part(wql_block* b, int32_t slot, uint32_t mask) {
   int rc = check_slot(b, slot);
   if (rc) return rc;
+  if (mask & ~(uint32_t)(WQL_F | WQL_MEMORY)) return fail(WQL_EINVAL, "mask: WQL_F and/or WQL_MEMORY");
   const long long P = b->P, ne = 6LL * b->d.nmech * P;
   char* sf = (char*)b->slot_F + (size_t)slot * slot_F_bytes(b);
-  HIP_TRY(hipMemcpyAsync(sf, b->F[b->cur], 9 * P * b->ssz, hipMemcpyDeviceToDevice, b->stream));
-  if (b->d.precision == WQL_FP32_KAHAN)
-    HIP_TRY(hipMemcpyAsync(sf + 9 * P * b->ssz, b->Fc, 9 * P * sizeof(float),
-                           hipMemcpyDeviceToDevice, b->stream));
-  if (ne)
-    HIP_TRY(hipMemcpyAsync((char*)b->slot_E + (size_t)slot * ne * b->esz, b->eta, ne * b->esz,
-                           hipMemcpyDeviceToDevice, b->stream));
-  b->slot_mz[slot] = b->mech_zero;
+  if (mask & WQL_F) {
+    HIP_TRY(hipMemcpyAsync(sf, b->F[b->cur], 9 * P * b->ssz, hipMemcpyDeviceToDevice, b->stream));
+    if (b->d.precision == WQL_FP32_KAHAN)
+      HIP_TRY(hipMemcpyAsync(sf + 9 * P * b->ssz, b->Fc, 9 * P * sizeof(float),
+                             hipMemcpyDeviceToDevice, b->stream));
+  }
+  if (mask & WQL_MEMORY) {
+    if (ne)
+      HIP_TRY(hipMemcpyAsync((char*)b->slot_E + (size_t)slot * ne * b->esz, b->eta, ne * b->esz,
+                             hipMemcpyDeviceToDevice, b->stream));
+    b->slot_mz[slot] = b->mech_zero;
+  }
   return WQL_OK;
 }
 
-extern "C" int wql_state_load(wql_block* b, int32_t slot) {
+extern "C" int wql_state_load_part(wql_block* b, int32_t slot, uint32_t mask) {
   int rc = check_slot(b, slot);
   if (rc) return rc;
+  if (mask & ~(uint32_t)(WQL_F | WQL_MEMORY)) return fail(WQL_EINVAL, "mask: WQL_F and/or WQL_MEMORY");
   const long long P = b->P, ne = 6LL * b->d.nmech * P;
-  b->fo_valid = false;
   const char* sf = (const char*)b->slot_F + (size_t)slot * slot_F_bytes(b);
-  HIP_TRY(hipMemcpyAsync(b->F[b->cur], sf, 9 * P * b->ssz, hipMemcpyDeviceToDevice, b->stream));
-  if (b->d.precision == WQL_FP32_KAHAN)
-    HIP_TRY(hipMemcpyAsync(b->Fc, sf + 9 * P * b->ssz, 9 * P * sizeof(float),
-                           hipMemcpyDeviceToDevice, b->stream));
-  if (ne)
-    HIP_TRY(hipMemcpyAsync(b->eta, (char*)b->slot_E + (size_t)slot * ne * b->esz, ne * b->esz,
-                           hipMemcpyDeviceToDevice, b->stream));
-  // the slot's zero mechanisms; Deta of those is zero as well (it only ever
-  // holds A*0 + 0)
-  b->mech_zero = b->slot_mz[slot];
+  if (mask & WQL_F) {
+    b->fo_valid = false;
+    HIP_TRY(hipMemcpyAsync(b->F[b->cur], sf, 9 * P * b->ssz, hipMemcpyDeviceToDevice, b->stream));
+    if (b->d.precision == WQL_FP32_KAHAN)
+      HIP_TRY(hipMemcpyAsync(b->Fc, sf + 9 * P * b->ssz, 9 * P * sizeof(float),
+                             hipMemcpyDeviceToDevice, b->stream));
+  }
+  if (mask & WQL_MEMORY) {
+    if (ne)
+      HIP_TRY(hipMemcpyAsync(b->eta, (char*)b->slot_E + (size_t)slot * ne * b->esz, ne * b->esz,
+                             hipMemcpyDeviceToDevice, b->stream));
+    // the slot's zero mechanisms; Deta of those is zero as well (it only ever
+    // holds A*0 + 0)
+    b->mech_zero = b->slot_mz[slot];
+  }
   return WQL_OK;
+}
+
+extern "C" int wql_state_save(wql_block* b, int32_t slot) {
+  return wql_state_save_part(b, slot, WQL_F | WQL_MEMORY);
+}
+
+extern "C" int wql_state_load(wql_block* b, int32_t slot) {
+  return wql_state_load_part(b, slot, WQL_F | WQL_MEMORY);
 }
 
 template <typename T, typename St = double>
