@@ -74,11 +74,12 @@ def byte_model(prob, precision):
 
     Per kernel (what each kernel of the stage must move, its W hand-off
     included): interior stencil = F read (9), geometry, material, the
-    velocity components' F write and DF read and write (3), W write (12: six
-    stress rates, six strain rates); shell = the same per node plus the PML
-    streams; memory-variable kernel = W read (12), lambda, mu, Q^-1 (4),
-    eta/Deta read and write, and the six stress components' F read and write
-    and DF read and write.  FP32-mixed: the stencils read F rounded to float
+    velocity components' F write and DF read and write (3), W write (6
+    strain rates); shell = the same per node with W write 12 (six stress
+    rates as well) plus the PML streams; memory-variable kernel = W read (6,
+    and 6 more at shell nodes), lambda, mu, Q^-1 (4), eta/Deta read and
+    write, and the six stress components' F read and write and DF read and
+    write.  FP32-mixed: the stencils read F rounded to float
     (9 x 4 B) plus the velocity components' FP64 F (3 x 8 B), and every F
     write also writes the float copy (4 B per component)."""
     nq, nr, ns = prob.n
@@ -107,12 +108,16 @@ def byte_model(prob, precision):
         operands = 9 * 4 + nupd * (8 + 4)
     else:
         operands = 9 * ss
-    stencil_node = operands + (ngeo + 3) * es + (12 * es + vel if prob.nmech else 9 * (state - ss))
+    # W hand-off: 12 planes (6 stress rates, 6 strain rates) from the shell;
+    # the strict interior hands over the strain rates only (its stress rates
+    # are Hooke's law of them, formed again by the memory-variable kernel)
+    shell_node = operands + (ngeo + 3) * es + (12 * es + vel if prob.nmech else 9 * (state - ss))
+    interior_node = operands + (ngeo + 3) * es + (6 * es + vel if prob.nmech else 9 * (state - ss))
     kernels = {
-        "interior": n_int * stencil_node,
-        "shell": (P - n_int) * stencil_node + slab * 36 * es,
-        "memory_update": (P * (12 * es + 4 * es + 24 * na * es + 6 * state + (6 * 4 if opcopy else 0))
-                          if prob.nmech else 0),
+        "interior": n_int * interior_node,
+        "shell": (P - n_int) * shell_node + slab * 36 * es,
+        "memory_update": ((P * (6 * es + 4 * es + 24 * na * es + 6 * state + (6 * 4 if opcopy else 0))
+                           + (P - n_int) * 6 * es) if prob.nmech else 0),
     }
     # the same count with every mechanism's eta and Deta streamed (the
     # per-point figure quoted for the workload, 1116 B FP32-mixed Q8 before

@@ -403,6 +403,40 @@ __device__ __forceinline__ void memory_update_node(const StepArgs<T, S>& a, unsi
                                      a.A, a.Bdt, id, mu, l2m, e, DU);
 }
 
+// Nodes of the strict interior outside every source box: no PML, face or
+// source term enters their stress rates, which are therefore Hooke's law of
+// the strain rates in W.  The interior kernel stores only the strain rates
+// for them, and the memory-variable kernels form the stress rates from those
+// (the same expressions as the stencil epilogue).  The test is geometric, so
+// it holds whichever kernel produced W.
+template <typename T, typename S>
+__device__ __forceinline__ bool hooke_node(const StepArgs<T, S>& a, unsigned long long id) {
+  const unsigned long long plane = (unsigned long long)a.nx * (unsigned long long)a.ny;
+  const int z = (int)(id / plane);
+  const unsigned long long r = id - (unsigned long long)z * plane;
+  const int y = (int)(r / (unsigned long long)a.nx);
+  const int x = (int)(r - (unsigned long long)y * a.nx);
+  if (x < a.ix0 || x > a.ix1 || y < a.iy0 || y > a.iy1 || z < a.iz0 || z > a.iz1) return false;
+  for (int k = 0; k < a.nsrc; ++k) {
+    const DevSource& Sr = a.src[k];
+    const int rr = Sr.order == 2 ? 2 : 4;   // the box of source_at
+    if (x >= Sr.near[0] - rr && x <= Sr.near[0] + rr && y >= Sr.near[1] - rr &&
+        y <= Sr.near[1] + rr && z >= Sr.near[2] - rr && z <= Sr.near[2] + rr)
+      return false;
+  }
+  return true;
+}
+
+template <typename T>
+__device__ __forceinline__ void hooke_rates(T lam, T l2m, T mu, const T (&e)[6], T* DU) {
+  DU[3] = l2m * e[0] + lam * (e[1] + e[2]);
+  DU[4] = l2m * e[1] + lam * (e[0] + e[2]);
+  DU[5] = l2m * e[2] + lam * (e[0] + e[1]);
+  DU[6] = mu * e[3];
+  DU[7] = mu * e[4];
+  DU[8] = mu * e[5];
+}
+
 template <typename T, int NMECH, bool NT, typename S>
 __global__ __launch_bounds__(256) void k_memory_update(const StepArgs<T, S> a) {
   const unsigned id = (unsigned)a.node0 + blockIdx.x * 256u + threadIdx.x;
@@ -412,11 +446,15 @@ __global__ __launch_bounds__(256) void k_memory_update(const StepArgs<T, S> a) {
 #pragma unroll
   for (int c = 0; c < 3; ++c) DU[c] = T(0);
 #pragma unroll
-  for (int c = 3; c < 9; ++c) DU[c] = ld_stream<NT>(atp(a.W + (c - 3) * P, id));
-#pragma unroll
   for (int c = 0; c < 6; ++c) e[c] = ld_stream<NT>(atp(a.W + (6 + c) * P, id));
   const T lam = ld_stream<NT>(atp(a.M, id)), mu = ld_stream<NT>(atp(a.M + P, id));
   const T l2m = lam + T(2) * mu;
+  if (hooke_node(a, id)) {
+    hooke_rates(lam, l2m, mu, e, DU);
+  } else {
+#pragma unroll
+    for (int c = 3; c < 9; ++c) DU[c] = ld_stream<NT>(atp(a.W + (c - 3) * P, id));
+  }
   memory_update_node<T, NMECH, NT>(a, id, mu, l2m, e, DU);
 #pragma unroll
   for (int c = 3; c < 9; ++c) {
@@ -529,13 +567,6 @@ __global__ __launch_bounds__(256) void k_memory_update_n(const StepArgs<T, S> a)
 #pragma unroll
     for (int c = 0; c < 3; ++c) DU[n][c] = T(0);
 #pragma unroll
-  for (int c = 3; c < 9; ++c) {
-    T t[N];
-    ldn<N, NT>(a.W + (c - 3) * P, id, t);
-#pragma unroll
-    for (int n = 0; n < N; ++n) DU[n][c] = t[n];
-  }
-#pragma unroll
   for (int c = 0; c < 6; ++c) {
     T t[N];
     ldn<N, NT>(a.W + (6 + c) * P, id, t);
@@ -547,6 +578,24 @@ __global__ __launch_bounds__(256) void k_memory_update_n(const StepArgs<T, S> a)
   ldn<N, NT>(a.M + P, id, mu);
 #pragma unroll
   for (int n = 0; n < N; ++n) l2m[n] = lam[n] + T(2) * mu[n];
+  bool hk[N], all_hooke = true;
+#pragma unroll
+  for (int n = 0; n < N; ++n) {
+    hk[n] = hooke_node(a, (unsigned long long)id + n);
+    all_hooke = all_hooke && hk[n];
+  }
+  if (!all_hooke) {
+#pragma unroll
+    for (int c = 3; c < 9; ++c) {
+      T t[N];
+      ldn<N, NT>(a.W + (c - 3) * P, id, t);
+#pragma unroll
+      for (int n = 0; n < N; ++n) DU[n][c] = t[n];
+    }
+  }
+#pragma unroll
+  for (int n = 0; n < N; ++n)
+    if (hk[n]) hooke_rates(lam[n], l2m[n], mu[n], e[n], DU[n]);
   memory_update_planes_n<T, NMECH, N, NT>(a.eta, a.Deta, a.Qs, a.Qp, P, a.mech_skip, a.wgt,
                                       a.tau_inv, a.A, a.Bdt, id, mu, l2m, e, DU);
 #pragma unroll
@@ -873,11 +922,12 @@ void k_interior(const StepArgs<T, S> a) {
         for (int c = 0; c < 3; ++c)
           rk_finish_pre<NT>(a, c, idu, scale_rate(a.A, pre_df[c]) + (double)DU[c], pre_f[c],
                             pre_c[c]);
+        // stress rates only where the source adds to them: elsewhere in the
+        // strict interior they are Hooke's law of e (hooke_node)
+        if (has_src) {
 #pragma unroll
-        for (int c = 3; c < 9; ++c) {
-          T r = DU[c];
-          if (has_src) r -= (T)srcv[c - 3];
-          st_stream<NT>(atp(a.W + (c - 3) * P, idu), r);
+          for (int c = 3; c < 9; ++c)
+            st_stream<NT>(atp(a.W + (c - 3) * P, idu), DU[c] - (T)srcv[c - 3]);
         }
 #pragma unroll
         for (int c = 0; c < 6; ++c) st_stream<NT>(atp(a.W + (6 + c) * P, idu), e[c]);
