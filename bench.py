@@ -259,8 +259,13 @@ def accuracy_config3(n=385, steps=1000, nmech=8, npml=20):
         blk = PB.make_device_block(prob, precision=prec, state_mirror=True)
         blk.set_receivers(prob.receivers, steps)
         w0 = time.perf_counter()
-        blk.time_step(prob.dt, steps, 0.0)
-        blk.sync()
+        t, done = 0.0, 0
+        while done < steps:
+            k = min(250, steps - done)
+            t = blk.time_step(prob.dt, k, t)
+            blk.sync()
+            done += k
+            print(f"[accuracy c3] {name}: {done} / {steps} steps", file=sys.stderr, flush=True)
         wall = time.perf_counter() - w0
         blk.download(L.WQL_F)
         res[name] = (blk.seismograms(), blk.F.cpu().numpy(), wall)

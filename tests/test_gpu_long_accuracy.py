@@ -22,11 +22,13 @@ from waveqlab3d_q_gpu_amd import _lib as L
 pytestmark = pytest.mark.gpu
 
 
-def test_config3_fp32_variants_1000_steps():
+def test_config3_fp32_variants_1000_steps(capsys):
     import bench
     dev = ctypes.c_int32()
     L.check(L.lib().wql_runtime_init(0, ctypes.byref(dev)))
-    out = bench.accuracy_config3(n=385, steps=1000)
+    # about five minutes: its progress lines go straight to the terminal
+    with capsys.disabled():
+        out = bench.accuracy_config3(n=385, steps=1000)
     for name in ("fp32_mixed", "fp32_kahan"):
         r = out[name]
         print(f"config 3, {name}: seismogram rel L2 all traces "
