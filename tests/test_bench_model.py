@@ -29,19 +29,23 @@ def test_active_mechanisms_and_boxes():
 
 def test_memory_kernel_bytes_per_node():
     prob = _c3()
-    # FP32-mixed: W 12 planes + lambda, mu, Qs^-1, Qp^-1 (4 B each), 6 active
-    # mechanisms x 6 components x (eta, Deta) read and written (4 B), the six
-    # stress components' F and DF read and written (8 B) and their float copy
-    # written (4 B)
-    per_node = 12 * 4 + 4 * 4 + 6 * 6 * 4 * 4 + 6 * 4 * 8 + 6 * 4
-    assert per_node == 856
-    assert bench.byte_model(prob, 1)["kernels"]["memory_update"] == NODES * per_node
+    n_int = 345 * 359 * 345
+    n_shell = NODES - n_int
+    # FP32-mixed: W's 6 strain rates + lambda, mu, Qs^-1, Qp^-1 (4 B each), 6
+    # active mechanisms x 6 components x (eta, Deta) read and written (4 B),
+    # the six stress components' F and DF read and written (8 B) and their
+    # float copy written (4 B); shell nodes also read W's 6 stress rates (the
+    # strict interior's are Hooke's law of the strain rates)
+    per_node = 6 * 4 + 4 * 4 + 6 * 6 * 4 * 4 + 6 * 4 * 8 + 6 * 4
+    assert per_node == 832
+    assert bench.byte_model(prob, 1)["kernels"]["memory_update"] == (
+        NODES * per_node + n_shell * 6 * 4)
     # FP64: everything in 8 B, no float copy
     assert bench.byte_model(prob, 0)["kernels"]["memory_update"] == NODES * (
-        12 * 8 + 4 * 8 + 6 * 6 * 4 * 8 + 6 * 4 * 8)
+        6 * 8 + 4 * 8 + 6 * 6 * 4 * 8 + 6 * 4 * 8) + n_shell * 6 * 8
     # FP32-Kahan: F, DF and the compensation in 4 B (F, DF, Fc read + write)
     assert bench.byte_model(prob, 2)["kernels"]["memory_update"] == NODES * (
-        12 * 4 + 4 * 4 + 6 * 6 * 4 * 4 + 6 * (4 * 4 + 2 * 4))
+        6 * 4 + 4 * 4 + 6 * 6 * 4 * 4 + 6 * (4 * 4 + 2 * 4)) + n_shell * 6 * 4
 
 
 def test_stencil_and_end_to_end_bytes():
@@ -50,12 +54,14 @@ def test_stencil_and_end_to_end_bytes():
     n_int = 345 * 359 * 345
     assert m["interior_nodes"] == n_int
     # FP32-mixed stencil node: F as float (36) + velocity FP64 F read (24) and its
-    # float copy written (12) + geometry 10 and material 3 (4 B) + W written (48)
-    # + velocity DF read and written and F written (3 x 24)
-    stencil = 36 + 3 * (8 + 4) + 13 * 4 + 12 * 4 + 3 * 24
-    assert stencil == 244
-    assert m["kernels"]["interior"] == n_int * stencil
-    assert m["kernels"]["shell"] == (NODES - n_int) * stencil + 5 * 20 * N * N * 36 * 4
+    # float copy written (12) + geometry 10 and material 3 (4 B) + W written
+    # (strict interior: the 6 strain rates, 24; shell: the 6 stress rates as
+    # well, 48) + velocity DF read and written and F written (3 x 24)
+    interior = 36 + 3 * (8 + 4) + 13 * 4 + 6 * 4 + 3 * 24
+    shell = 36 + 3 * (8 + 4) + 13 * 4 + 12 * 4 + 3 * 24
+    assert (interior, shell) == (220, 244)
+    assert m["kernels"]["interior"] == n_int * interior
+    assert m["kernels"]["shell"] == (NODES - n_int) * shell + 5 * 20 * N * N * 36 * 4
     # end to end: F, DF read + write in FP64, geometry + material, eta/Deta
     # + Q^-1, PML auxiliaries
     e2e = NODES * (36 * 8 + 13 * 4 + (24 * 6 + 2) * 4) + 5 * 20 * N * N * 36 * 4

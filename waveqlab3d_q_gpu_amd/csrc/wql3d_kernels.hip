@@ -722,6 +722,11 @@ __device__ __forceinline__ void pin(V& v) {
 #define WQL_IW64 2
 #endif
 #define WQL_INTERIOR_WAVES(T) (sizeof(T) == 4 ? WQL_IW32 : WQL_IW64)
+// epilogue operands of the FP32 interior kernel loaded ahead (1) or in the
+// epilogue (0)
+#ifndef WQL_IPRE
+#define WQL_IPRE 1
+#endif
 
 // MODE 0: fused stage (memory variables in the epilogue); MODE 3: rates and
 // strain rates to W, memory variables and RK update by k_memory_update
@@ -793,11 +798,41 @@ void k_interior(const StepArgs<T, S> a) {
         }
       }
     }
+    // FP32 storage: the epilogue's material and geometry at the node
+    // (lambda, mu, rho and the metric entries the staging below does not
+    // load: X_m(2), or the Cartesian scales) are loaded here and the rest
+    // kept from the staging, so no load of the epilogue waits behind the
+    // barrier.  FP64 storage reloads them in the epilogue (the extra live
+    // registers would spill).
+    constexpr bool PRE = WQL_IPRE && sizeof(T) == 4;
+    T em[3], eg[MESH ? 10 : 3];
+    if (PRE && active) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) em[k] = ld_stream<NT>(atp(a.M + k * P, p));
+      if constexpr (MESH) {
+#pragma unroll
+        for (int m = 0; m < 3; ++m) eg[3 * m + 2] = ldp(a.G + (3 * m + 2) * P, p);
+      } else {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) eg[d] = ldp(a.G + d * P, p);
+      }
+    }
     // own node of level z: v from the queue, fl_q and fl_r
     if (inblk) {
       T vq[3], fq[3], fr[3];
-      node_operands<T, MESH, 0>(a, p, vq, fq);
-      node_operands<T, MESH, 1>(a, p, vq, fr);
+      RawOps<T, MESH, S> r0, r1;
+      load_ops<T, MESH, 0>(a, p, r0);
+      load_ops<T, MESH, 1>(a, p, r1);
+      make_ops<T, MESH, 0>(r0, vq, fq);
+      make_ops<T, MESH, 1>(r1, vq, fr);
+      if constexpr (PRE && MESH) {
+        eg[9] = r0.g[0];
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+          eg[3 * m + 0] = r0.g[1 + m];
+          eg[3 * m + 1] = r1.g[1 + m];
+        }
+      }
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         L[c][ty + 3][tx + 3] = qv[3][c];
@@ -872,30 +907,33 @@ void k_interior(const StepArgs<T, S> a) {
         }
       });
       const unsigned idu = p;
-      const T lam = ld_stream<NT>(atp(a.M, idu)), mu = ld_stream<NT>(atp(a.M + P, idu)),
-              rho = ld_stream<NT>(atp(a.M + 2 * P, idu));
+      const T lam = PRE ? em[0] : ld_stream<NT>(atp(a.M, idu));
+      const T mu = PRE ? em[1] : ld_stream<NT>(atp(a.M + P, idu));
+      const T rho = PRE ? em[2] : ld_stream<NT>(atp(a.M + 2 * P, idu));
+#define WQL_EG(k) (PRE ? eg[k] : ldp(a.G + (k) * P, idu))
       const T l2m = lam + T(2) * mu;
       T gr[3][3];   // gr[m][c] = dv_c/dx_m
       T DU[9];
-      if (MESH) {
+      if constexpr (MESH) {
         const T ihd[3] = {(T)a.ih[0], (T)a.ih[1], (T)a.ih[2]};
-        const T J = ldp(a.G + 9 * P, idu);
+        const T J = WQL_EG(9);
         const T rhoJ_inv = T(1) / (rho * J);
 #pragma unroll
         for (int c = 0; c < 3; ++c)
           DU[c] = (ihd[0] * Df[0][c] + ihd[1] * Df[1][c] + ihd[2] * Df[2][c]) * rhoJ_inv;
 #pragma unroll
         for (int m = 0; m < 3; ++m) {
-          const T X0 = ldp(a.G + (3 * m + 0) * P, idu) * ihd[0];
-          const T X1 = ldp(a.G + (3 * m + 1) * P, idu) * ihd[1];
-          const T X2 = ldp(a.G + (3 * m + 2) * P, idu) * ihd[2];
+          const T X0 = WQL_EG(3 * m + 0) * ihd[0];
+          const T X1 = WQL_EG(3 * m + 1) * ihd[1];
+          const T X2 = WQL_EG(3 * m + 2) * ihd[2];
 #pragma unroll
           for (int c = 0; c < 3; ++c) gr[m][c] = X0 * Dv[0][c] + X1 * Dv[1][c] + X2 * Dv[2][c];
         }
       } else {
         T sd[3];
 #pragma unroll
-        for (int d = 0; d < 3; ++d) sd[d] = ldp(a.G + d * P, idu) * (T)a.ih[d];
+        for (int d = 0; d < 3; ++d) sd[d] = WQL_EG(d) * (T)a.ih[d];
+#undef WQL_EG
         const T rho_inv = T(1) / rho;
 #pragma unroll
         for (int c = 0; c < 3; ++c)
