@@ -284,9 +284,30 @@ int wql_block_kernel_times(wql_block *b, double ms[3], int32_t *stages);
  * on T0_l (:2324-2378).  The initial state is desc->W0 ([nz][ny]) or 0 as
  * init_iface sets it (iface.f90:38; the reference's Init_Vel_State,
  * :2722-2850, which would set the steady state, is not called there). */
+/* Linear friction (coupling 'linear_friction', Interface_Condition.f90:
+ * 360-366, 639-742): tau = alpha v on the shear components.  The hats are
+ * the closed form of Linear_Friction with alpha = desc->alpha (the reference
+ * fixes alpha = 1e14, which a negative desc->alpha selects; 0 is a
+ * frictionless interface).  The reference's routine leaves its u_out(2)
+ * (the minus side's m velocity hat) unset; here it is u_m_hat of the same
+ * derivation.  The fault output adds desc->T0L / T0R (the problem's
+ * prestress, [3][nz][ny]) when given, else zero. */
+/* Rate-and-state friction, problem 'rate-weakening' (Interface_Condition.
+ * 375-446, 2387-2426; initial_stress_condition.f90:184-191; Init_Vel_State
+ * :2822-2845): L0 0.5144, f0 0.7, b 0.02, V0 1e-6, a(y, z) 0.016 inside the
+ * 20 km box, 0.032 outside (smooth_boxcar width 2.5); the prestress is the
+ * traction of the stress with psi = 50 degrees (sxx -126, sxz 42, szz from
+ * the angle, syy their mean) plus a Gaussian pulse 2 tau_pulse F(r) on T0_l
+ * from t > 0 (tau_pulse = -0.2429 T0_n, r0 1.35 km around (y, z) = (8, 0));
+ * the state evolves as dpsi/dt = -V / L0 (a asinh(V / (2 V0) e^(psi / a))
+ * - Fss) with the flash-weakened steady state Fss = 0.13 + (f0 - (b - a)
+ * ln(V / V0) - 0.13) / (1 + (V / V_w)^8)^(1/8), V_w(y, z) = 0.17 inside the
+ * box, 5.17 outside.  The slip-rate solve is the TPV101 one.  The initial
+ * state is desc->W0 or 0 (Init_Vel_State's value is 0.4367). */
 enum { WQL_IFACE_LOCKED = 0, WQL_IFACE_SLIP_WEAKENING_TPV5 = 1,
        WQL_IFACE_SLIP_WEAKENING_TPV28 = 2, WQL_IFACE_SLIP_WEAKENING = 3,
-       WQL_IFACE_RATE_STATE_TPV101 = 4, WQL_IFACE_RATE_STATE_TPV102 = 5 };
+       WQL_IFACE_RATE_STATE_TPV101 = 4, WQL_IFACE_RATE_STATE_TPV102 = 5,
+       WQL_IFACE_LINEAR_FRICTION = 6, WQL_IFACE_RATE_STATE_RW = 7 };
 typedef struct wql_iface_desc {
   int32_t coupling;          /* WQL_IFACE_*: coupling 'locked' / 'slip-weakening_friction'
                                 with problem 'TPV5' / 'TPV28' / caller tables */
@@ -295,10 +316,12 @@ typedef struct wql_iface_desc {
                                 the y and z of block L's x-high face (G%X(pq, :, :, 2:3)) */
   const double *s0;          /* TPV28 only: initial stress (xx, yy, zz, xy, xz, yz), host
                                 array of 6; null = (-60, 0, 60, 0, 29.38, 0) */
-  const double *T0L, *T0R;   /* WQL_IFACE_SLIP_WEAKENING only: prestress [3][nz][ny] in the
-                                (n, m, l) frame of block L's face / block R's face */
+  const double *T0L, *T0R;   /* WQL_IFACE_SLIP_WEAKENING: prestress [3][nz][ny] in the
+                                (n, m, l) frame of block L's face / block R's face;
+                                WQL_IFACE_LINEAR_FRICTION: the output's prestress (or null) */
   const double *fs, *fd, *Dc;  /* WQL_IFACE_SLIP_WEAKENING only: [nz][ny] each */
   const double *W0;          /* WQL_IFACE_RATE_STATE_*: initial state [nz][ny]; null = 0 */
+  double alpha;              /* WQL_IFACE_LINEAR_FRICTION: tau = alpha v; < 0 = 1e14 */
 } wql_iface_desc;
 typedef struct wql_iface wql_iface;
 /* init_iface (iface.f90:13-55): S = 0, DS = 1e40, rupture time 1e9; for

@@ -63,11 +63,12 @@ class _Block(C.Structure):
 class _Iface(C.Structure):
     _fields_ = [("ny", C.c_int), ("nz", C.c_int), ("coupling", C.c_int),
                 ("s0", C.c_double * 6), ("yf", _P), ("zf", _P), ("S", _P), ("DS", _P), ("W", _P), ("DW", _P),
-                ("Uhat", _P), ("Vhat", _P), ("Svel", _P), ("trup", _P)]
+                ("Uhat", _P), ("Vhat", _P), ("Svel", _P), ("trup", _P), ("alpha", C.c_double)]
 
 
 IFACE_LOCKED, IFACE_SLIP_WEAKENING_TPV5, IFACE_SLIP_WEAKENING_TPV28 = 0, 1, 2
 IFACE_RATE_STATE_TPV101, IFACE_RATE_STATE_TPV102 = 4, 5   # the library's coupling codes
+IFACE_LINEAR_FRICTION, IFACE_RATE_STATE_RW = 6, 7
 
 _lib = None
 
@@ -137,6 +138,14 @@ def _load(path):
     L.orc_rs_state_rate.restype = C.c_double
     L.orc_rs_a.argtypes = [C.c_double, C.c_double]
     L.orc_rs_a.restype = C.c_double
+    for name in ("orc_rw_a", "orc_rw_vw"):
+        getattr(L, name).argtypes = [C.c_double, C.c_double]
+        getattr(L, name).restype = C.c_double
+    L.orc_rw_state_rate.argtypes = [C.c_double] * 4
+    L.orc_rw_state_rate.restype = C.c_double
+    L.orc_rw_prestress.argtypes = [_P, _P, _P] + [C.c_double] * 3 + [_P]
+    L.orc_welded_hats.argtypes = [_P, _P] + [C.c_double] * 4 + [_P, _P]
+    L.orc_linear_friction_hats.argtypes = [_P, _P] + [C.c_double] * 5 + [_P, _P]
     return L
 
 
@@ -151,6 +160,19 @@ def rs_state_rate(W, Vel, L0=0.02, f0=0.6, b=0.012, V0=1.0e-6):
 
 def rs_a(y, z):
     return lib().orc_rs_a(y, z)
+
+
+def rw_state_rate(W, Vel, a, Vw):
+    """State rate of problem 'rate-weakening' (flash-weakened steady state)."""
+    return lib().orc_rw_state_rate(W, Vel, a, Vw)
+
+
+def rw_a(y, z):
+    return lib().orc_rw_a(y, z)
+
+
+def rw_vw(y, z):
+    return lib().orc_rw_vw(y, z)
 
 
 def _ptr(a):
@@ -334,6 +356,33 @@ class OracleBlock:
         return float(tt[0]), seis[:, :nrec] if nrec else None
 
 
+def rw_prestress(n, m, l, y, z, t):
+    """Prestress (T0_n, T0_m, T0_l) of problem 'rate-weakening' on a face with
+    unit frame n, m, l at (y, z) km, time t."""
+    vecs = [np.ascontiguousarray(v, dtype=np.float64) for v in (n, m, l)]
+    T0 = np.zeros(3)
+    lib().orc_rw_prestress(*[_ptr(v) for v in vecs], y, z, t, _ptr(T0))
+    return T0
+
+
+def welded_hats(U, V, Z1s, Z2s, Z1p, Z2p):
+    """Welded_Interface hats (P, Q) of rotated grid values U, V (6 each)."""
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    V = np.ascontiguousarray(V, dtype=np.float64)
+    P, Q = np.zeros(6), np.zeros(6)
+    lib().orc_welded_hats(_ptr(U), _ptr(V), Z1s, Z2s, Z1p, Z2p, _ptr(P), _ptr(Q))
+    return P, Q
+
+
+def linear_friction_hats(U, V, Z1s, Z2s, Z1p, Z2p, alpha):
+    """Linear_Friction hats (P, Q) with tau = alpha v."""
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    V = np.ascontiguousarray(V, dtype=np.float64)
+    P, Q = np.zeros(6), np.zeros(6)
+    lib().orc_linear_friction_hats(_ptr(U), _ptr(V), Z1s, Z2s, Z1p, Z2p, alpha, _ptr(P), _ptr(Q))
+    return P, Q
+
+
 class OracleIface:
     """The q-interface between the x-high face of block L and the x-low face
     of block R (wql_oracle_iface.c).  yf, zf: coordinates of the face nodes
@@ -342,8 +391,10 @@ class OracleIface:
     # TPV28 initial stress of the source (initial_stress_condition.f90:78-84)
     TPV28_STRESS = (-60.0, 0.0, 60.0, 0.0, 29.38, 0.0)
 
-    def __init__(self, ny, nz, yf, zf, coupling=IFACE_SLIP_WEAKENING_TPV5, s0=None, W0=None):
+    def __init__(self, ny, nz, yf, zf, coupling=IFACE_SLIP_WEAKENING_TPV5, s0=None, W0=None,
+                 alpha=1.0e14):
         self.ny, self.nz, self.coupling = ny, nz, coupling
+        self.alpha = float(alpha)   # linear friction (the reference's value, :365)
         self.s0 = tuple(self.TPV28_STRESS if s0 is None else s0)
         shp = (nz, ny)
         self.yf = np.ascontiguousarray(yf, dtype=np.float64).reshape(shp)
@@ -361,6 +412,7 @@ class OracleIface:
     def _sync(self):
         s = _Iface()
         s.ny, s.nz, s.coupling = self.ny, self.nz, self.coupling
+        s.alpha = self.alpha
         s.s0[:] = list(self.s0)
         for name in ("yf", "zf", "S", "DS", "W", "DW", "Uhat", "Vhat", "Svel", "trup"):
             setattr(s, name, _ptr(getattr(self, name)))

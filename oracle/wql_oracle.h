@@ -111,7 +111,10 @@ enum { ORC_IFACE_LOCKED = 0, ORC_IFACE_SLIP_WEAKENING_TPV5 = 1, ORC_IFACE_SLIP_W
        /* coupling 'rate-and-state_friction' with problem 'TPV101' / 'TPV102'
         * (the values of the library's WQL_IFACE_* codes; 3 is its caller-table
         * slip weakening, which the oracle has no use for) */
-       ORC_IFACE_RATE_STATE_TPV101 = 4, ORC_IFACE_RATE_STATE_TPV102 = 5 };
+       ORC_IFACE_RATE_STATE_TPV101 = 4, ORC_IFACE_RATE_STATE_TPV102 = 5,
+       /* coupling 'linear_friction' (alpha from orc_iface.alpha) and
+        * 'rate-and-state_friction' with problem 'rate-weakening' */
+       ORC_IFACE_LINEAR_FRICTION = 6, ORC_IFACE_RATE_STATE_RW = 7 };
 
 /* rate-and-state pieces (Interface_Condition.f90), exposed for their pins:
  * Regula_Falsi (:1170-1239) on f (:1241-1265) from the guess V, returning the
@@ -122,6 +125,23 @@ double orc_rs_state_rate(double W, double Vel, double L0, double f0, double b, d
 /* the friction parameter a of TPV101 / TPV102 at (y, z) (:386-400, with
  * smooth_boxcar :2460-2476) */
 double orc_rs_a(double y, double z);
+/* problem 'rate-weakening': a(y, z) and the flash-weakening velocity
+ * V_w(y, z) (:410-416, 438-444), and the state rate
+ * -V/L0 (a asinh(V/(2 V0) e^(W/a)) - Fss) (:438-444) */
+double orc_rw_a(double y, double z);
+double orc_rw_vw(double y, double z);
+double orc_rw_state_rate(double W, double Vel, double a, double Vw);
+/* its prestress (n, m, l) on a face with frame n, m, l at (y, z), time t
+ * (:2387-2426) */
+void orc_rw_prestress(const double *n, const double *m, const double *l, double y, double z,
+                      double t, double T0[3]);
+/* hat variables P (minus side), Q (plus side) in (n, m, l) from the rotated
+ * grid values U, V (velocities 0..2, tractions 3..5) of Welded_Interface
+ * (:515-590) and of Linear_Friction (:639-742) with tau = alpha v */
+void orc_welded_hats(const double U[6], const double V[6], double Z1s, double Z2s, double Z1p,
+                     double Z2p, double P[6], double Q[6]);
+void orc_linear_friction_hats(const double U[6], const double V[6], double Z1s, double Z2s,
+                              double Z1p, double Z2p, double alpha, double P[6], double Q[6]);
 
 /* Interface between the x-high face of block L and the x-low face of block
  * R (same ny, nz).  Face arrays [k][NF], fidx = (y-1) + ny*(z-1).
@@ -140,6 +160,7 @@ typedef struct orc_iface {
   double *yf, *zf;
   double *S, *DS, *W, *DW;
   double *Uhat, *Vhat, *Svel, *trup;
+  double alpha;     /* linear friction: tau = alpha v */
 } orc_iface;
 
 /* enforce_iface_conditions for one side: side 2 = block L's call (SAT into

@@ -2,7 +2,8 @@
  * wql_oracle_iface.c -- CPU restatement (FP64) of the two-block interface of
  * WaveQLab3D-Q: the q-direction interface between the x-high face of block L
  * and the x-low face of block R: locked, TPV5 / TPV28 linear slip-weakening,
- * or TPV101 / TPV102 rate-and-state friction.
+ * TPV101 / TPV102 or rate-weakening rate-and-state friction, or linear
+ * friction.
  * TEST INFRASTRUCTURE ONLY (see wql_oracle.h).
  *
  * Follows, step by step and in the same order of operations:
@@ -20,7 +21,11 @@
  *     TPV28 (:1802-1830, 2049-2079), boxcar (:2438-2457); or
  *     rate_state_friction (:918-1067) with solve's rate-and-state branch
  *     (:1120-1146), Regula_Falsi and f (:1170-1265), prestress TPV101 /
- *     TPV102 (:2324-2378) and the state rate (:420-436); DS(1:3) += Q - P,
+ *     TPV102 (:2324-2378) and the state rate (:420-436); or the same
+ *     solve for problem 'rate-weakening' (parameters :410-416, prestress
+ *     :2387-2426 with initial_stress_tensor, initial_stress_condition.f90:
+ *     184-191, state rate :438-444); or Linear_Friction (:639-742, alpha
+ *     :360-366); DS(1:3) += Q - P,
  *     DS(4) += |slip rate|;
  *   rupture_front (CouplingForcing.f90:580-600);
  *   scale_rates_iface / update_fields_iface (iface.f90:58-85).
@@ -97,8 +102,48 @@ static void prestress_tpv101(const double *n, const double *m, const double *l, 
   *T0l = *T0l + 25.0 * F * G;
 }
 
+/* prestress, problem 'rate-weakening' (Interface_Condition.f90:2387-2426):
+ * the traction of initial_stress_tensor('rate-weakening')
+ * (initial_stress_condition.f90:184-191: psi = 50 degrees with that file's
+ * pi = 3.14159265359; sxx = -126, sxz = 42, szz from psi, syy their mean)
+ * on the face, in (n, m, l), plus the Gaussian pulse 2 tau_pulse F G on T0_l,
+ * tau_pulse = -0.2429 T0_n, F = exp(-(r/r0)^2/2), r0 = 3*1.5*0.3 around
+ * (y, z) = (8, 0), G = 1 for t > 0 */
+static void prestress_rw(const double *n, const double *m, const double *l, double y, double z,
+                         double t, double *T0n, double *T0m, double *T0l) {
+  const double pi = 3.14159265359;
+  const double psi = 50.0 / 180.0 * pi;
+  double s0[6];
+  s0[0] = -126.0;
+  s0[4] = 42.0;
+  s0[2] = (1.0 - 2.0 * s0[4] / (s0[0] * tan(2.0 * psi))) * s0[0];
+  s0[1] = 0.5 * (s0[0] + s0[2]);
+  s0[3] = 0.0;
+  s0[5] = 0.0;
+  const double Tx = n[0] * s0[0] + n[1] * s0[3] + n[2] * s0[4];
+  const double Ty = n[0] * s0[3] + n[1] * s0[1] + n[2] * s0[5];
+  const double Tz = n[0] * s0[4] + n[1] * s0[5] + n[2] * s0[2];
+  *T0n = n[0] * Tx + n[1] * Ty + n[2] * Tz;
+  *T0m = m[0] * Tx + m[1] * Ty + m[2] * Tz;
+  *T0l = l[0] * Tx + l[1] * Ty + l[2] * Tz;
+  const double tau_pulse = -0.2429 * *T0n;
+  const double r0 = 3.0 * 1.5 * 0.3;
+  const double y0 = 8.0, z0 = 0.0;
+  const double r = sqrt((y - y0) * (y - y0) + (z + z0) * (z + z0));
+  const double F = exp(-((r / r0) * (r / r0)) / 2.0);
+  double G = 0.0;
+  if (t > 0.0) G = 1.0;
+  *T0l = *T0l + 2.0 * tau_pulse * F * G;
+}
+
+void orc_rw_prestress(const double *n, const double *m, const double *l, double y, double z,
+                      double t, double T0[3]) {
+  prestress_rw(n, m, l, y, z, t, &T0[0], &T0[1], &T0[2]);
+}
+
 static int is_rate_state(int coupling) {
-  return coupling == ORC_IFACE_RATE_STATE_TPV101 || coupling == ORC_IFACE_RATE_STATE_TPV102;
+  return coupling == ORC_IFACE_RATE_STATE_TPV101 || coupling == ORC_IFACE_RATE_STATE_TPV102 ||
+         coupling == ORC_IFACE_RATE_STATE_RW;
 }
 
 /* prestress of the friction problems in (n, m, l) at time t */
@@ -106,6 +151,10 @@ static void prestress(const orc_iface *I, const double *n, const double *m, cons
                       double y, double z, double t, double T0[3]) {
   if (I->coupling == ORC_IFACE_SLIP_WEAKENING_TPV28)
     prestress_tpv28(I->s0, n, m, l, y, z, &T0[0], &T0[1], &T0[2]);
+  else if (I->coupling == ORC_IFACE_RATE_STATE_RW)
+    prestress_rw(n, m, l, y, z, t, &T0[0], &T0[1], &T0[2]);
+  else if (I->coupling == ORC_IFACE_LINEAR_FRICTION)
+    T0[0] = T0[1] = T0[2] = 0.0;
   else if (is_rate_state(I->coupling))
     prestress_tpv101(n, m, l, y, z, t, &T0[0], &T0[1], &T0[2]);
   else
@@ -122,6 +171,71 @@ static double smooth_boxcar(double x, double w, double wr) {
 
 double orc_rs_a(double y, double z) {
   return 0.008 + 0.008 * (1.0 - smooth_boxcar(y, 15.0, 3.0) * smooth_boxcar(z, 15.0, 3.0));
+}
+
+double orc_rw_a(double y, double z) {
+  return 0.016 + 0.016 * (1.0 - smooth_boxcar(y, 20.0, 2.5) * smooth_boxcar(z, 20.0, 2.5));
+}
+
+double orc_rw_vw(double y, double z) {
+  return 0.17 + 5.0 * (1.0 - smooth_boxcar(y, 20.0, 2.5) * smooth_boxcar(z, 20.0, 2.5));
+}
+
+/* the state rate of problem 'rate-weakening' (Interface_Condition.f90:438-444),
+ * L0 = 2*0.2572, f0 = 0.7, b = 0.02, V0 = 1e-6 (:410-416); (V/V_w)^8 as
+ * three squarings */
+double orc_rw_state_rate(double W, double Vel, double a, double Vw) {
+  const double L0 = 2.0 * 0.2572, f0 = 0.7, b = 0.02, V0 = 1.0e-6;
+  double Fss = f0 - (b - a) * log(Vel / V0);
+  const double r = Vel / Vw, r2 = r * r, r4 = r2 * r2;
+  Fss = 0.13 + (Fss - 0.13) / pow(1.0 + r4 * r4, 0.125);
+  return -(Vel / L0 * (a * asinh(Vel / (2.0 * V0) * exp(W / a)) - Fss));
+}
+
+void orc_welded_hats(const double U[6], const double V[6], double Z1s, double Z2s, double Z1p,
+                     double Z2p, double P[6], double Q[6]) {
+  const double p_n = Z1p * U[0] - U[3], p_m = Z1s * U[1] - U[4], p_l = Z1s * U[2] - U[5];
+  const double q_n = Z2p * V[0] + V[3], q_m = Z2s * V[1] + V[4], q_l = Z2s * V[2] + V[5];
+  const double eta_p = 1.0 / (1.0 / Z1p + 1.0 / Z2p);
+  const double eta_s = 1.0 / (1.0 / Z1s + 1.0 / Z2s);
+  const double phi_n = eta_p * (q_n / Z2p - p_n / Z1p);
+  const double phi_m = eta_s * (q_m / Z2s - p_m / Z1s);
+  const double phi_l = eta_s * (q_l / Z2s - p_l / Z1s);
+  P[0] = (q_n - phi_n) / Z2p; P[3] = phi_n;
+  Q[0] = (p_n + phi_n) / Z1p; Q[3] = phi_n;
+  P[2] = (q_l - phi_l) / Z2s; P[5] = phi_l;
+  Q[2] = (p_l + phi_l) / Z1s; Q[5] = phi_l;
+  P[1] = (q_m - phi_m) / Z2s; P[4] = phi_m;
+  Q[1] = (p_m + phi_m) / Z1s; Q[4] = phi_m;
+}
+
+/* Linear_Friction (Interface_Condition.f90:639-742): the normal components
+ * as the welded interface, the shear components from tau = alpha v and the
+ * characteristics; the reference does not store u_out(2) (the minus side's m
+ * velocity), which is u_m_hat of the same derivation here */
+void orc_linear_friction_hats(const double U[6], const double V[6], double Z1s, double Z2s,
+                              double Z1p, double Z2p, double alpha, double P[6], double Q[6]) {
+  const double p_n = Z1p * U[0] - U[3], p_m = Z1s * U[1] - U[4], p_l = Z1s * U[2] - U[5];
+  const double q_n = Z2p * V[0] + V[3], q_m = Z2s * V[1] + V[4], q_l = Z2s * V[2] + V[5];
+  const double eta_p = 1.0 / (1.0 / Z1p + 1.0 / Z2p);
+  const double eta_s = 1.0 / (1.0 / Z1s + 1.0 / Z2s);
+  const double phi_n = eta_p * (q_n / Z2p - p_n / Z1p);
+  const double phi_m = eta_s * (q_m / Z2s - p_m / Z1s);
+  const double phi_l = eta_s * (q_l / Z2s - p_l / Z1s);
+  const double u_n_hat = (q_n - phi_n) / Z2p, Tu_n_hat = phi_n;
+  const double v_n_hat = (p_n + phi_n) / Z1p, Tv_n_hat = phi_n;
+  const double Tu_m_hat = alpha / (alpha + eta_s) * phi_m;
+  const double Tu_l_hat = alpha / (alpha + eta_s) * phi_l;
+  const double Tv_m_hat = alpha / (alpha + eta_s) * phi_m;
+  const double Tv_l_hat = alpha / (alpha + eta_s) * phi_l;
+  const double v_m_hat = (p_m + Tu_m_hat) / (Z1s) + 1.0 / (alpha + eta_s) * phi_m;
+  const double v_l_hat = (p_l + Tu_l_hat) / (Z1s) + 1.0 / (alpha + eta_s) * phi_l;
+  const double u_m_hat = (q_m - Tv_m_hat) / (Z2s) - 1.0 / (alpha + eta_s) * phi_m;
+  const double u_l_hat = (q_l - Tv_l_hat) / (Z2s) - 1.0 / (alpha + eta_s) * phi_l;
+  P[0] = u_n_hat; P[1] = u_m_hat; P[2] = u_l_hat;
+  P[3] = Tu_n_hat; P[4] = Tu_m_hat; P[5] = Tu_l_hat;
+  Q[0] = v_n_hat; Q[1] = v_m_hat; Q[2] = v_l_hat;
+  Q[3] = Tv_n_hat; Q[4] = Tv_m_hat; Q[5] = Tv_l_hat;
 }
 
 /* f (Interface_Condition.f90:1241-1265) */
@@ -192,6 +306,10 @@ static void hats(const orc_iface *I, const double *n, const double *m, const dou
   *Vel = 0.0;
   const double p_n = Z1p * U[0] - U[3], p_m = Z1s * U[1] - U[4], p_l = Z1s * U[2] - U[5];
   const double q_n = Z2p * V[0] + V[3], q_m = Z2s * V[1] + V[4], q_l = Z2s * V[2] + V[5];
+  if (coupling == ORC_IFACE_LINEAR_FRICTION) {
+    orc_linear_friction_hats(U, V, Z1s, Z2s, Z1p, Z2p, I->alpha, P, Q);
+    return;
+  }
   if (coupling == ORC_IFACE_LOCKED) {
     /* Welded_Interface (Interface_Condition.f90:515-590) */
     const double eta_p = 1.0 / (1.0 / Z1p + 1.0 / Z2p);
@@ -210,8 +328,9 @@ static void hats(const orc_iface *I, const double *n, const double *m, const dou
   if (is_rate_state(coupling)) {
     /* rate_state_friction (Interface_Condition.f90:918-1067) with solve's
      * rate-and-state branch (:1120-1146); parameters of TPV101 / TPV102
-     * (:386-400) */
-    const double V0 = 1.0e-6, a = orc_rs_a(y, z);
+     * (:386-400) or rate-weakening (:410-416) */
+    const double V0 = 1.0e-6;
+    const double a = coupling == ORC_IFACE_RATE_STATE_RW ? orc_rw_a(y, z) : orc_rs_a(y, z);
     const double eta_s = 1.0 / (1.0 / Z1s + 1.0 / Z2s);
     const double eta_p = 1.0 / (1.0 / Z1p + 1.0 / Z2p);
     double T0[3];
@@ -375,7 +494,10 @@ static void eval_side(orc_iface *I, orc_block *L, orc_block *R, int ib, double t
       if (ib == 2) {
         for (int c = 0; c < 3; ++c) I->DS[c * NF + fi] = I->DS[c * NF + fi] + Q[c] - P[c];
         I->DS[3 * NF + fi] = I->DS[3 * NF + fi] + slip_rate;
-        if (is_rate_state(I->coupling))
+        if (I->coupling == ORC_IFACE_RATE_STATE_RW)
+          I->DW[fi] = I->DW[fi] + orc_rw_state_rate(I->W[fi], Vel, orc_rw_a(I->yf[fi], I->zf[fi]),
+                                                    orc_rw_vw(I->yf[fi], I->zf[fi]));
+        else if (is_rate_state(I->coupling))
           I->DW[fi] = I->DW[fi] + orc_rs_state_rate(I->W[fi], Vel, 0.02, 0.6, 0.012, 1.0e-6);
       }
       if (stage == 1) {

@@ -39,4 +39,5 @@ program desc_layout
   write(*, '(A,I0)') 'iface_y ', transfer(c_loc(i%y), b) - b
   write(*, '(A,I0)') 'iface_Dc ', transfer(c_loc(i%Dc), b) - b
   write(*, '(A,I0)') 'iface_W0 ', transfer(c_loc(i%W0), b) - b
+  write(*, '(A,I0)') 'iface_alpha ', transfer(c_loc(i%alpha), b) - b
 end program desc_layout

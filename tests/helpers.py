@@ -62,8 +62,9 @@ def oracle_two_block(O, tp):
     L_ = oracle_block(O, tp.left, O.metrics(tp.left.X))
     R_ = oracle_block(O, tp.right, O.metrics(tp.right.X))
     nz, ny = tp.yf.shape
+    alpha = getattr(tp, "alpha", None)
     I = O.OracleIface(ny, nz, tp.yf, tp.zf, coupling=tp.coupling, s0=tp.s0,
-                      W0=getattr(tp, "W0", None))
+                      W0=getattr(tp, "W0", None), alpha=1.0e14 if alpha is None else alpha)
     return L_, R_, I
 
 

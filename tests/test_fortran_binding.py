@@ -39,6 +39,7 @@ C_FIELDS = {
     "iface_y": "offsetof(wql_iface_desc, y)",
     "iface_Dc": "offsetof(wql_iface_desc, Dc)",
     "iface_W0": "offsetof(wql_iface_desc, W0)",
+    "iface_alpha": "offsetof(wql_iface_desc, alpha)",
 }
 
 

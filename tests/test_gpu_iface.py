@@ -374,3 +374,91 @@ def test_rate_state_fp32_against_fp64(oracle, precision):
     print(precision, {k: "%.2e" % e for k, e in errs.items()}, flush=True)
     assert max(errs.values()) <= 1e-4, errs
     itf.close()
+
+
+def test_rate_weakening_fp64_matches_oracle(oracle):
+    """Problem 'rate-weakening' (flash-weakened rate-and-state friction with
+    the Gaussian prestress pulse) on two 21^3 blocks for 0.6 s: FP64 fault
+    output, state and fields against the oracle.  Same regula falsi as
+    TPV101, so the same stated tolerance: 1e-9 relative L2."""
+    tp = PB.rate_weakening_cartesian(21)
+    print(f"case {tp.name}: {tp.nsteps} steps", flush=True)
+    Lo, Ro, Io = oracle_two_block(oracle, tp)
+    _, fr_o = oracle.two_block_time_step(Lo, Ro, Io, tp.dt, tp.nsteps, 0.0)
+    bl = gpu_block(tp.left, oracle.metrics(tp.left.X))
+    br = gpu_block(tp.right, oracle.metrics(tp.right.X))
+    itf = Interface(bl, br, tp.yf, tp.zf, coupling=tp.coupling, W0=tp.W0)
+    itf.set_frames(tp.nsteps)
+    itf.time_step(tp.dt, tp.nsteps, 0.0)
+    fr = itf.frames()
+    errs = {k: rel_l2(fr[k], fr_o[k])
+            for k in ("S", "Svel", "Uface", "Vface", "Uhat", "Vhat", "state")}
+    errs["W"] = rel_l2(itf.friction_state(), Io.W)
+    print({k: "%.2e" % e for k, e in errs.items()}, flush=True)
+    assert max(errs.values()) <= 1e-9, errs
+    assert np.sqrt((fr["Svel"][:, 1:] ** 2).sum(axis=1)).max() > 1.0
+    bl.download(L.WQL_F)
+    br.download(L.WQL_F)
+    assert max(field_errors(bl.F, Lo.F) + field_errors(br.F, Ro.F)) <= 1e-9
+    itf.close()
+
+
+@pytest.mark.parametrize("alpha", [5.0, 0.0, None])
+def test_linear_friction_fp64_matches_oracle(oracle, alpha):
+    """Linear friction tau = alpha v across the curved interface of the
+    two-block curvilinear set-up (different materials, Q4, PML), the pulse
+    crossing with slip: FP64 fault output and fields against the oracle
+    (<= 1e-12), for alpha = 5 MPa s/m, a frictionless interface and the
+    reference's alpha = 1e14 (None)."""
+    tp = PB.linear_friction_curvilinear(24, alpha=5.0 if alpha is None else alpha)
+    tp.alpha = alpha
+    print(f"case {tp.name} alpha={alpha}", flush=True)
+    Lo, Ro, Io = oracle_two_block(oracle, tp)
+    _, fr_o = oracle.two_block_time_step(Lo, Ro, Io, tp.dt, tp.nsteps, 0.0)
+    bl = gpu_block(tp.left, oracle.metrics(tp.left.X))
+    br = gpu_block(tp.right, oracle.metrics(tp.right.X))
+    itf = Interface(bl, br, tp.yf, tp.zf, coupling=tp.coupling, alpha=alpha)
+    itf.set_frames(tp.nsteps)
+    itf.time_step(tp.dt, tp.nsteps, 0.0)
+    fr = itf.frames()
+    keys = ("S", "Svel", "Uface", "Vface", "Uhat", "Vhat")
+    if alpha is None:
+        # locked: the slip rate is the rounding of a velocity difference,
+        # compared on the scale of the face velocities
+        keys = keys[2:]
+        vscale = np.abs(fr_o["Uface"][:, :3]).max()
+        for k in ("S", "Svel"):
+            d = float(np.abs(fr[k] - fr_o[k]).max())
+            print(k, "max |diff|", d, "velocity scale", vscale, flush=True)
+            assert d <= 1e-12 * vscale
+    errs = {k: rel_l2(fr[k], fr_o[k]) for k in keys}
+    print({k: "%.2e" % e for k, e in errs.items()}, flush=True)
+    assert max(errs.values()) <= TOL64, errs
+    bl.download(L.WQL_F)
+    br.download(L.WQL_F)
+    assert max(field_errors(bl.F, Lo.F) + field_errors(br.F, Ro.F)) <= TOL64
+    slip = np.abs(fr["S"][-1, 3]).max()
+    if alpha is None:
+        assert slip <= 1e-9           # the reference's alpha: locked
+    else:
+        assert slip > 1e-3
+    itf.close()
+
+
+def test_linear_friction_reference_alpha_is_the_locked_interface(oracle):
+    """With the reference's alpha = 1e14 the linear-friction interface is the
+    welded one: the GPU fields of the two couplings agree to 1e-12."""
+    outs = []
+    for coupling in (L.WQL_IFACE_LINEAR_FRICTION, L.WQL_IFACE_LOCKED):
+        tp = PB.two_block_curvilinear(24, nmech=4, coupling=coupling, nsteps=30)
+        bl = gpu_block(tp.left, oracle.metrics(tp.left.X))
+        br = gpu_block(tp.right, oracle.metrics(tp.right.X))
+        itf = Interface(bl, br, tp.yf, tp.zf, coupling=coupling)
+        itf.time_step(tp.dt, tp.nsteps, 0.0)
+        bl.download(L.WQL_F)
+        br.download(L.WQL_F)
+        outs.append((bl.F.copy(), br.F.copy()))
+        itf.close()
+    errs = field_errors(outs[0][0], outs[1][0]) + field_errors(outs[0][1], outs[1][1])
+    print("linear(1e14) vs locked:", max(errs), flush=True)
+    assert max(errs) <= TOL64

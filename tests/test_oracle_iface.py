@@ -301,3 +301,94 @@ def test_tpv101_nucleates_and_ruptures(oracle):
     from scipy.stats import spearmanr
     rho = spearmanr(r[ok], tr[ok]).correlation
     assert rho > 0.8, rho
+
+
+def test_linear_friction_limits(oracle):
+    """Linear_Friction (Interface_Condition.f90:639-742): with tau = alpha v
+    the hats tend to the welded interface's as alpha grows (the reference's
+    alpha = 1e14 is its locked limit, :360-366), and alpha = 0 is a
+    frictionless interface (zero shear traction hats) with the normal
+    components still welded.  For any alpha the hats keep both outgoing
+    characteristics (p = Z1 u - T on the minus side, q = Z2 v + T on the
+    plus side) and the shear traction is alpha times the slip rate."""
+    rng = np.random.default_rng(3)
+    for _ in range(20):
+        U, V = rng.normal(size=6), rng.normal(size=6)
+        Z1s, Z2s = rng.uniform(3, 10, 2)
+        Z1p, Z2p = Z1s * 1.7, Z2s * 1.8
+        Pw, Qw = oracle.welded_hats(U, V, Z1s, Z2s, Z1p, Z2p)
+        Pl, Ql = oracle.linear_friction_hats(U, V, Z1s, Z2s, Z1p, Z2p, 1.0e14)
+        assert np.allclose(Pl, Pw, rtol=0, atol=1e-12) and np.allclose(Ql, Qw, rtol=0, atol=1e-12)
+        P0, Q0 = oracle.linear_friction_hats(U, V, Z1s, Z2s, Z1p, Z2p, 0.0)
+        assert np.all(P0[4:] == 0.0) and np.all(Q0[4:] == 0.0)
+        assert P0[0] == Pw[0] and Q0[0] == Qw[0] and P0[3] == Pw[3]
+        for alpha in (0.0, 0.7, 5.0, 300.0):
+            P, Q = oracle.linear_friction_hats(U, V, Z1s, Z2s, Z1p, Z2p, alpha)
+            Zs1, Zs2 = (Z1p, Z1s, Z1s), (Z2p, Z2s, Z2s)
+            for c in range(3):
+                assert abs((Zs1[c] * P[c] - P[3 + c]) - (Zs1[c] * U[c] - U[3 + c])) <= 1e-12 * 20
+                assert abs((Zs2[c] * Q[c] + Q[3 + c]) - (Zs2[c] * V[c] + V[3 + c])) <= 1e-12 * 20
+            for c in (1, 2):     # tau = alpha (v_plus - v_minus), same traction both sides
+                assert abs(P[3 + c] - Q[3 + c]) <= 1e-14
+                assert abs(P[3 + c] - alpha * (Q[c] - P[c])) <= 1e-10 * max(1.0, alpha)
+
+
+def test_rate_weakening_state_law(oracle):
+    """The state rate of problem 'rate-weakening' (:438-444) drives psi to
+    the steady state of its flash-weakened friction: it vanishes where
+    a asinh(V/(2 V0) e^(psi/a)) equals Fss(V), is negative above and positive
+    below; Fss is the classical f0 - (b - a) ln(V/V0) far below V_w (at V = V0
+    it is f0 to ~1e-43) and weakens towards 0.13 far above V_w."""
+    f0, b, V0 = 0.7, 0.02, 1e-6
+    for a, Vw in ((0.016, 0.17), (0.032, 5.17)):
+        for V, Fss_expect in ((V0, f0), (1e-9, f0 - (b - a) * np.log(1e-9 / V0)),
+                              (1e4 * Vw, None)):
+            if Fss_expect is None:
+                # V = 1e4 V_w: Fss = 0.13 + (f - 0.13) / (1 + 1e32)^(1/8) = 0.13 + ~1e-4 (f - 0.13)
+                Fss_expect = 0.13 + (f0 - (b - a) * np.log(V / V0) - 0.13) * 1e-4
+            psi = a * np.log(2.0 * V0 / V * np.sinh(Fss_expect / a))
+            r = oracle.rw_state_rate(psi, V, a, Vw)
+            scale = V / (2.0 * 0.2572) * a
+            assert abs(r) <= 1e-6 * scale, (a, V, r)
+            assert oracle.rw_state_rate(psi + 0.01, V, a, Vw) < 0
+            assert oracle.rw_state_rate(psi - 0.01, V, a, Vw) > 0
+
+
+def test_rate_weakening_parameters_and_prestress(oracle):
+    """a(y, z) 0.016 / 0.032 and V_w 0.17 / 5.17 m/s inside / outside the
+    20 km box (smooth_boxcar width 2.5); the prestress is the traction of a
+    stress whose principal axis lies at psi = 50 degrees from x in the x-z
+    plane (no shear traction on that plane), -126 MPa normal and 42 MPa
+    shear on the fault plane x = const, plus the pulse 2 (0.2429 * 126) F(r)
+    on T0_l from t > 0, F = 1 at (8, 0), exp(-1/2) at r = 1.35 km."""
+    assert oracle.rw_a(8.0, 0.0) == 0.016 and oracle.rw_a(23.0, 0.0) == 0.032
+    assert oracle.rw_vw(8.0, 0.0) == 0.17 and oracle.rw_vw(0.0, -23.0) == 5.17
+    n, m, l = (1.0, 0.0, 0.0), (0.0, 1.0, 0.0), (0.0, 0.0, 1.0)
+    T0 = oracle.rw_prestress(n, m, l, 2.0, 0.0, 0.0)      # t = 0: no pulse
+    assert T0[0] == -126.0 and T0[1] == 0.0 and abs(T0[2] - 42.0) <= 1e-13
+    psi = 50.0 / 180.0 * 3.14159265359
+    nn = (np.cos(psi), 0.0, np.sin(psi))
+    mm = (0.0, 1.0, 0.0)
+    ll = (-np.sin(psi), 0.0, np.cos(psi))
+    Tp = oracle.rw_prestress(nn, mm, ll, 2.0, 0.0, 0.0)
+    assert abs(Tp[1]) <= 1e-12 and abs(Tp[2]) <= 1e-12
+    pulse = oracle.rw_prestress(n, m, l, 8.0, 0.0, 1e-3)[2] - T0[2]
+    assert abs(pulse - 2.0 * 0.2429 * 126.0) <= 1e-12
+    pulse_r0 = oracle.rw_prestress(n, m, l, 8.0, 1.35, 1e-3)[2] - T0[2]
+    assert abs(pulse_r0 - 2.0 * 0.2429 * 126.0 * np.exp(-0.5)) <= 1e-12
+
+
+def test_rate_weakening_ruptures(oracle):
+    """Problem 'rate-weakening' on two 21^3 blocks: from psi = 0.4367 (far
+    below the steady state of its prestress) the fault slides at once; the
+    slip rate peaks at several m/s, every fault node ruptures and the state
+    evolves downwards."""
+    from helpers import oracle_two_block
+    tp = PB.rate_weakening_cartesian(21)
+    Lo, Ro, Io = oracle_two_block(oracle, tp)
+    _, fr = oracle.two_block_time_step(Lo, Ro, Io, tp.dt, tp.nsteps, 0.0)
+    assert np.isfinite(Lo.F).all() and np.isfinite(Ro.F).all()
+    sv = np.sqrt((fr["Svel"][:, 1:] ** 2).sum(axis=1)).max()
+    assert sv > 1.0
+    assert (fr["trup"][-1, 0] < 1e8).all()
+    assert Io.W.max() < 0.4367

@@ -20,6 +20,7 @@ WQL_SAT_BC, WQL_SAT_ALPHA_THETA = 0, 1
 WQL_IFACE_LOCKED, WQL_IFACE_SLIP_WEAKENING_TPV5, WQL_IFACE_SLIP_WEAKENING_TPV28 = 0, 1, 2
 WQL_IFACE_SLIP_WEAKENING = 3
 WQL_IFACE_RATE_STATE_TPV101, WQL_IFACE_RATE_STATE_TPV102 = 4, 5
+WQL_IFACE_LINEAR_FRICTION, WQL_IFACE_RATE_STATE_RW = 6, 7
 WQL_F, WQL_DF, WQL_GEOMETRY, WQL_MATERIAL = 1, 2, 4, 8
 WQL_MEMORY, WQL_QINV, WQL_PML, WQL_FACES, WQL_ALL = 16, 32, 64, 128, 255
 
@@ -52,7 +53,7 @@ class BlockDesc(C.Structure):
 class IfaceDesc(C.Structure):
     _fields_ = [("coupling", C.c_int32), ("arrays_on_device", C.c_int32),
                 ("y", _D), ("z", _D), ("s0", _D), ("T0L", _D), ("T0R", _D),
-                ("fs", _D), ("fd", _D), ("Dc", _D), ("W0", _D)]
+                ("fs", _D), ("fd", _D), ("Dc", _D), ("W0", _D), ("alpha", C.c_double)]
 
 
 class Source(C.Structure):

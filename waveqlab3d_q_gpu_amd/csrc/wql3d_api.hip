@@ -8,6 +8,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <mutex>
 #include <thread>
 #include <vector>
@@ -158,12 +159,13 @@ struct wql_iface {
   int coupling;
   // one allocation of kIfacePlanes planes: S 4, DS 4, trup, prestress of L 3
   // and of R 3, fs, fd, Dc, SAT forcing of L 9 and of R 9, the rate-and-state
-  // state W, its rate DW, the parameter a and the nucleation amplitude, and
-  // FP64 copies of the coupled faces' 13 frame planes and 3 material planes
-  // of each side
+  // state W, its rate DW, the parameter a, the nucleation amplitude and the
+  // flash-weakening velocity V_w (rate-weakening), and FP64 copies of the
+  // coupled faces' 13 frame planes and 3 material planes of each side
   double* buf;
   double *S, *DS, *trup, *T0L, *T0R, *fs, *fd, *Dc, *satL, *satR;
-  double *W, *DW, *rs_a, *rs_nuc, *frL, *frR, *matL, *matR;
+  double *W, *DW, *rs_a, *rs_nuc, *rs_vw, *frL, *frR, *matL, *matR;
+  double alpha;     // linear friction: tau = alpha v
   double* frames;   // nslots * kFramePlanes planes of fault output
   int nslots, slot;
   hipEvent_t ev_iface;  // on L's stream after k_iface: block R's stage waits for it
@@ -1703,7 +1705,8 @@ static int iface_fill_faces(wql_iface* I) {
 }
 
 static bool is_rate_state(int coupling) {
-  return coupling == WQL_IFACE_RATE_STATE_TPV101 || coupling == WQL_IFACE_RATE_STATE_TPV102;
+  return coupling == WQL_IFACE_RATE_STATE_TPV101 || coupling == WQL_IFACE_RATE_STATE_TPV102 ||
+         coupling == WQL_IFACE_RATE_STATE_RW;
 }
 
 // smooth_boxcar (Interface_Condition.f90:2460-2476)
@@ -1721,6 +1724,76 @@ static int iface_tables(wql_iface* I, wql_block* L, wql_block* R, const wql_ifac
   if (d->coupling == WQL_IFACE_LOCKED) return WQL_OK;
   const bool dev = d->arrays_on_device != 0;
   std::vector<double> T0L(3 * NF), T0R(3 * NF), fs(NF), fd(NF), Dc(NF);
+  if (d->coupling == WQL_IFACE_LINEAR_FRICTION) {
+    // no friction tables; the prestress the fault output adds, if given
+    if (d->T0L && d->T0R) {
+      if (int rc = to_host(T0L, d->T0L, 3 * NF, dev)) return rc;
+      if (int rc = to_host(T0R, d->T0R, 3 * NF, dev)) return rc;
+    } else {
+      std::fill(T0L.begin(), T0L.end(), 0.0);
+      std::fill(T0R.begin(), T0R.end(), 0.0);
+    }
+    const size_t nb = (size_t)NF * sizeof(double);
+    HIP_TRY(hipMemcpy(I->T0L, T0L.data(), 3 * nb, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(I->T0R, T0R.data(), 3 * nb, hipMemcpyHostToDevice));
+    return WQL_OK;
+  }
+  if (d->coupling == WQL_IFACE_RATE_STATE_RW) {
+    // problem 'rate-weakening' (Interface_Condition.f90:410-416, 438-444,
+    // 2387-2426): the traction of initial_stress_tensor('rate-weakening')
+    // (initial_stress_condition.f90:184-191, with its pi = 3.14159265359) in
+    // each side's frame, a(y, z) and V_w(y, z) with smooth_boxcar(20, 2.5),
+    // and the pulse shape F(r) = exp(-(r/r0)^2/2), r0 = 3*1.5*0.3, around
+    // (y, z) = (8, 0) (the stage adds 2 (-0.2429 T0_n) F(r) to T0_l for t > 0)
+    std::vector<double> y, z, a(NF), vw(NF), nuc(NF), W0(NF, 0.0);
+    if (int rc = to_host(y, d->y, NF, dev)) return rc;
+    if (int rc = to_host(z, d->z, NF, dev)) return rc;
+    if (d->W0)
+      if (int rc = to_host(W0, d->W0, NF, dev)) return rc;
+    const double pi_s = 3.14159265359;
+    const double psi = 50.0 / 180.0 * pi_s;
+    double s0[6];
+    s0[0] = -126.0;
+    s0[4] = 42.0;
+    s0[2] = (1.0 - 2.0 * s0[4] / (s0[0] * tan(2.0 * psi))) * s0[0];
+    s0[1] = 0.5 * (s0[0] + s0[2]);
+    s0[3] = 0.0;
+    s0[5] = 0.0;
+    const double* frs[2] = {L->face64[1], R->face64[0]};
+    std::vector<double>* outs[2] = {&T0L, &T0R};
+    const double r0 = 3.0 * 1.5 * 0.3;
+    for (long long k = 0; k < NF; ++k) {
+      for (int side = 0; side < 2; ++side) {
+        const double* fr = frs[side];
+        double l[3], m[3], n[3];
+        for (int c = 0; c < 3; ++c) {
+          l[c] = fr[c * NF + k];
+          m[c] = fr[(3 + c) * NF + k];
+          n[c] = fr[(6 + c) * NF + k];
+        }
+        const double Tx = n[0] * s0[0] + n[1] * s0[3] + n[2] * s0[4];
+        const double Ty = n[0] * s0[3] + n[1] * s0[1] + n[2] * s0[5];
+        const double Tz = n[0] * s0[4] + n[1] * s0[5] + n[2] * s0[2];
+        std::vector<double>& t = *outs[side];
+        t[k] = n[0] * Tx + n[1] * Ty + n[2] * Tz;
+        t[NF + k] = m[0] * Tx + m[1] * Ty + m[2] * Tz;
+        t[2 * NF + k] = l[0] * Tx + l[1] * Ty + l[2] * Tz;
+      }
+      const double box = smooth_boxcar(y[k], 20.0, 2.5) * smooth_boxcar(z[k], 20.0, 2.5);
+      a[k] = 0.016 + 0.016 * (1.0 - box);
+      vw[k] = 0.17 + 5.0 * (1.0 - box);
+      const double r = sqrt((y[k] - 8.0) * (y[k] - 8.0) + (z[k] + 0.0) * (z[k] + 0.0));
+      nuc[k] = exp(-((r / r0) * (r / r0)) / 2.0);
+    }
+    const size_t nb = (size_t)NF * sizeof(double);
+    HIP_TRY(hipMemcpy(I->T0L, T0L.data(), 3 * nb, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(I->T0R, T0R.data(), 3 * nb, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(I->rs_a, a.data(), nb, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(I->rs_nuc, nuc.data(), nb, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(I->rs_vw, vw.data(), nb, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(I->W, W0.data(), nb, hipMemcpyHostToDevice));
+    return WQL_OK;
+  }
   if (is_rate_state(d->coupling)) {
     // TPV101 / TPV102 (Interface_Condition.f90:386-400, 2324-2378): the
     // traction of the initial stress (-120, 0, -120, 0, 75, 0)
@@ -1845,8 +1918,10 @@ extern "C" int wql_iface_create(wql_block* L, wql_block* R, const wql_iface_desc
   if (!out) return fail(WQL_EINVAL, "null output pointer");
   *out = nullptr;
   if (!L || !R || !d || L == R) return fail(WQL_EINVAL, "need two distinct blocks and a descriptor");
-  if (d->coupling < WQL_IFACE_LOCKED || d->coupling > WQL_IFACE_RATE_STATE_TPV102)
+  if (d->coupling < WQL_IFACE_LOCKED || d->coupling > WQL_IFACE_RATE_STATE_RW)
     return fail(WQL_EINVAL, "invalid coupling %d", d->coupling);
+  if (d->coupling == WQL_IFACE_LINEAR_FRICTION && !std::isfinite(d->alpha))
+    return fail(WQL_EINVAL, "linear friction needs a finite alpha (< 0: the reference's 1e14)");
   if ((d->coupling == WQL_IFACE_SLIP_WEAKENING_TPV5 || d->coupling == WQL_IFACE_SLIP_WEAKENING_TPV28 ||
        is_rate_state(d->coupling)) &&
       (!d->y || !d->z))
@@ -1870,8 +1945,9 @@ extern "C" int wql_iface_create(wql_block* L, wql_block* R, const wql_iface_desc
   I->ny = L->ny; I->nz = L->nz;
   I->NF = (long long)I->ny * I->nz;
   I->coupling = d->coupling;
+  I->alpha = d->alpha < 0.0 ? 1.0e14 : d->alpha;
   const long long NF = I->NF;
-  constexpr int kIfacePlanes = 36 + 4 + 26 + 6;
+  constexpr int kIfacePlanes = 36 + 5 + 26 + 6;
   hipError_t e = hipMalloc((void**)&I->buf, kIfacePlanes * NF * sizeof(double));
   if (e != hipSuccess) {
     free(I);
@@ -1881,7 +1957,8 @@ extern "C" int wql_iface_create(wql_block* L, wql_block* R, const wql_iface_desc
   I->T0R = I->T0L + 3 * NF; I->fs = I->T0R + 3 * NF; I->fd = I->fs + NF; I->Dc = I->fd + NF;
   I->satL = I->Dc + NF; I->satR = I->satL + 9 * NF;
   I->W = I->satR + 9 * NF; I->DW = I->W + NF; I->rs_a = I->DW + NF; I->rs_nuc = I->rs_a + NF;
-  I->frL = I->rs_nuc + NF; I->frR = I->frL + 13 * NF; I->matL = I->frR + 13 * NF;
+  I->rs_vw = I->rs_nuc + NF;
+  I->frL = I->rs_vw + NF; I->frR = I->frL + 13 * NF; I->matL = I->frR + 13 * NF;
   I->matR = I->matL + 3 * NF;
   auto cleanup = [&](int rc) {
     (void)hipFree(I->buf);
@@ -1891,7 +1968,7 @@ extern "C" int wql_iface_create(wql_block* L, wql_block* R, const wql_iface_desc
     return rc;
   };
   if ((e = launch_fill<double>(I->W, NF, 0.0, L->stream)) != hipSuccess ||
-      (e = launch_fill<double>(I->rs_a, 2 * NF, 0.0, L->stream)) != hipSuccess ||
+      (e = launch_fill<double>(I->rs_a, 3 * NF, 0.0, L->stream)) != hipSuccess ||
       (e = hipStreamSynchronize(L->stream)) != hipSuccess)
     return cleanup(fail(WQL_EHIP, "interface init: %s", hipGetErrorString(e)));
   if (int rc = iface_fill_faces(I)) return cleanup(rc);
@@ -1958,9 +2035,15 @@ static int iface_stage_t(wql_iface* I, double A, double Bdt, double t, int stage
   a.S = I->S; a.DS = I->DS; a.trup = I->trup;
   a.T0L = I->T0L; a.T0R = I->T0R; a.fs = I->fs; a.fd = I->fd; a.Dc = I->Dc;
   a.satL = I->satL; a.satR = I->satR;
-  a.W = I->W; a.DW = I->DW; a.rs_a = I->rs_a; a.rs_nuc = I->rs_nuc;
-  // TPV101 / TPV102 friction constants (Interface_Condition.f90:386-400)
-  a.rs_L0 = 0.02; a.rs_f0 = 0.6; a.rs_b = 0.012; a.rs_V0 = 1.0e-6;
+  a.W = I->W; a.DW = I->DW; a.rs_a = I->rs_a; a.rs_nuc = I->rs_nuc; a.rs_vw = I->rs_vw;
+  if (I->coupling == WQL_IFACE_RATE_STATE_RW) {
+    // rate-weakening friction constants (Interface_Condition.f90:410-416)
+    a.rs_L0 = 2.0 * 0.2572; a.rs_f0 = 0.7; a.rs_b = 0.02; a.rs_V0 = 1.0e-6;
+  } else {
+    // TPV101 / TPV102 friction constants (Interface_Condition.f90:386-400)
+    a.rs_L0 = 0.02; a.rs_f0 = 0.6; a.rs_b = 0.012; a.rs_V0 = 1.0e-6;
+  }
+  a.alpha = I->alpha;
   a.A = A; a.Bdt = Bdt; a.t = t;
   a.stage1 = stage1 ? 1 : 0;
   a.frame = (stage1 && I->frames && I->slot < I->nslots)

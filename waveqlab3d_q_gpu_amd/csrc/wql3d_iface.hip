@@ -14,7 +14,8 @@
 //     prestress and the linear slip-weakening strength of the problem taken
 //     from per-node tables (built at wql_iface_create), or
 //     rate_state_friction (:918-1067) with the regula falsi slip-rate solve
-//     (:1120-1265) and the state rate (:420-436); DS(1:3) += Q - P,
+//     (:1120-1265) and the state rate (:420-444), or Linear_Friction
+//     (:639-742); DS(1:3) += Q - P,
 //     DS(4) += |slip rate|;
 //   rupture_front (CouplingForcing.f90:580-600);
 //   scale_rates_iface / update_fields_iface (iface.f90:58-85).
@@ -49,6 +50,21 @@ __device__ __forceinline__ void hats(int coupling, const double U[6], const doub
     Q[2] = (p_l + phi_l) / Z1s; Q[5] = phi_l;
     P[1] = (q_m - phi_m) / Z2s; P[4] = phi_m;
     Q[1] = (p_m + phi_m) / Z1s; Q[4] = phi_m;
+    return;
+  }
+  if (coupling == WQL_IFACE_LINEAR_FRICTION) {
+    // Linear_Friction (Interface_Condition.f90:639-742), alpha in fd: normal
+    // components as welded, shear tau = alpha/(alpha + eta_s) phi
+    const double alpha = fd;
+    P[0] = (q_n - phi_n) / Z2p; P[3] = phi_n;
+    Q[0] = (p_n + phi_n) / Z1p; Q[3] = phi_n;
+    const double Tm = alpha / (alpha + eta_s) * phi_m, Tl = alpha / (alpha + eta_s) * phi_l;
+    P[4] = Tm; P[5] = Tl;
+    Q[4] = Tm; Q[5] = Tl;
+    Q[1] = (p_m + Tm) / Z1s + 1.0 / (alpha + eta_s) * phi_m;
+    Q[2] = (p_l + Tl) / Z1s + 1.0 / (alpha + eta_s) * phi_l;
+    P[1] = (q_m - Tm) / Z2s - 1.0 / (alpha + eta_s) * phi_m;
+    P[2] = (q_l - Tl) / Z2s - 1.0 / (alpha + eta_s) * phi_l;
     return;
   }
   const double u_n_hat = (q_n - phi_n) / Z2p, Tu_n_hat = phi_n;
@@ -190,6 +206,22 @@ struct Side {
   double lam, mu, rho;
 };
 
+__device__ __forceinline__ bool is_rate_state(int coupling) {
+  return coupling == WQL_IFACE_RATE_STATE_TPV101 || coupling == WQL_IFACE_RATE_STATE_TPV102 ||
+         coupling == WQL_IFACE_RATE_STATE_RW;
+}
+
+// the state rate of problem 'rate-weakening' (Interface_Condition.f90:
+// 438-444): -V/L0 (a asinh(V/(2 V0) e^(psi/a)) - Fss), Fss the steady state
+// f0 - (b - a) ln(V/V0), flash-weakened towards 0.13 above V_w
+__device__ __forceinline__ double rw_state_rate(double W, double Vel, double a, double Vw,
+                                                double L0, double f0, double b, double V0) {
+  double Fss = f0 - (b - a) * log(Vel / V0);
+  const double r = Vel / Vw, r2 = r * r, r4 = r2 * r2;
+  Fss = 0.13 + (Fss - 0.13) / pow(1.0 + r4 * r4, 0.125);
+  return -(Vel / L0 * (a * asinh(Vel / (2.0 * V0) * exp(W / a)) - Fss));
+}
+
 // one evaluation (the calling block's frame and metric column): hats,
 // slip-rate magnitude and the calling block's SAT forcing bc[9]
 template <typename T, typename S>
@@ -225,11 +257,11 @@ __device__ __forceinline__ void evaluate(const IfaceArgs<T, S>& a, int ib, const
   const double c2p = sqrt((2.0 * s2.mu + s2.lam) / s2.rho), c2s = sqrt(s2.mu / s2.rho);
   const double Z1s = s1.rho * c1s, Z2s = s2.rho * c2s, Z1p = s1.rho * c1p, Z2p = s2.rho * c2p;
   *Vel = 0.0;
-  if (a.coupling == WQL_IFACE_RATE_STATE_TPV101 || a.coupling == WQL_IFACE_RATE_STATE_TPV102) {
+  if (is_rate_state(a.coupling)) {
     hats_rate_state(U, V, Z1s, Z2s, Z1p, Z2p, T0, psi, a.rs_a[fi], a.rs_V0, P, Q, Vel);
   } else {
-    double fs = 0.0, fd = 0.0, Dc = 1.0;
-    if (a.coupling != WQL_IFACE_LOCKED) {
+    double fs = 0.0, fd = a.alpha, Dc = 1.0;
+    if (a.coupling != WQL_IFACE_LOCKED && a.coupling != WQL_IFACE_LINEAR_FRICTION) {
       fs = a.fs[fi];
       fd = a.fd[fi];
       Dc = a.Dc[fi];
@@ -353,19 +385,25 @@ __global__ __launch_bounds__(256) void k_iface(const IfaceArgs<T, S> a) {
   s1.lam = a.matL[fi]; s1.mu = a.matL[NF + fi]; s1.rho = a.matL[2 * NF + fi];
   s2.lam = a.matR[fi]; s2.mu = a.matR[NF + fi]; s2.rho = a.matR[2 * NF + fi];
   const double slip = a.S[3 * NF + fi];
-  const bool rate_state =
-      a.coupling == WQL_IFACE_RATE_STATE_TPV101 || a.coupling == WQL_IFACE_RATE_STATE_TPV102;
+  const bool rate_state = is_rate_state(a.coupling);
   const double psi = rate_state ? a.W[fi] : 0.0;
   // prestress of each side (zero for a welded interface, whose output
-  // carries no prestress, Interface_Condition.f90:482-486)
+  // carries no prestress, Interface_Condition.f90:482-486, and for a linear
+  // friction interface without tables)
   double T0L[3] = {0.0, 0.0, 0.0}, T0R[3] = {0.0, 0.0, 0.0};
-  if (a.coupling != WQL_IFACE_LOCKED) {
+  if (a.coupling != WQL_IFACE_LOCKED && a.T0L) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       T0L[c] = a.T0L[c * NF + fi];
       T0R[c] = a.T0R[c * NF + fi];
     }
-    if (rate_state) {
+    if (a.coupling == WQL_IFACE_RATE_STATE_RW) {
+      // Gaussian pulse 2 tau_pulse F(r) G(t) on T0_l, tau_pulse = -0.2429 T0_n,
+      // G = 1 from t > 0 (Interface_Condition.f90:2408-2424)
+      const double G = a.t > 0.0 ? 1.0 : 0.0;
+      T0L[2] = T0L[2] + 2.0 * (-0.2429 * T0L[0]) * a.rs_nuc[fi] * G;
+      T0R[2] = T0R[2] + 2.0 * (-0.2429 * T0R[0]) * a.rs_nuc[fi] * G;
+    } else if (rate_state) {
       // nucleation 25 F(r) G(t) on T0_l (Interface_Condition.f90:2340-2347)
       const double t = a.t;
       double G = 0.0;
@@ -381,7 +419,10 @@ __global__ __launch_bounds__(256) void k_iface(const IfaceArgs<T, S> a) {
   // block L, side 2: SAT into L, the slip-rate increment
   evaluate(a, 2, a.frL, fi, NF, u, v, s1, s2, slip, psi, T0L, P, Q, &srate, &vel, bc);
   double dw = 0.0;
-  if (rate_state)   // the state rate of TPV101 / TPV102 (Interface_Condition.f90:431-436)
+  if (a.coupling == WQL_IFACE_RATE_STATE_RW)
+    dw = a.A * a.DW[fi] + rw_state_rate(psi, vel, a.rs_a[fi], a.rs_vw[fi], a.rs_L0, a.rs_f0,
+                                        a.rs_b, a.rs_V0);
+  else if (rate_state)   // the state rate of TPV101 / TPV102 (Interface_Condition.f90:431-436)
     dw = a.A * a.DW[fi] + (a.rs_b * a.rs_V0 / a.rs_L0 * exp(-(psi - a.rs_f0) / a.rs_b) -
                            vel * a.rs_b / a.rs_L0);
   double ds[4];

@@ -197,14 +197,19 @@ struct IfaceArgs {
   const double* Dc;
   double* satL;            // 9 planes each: tau0/hq * SAT forcing
   double* satR;
-  // rate-and-state friction (TPV101 / TPV102): state psi and its rate, the
-  // parameter a and the nucleation amplitude 25 F(r) per node, and the
-  // constants L0, f0, b, V0
+  // rate-and-state friction (TPV101 / TPV102 / rate-weakening): state psi
+  // and its rate, the parameter a and the nucleation amplitude per node
+  // (TPV101: 25 F(r); rate-weakening: 2 F(r), times its -0.2429 T0_n), and
+  // the constants L0, f0, b, V0
   double* W;
   double* DW;
   const double* rs_a;
   const double* rs_nuc;
+  // rate-weakening: the flash-weakening velocity V_w per node (else null)
+  const double* rs_vw;
   double rs_L0, rs_f0, rs_b, rs_V0;
+  // linear friction: tau = alpha v
+  double alpha;
   double A, Bdt, t;
   int stage1;
   double* frame;           // null, or kFramePlanes planes of the current output slot

@@ -21,7 +21,8 @@ module wql3d_hip_binding
        WQL_MEMORY = 16, WQL_QINV = 32, WQL_PML = 64, WQL_FACES = 128, WQL_ALL = 255
   integer(c_int32_t), parameter :: WQL_IFACE_LOCKED = 0, WQL_IFACE_SLIP_WEAKENING_TPV5 = 1, &
        WQL_IFACE_SLIP_WEAKENING_TPV28 = 2, WQL_IFACE_SLIP_WEAKENING = 3, &
-       WQL_IFACE_RATE_STATE_TPV101 = 4, WQL_IFACE_RATE_STATE_TPV102 = 5
+       WQL_IFACE_RATE_STATE_TPV101 = 4, WQL_IFACE_RATE_STATE_TPV102 = 5, &
+       WQL_IFACE_LINEAR_FRICTION = 6, WQL_IFACE_RATE_STATE_RW = 7
 
   type, bind(C) :: wql_box
      integer(c_int32_t) :: lo(3), hi(3)
@@ -51,6 +52,7 @@ module wql3d_hip_binding
   type, bind(C) :: wql_iface_desc
      integer(c_int32_t) :: coupling, arrays_on_device
      type(c_ptr)        :: y, z, s0, T0L, T0R, fs, fd, Dc, W0
+     real(c_double)     :: alpha
   end type wql_iface_desc
 
   interface

@@ -32,17 +32,22 @@ class Interface:
     (xx, yy, zz, xy, xz, yz), default the source's), or
     ``WQL_IFACE_SLIP_WEAKENING`` with ``tables`` = dict of host arrays T0L,
     T0R (3, nz, ny) (prestress in each side's (n, m, l) face frame) and fs,
-    fd, Dc (nz, ny), or ``WQL_IFACE_RATE_STATE_TPV101`` / ``_TPV102`` with
-    the initial state ``W0`` (nz, ny) (default 0, as init_iface sets it).
+    fd, Dc (nz, ny), or ``WQL_IFACE_RATE_STATE_TPV101`` / ``_TPV102`` /
+    ``_RW`` (problem 'rate-weakening') with the initial state ``W0`` (nz, ny)
+    (default 0, as init_iface sets it), or ``WQL_IFACE_LINEAR_FRICTION``
+    with ``alpha`` (tau = alpha v; None = the reference's 1e14) and
+    optionally ``tables`` = dict with T0L, T0R, the prestress the fault
+    output adds.
 
     The blocks must outlive the interface; call :meth:`close` first."""
 
     def __init__(self, left: Block, right: Block, y, z, coupling=L.WQL_IFACE_SLIP_WEAKENING_TPV5,
-                 s0=None, tables=None, W0=None):
+                 s0=None, tables=None, W0=None, alpha=None):
         self.left, self.right = left, right
         self.ny, self.nz = left.nr, left.ns
         d = L.IfaceDesc()
         d.coupling = int(coupling)
+        d.alpha = -1.0 if alpha is None else float(alpha)
         on_dev = not isinstance(y, np.ndarray)
         if not on_dev:
             y = np.ascontiguousarray(y, dtype=np.float64)
@@ -60,6 +65,8 @@ class Interface:
             if on_dev:
                 raise ValueError("tables are host arrays; pass host y, z with them")
             for name, shape in (("T0L", (3,)), ("T0R", (3,)), ("fs", ()), ("fd", ()), ("Dc", ())):
+                if name not in tables and coupling == L.WQL_IFACE_LINEAR_FRICTION:
+                    continue
                 a = np.ascontiguousarray(tables[name], dtype=np.float64).reshape(
                     shape + (self.nz, self.ny))
                 keep.append(a)

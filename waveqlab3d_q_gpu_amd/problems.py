@@ -393,6 +393,7 @@ class TwoBlockProblem:
     name: str = ""
     s0: tuple = None               # TPV28 initial stress (None: the source's values)
     W0: np.ndarray = None          # rate-and-state initial state (None: 0)
+    alpha: float = None            # linear friction coefficient (None: the reference's 1e14)
 
 
 def tpv5_cartesian(n=21, sat_form=1, coupling=None):
@@ -513,6 +514,46 @@ def tpv101_cartesian(n=21, problem=101, nsteps=None, t_final=1.5):
     tp = TwoBlockProblem(left=left, right=right, coupling=coupling, yf=yf, zf=zf, dt=dt,
                          nsteps=nsteps, name=f"tpv{problem}_cartesian_{n}")
     tp.W0 = W0
+    return tp
+
+
+def rate_weakening_cartesian(n=21, nsteps=None, t_final=0.6):
+    """Rate-and-state friction, problem 'rate-weakening' (Interface_Condition.
+    f90:410-416, 438-444, 2387-2426), on two n^3 Cartesian blocks
+    [-3, 0] / [0, 3] x [6.5, 9.5] x [-1.5, 1.5] km around the pulse centre
+    (y, z) = (8, 0), rho/cs/cp = 2.67/3.464/6, characteristic SAT outside and
+    a free surface on y-min, CFL 0.5.  Initial state psi = 0.4367 and zero
+    particle velocity (Init_Vel_State, :2822-2845); the prestress pulse is on
+    from t > 0."""
+    nn = (n, n, n)
+    XL = cartesian_grid(nn, (-3.0, 6.5, -1.5), (0.0, 9.5, 1.5))
+    XR = cartesian_grid(nn, (0.0, 6.5, -1.5), (3.0, 9.5, 1.5))
+    M = material_from_rho_s_p(nn, 2.67, 3.464, 6.0)
+    dt = block_time_step(3.0 / float(n - 1), 0.5, 3.464, 6.0)
+    if nsteps is None:
+        nsteps = int(math.floor(t_final / dt))
+    yf = np.ascontiguousarray(XL[1][:, :, -1])
+    zf = np.ascontiguousarray(XL[2][:, :, -1])
+    left = Problem(n=nn, mesh=L.WQL_CARTESIAN, X=XL, M=M.copy(), bc=(1, 0, 2, 1, 1, 1), dt=dt,
+                   nsteps=nsteps, name="rw_block1")
+    right = Problem(n=nn, mesh=L.WQL_CARTESIAN, X=XR, M=M.copy(), bc=(0, 1, 2, 1, 1, 1), dt=dt,
+                    nsteps=nsteps, name="rw_block2")
+    tp = TwoBlockProblem(left=left, right=right, coupling=L.WQL_IFACE_RATE_STATE_RW, yf=yf,
+                         zf=zf, dt=dt, nsteps=nsteps, name=f"rate_weakening_cartesian_{n}")
+    tp.W0 = np.full(yf.shape, 0.4367)
+    return tp
+
+
+def linear_friction_curvilinear(n=24, alpha=5.0, nmech=4, npml=6, nsteps=30):
+    """Linear friction tau = alpha v (coupling 'linear_friction',
+    Interface_Condition.f90:360-366, 639-742) across the curved interface of
+    two_block_curvilinear: the Gaussian pulse of block L crosses it with slip
+    (alpha in MPa s/m; the shear impedance harmonic mean of the two sides is
+    about 4.7)."""
+    tp = two_block_curvilinear(n=n, nmech=nmech, npml=npml, coupling=L.WQL_IFACE_LINEAR_FRICTION,
+                               nsteps=nsteps)
+    tp.alpha = alpha
+    tp.name = f"linear_friction_curvilinear_{n}_alpha{alpha:g}"
     return tp
 
 
