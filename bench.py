@@ -63,6 +63,11 @@ def slab_nodes(prob):
     return tot
 
 
+# share of the stages of the KC(5,4) low-storage scheme with A = 0 (the
+# first of five), which read no old rate
+FIRST_STAGE_SHARE = 1 / 5
+
+
 def byte_model(prob, precision):
     """Algorithmic HBM bytes of one stage.
 
@@ -81,7 +86,12 @@ def byte_model(prob, precision):
     write, and the six stress components' F read and write and DF read and
     write.  FP32-mixed: the stencils read F rounded to float
     (9 x 4 B) plus the velocity components' FP64 F (3 x 8 B), and every F
-    write also writes the float copy (4 B per component)."""
+    write also writes the float copy (4 B per component).
+
+    The first of the five stages has A = 0 and reads no old rate (DF, Deta,
+    PML DQ): every figure is the average over the five stages, i.e. the
+    full-stage count less a fifth of the old-rate reads (fields
+    "rate_reads" and "first_stage_share")."""
     nq, nr, ns = prob.n
     P = nq * nr * ns
     es = 8 if precision == 0 else 4
@@ -119,13 +129,27 @@ def byte_model(prob, precision):
         "memory_update": ((P * (6 * es + 4 * es + 24 * na * es + 6 * state + (6 * 4 if opcopy else 0))
                            + (P - n_int) * 6 * es) if prob.nmech else 0),
     }
-    # the same count with every mechanism's eta and Deta streamed (the
-    # per-point figure quoted for the workload, 1116 B FP32-mixed Q8 before
-    # PML), whatever the mechanisms' weights
+    # old-rate reads that the first stage (A = 0) skips: DF (ss per
+    # component), Deta (6 per active mechanism) and PML DQ (9 per slab node)
+    vel_rd = 3 * ss if prob.nmech else 9 * ss
+    rate_reads = {
+        "interior": n_int * vel_rd,
+        "shell": (P - n_int) * vel_rd + slab * 9 * es,
+        "memory_update": P * (6 * ss + 6 * na * es) if prob.nmech else 0,
+    }
+    rate_reads_e2e = P * (9 * ss + (6 * na * es if prob.nmech else 0)) + slab * 9 * es
+    share = FIRST_STAGE_SHARE
+    kernels = {k: int(round(v - share * rate_reads[k])) for k, v in kernels.items()}
+    end_to_end = int(round(end_to_end - share * rate_reads_e2e))
+    # the same count with every mechanism's eta and Deta streamed at every
+    # stage (the per-point figure quoted for the workload, 1116 B FP32-mixed
+    # Q8 before PML), whatever the mechanisms' weights
     all_mech = (P * (9 * state + (ngeo + 3) * es + ((24 * prob.nmech + 2) * es if prob.nmech else 0))
                 + slab * 36 * es)
     return {"P": P, "interior_nodes": n_int, "active_mechanisms": na,
-            "end_to_end": end_to_end, "end_to_end_all_mechanisms": all_mech, "kernels": kernels}
+            "end_to_end": end_to_end, "end_to_end_all_mechanisms": all_mech, "kernels": kernels,
+            "rate_reads": dict(rate_reads, end_to_end=rate_reads_e2e),
+            "first_stage_share": share}
 
 
 def near_source_receivers(prob, offsets=((4, 0, 0), (-4, 0, 0), (0, 4, 0), (0, -4, 0),

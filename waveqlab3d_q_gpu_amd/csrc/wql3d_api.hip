@@ -1185,7 +1185,9 @@ static StepArgs<T, S> make_args(wql_block* b, double t) {
 template <typename T, typename S>
 static hipError_t arena_probe_launch(wql_block* b) {
   StepArgs<T, S> a = make_args<T, S>(b, 0.0);
-  a.A = 0.0;
+  // A != 0: the variant of stages 2..5, which also reads the old rates
+  // (all zero here, so every plane stays zero)
+  a.A = 1.0;
   a.Bdt = 0.0;
   return launch_stage_part_z<T, S>(a, b->d.mesh, StagePart::kMemory, 0, b->nz, b->stream);
 }

@@ -468,6 +468,16 @@ __device__ __forceinline__ void st_stream(V* p, V v) {
   else *p = v;
 }
 
+// Old rate register of the low-storage RK scheme, to be scaled by A.  The
+// first stage has A = 0 and scale_rate maps any old rate to 0 there, so the
+// register is not read at all (a uniform branch on a kernel argument): that
+// stage moves no old-rate bytes.
+template <bool NT, typename V>
+__device__ __forceinline__ V ld_rate(double A, const V* p) {
+  if (A == 0.0) return V(0);
+  return ld_stream<NT>(p);
+}
+
 // DF = df and the RK state update F_out = F_in + Bdt*DF of component c at
 // node idu: FP64 state, or FP32 state with a Kahan-compensated sum whose
 // compensation planes Fc are updated in place.

@@ -201,7 +201,7 @@ __device__ __forceinline__ void node_epilogue(const StepArgs<T, S>& a, int x, in
         T* De = atp(a.Deta + (long long)(c * NMECH + i) * P, idu);
         T* Et = atp(a.eta + (long long)(c * NMECH + i) * P, idu);
         if (MODE == 0) {
-          const T dn = scale_rate(a.A, ld_stream<NT>(De)) + r;
+          const T dn = scale_rate(a.A, ld_rate<NT>(a.A, De)) + r;
           st_stream<NT>(De, dn);
           st_stream<NT>(Et, ev[i] + (T)a.Bdt * dn);
         } else {
@@ -298,7 +298,7 @@ __device__ __forceinline__ void node_epilogue(const StepArgs<T, S>& a, int x, in
         T r = s[c] - (d + cfs) * q[c];
         if (fsat) r -= pen * sat[c];
         if (FUSED) {
-          const T dn = scale_rate(a.A, ld_stream<NT>(DQ + c * Ps + si)) + r;
+          const T dn = scale_rate(a.A, ld_rate<NT>(a.A, DQ + c * Ps + si)) + r;
           st_stream<NT>(DQ + c * Ps + si, dn);
           st_stream<NT>(Q + c * Ps + si, q[c] + (T)a.Bdt * dn);
         } else {
@@ -313,7 +313,7 @@ __device__ __forceinline__ void node_epilogue(const StepArgs<T, S>& a, int x, in
     const bool has_src = a.nsrc > 0 && source_at<T, MESH>(a, x, y, z, id, srcv);
 #pragma unroll
     for (int c = 0; c < 9; ++c) {
-      double df = scale_rate(a.A, ld_stream<NT>(atp(a.DF + c * P, idu))) + (double)DU[c];
+      double df = scale_rate(a.A, ld_rate<NT>(a.A, atp(a.DF + c * P, idu))) + (double)DU[c];
       if (has_src && c >= 3) df -= srcv[c - 3];
       if (has_sat) df -= satDF[c];
       rk_finish<NT>(a, c, idu, df);
@@ -330,7 +330,7 @@ __device__ __forceinline__ void node_epilogue(const StepArgs<T, S>& a, int x, in
       if (has_src && c >= 3) r -= (T)srcv[c - 3];
       if (has_sat) r -= (T)satDF[c];
       if (c < 3) {
-        rk_finish<NT>(a, c, idu, scale_rate(a.A, ld_stream<NT>(atp(a.DF + c * P, idu))) + (double)r);
+        rk_finish<NT>(a, c, idu, scale_rate(a.A, ld_rate<NT>(a.A, atp(a.DF + c * P, idu))) + (double)r);
       } else {
         st_stream<NT>(atp(a.W + (c - 3) * P, idu), r);
       }
@@ -378,7 +378,7 @@ __device__ __forceinline__ void memory_update_planes(T* __restrict__ eta, T* __r
     for (int i = 0; i < NMECH; ++i) {
       if (mech_skip & (1 << i)) { ev[i] = T(0); dv[i] = T(0); continue; }
       ev[i] = ld_stream<NT>(atp(eta + (long long)(c * NMECH + i) * P, id));
-      dv[i] = ld_stream<NT>(atp(Deta + (long long)(c * NMECH + i) * P, id));
+      dv[i] = ld_rate<NT>(A, atp(Deta + (long long)(c * NMECH + i) * P, id));
       sum += ev[i];
     }
     DU[3 + c] -= sum;
@@ -458,7 +458,7 @@ __global__ __launch_bounds__(256) void k_memory_update(const StepArgs<T, S> a) {
   memory_update_node<T, NMECH, NT>(a, id, mu, l2m, e, DU);
 #pragma unroll
   for (int c = 3; c < 9; ++c) {
-    const double df = scale_rate(a.A, ld_stream<NT>(atp(a.DF + c * P, id))) + (double)DU[c];
+    const double df = scale_rate(a.A, ld_rate<NT>(a.A, atp(a.DF + c * P, id))) + (double)DU[c];
     rk_finish<NT>(a, c, id, df);
   }
 }
@@ -500,7 +500,7 @@ __device__ __forceinline__ void stn(V* base, unsigned id, const V (&v)[N]) {
   else *p = t;
 }
 
-template <typename T, int NMECH, int N, bool NT>
+template <typename T, int NMECH, int N, bool NT, bool RD>
 __device__ __forceinline__ void memory_update_planes_n(
     T* __restrict__ eta, T* __restrict__ Deta, const T* __restrict__ Qsp, const T* __restrict__ Qpp,
     long long P, int mech_skip, const double* wgt, const double* tau_inv, double A, double Bdt,
@@ -530,7 +530,10 @@ __device__ __forceinline__ void memory_update_planes_n(
         continue;
       }
       ldn<N, NT>(eta + (long long)(c * NMECH + i) * P, id, ev[i]);
-      ldn<N, NT>(Deta + (long long)(c * NMECH + i) * P, id, dv[i]);
+      if constexpr (RD) ldn<N, NT>(Deta + (long long)(c * NMECH + i) * P, id, dv[i]);
+      else
+#pragma unroll
+        for (int n = 0; n < N; ++n) dv[i][n] = T(0);
 #pragma unroll
       for (int n = 0; n < N; ++n) sum[n] += ev[i][n];
     }
@@ -556,7 +559,9 @@ __device__ __forceinline__ void memory_update_planes_n(
   }
 }
 
-template <typename T, int NMECH, int N, bool NT, typename S>
+// RD = false at the first RK stage (A = 0): the old rates Deta and DF are
+// not read (scale_rate maps them to 0 there whatever they hold).
+template <typename T, int NMECH, int N, bool NT, typename S, bool RD>
 __global__ __launch_bounds__(256) void k_memory_update_n(const StepArgs<T, S> a) {
   const unsigned id = (unsigned)a.node0 + (blockIdx.x * 256u + threadIdx.x) * (unsigned)N;
   const long long P = a.P;
@@ -596,12 +601,15 @@ __global__ __launch_bounds__(256) void k_memory_update_n(const StepArgs<T, S> a)
 #pragma unroll
   for (int n = 0; n < N; ++n)
     if (hk[n]) hooke_rates(lam[n], l2m[n], mu[n], e[n], DU[n]);
-  memory_update_planes_n<T, NMECH, N, NT>(a.eta, a.Deta, a.Qs, a.Qp, P, a.mech_skip, a.wgt,
+  memory_update_planes_n<T, NMECH, N, NT, RD>(a.eta, a.Deta, a.Qs, a.Qp, P, a.mech_skip, a.wgt,
                                       a.tau_inv, a.A, a.Bdt, id, mu, l2m, e, DU);
 #pragma unroll
   for (int c = 3; c < 9; ++c) {
     S dfo[N], fin[N];
-    ldn<N, NT>(a.DF + c * P, id, dfo);
+    if constexpr (RD) ldn<N, NT>(a.DF + c * P, id, dfo);
+    else
+#pragma unroll
+      for (int n = 0; n < N; ++n) dfo[n] = S(0);
     ldn<N, NT>(a.Fin + c * P, id, fin);
     if constexpr (std::is_same<S, double>::value) {
       double df[N], fo[N];
@@ -792,7 +800,7 @@ void k_interior(const StepArgs<T, S> a) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
           pre_f[c] = ldp(a.Fin + c * P, p);
-          pre_df[c] = ld_stream<NT>(atp(a.DF + c * P, p));
+          pre_df[c] = ld_rate<NT>(a.A, atp(a.DF + c * P, p));
           if constexpr (std::is_same<S, float>::value) pre_c[c] = ld_stream<NT>(atp(a.Fc + c * P, p));
           else pre_c[c] = S(0);
         }
@@ -977,7 +985,7 @@ void k_interior(const StepArgs<T, S> a) {
         }
 #pragma unroll
         for (int c = 0; c < 9; ++c) {
-          double df = scale_rate(a.A, ld_stream<NT>(atp(a.DF + c * P, idu))) + (double)DU[c];
+          double df = scale_rate(a.A, ld_rate<NT>(a.A, atp(a.DF + c * P, idu))) + (double)DU[c];
           if (has_src && c >= 3) df -= srcv[c - 3];
           rk_finish<NT>(a, c, idu, df);
         }
@@ -1186,20 +1194,26 @@ static hipError_t launch_node(const StepArgs<T, S>& a, int mesh, hipStream_t s) 
   return launch_node_nt<T, MODE, false>(a, mesh, s);
 }
 
+template <typename T, int NM, int NP, bool NT, typename S>
+static void launch_memory_update_n(const StepArgs<T, S>& a, dim3 g, hipStream_t s) {
+  if (a.A != 0.0) hipLaunchKernelGGL((k_memory_update_n<T, NM, NP, NT, S, true>), g, dim3(256), 0, s, a);
+  else hipLaunchKernelGGL((k_memory_update_n<T, NM, NP, NT, S, false>), g, dim3(256), 0, s, a);
+}
+
 template <typename T, bool NT, typename S>
 static hipError_t launch_memory_update(const StepArgs<T, S>& a, hipStream_t s) {
   if (a.node1 <= a.node0) return hipSuccess;
   if (a.mem_npt == 4 && a.node0 % 4 == 0 && a.node1 % 4 == 0) {
     const dim3 g((unsigned)((a.node1 - a.node0 + 1023) / 1024));
-    if (a.nmech == 4) hipLaunchKernelGGL((k_memory_update_n<T, 4, 4, NT, S>), g, dim3(256), 0, s, a);
-    else if (a.nmech == 8) hipLaunchKernelGGL((k_memory_update_n<T, 8, 4, NT, S>), g, dim3(256), 0, s, a);
+    if (a.nmech == 4) launch_memory_update_n<T, 4, 4, NT>(a, g, s);
+    else if (a.nmech == 8) launch_memory_update_n<T, 8, 4, NT>(a, g, s);
     else return hipErrorInvalidValue;
     return hipGetLastError();
   }
   if (a.mem_npt >= 2 && a.node0 % 2 == 0 && a.node1 % 2 == 0) {
     const dim3 g((unsigned)((a.node1 - a.node0 + 511) / 512));
-    if (a.nmech == 4) hipLaunchKernelGGL((k_memory_update_n<T, 4, 2, NT, S>), g, dim3(256), 0, s, a);
-    else if (a.nmech == 8) hipLaunchKernelGGL((k_memory_update_n<T, 8, 2, NT, S>), g, dim3(256), 0, s, a);
+    if (a.nmech == 4) launch_memory_update_n<T, 4, 2, NT>(a, g, s);
+    else if (a.nmech == 8) launch_memory_update_n<T, 8, 2, NT>(a, g, s);
     else return hipErrorInvalidValue;
     return hipGetLastError();
   }
