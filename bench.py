@@ -221,6 +221,39 @@ def cpu_baseline(n_sample, n_single, nmech, npml, seed):
                               "seconds": round(el1, 3)}}
 
 
+def stream_ceiling(gib=4, reps=10):
+    """The rate that plain streaming reaches on this GPU, for reading the
+    dominant kernel's fraction against what is attainable rather than the
+    nominal 8 TB/s: the library's own streaming kernels (k_scale: one read
+    and one write per value; k_update: two reads and one write) over two
+    `gib` GiB arrays of doubles, timed with events on the default stream,
+    which these entry points launch on and synchronize after each call."""
+    import torch
+    from waveqlab3d_q_gpu_amd import scale_rate_array, update_state_array
+    n = gib * 2 ** 30 // 8
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.ones(n, dtype=torch.float64, device="cuda")
+
+    def rate(fn, nbytes):
+        fn()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        e1.synchronize()
+        return nbytes / (e0.elapsed_time(e1) / reps / 1e3) / 1e12
+
+    sc = rate(lambda: scale_rate_array(x, 1.0), 16 * n)
+    up = rate(lambda: update_state_array(y, x, 0.0), 24 * n)
+    del x, y
+    torch.cuda.empty_cache()
+    return {"k_scale_TBs": round(sc, 3), "k_update_TBs": round(up, 3), "TBs": round(max(sc, up), 3),
+            "arrays": f"2 x {gib} GiB of doubles, {reps} launches each"}
+
+
 def read_traffic(name):
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
@@ -567,6 +600,9 @@ def main():
                                      "bytes_per_node": round(r32["stage_bytes_all_mechanisms"] / P, 1),
                                      "frac": round(r32["stage_bytes_all_mechanisms"]
                                                    / (r32["stage_ms"] / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}}}
+    ceil = stream_ceiling()
+    roof["stream_ceiling"] = ceil
+    roof["frac_of_stream_ceiling"] = round(k32[dom]["GBs"] / 1e3 / ceil["TBs"], 4)
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_sample, args.cpu_sample_single, args.nmech, args.npml, 2024)
     out = {
         "metric": METRIC, "value": round(r32["value"], 2), "unit": "Mpt-RK-stage/s",
