@@ -364,6 +364,11 @@ int wql_iface_get_friction_state(wql_iface *I, double *W);
  * Dense arrays below are DEVICE pointers, node order x fastest, no ghosts. */
 /* adjoint state, rate adjoints, gradients (zeroed) and nslots state slots */
 int wql_adjoint_create(wql_block *b, int32_t nslots);
+/* the same with nslots_f more slots, numbered nslots .. nslots+nslots_f-1,
+ * that hold F only (the memory a slot of F needs, without eta): for the
+ * stage inputs of a checkpoint segment, which a reverse stage reads.
+ * wql_adjoint_create(b, n) is wql_adjoint_create_f(b, n, 0). */
+int wql_adjoint_create_f(wql_block *b, int32_t nslots, int32_t nslots_f);
 /* ub = rb = 0, g = 0 */
 int wql_adjoint_reset(wql_block *b);
 /* copy the current state (F, eta; under FP32-Kahan also F's compensation)
@@ -371,8 +376,9 @@ int wql_adjoint_reset(wql_block *b);
 int wql_state_save(wql_block *b, int32_t slot);
 int wql_state_load(wql_block *b, int32_t slot);
 /* the same for part of the state: mask WQL_F (F and, under FP32-Kahan, its
- * compensation) and/or WQL_MEMORY (eta).  A reverse stage reads only F of its
- * input state, so the stage inputs of a step need only F. */
+ * compensation) and/or WQL_MEMORY (eta; full slots only).  A reverse stage
+ * reads only F of its input state, so the stage inputs of a step need only
+ * F. */
 int wql_state_save_part(wql_block *b, int32_t slot, uint32_t mask);
 int wql_state_load_part(wql_block *b, int32_t slot, uint32_t mask);
 /* reverse one stage with RK coefficients A (a_s) and Bdt (b_s * dt) */

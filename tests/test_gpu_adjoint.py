@@ -195,6 +195,38 @@ def test_checkpoint_reload_steps_bit_identically(precision):
     assert moved > 0 and all(same)
 
 
+def test_f_only_slots():
+    """wql_adjoint_create_f: the F-only slots after the full ones take and
+    give back F (with eta from a full slot, a reloaded stage input repeats
+    the stage bit for bit) and refuse the memory variables; slots past the
+    last are refused."""
+    import torch
+    prob = _problem("curv_q8", n=24, nsteps=3)
+    blk = PB.make_device_block(prob, precision=L.WQL_FP64, state_mirror=True)
+    lib, h = L.lib(), blk._h
+    L.check(lib.wql_adjoint_create_f(h, 1, 2))
+    blk.time_step(prob.dt, 2, 0.0)
+    L.check(lib.wql_state_save_part(h, 0, L.WQL_MEMORY))
+    L.check(lib.wql_state_save_part(h, 2, L.WQL_F))
+    assert lib.wql_state_save_part(h, 2, L.WQL_MEMORY) == L.WQL_EINVAL
+    assert lib.wql_state_load(h, 1) == L.WQL_EINVAL
+    assert lib.wql_state_save_part(h, 3, L.WQL_F) == L.WQL_EINVAL
+    # a first stage (A = 0 reads no old rate) from the saved state, twice
+    t = 2 * prob.dt
+    L.check(lib.wql_stage_fused(h, 0.0, 0.1 * prob.dt, t))
+    blk.download(L.WQL_F)
+    torch.cuda.synchronize()
+    first = blk.F.clone()
+    L.check(lib.wql_state_load_part(h, 0, L.WQL_MEMORY))
+    L.check(lib.wql_state_load_part(h, 2, L.WQL_F))
+    L.check(lib.wql_stage_fused(h, 0.0, 0.1 * prob.dt, t))
+    blk.download(L.WQL_F)
+    torch.cuda.synchronize()
+    same = bool(torch.equal(first, blk.F))
+    blk.close()
+    assert same and float(torch.abs(first).max()) > 0
+
+
 def test_config4_adjoint_gradient_129():
     """The 129^3 Q4, 400-step workload with checkpoints every 20 steps: the
     loss is the L2 velocity misfit at 64 surface receivers against synthetic

@@ -107,6 +107,7 @@ SIGNATURES = {
     "wql_block_set_timing": (C.c_int, [_P, C.c_int32]),
     "wql_block_kernel_times": (C.c_int, [_P, _D, C.POINTER(C.c_int32)]),
     "wql_adjoint_create": (C.c_int, [_P, C.c_int32]),
+    "wql_adjoint_create_f": (C.c_int, [_P, C.c_int32, C.c_int32]),
     "wql_adjoint_reset": (C.c_int, [_P]),
     "wql_state_save": (C.c_int, [_P, C.c_int32]),
     "wql_state_load": (C.c_int, [_P, C.c_int32]),

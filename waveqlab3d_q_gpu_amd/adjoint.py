@@ -8,13 +8,13 @@ library's ``wql_time_step``.
 
 Memory: checkpoint-and-recompute.  The forward run keeps the state (F, eta) at
 every C-th step start (C = ceil(sqrt(N)) by default) in library-held slots.
-The reverse sweep walks the segments backwards: it recomputes the step starts
-of one segment from its checkpoint, then for each step (last first) recomputes
-the five stage inputs and reverses the five stages; the adjoint of the step's
-receiver sample (taken from the step's input state) is added after the five
-stages of that step have been reversed.  Slots:
-ceil(N/C) checkpoints + C step starts + 5 stage inputs; forward work 3N steps
-in total.
+The reverse sweep walks the segments backwards: it recomputes one segment from
+its checkpoint stage by stage, keeping F of every stage input (a reverse stage
+reads F of its input state only), then for each step (last first) reverses
+the five stages from those inputs; the adjoint of the step's receiver sample
+(taken from the step's input state) is added after the five stages of that
+step have been reversed.  Slots: ceil(N/C) checkpoints (F and eta) and 5C
+stage inputs (F only); forward work 2N steps in total.
 """
 from __future__ import annotations
 
@@ -46,12 +46,11 @@ class AdjointRunner:
         self.t0 = float(t0)
         self.C = int(every) if every else max(1, int(math.ceil(math.sqrt(self.N))))
         self.nck = (self.N + self.C - 1) // self.C
-        self.s_seg = self.nck
-        self.s_stage = self.nck + self.C
+        self.s_stage = self.nck          # first F-only slot: stage inputs
         self.receivers = np.asarray(receivers)
         self.nrec = self.receivers.shape[0]
         self.ts = None
-        check(lib().wql_adjoint_create(blk._h, self.s_stage + 5))
+        check(lib().wql_adjoint_create_f(blk._h, self.nck, 5 * self.C))
 
     def set_material(self, rho, lam, mu):
         """Per-node material (torch float64 tensors (nz, ny, nx) on the GPU)."""
@@ -78,16 +77,22 @@ class AdjointRunner:
         return torch.from_numpy(blk.seismograms()).to(blk.M.device)
 
     # -- reverse sweep -------------------------------------------------------
-    def _stage_adjoint(self, n):
-        """Reverse the five stages of step n (its start state is current).
-        A reverse stage reads F of its input state only, so the stage inputs
-        are kept without eta."""
-        h, dt, t = self.blk._h, self.dt, self.ts[n]
-        for s in range(5):
-            check(lib().wql_state_save_part(h, self.s_stage + s, L.WQL_F))
-            check(lib().wql_stage_fused(h, RK_A[s], RK_B[s] * dt, t + RK_C[s] * dt))
+    def _segment_stage_inputs(self, n0, n1):
+        """Recompute steps n0 .. n1-1 from the current state (the segment's
+        checkpoint), keeping F of each stage input in the F-only slots (the
+        same fused stages, coefficients and stage times as wql_time_step)."""
+        h, dt = self.blk._h, self.dt
+        for n in range(n0, n1):
+            t = self.ts[n]
+            for s in range(5):
+                check(lib().wql_state_save_part(h, self.s_stage + 5 * (n - n0) + s, L.WQL_F))
+                check(lib().wql_stage_fused(h, RK_A[s], RK_B[s] * dt, t + RK_C[s] * dt))
+
+    def _stage_adjoint(self, n, n0):
+        """Reverse the five stages of step n from their kept inputs."""
+        h, dt = self.blk._h, self.dt
         for s in reversed(range(5)):
-            check(lib().wql_state_load_part(h, self.s_stage + s, L.WQL_F))
+            check(lib().wql_state_load_part(h, self.s_stage + 5 * (n - n0) + s, L.WQL_F))
             check(lib().wql_adjoint_stage(h, RK_A[s], RK_B[s] * dt))
 
     def backward(self, g_traces):
@@ -101,12 +106,9 @@ class AdjointRunner:
         for seg in reversed(range(self.nck)):
             n0, n1 = seg * self.C, min(self.N, (seg + 1) * self.C)
             check(lib().wql_state_load(h, seg))
-            for n in range(n0, n1):
-                check(lib().wql_state_save(h, self.s_seg + n - n0))
-                blk.time_step(self.dt, 1, self.ts[n])
+            self._segment_stage_inputs(n0, n1)
             for n in reversed(range(n0, n1)):
-                check(lib().wql_state_load(h, self.s_seg + n - n0))
-                self._stage_adjoint(n)
+                self._stage_adjoint(n, n0)
                 check(lib().wql_adjoint_add_receivers(h, _vp(g[n])))
         nz, ny, nx = blk.ns, blk.nr, blk.nq
         out = [torch.zeros((nz, ny, nx), dtype=torch.float64, device=dev) for _ in range(3)]

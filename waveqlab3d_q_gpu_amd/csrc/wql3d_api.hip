@@ -110,7 +110,8 @@ struct wql_block {
   hipEvent_t* ev;
   // discrete adjoint (wql_adjoint_create): adjoint of (F, eta) and of
   // (DF, Deta), gradient planes (rho, lambda, mu), workspace, state slots
-  int adj_nslots;
+  int adj_nslots;    // full state slots (F and eta)
+  int adj_nslots_f;  // slots after those that hold F only
   double* adj_Fb;
   void* adj_Eb;
   double* adj_DFb;
@@ -2169,6 +2170,7 @@ static void adjoint_free(wql_block* b) {
   free(b->slot_mz);
   b->slot_mz = nullptr;
   b->adj_nslots = 0;
+  b->adj_nslots_f = 0;
 }
 
 // bytes of one checkpoint of F: the RK state, and under FP32-Kahan its
@@ -2179,8 +2181,12 @@ static size_t slot_F_bytes(const wql_block* b) {
 }
 
 extern "C" int wql_adjoint_create(wql_block* b, int32_t nslots) {
+  return wql_adjoint_create_f(b, nslots, 0);
+}
+
+extern "C" int wql_adjoint_create_f(wql_block* b, int32_t nslots, int32_t nslots_f) {
   if (!b) return fail(WQL_EINVAL, "null block");
-  if (nslots < 0) return fail(WQL_EINVAL, "nslots must be >= 0");
+  if (nslots < 0 || nslots_f < 0) return fail(WQL_EINVAL, "slot counts must be >= 0");
   for (int f = 0; f < 6; ++f)
     if (b->d.pml[f]) return fail(WQL_EINVAL, "the adjoint does not differentiate PML slabs (face %d)", f);
   if (b->iface) return fail(WQL_EINVAL, "the adjoint does not differentiate interfaces");
@@ -2197,7 +2203,7 @@ extern "C" int wql_adjoint_create(wql_block* b, int32_t nslots) {
   if (!rc) rc = dalloc(b, &b->adj_DEb, ne * es);
   if (!rc) rc = dalloc(b, (void**)&b->adj_g, 3 * P * sizeof(double));
   if (!rc) rc = dalloc(b, &b->adj_Z, 18 * P * es);
-  if (!rc) rc = dalloc(b, (void**)&b->slot_F, (size_t)nslots * slot_F_bytes(b));
+  if (!rc) rc = dalloc(b, (void**)&b->slot_F, ((size_t)nslots + (size_t)nslots_f) * slot_F_bytes(b));
   if (!rc) rc = dalloc(b, &b->slot_E, (size_t)nslots * ne * es);
   if (!rc) {
     b->slot_mz = (int*)calloc(nslots > 0 ? nslots : 1, sizeof(int));
@@ -2205,6 +2211,7 @@ extern "C" int wql_adjoint_create(wql_block* b, int32_t nslots) {
   }
   if (rc) { adjoint_free(b); return rc; }
   b->adj_nslots = nslots;
+  b->adj_nslots_f = nslots_f;
   return wql_adjoint_reset(b);
 }
 
@@ -2226,17 +2233,21 @@ extern "C" int wql_adjoint_reset(wql_block* b) {
   return WQL_OK;
 }
 
-static int check_slot(wql_block* b, int slot) {
+// slots 0 .. adj_nslots-1 hold F and eta, the adj_nslots_f after them F only
+static int check_slot(wql_block* b, int slot, uint32_t mask) {
   if (!b || !b->adj_Fb) return fail(WQL_ESTATE, "no adjoint context (wql_adjoint_create)");
-  if (slot < 0 || slot >= b->adj_nslots)
-    return fail(WQL_EINVAL, "slot %d outside 0..%d", slot, b->adj_nslots - 1);
+  const int n = b->adj_nslots + b->adj_nslots_f;
+  if (slot < 0 || slot >= n) return fail(WQL_EINVAL, "slot %d outside 0..%d", slot, n - 1);
+  if ((mask & WQL_MEMORY) && slot >= b->adj_nslots)
+    return fail(WQL_EINVAL, "slot %d holds F only (memory variables: slots 0..%d)", slot,
+                b->adj_nslots - 1);
   return WQL_OK;
 }
 
 extern "C" int wql_state_save_part(wql_block* b, int32_t slot, uint32_t mask) {
-  int rc = check_slot(b, slot);
-  if (rc) return rc;
   if (mask & ~(uint32_t)(WQL_F | WQL_MEMORY)) return fail(WQL_EINVAL, "mask: WQL_F and/or WQL_MEMORY");
+  int rc = check_slot(b, slot, mask);
+  if (rc) return rc;
   const long long P = b->P, ne = 6LL * b->d.nmech * P;
   char* sf = (char*)b->slot_F + (size_t)slot * slot_F_bytes(b);
   if (mask & WQL_F) {
@@ -2255,9 +2266,9 @@ extern "C" int wql_state_save_part(wql_block* b, int32_t slot, uint32_t mask) {
 }
 
 extern "C" int wql_state_load_part(wql_block* b, int32_t slot, uint32_t mask) {
-  int rc = check_slot(b, slot);
-  if (rc) return rc;
   if (mask & ~(uint32_t)(WQL_F | WQL_MEMORY)) return fail(WQL_EINVAL, "mask: WQL_F and/or WQL_MEMORY");
+  int rc = check_slot(b, slot, mask);
+  if (rc) return rc;
   const long long P = b->P, ne = 6LL * b->d.nmech * P;
   const char* sf = (const char*)b->slot_F + (size_t)slot * slot_F_bytes(b);
   if (mask & WQL_F) {
