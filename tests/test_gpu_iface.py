@@ -355,12 +355,16 @@ def test_rate_state_fp64_matches_oracle(oracle, problem):
     itf.close()
 
 
+@pytest.mark.parametrize("problem", ["tpv101", "rate-weakening"])
 @pytest.mark.parametrize("precision", [L.WQL_FP32_MIXED, L.WQL_FP32_KAHAN])
-def test_rate_state_fp32_against_fp64(oracle, precision):
+def test_rate_state_fp32_against_fp64(oracle, precision, problem):
     """FP32 blocks with the rate-and-state interface in FP64 (state, solve and
     SAT on FP64 copies of the faces' frames and material): slip, slip rate
-    and state against the FP64 oracle over the whole rupture."""
-    tp = PB.tpv101_cartesian(21)
+    and state against the FP64 oracle over the whole rupture (TPV101, and the
+    flash-weakened 'rate-weakening' problem with its prestress pulse)."""
+    tp = (PB.tpv101_cartesian(21) if problem == "tpv101"
+          else PB.rate_weakening_cartesian(21))
+    print(f"case {tp.name}, precision {precision}: {tp.nsteps} steps", flush=True)
     Lo, Ro, Io = oracle_two_block(oracle, tp)
     _, fr_o = oracle.two_block_time_step(Lo, Ro, Io, tp.dt, tp.nsteps, 0.0)
     bl = gpu_block(tp.left, oracle.metrics(tp.left.X), precision=precision)
@@ -443,6 +447,29 @@ def test_linear_friction_fp64_matches_oracle(oracle, alpha):
     else:
         assert slip > 1e-3
     itf.close()
+
+
+@pytest.mark.parametrize("precision", [L.WQL_FP32_MIXED, L.WQL_FP32_KAHAN])
+def test_linear_friction_fp32_against_fp64(oracle, precision):
+    """FP32 blocks (Q4, PML, curved interface) with the linear-friction
+    interface in FP64: fault output and fields against the FP64 oracle, to
+    the FP32 budget of the other interface tests (1e-4 relative L2)."""
+    tp = PB.linear_friction_curvilinear(24, alpha=5.0)
+    Lo, Ro, Io = oracle_two_block(oracle, tp)
+    _, fr_o = oracle.two_block_time_step(Lo, Ro, Io, tp.dt, tp.nsteps, 0.0)
+    bl = gpu_block(tp.left, oracle.metrics(tp.left.X), precision=precision)
+    br = gpu_block(tp.right, oracle.metrics(tp.right.X), precision=precision)
+    itf = Interface(bl, br, tp.yf, tp.zf, coupling=tp.coupling, alpha=5.0)
+    itf.set_frames(tp.nsteps)
+    itf.time_step(tp.dt, tp.nsteps, 0.0)
+    fr = itf.frames()
+    errs = {k: rel_l2(fr[k], fr_o[k]) for k in ("S", "Svel", "Uface", "Vface", "Uhat", "Vhat")}
+    bl.download(L.WQL_F)
+    br.download(L.WQL_F)
+    errs["fields"] = max(field_errors(bl.F, Lo.F) + field_errors(br.F, Ro.F))
+    print(precision, {k: "%.2e" % e for k, e in errs.items()}, flush=True)
+    itf.close()
+    assert max(errs.values()) <= 1e-4, errs
 
 
 def test_linear_friction_reference_alpha_is_the_locked_interface(oracle):
