@@ -125,9 +125,11 @@ def test_clamped_faces_match_oracle(oracle, fused):
 
 
 # Launch shapes of the fused stage (WQL3D_NODE_TUNING = "nontemporal,xcd,band,
-# split,interior,zchunk"): strict-interior kernel off, or tiles 64x8 / 32x16 /
-# 64x4 with z chunks that cut the interior box unevenly; shell memory
-# variables in their own kernel (split 1) or in the node kernel (split 0).
+# split,interior,zchunk"): workgroup order over the XCDs round-robin (0),
+# contiguous (1) or in groups of 3 or 16 (whose last workgroups keep their
+# index); strict-interior kernel off, or tiles 64x8 / 32x16 / 64x4 with z
+# chunks that cut the interior box unevenly; shell memory variables in their
+# own kernel (split 1) or in the node kernel (split 0).
 # WQL3D_STAGE_CHUNKS: z chunks of the two-stream pipeline of anelastic stages
 # (1 = one stream; 13 chunks of 3 z levels = the halo width; 5 uneven chunks).
 # WQL3D_MEM_NPT / WQL3D_MEM_NT: nodes per thread (1, 2, 4) and non-temporal
@@ -138,7 +140,8 @@ def test_clamped_faces_match_oracle(oracle, fused):
     ("0,1,0,0,1,16", "4", "2", "1"), ("0,0,0,1,1,9", "4", "2", "1"),
     ("0,1,0,1,1,32", "1", "2", "1"), ("0,1,0,1,1,32", "13", "2", "1"),
     ("0,1,0,1,0,32", "13", "2", "1"), ("0,0,0,1,1,32", "1", "1", "0"),
-    ("0,0,0,1,1,32", "1", "4", "1"), ("0,0,0,1,1,32", "1", "2", "0")])
+    ("0,0,0,1,1,32", "1", "4", "1"), ("0,0,0,1,1,32", "1", "2", "0"),
+    ("0,16,0,1,1,7", "4", "2", "1"), ("0,3,0,1,2,5", "1", "2", "1")])
 def test_fused_stage_launch_shapes(oracle, monkeypatch, tuning, chunks, npt, nt):
     monkeypatch.setenv("WQL3D_NODE_TUNING", tuning)
     monkeypatch.setenv("WQL3D_STAGE_CHUNKS", chunks)

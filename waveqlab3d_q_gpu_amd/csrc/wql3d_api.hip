@@ -499,15 +499,19 @@ extern "C" int wql_block_create(const wql_block_desc* d, wql_block** out) {
   b->esz = d->precision == WQL_FP64 ? sizeof(double) : sizeof(float);
   b->ssz = d->precision == WQL_FP32_KAHAN ? sizeof(float) : sizeof(double);
   // launch shape (env WQL3D_NODE_TUNING = "nontemporal,xcd_remap,band_rows,
-  // split,interior,zchunk"): band_rows 0 = whole y extent; split = memory
+  // split,interior,zchunk"): xcd_remap 0 = round-robin, 1 = contiguous,
+  // G >= 2 = groups of G; band_rows 0 = whole y extent; split = memory
   // variables of the shell in their own kernel; interior = tile of the
   // strict-interior kernel (0 off, 1 64x8, 2 32x16, 3 64x4); zchunk = z
   // levels per interior workgroup
   b->tune_nt = 0;
-  // XCD-contiguous workgroup order: measured slower at 385^3 Q8 (shell 4.37
-  // vs 3.81 ms, interior 3.82 vs 3.71 ms), so the hardware's round-robin
-  // order across the XCDs is the default
-  b->tune_xcd = 0;
+  // workgroup order over the XCDs: groups of 16 consecutive workgroups per
+  // XCD (xcd_logical).  At 385^3 Q8 FP32-mixed on one allocation the
+  // interior kernel takes 2.74-2.81 ms against 2.96-3.00 with the hardware's
+  // round-robin order and the shell 3.22-3.30 against 3.34-3.35 (groups of
+  // 4-64 all gain; FP64 is unchanged within 0.1 ms); the contiguous order
+  // (1) is slower (shell 4.05 ms)
+  b->tune_xcd = 16;
   b->tune_band = 0;
   b->tune_split = 1;
   b->tune_zm = 1;
@@ -515,10 +519,10 @@ extern "C" int wql_block_create(const wql_block_desc* d, wql_block** out) {
   // 3.46 / 3.34 -> 3.30 ms (FP32-mixed) and 5.16 -> 5.12 ms (FP64) against 32
   b->tune_zchunk = 24;
   if (const char* tv = getenv("WQL3D_NODE_TUNING")) {
-    int v0 = 0, v1 = 0, v2 = 0, v3 = 1, v4 = 1, v5 = 24;
+    int v0 = 0, v1 = 16, v2 = 0, v3 = 1, v4 = 1, v5 = 24;
     if (sscanf(tv, "%d,%d,%d,%d,%d,%d", &v0, &v1, &v2, &v3, &v4, &v5) >= 3) {
       b->tune_nt = v0 != 0;
-      b->tune_xcd = v1 != 0;
+      b->tune_xcd = v1 > 0 ? v1 : 0;
       b->tune_band = v2 > 0 ? v2 : 0;
       b->tune_split = v3 != 0;
       b->tune_zm = (v4 >= 0 && v4 <= 3) ? v4 : 1;

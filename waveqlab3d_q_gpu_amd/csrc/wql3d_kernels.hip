@@ -24,19 +24,33 @@ namespace wql {
 //   MODE 2: rates only, strict interior nodes (RHS_Center)
 // In MODE 0 rates include the moment-tensor source and the SAT penalty.
 // ---------------------------------------------------------------------------
-// Workgroup -> node mapping.  Workgroups are issued round-robin over the 8
-// XCDs; with xcd_remap the hardware index is permuted so each XCD receives a
-// contiguous range of logical indices.  Logical order is band-major: bands of
-// band_rows y-rows, then z, then 256-node segments of the band's (x,y) slab,
-// so consecutive work on one XCD shares y and z stencil neighbours in its L2.
+// Workgroup -> logical index.  Workgroups are issued round-robin over the 8
+// XCDs (workgroup h to XCD h % 8).  xcd_remap = 1 permutes the index so each
+// XCD receives a contiguous range of logical indices; xcd_remap = G >= 2 so
+// each XCD receives groups of G consecutive logical indices (XCD j the groups
+// j, j + 8, j + 16, ...; the last B mod 8G workgroups keep their index), so
+// that neighbouring work shares one L2 while the XCDs stay interleaved over
+// the array.  Both are bijections on [0, B).
+__device__ __forceinline__ unsigned xcd_logical(unsigned h, unsigned B, int remap) {
+  if (remap == 1) {
+    const unsigned q = B >> 3, r = B & 7u, j = h & 7u, k = h >> 3;
+    return j * q + (j < r ? j : r) + k;
+  }
+  if (remap >= 2) {
+    const unsigned G = (unsigned)remap, B8 = B / (8u * G) * (8u * G);
+    if (h < B8) {
+      const unsigned j = h & 7u, k = h >> 3;
+      return ((k / G) * 8u + j) * G + k % G;
+    }
+  }
+  return h;
+}
+
+// Logical order is band-major: bands of band_rows y-rows, then z, then
+// 256-node segments of the band's (x,y) slab.
 template <typename T, typename S>
 __device__ __forceinline__ bool map_node(const StepArgs<T, S>& a, int& x, int& y, int& z) {
-  const unsigned h = blockIdx.x;
-  unsigned L = h;
-  if (a.xcd_remap) {
-    const unsigned B = gridDim.x, q = B >> 3, r = B & 7u, j = h & 7u, k = h >> 3;
-    L = j * q + (j < r ? j : r) + k;
-  }
+  const unsigned L = xcd_logical(blockIdx.x, gridDim.x, a.xcd_remap);
   const unsigned seg = L % (unsigned)a.segs;
   const unsigned bz = L / (unsigned)a.segs;
   z = (int)(bz % (unsigned)a.nz);
@@ -56,11 +70,7 @@ __device__ __forceinline__ void node_epilogue(const StepArgs<T, S>& a, int x, in
 // concatenated boxes, x fastest inside each box.
 template <typename T, typename S>
 __device__ __forceinline__ bool map_box_node(const StepArgs<T, S>& a, int& x, int& y, int& z) {
-  unsigned h = blockIdx.x;
-  if (a.xcd_remap) {
-    const unsigned B = gridDim.x, q = B >> 3, r = B & 7u, j = h & 7u, k = h >> 3;
-    h = j * q + (j < r ? j : r) + k;
-  }
+  const unsigned h = xcd_logical(blockIdx.x, gridDim.x, a.xcd_remap);
   const long long g = (long long)h * 256 + threadIdx.x;
   if (g >= a.box_start[a.nbox]) return false;
   int b = 0;
@@ -747,11 +757,7 @@ void k_interior(const StepArgs<T, S> a) {
 
   const int bx = a.ix1 - a.ix0 + 1, by = a.iy1 - a.iy0 + 1;
   const unsigned ntx = (unsigned)((bx + TX - 1) / TX), nty = (unsigned)((by + TY - 1) / TY);
-  unsigned Lidx = blockIdx.x;
-  if (a.xcd_remap) {
-    const unsigned B = gridDim.x, q = B >> 3, r = B & 7u, j = Lidx & 7u, k = Lidx >> 3;
-    Lidx = j * q + (j < r ? j : r) + k;
-  }
+  const unsigned Lidx = xcd_logical(blockIdx.x, gridDim.x, a.xcd_remap);
   const int tix = (int)(Lidx % ntx);
   const int tiy = (int)((Lidx / ntx) % nty);
   const int zc = (int)(Lidx / (ntx * nty));
