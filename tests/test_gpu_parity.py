@@ -132,21 +132,22 @@ def test_clamped_faces_match_oracle(oracle, fused):
 # own kernel (split 1) or in the node kernel (split 0).
 # WQL3D_STAGE_CHUNKS: z chunks of the two-stream pipeline of anelastic stages
 # (1 = one stream; 13 chunks of 3 z levels = the halo width; 5 uneven chunks).
-# WQL3D_MEM_NPT / WQL3D_MEM_NT: nodes per thread (1, 2, 4) and non-temporal
-# streams of the memory-variable kernel.
-@pytest.mark.parametrize("tuning,chunks,npt,nt", [
-    ("0,1,0,1,0,64", "4", "2", "1"), ("0,1,0,1,1,7", "4", "2", "1"),
-    ("0,1,0,1,2,5", "5", "2", "1"), ("0,1,0,1,3,64", "4", "2", "1"),
-    ("0,1,0,0,1,16", "4", "2", "1"), ("0,0,0,1,1,9", "4", "2", "1"),
-    ("0,1,0,1,1,32", "1", "2", "1"), ("0,1,0,1,1,32", "13", "2", "1"),
-    ("0,1,0,1,0,32", "13", "2", "1"), ("0,0,0,1,1,32", "1", "1", "0"),
-    ("0,0,0,1,1,32", "1", "4", "1"), ("0,0,0,1,1,32", "1", "2", "0"),
-    ("0,16,0,1,1,7", "4", "2", "1"), ("0,3,0,1,2,5", "1", "2", "1")])
-def test_fused_stage_launch_shapes(oracle, monkeypatch, tuning, chunks, npt, nt):
+# WQL3D_MEM_NPT / WQL3D_MEM_NT / WQL3D_MEM_XCD: nodes per thread (1, 2, 4),
+# non-temporal streams and workgroup order of the memory-variable kernel.
+@pytest.mark.parametrize("tuning,chunks,npt,nt,mx", [
+    ("0,1,0,1,0,64", "4", "2", "1", "16"), ("0,1,0,1,1,7", "4", "2", "1", "16"),
+    ("0,1,0,1,2,5", "5", "2", "1", "16"), ("0,1,0,1,3,64", "4", "2", "1", "16"),
+    ("0,1,0,0,1,16", "4", "2", "1", "16"), ("0,0,0,1,1,9", "4", "2", "1", "16"),
+    ("0,1,0,1,1,32", "1", "2", "1", "16"), ("0,1,0,1,1,32", "13", "2", "1", "16"),
+    ("0,1,0,1,0,32", "13", "2", "1", "16"), ("0,0,0,1,1,32", "1", "1", "0", "16"),
+    ("0,0,0,1,1,32", "1", "4", "1", "16"), ("0,0,0,1,1,32", "1", "2", "0", "0"),
+    ("0,16,0,1,1,7", "4", "2", "1", "16"), ("0,3,0,1,2,5", "1", "2", "1", "3")])
+def test_fused_stage_launch_shapes(oracle, monkeypatch, tuning, chunks, npt, nt, mx):
     monkeypatch.setenv("WQL3D_NODE_TUNING", tuning)
     monkeypatch.setenv("WQL3D_STAGE_CHUNKS", chunks)
     monkeypatch.setenv("WQL3D_MEM_NPT", npt)
     monkeypatch.setenv("WQL3D_MEM_NT", nt)
+    monkeypatch.setenv("WQL3D_MEM_XCD", mx)
     prob = small_curvilinear(n=40, nmech=8, nsteps=3, seed=11)
     met = oracle.metrics(prob.X)
     rng = np.random.default_rng(7)

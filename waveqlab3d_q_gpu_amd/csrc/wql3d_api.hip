@@ -99,6 +99,7 @@ struct wql_block {
   int tune_zm, tune_zchunk;                      // z-marching kernel, z levels per chunk
   int tune_mem_npt;                              // nodes per thread of k_memory_update
   int tune_mem_nt;                               // non-temporal k_memory_update streams
+  int tune_mem_xcd;                              // its workgroup order over the XCDs
   void* arena;    // one allocation holding F[2], DF, G, M, eta, Deta, Qs, Qp, W
   void* W;        // 12 planes of T: split anelastic stage hand-off
   int mech_zero;  // mechanisms whose eta and Deta are known to be all zero
@@ -533,6 +534,13 @@ extern "C" int wql_block_create(const wql_block_desc* d, wql_block** out) {
   // at 385^3 Q8 FP32-mixed (every plane is read or written once per stage)
   b->tune_mem_nt = 1;
   if (const char* nv = getenv("WQL3D_MEM_NT")) b->tune_mem_nt = atoi(nv) != 0;
+  // the same grouped workgroup order for the memory-variable kernel (no
+  // stencil reuse, but the ~180 plane streams in flight then come from one
+  // compact node range): at 385^3 Q8 on arena-searched blocks, FP32-mixed
+  // 7.78-7.82 -> 7.61-7.62 ms, FP64 12.77-12.97 -> 12.39-12.51 ms (groups
+  // of 64 the same, 256 and the contiguous order no gain)
+  b->tune_mem_xcd = 16;
+  if (const char* xv = getenv("WQL3D_MEM_XCD")) b->tune_mem_xcd = atoi(xv) > 0 ? atoi(xv) : 0;  // 0 = round-robin
   b->tune_mem_npt = 2;
   if (const char* mv = getenv("WQL3D_MEM_NPT")) {
     const int v = atoi(mv);
@@ -1178,6 +1186,7 @@ static StepArgs<T, S> make_args(wql_block* b, double t) {
   a.node1 = b->P;
   a.mem_npt = b->tune_mem_npt;
   a.mem_nt = b->tune_mem_nt;
+  a.mem_xcd = b->tune_mem_xcd;
   // a zero-weight mechanism whose state is all zero stays zero:
   // Deta = A*0 + (0 - 0)/tau = 0, eta = 0 + Bdt*0 = 0
   int zero_weight = 0;

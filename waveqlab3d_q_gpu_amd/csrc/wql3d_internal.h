@@ -120,6 +120,7 @@ struct StepArgs {
   long long node0, node1;
   int mem_npt;
   int mem_nt;       // non-temporal accesses in k_memory_update only
+  int mem_xcd;      // workgroup order of k_memory_update_n over the XCDs (xcd_logical)
 };
 
 // Launchers (wql3d_kernels.hip).  All return hipError_t of the launch.

@@ -573,7 +573,8 @@ __device__ __forceinline__ void memory_update_planes_n(
 // not read (scale_rate maps them to 0 there whatever they hold).
 template <typename T, int NMECH, int N, bool NT, typename S, bool RD>
 __global__ __launch_bounds__(256) void k_memory_update_n(const StepArgs<T, S> a) {
-  const unsigned id = (unsigned)a.node0 + (blockIdx.x * 256u + threadIdx.x) * (unsigned)N;
+  const unsigned wg = xcd_logical(blockIdx.x, gridDim.x, a.mem_xcd);
+  const unsigned id = (unsigned)a.node0 + (wg * 256u + threadIdx.x) * (unsigned)N;
   const long long P = a.P;
   if (id >= (unsigned long long)a.node1) return;
   T DU[N][9], e[N][6];
