@@ -157,7 +157,7 @@ def test_adjoint_fp32_matches_fp64(precision):
         d = [torch.from_numpy(rng.uniform(-1, 1, lam0.shape)).to(dev) * m
              for m in (rho0, lam0, mu0)]
         dJ = sum(float((g * dk).sum()) for g, dk in zip((rho.grad, lam.grad, mu.grad), d))
-        out[prec] = (float(J), dJ)
+        out[prec] = (float(J.detach()), dJ)
         blk.close()
     (J64, g64), (J32, g32) = out[L.WQL_FP64], out[precision]
     print(f"misfit FP64 {J64!r} FP32 (precision {precision}) {J32!r}; "

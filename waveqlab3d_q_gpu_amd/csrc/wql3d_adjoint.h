@@ -16,6 +16,7 @@ struct AdjArgs {
   T* DEb;
   double* g[3];
   T* Z;
+  int xcd;   // workgroup order of the per-node kernels over the XCDs (xcd_logical)
 };
 
 // reverse of one fused stage whose input state is the block's current F / eta

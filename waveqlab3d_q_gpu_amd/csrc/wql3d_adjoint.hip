@@ -156,7 +156,7 @@ template <typename T, typename St, int MESH, int NMECH>
 __global__ __launch_bounds__(256) void k_adj_grad(const StepArgs<T, St> a, const AdjArgs<T> j,
                                                   double Bdt) {
   int x, y, z;
-  const long long id = (long long)blockIdx.x * 256 + threadIdx.x;
+  const long long id = (long long)xcd_logical(blockIdx.x, gridDim.x, j.xcd) * 256 + threadIdx.x;
   if (!flat_node(a, id, x, y, z)) return;
   const long long P = a.P;
   const int pos[3] = {x, y, z};
@@ -240,7 +240,7 @@ template <typename T, typename St, int MESH, int NMECH>
 __global__ __launch_bounds__(256) void k_adj_pre(const StepArgs<T, St> a, const AdjArgs<T> j,
                                                  double A, double Bdt) {
   int x, y, z;
-  const long long id = (long long)blockIdx.x * 256 + threadIdx.x;
+  const long long id = (long long)xcd_logical(blockIdx.x, gridDim.x, j.xcd) * 256 + threadIdx.x;
   if (!flat_node(a, id, x, y, z)) return;
   const long long P = a.P;
   const T lam = a.M[id], mu = a.M[P + id], rho = a.M[2 * P + id];
@@ -429,7 +429,7 @@ __global__ __launch_bounds__(256) void k_adj_sat(const StepArgs<T, St> a, const 
 template <typename T, typename St, int MESH>
 __global__ __launch_bounds__(256) void k_adj_stencil(const StepArgs<T, St> a, const AdjArgs<T> j) {
   int x, y, z;
-  const long long id = (long long)blockIdx.x * 256 + threadIdx.x;
+  const long long id = (long long)xcd_logical(blockIdx.x, gridDim.x, j.xcd) * 256 + threadIdx.x;
   if (!flat_node(a, id, x, y, z)) return;
   const long long P = a.P;
   const int pos[3] = {x, y, z};

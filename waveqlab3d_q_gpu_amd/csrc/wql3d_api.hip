@@ -100,6 +100,7 @@ struct wql_block {
   int tune_mem_npt;                              // nodes per thread of k_memory_update
   int tune_mem_nt;                               // non-temporal k_memory_update streams
   int tune_mem_xcd;                              // its workgroup order over the XCDs
+  int tune_adj_xcd;                              // adjoint node kernels' order
   void* arena;    // one allocation holding F[2], DF, G, M, eta, Deta, Qs, Qp, W
   void* W;        // 12 planes of T: split anelastic stage hand-off
   int mech_zero;  // mechanisms whose eta and Deta are known to be all zero
@@ -540,6 +541,10 @@ extern "C" int wql_block_create(const wql_block_desc* d, wql_block** out) {
   // 7.78-7.82 -> 7.61-7.62 ms, FP64 12.77-12.97 -> 12.39-12.51 ms (groups
   // of 64 the same, 256 and the contiguous order no gain)
   b->tune_mem_xcd = 16;
+  // and for the adjoint's per-node kernels: reverse step at 129^3 Q4 8.64-8.74
+  // -> 8.41-8.45 ms FP64, 4.67-4.68 -> 4.46 ms FP32-mixed
+  b->tune_adj_xcd = 16;
+  if (const char* av = getenv("WQL3D_ADJ_XCD")) b->tune_adj_xcd = atoi(av) > 0 ? atoi(av) : 0;
   if (const char* xv = getenv("WQL3D_MEM_XCD")) b->tune_mem_xcd = atoi(xv) > 0 ? atoi(xv) : 0;  // 0 = round-robin
   b->tune_mem_npt = 2;
   if (const char* mv = getenv("WQL3D_MEM_NPT")) {
@@ -2321,6 +2326,7 @@ static int adjoint_stage_t(wql_block* b, double A, double Bdt) {
   j.DEb = (T*)b->adj_DEb;
   for (int k = 0; k < 3; ++k) j.g[k] = b->adj_g + k * b->P;
   j.Z = (T*)b->adj_Z;
+  j.xcd = b->tune_adj_xcd;
   HIP_TRY((launch_adjoint_stage<T, St>(a, j, b->d.mesh, A, Bdt, b->stream)));
   return WQL_OK;
 }

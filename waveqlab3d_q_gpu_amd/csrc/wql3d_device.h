@@ -468,6 +468,28 @@ __device__ __forceinline__ void st_stream(V* p, V v) {
   else *p = v;
 }
 
+// Workgroup -> logical index.  Workgroups are issued round-robin over the 8
+// XCDs (workgroup h to XCD h % 8).  xcd_remap = 1 permutes the index so each
+// XCD receives a contiguous range of logical indices; xcd_remap = G >= 2 so
+// each XCD receives groups of G consecutive logical indices (XCD j the groups
+// j, j + 8, j + 16, ...; the last B mod 8G workgroups keep their index), so
+// that neighbouring work shares one L2 while the XCDs stay interleaved over
+// the array.  Both are bijections on [0, B).
+__device__ __forceinline__ unsigned xcd_logical(unsigned h, unsigned B, int remap) {
+  if (remap == 1) {
+    const unsigned q = B >> 3, r = B & 7u, j = h & 7u, k = h >> 3;
+    return j * q + (j < r ? j : r) + k;
+  }
+  if (remap >= 2) {
+    const unsigned G = (unsigned)remap, B8 = B / (8u * G) * (8u * G);
+    if (h < B8) {
+      const unsigned j = h & 7u, k = h >> 3;
+      return ((k / G) * 8u + j) * G + k % G;
+    }
+  }
+  return h;
+}
+
 // Old rate register of the low-storage RK scheme, to be scaled by A.  The
 // first stage has A = 0 and scale_rate maps any old rate to 0 there, so the
 // register is not read at all (a uniform branch on a kernel argument): that

@@ -24,28 +24,6 @@ namespace wql {
 //   MODE 2: rates only, strict interior nodes (RHS_Center)
 // In MODE 0 rates include the moment-tensor source and the SAT penalty.
 // ---------------------------------------------------------------------------
-// Workgroup -> logical index.  Workgroups are issued round-robin over the 8
-// XCDs (workgroup h to XCD h % 8).  xcd_remap = 1 permutes the index so each
-// XCD receives a contiguous range of logical indices; xcd_remap = G >= 2 so
-// each XCD receives groups of G consecutive logical indices (XCD j the groups
-// j, j + 8, j + 16, ...; the last B mod 8G workgroups keep their index), so
-// that neighbouring work shares one L2 while the XCDs stay interleaved over
-// the array.  Both are bijections on [0, B).
-__device__ __forceinline__ unsigned xcd_logical(unsigned h, unsigned B, int remap) {
-  if (remap == 1) {
-    const unsigned q = B >> 3, r = B & 7u, j = h & 7u, k = h >> 3;
-    return j * q + (j < r ? j : r) + k;
-  }
-  if (remap >= 2) {
-    const unsigned G = (unsigned)remap, B8 = B / (8u * G) * (8u * G);
-    if (h < B8) {
-      const unsigned j = h & 7u, k = h >> 3;
-      return ((k / G) * 8u + j) * G + k % G;
-    }
-  }
-  return h;
-}
-
 // Logical order is band-major: bands of band_rows y-rows, then z, then
 // 256-node segments of the band's (x,y) slab.
 template <typename T, typename S>
