@@ -68,7 +68,7 @@ def slab_nodes(prob):
 FIRST_STAGE_SHARE = 1 / 5
 
 
-def byte_model(prob, precision):
+def byte_model(prob, precision, steps=None):
     """Algorithmic HBM bytes of one stage.
 
     End to end (what the step must move, intermediates excluded): F read and
@@ -89,9 +89,13 @@ def byte_model(prob, precision):
     write also writes the float copy (4 B per component).
 
     The first of the five stages has A = 0 and reads no old rate (DF, Deta,
-    PML DQ): every figure is the average over the five stages, i.e. the
-    full-stage count less a fifth of the old-rate reads (fields
-    "rate_reads" and "first_stage_share")."""
+    PML DQ), so the rates the fifth stage stores are dead whenever another
+    step of the same wql_time_step call follows, and they are not stored:
+    every figure is the average over the stages of a call of ``steps`` steps
+    (None: a long run), i.e. the full-stage count less a fifth of the
+    old-rate reads and (steps - 1) / steps of a fifth of the rate stores,
+    which move the same planes (fields "rate_reads", "first_stage_share",
+    "dead_store_share")."""
     nq, nr, ns = prob.n
     P = nq * nr * ns
     es = 8 if precision == 0 else 4
@@ -139,8 +143,10 @@ def byte_model(prob, precision):
     }
     rate_reads_e2e = P * (9 * ss + (6 * na * es if prob.nmech else 0)) + slab * 9 * es
     share = FIRST_STAGE_SHARE
-    kernels = {k: int(round(v - share * rate_reads[k])) for k, v in kernels.items()}
-    end_to_end = int(round(end_to_end - share * rate_reads_e2e))
+    dead = FIRST_STAGE_SHARE * (1.0 if steps is None else (steps - 1) / steps)
+    # the rate stores move the planes the old-rate reads move
+    kernels = {k: int(round(v - (share + dead) * rate_reads[k])) for k, v in kernels.items()}
+    end_to_end = int(round(end_to_end - (share + dead) * rate_reads_e2e))
     # the same count with every mechanism's eta and Deta streamed at every
     # stage (the per-point figure quoted for the workload, 1116 B FP32-mixed
     # Q8 before PML), whatever the mechanisms' weights
@@ -149,7 +155,7 @@ def byte_model(prob, precision):
     return {"P": P, "interior_nodes": n_int, "active_mechanisms": na,
             "end_to_end": end_to_end, "end_to_end_all_mechanisms": all_mech, "kernels": kernels,
             "rate_reads": dict(rate_reads, end_to_end=rate_reads_e2e),
-            "first_stage_share": share}
+            "first_stage_share": share, "dead_store_share": dead}
 
 
 def near_source_receivers(prob, offsets=((4, 0, 0), (-4, 0, 0), (0, 4, 0), (0, -4, 0),
@@ -525,12 +531,13 @@ def main():
               f"{args.warmup}+{args.steps} steps", file=sys.stderr, flush=True)
         ms, wall, _ = timed_run(blk, prob, args.steps, args.warmup, t0)
         seis = blk.seismograms()
-        # per-kernel durations on the block stream over 2 more steps
+        # per-kernel durations on the block stream over one more call of the
+        # same number of steps
         blk.set_timing(True)
-        blk.time_step(prob.dt, 2, 0.0)
+        blk.time_step(prob.dt, args.steps, 0.0)
         kt, nst = blk.kernel_times()
         blk.set_timing(False)
-        bm = byte_model(prob, prec)
+        bm = byte_model(prob, prec, steps=args.steps)
         stage_ms = ms / (5 * args.steps)
         kernels = {}
         for k, key in enumerate(("interior", "shell", "memory_update")):

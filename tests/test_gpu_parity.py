@@ -407,3 +407,30 @@ def test_first_stage_does_not_read_old_rates(oracle, precision):
     for a, b in zip(*runs):
         assert np.isfinite(a).all() and np.isfinite(b).all()
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("precision", [L.WQL_FP64, L.WQL_FP32_MIXED, L.WQL_FP32_KAHAN])
+def test_dead_rate_stores_change_nothing(oracle, precision):
+    """Inside one wql_time_step call the fifth stage of every step but the
+    last does not store its rates (DF, Deta, PML DQ): the next step's first
+    stage (A = 0) reads none of them.  One call of 4 steps must leave the
+    whole state (F, DF, eta, Deta, PML Q and DQ) and the seismograms
+    bit-identical to 4 calls of one step, which store every rate."""
+    prob = small_curvilinear(n=30, nmech=8, npml=8, nsteps=4)
+    met = oracle.metrics(prob.X)
+    runs = []
+    for per_call in (4, 1):
+        blk = gpu_block(prob, met, precision=precision)
+        blk.set_receivers(prob.receivers, 4)
+        t = prob.sources[0].t_init - 2 * prob.dt
+        for _ in range(4 // per_call):
+            t = blk.time_step(prob.dt, per_call, t)
+        _download_all(blk)
+        runs.append([blk.F.copy(), blk.DF.copy(), blk.eta.copy(), blk.Deta.copy(),
+                     blk.seismograms()]
+                    + [q.copy() for q in blk.pml_Q if q is not None]
+                    + [q.copy() for q in blk.pml_DQ if q is not None])
+        blk.close()
+    assert np.abs(runs[0][0]).max() > 0
+    for a, b in zip(*runs):
+        assert np.array_equal(a, b)

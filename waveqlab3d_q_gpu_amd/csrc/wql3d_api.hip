@@ -1563,9 +1563,11 @@ static int ensure_operand_copy(wql_block* b) {
 }
 
 template <typename T, typename S = double>
-static int stage_fused_t(wql_block* b, double A, double Bdt, double t, bool keep_pending = false) {
+static int stage_fused_t(wql_block* b, double A, double Bdt, double t, bool keep_pending = false,
+                         bool dead_rates = false) {
   StepArgs<T, S> a = make_args<T, S>(b, t);
   a.A = A; a.Bdt = Bdt;
+  a.dead_rates = dead_rates ? 1 : 0;
   if (int rc = ensure_operand_copy(b)) return rc;
   if (!b->timing) {
     if (b->nchunks > 1 && stage_has_memory_part(a)) {
@@ -1593,11 +1595,12 @@ static int stage_fused_t(wql_block* b, double A, double Bdt, double t, bool keep
 
 // one fused stage; keep_pending leaves the pipelined memory kernels of the
 // stage unjoined (the next stage of the same block orders itself behind them)
-static int stage_fused(wql_block* b, double A, double Bdt, double t, bool keep_pending) {
-  int rc = b->d.precision == WQL_FP64 ? stage_fused_t<double>(b, A, Bdt, t, keep_pending)
+static int stage_fused(wql_block* b, double A, double Bdt, double t, bool keep_pending,
+                       bool dead_rates = false) {
+  int rc = b->d.precision == WQL_FP64 ? stage_fused_t<double>(b, A, Bdt, t, keep_pending, dead_rates)
            : b->d.precision == WQL_FP32_MIXED
-               ? stage_fused_t<float>(b, A, Bdt, t, keep_pending)
-               : stage_fused_t<float, float>(b, A, Bdt, t, keep_pending);
+               ? stage_fused_t<float>(b, A, Bdt, t, keep_pending, dead_rates)
+               : stage_fused_t<float, float>(b, A, Bdt, t, keep_pending, dead_rates);
   if (rc) return rc;
   b->cur ^= 1;
   return WQL_OK;
@@ -1643,7 +1646,10 @@ extern "C" int wql_time_step(wql_block* b, double dt, int32_t nsteps, double* t)
         if (!rc) rc = record(b);  // receivers sample F at t0 (time_step.f90:137)
         if (rc) return rc;
       }
-      int rc = stage_fused(b, kRkA[s], kRkB[s] * dt, *t, true);
+      // the last stage's rates are dead when another step of this call
+      // follows: its first stage has A = 0 and reads none of them
+      const bool dead = s == 4 && n + 1 < nsteps && kRkA[0] == 0.0;
+      int rc = stage_fused(b, kRkA[s], kRkB[s] * dt, *t, true, dead);
       if (rc) return rc;
       *t = t0 + kRkC[s + 1] * dt;
     }

@@ -502,18 +502,19 @@ __device__ __forceinline__ V ld_rate(double A, const V* p) {
 
 // DF = df and the RK state update F_out = F_in + Bdt*DF of component c at
 // node idu: FP64 state, or FP32 state with a Kahan-compensated sum whose
-// compensation planes Fc are updated in place.
+// compensation planes Fc are updated in place.  DF is not stored when the
+// stage's rates are dead (StepArgs::dead_rates).
 template <bool NT, typename T, typename S>
 __device__ __forceinline__ void rk_finish(const StepArgs<T, S>& a, int c, unsigned idu, double df) {
   const long long P = a.P;
   if constexpr (std::is_same<S, double>::value) {
-    st_stream<NT>(atp(a.DF + c * P, idu), df);
+    if (!a.dead_rates) st_stream<NT>(atp(a.DF + c * P, idu), df);
     const double fo = ldp(a.Fin + c * P, idu) + a.Bdt * df;
     st_stream<NT>(atp(a.Fout + c * P, idu), fo);
     if constexpr (kOpCopy<T, S>) st_stream<NT>(atp(a.Fo_out + c * P, idu), (T)fo);
   } else {
     const float d = (float)df;
-    st_stream<NT>(atp(a.DF + c * P, idu), d);
+    if (!a.dead_rates) st_stream<NT>(atp(a.DF + c * P, idu), d);
     const float f = ldp(a.Fin + c * P, idu);
     float* cp = atp(a.Fc + c * P, idu);
     const float y = (float)a.Bdt * d - ld_stream<NT>(cp);
@@ -531,13 +532,13 @@ __device__ __forceinline__ void rk_finish_pre(const StepArgs<T, S>& a, int c, un
   const long long P = a.P;
   if constexpr (std::is_same<S, double>::value) {
     (void)comp;
-    st_stream<NT>(atp(a.DF + c * P, idu), df);
+    if (!a.dead_rates) st_stream<NT>(atp(a.DF + c * P, idu), df);
     const double fo = fin + a.Bdt * df;
     st_stream<NT>(atp(a.Fout + c * P, idu), fo);
     if constexpr (kOpCopy<T, S>) st_stream<NT>(atp(a.Fo_out + c * P, idu), (T)fo);
   } else {
     const float d = (float)df;
-    st_stream<NT>(atp(a.DF + c * P, idu), d);
+    if (!a.dead_rates) st_stream<NT>(atp(a.DF + c * P, idu), d);
     const float y = (float)a.Bdt * d - comp;
     const float t = fin + y;
     st_stream<NT>(atp(a.Fc + c * P, idu), (t - fin) - y);

@@ -93,6 +93,10 @@ struct StepArgs {
   int nsrc;
   DevSource src[kMaxSources];
   double A, Bdt;    // fused stage coefficients
+  // the rates this stage stores (DF, Deta, PML DQ) are never read: the next
+  // stage is the first of a step (A = 0) inside the same wql_time_step call,
+  // so the stores are skipped (the call's last step stores them)
+  int dead_rates;
   int ix0, ix1, iy0, iy1, iz0, iz1;  // strict interior [lo, hi] 0-based (RHS_Center)
   // node-kernel launch shape (see map_node): y-band height, 256-node segments
   // per band slab, XCD-contiguous remap, non-temporal streams

@@ -190,7 +190,7 @@ __device__ __forceinline__ void node_epilogue(const StepArgs<T, S>& a, int x, in
         T* Et = atp(a.eta + (long long)(c * NMECH + i) * P, idu);
         if (MODE == 0) {
           const T dn = scale_rate(a.A, ld_rate<NT>(a.A, De)) + r;
-          st_stream<NT>(De, dn);
+          if (!a.dead_rates) st_stream<NT>(De, dn);
           st_stream<NT>(Et, ev[i] + (T)a.Bdt * dn);
         } else {
           *De = *De + r;
@@ -287,7 +287,7 @@ __device__ __forceinline__ void node_epilogue(const StepArgs<T, S>& a, int x, in
         if (fsat) r -= pen * sat[c];
         if (FUSED) {
           const T dn = scale_rate(a.A, ld_rate<NT>(a.A, DQ + c * Ps + si)) + r;
-          st_stream<NT>(DQ + c * Ps + si, dn);
+          if (!a.dead_rates) st_stream<NT>(DQ + c * Ps + si, dn);
           st_stream<NT>(Q + c * Ps + si, q[c] + (T)a.Bdt * dn);
         } else {
           DQ[c * Ps + si] = DQ[c * Ps + si] + r;
@@ -352,7 +352,7 @@ __device__ __forceinline__ void memory_update_planes(T* __restrict__ eta, T* __r
                                                      int mech_skip, const double* wgt,
                                                      const double* tau_inv, double A, double Bdt,
                                                      unsigned id, T mu, T l2m, const T (&e)[6],
-                                                     T (&DU)[9]) {
+                                                     T (&DU)[9], bool store_rates = true) {
   const T tr = e[0] + e[1] + e[2];
   const T Qs = ld_stream<NT>(atp(Qsp, id)), Qp = ld_stream<NT>(atp(Qpp, id));
   const T twomuQs = T(2) * mu * Qs;
@@ -378,7 +378,7 @@ __device__ __forceinline__ void memory_update_planes(T* __restrict__ eta, T* __r
       const T src = c < 3 ? (w * twomuQs) * e[c] + (w * pfac) * tr : (w * muQs) * e[c];
       const T r = (src - ev[i]) * ti;
       const T dn = scale_rate(A, dv[i]) + r;
-      st_stream<NT>(atp(Deta + (long long)(c * NMECH + i) * P, id), dn);
+      if (store_rates) st_stream<NT>(atp(Deta + (long long)(c * NMECH + i) * P, id), dn);
       st_stream<NT>(atp(eta + (long long)(c * NMECH + i) * P, id), ev[i] + (T)Bdt * dn);
     }
   }
@@ -388,7 +388,7 @@ template <typename T, int NMECH, bool NT, typename S>
 __device__ __forceinline__ void memory_update_node(const StepArgs<T, S>& a, unsigned id, T mu, T l2m,
                                                    const T (&e)[6], T (&DU)[9]) {
   memory_update_planes<T, NMECH, NT>(a.eta, a.Deta, a.Qs, a.Qp, a.P, a.mech_skip, a.wgt, a.tau_inv,
-                                     a.A, a.Bdt, id, mu, l2m, e, DU);
+                                     a.A, a.Bdt, id, mu, l2m, e, DU, !a.dead_rates);
 }
 
 // Nodes of the strict interior outside every source box: no PML, face or
@@ -488,7 +488,7 @@ __device__ __forceinline__ void stn(V* base, unsigned id, const V (&v)[N]) {
   else *p = t;
 }
 
-template <typename T, int NMECH, int N, bool NT, bool RD>
+template <typename T, int NMECH, int N, bool NT, bool RD, bool WR>
 __device__ __forceinline__ void memory_update_planes_n(
     T* __restrict__ eta, T* __restrict__ Deta, const T* __restrict__ Qsp, const T* __restrict__ Qpp,
     long long P, int mech_skip, const double* wgt, const double* tau_inv, double A, double Bdt,
@@ -541,15 +541,17 @@ __device__ __forceinline__ void memory_update_planes_n(
         dn[n] = scale_rate(A, dv[i][n]) + r;
         en[n] = ev[i][n] + (T)Bdt * dn[n];
       }
-      stn<N, NT>(Deta + (long long)(c * NMECH + i) * P, id, dn);
+      if constexpr (WR) stn<N, NT>(Deta + (long long)(c * NMECH + i) * P, id, dn);
       stn<N, NT>(eta + (long long)(c * NMECH + i) * P, id, en);
     }
   }
 }
 
 // RD = false at the first RK stage (A = 0): the old rates Deta and DF are
-// not read (scale_rate maps them to 0 there whatever they hold).
-template <typename T, int NMECH, int N, bool NT, typename S, bool RD>
+// not read (scale_rate maps them to 0 there whatever they hold).  WR = false
+// when the stage's rates are dead (StepArgs::dead_rates): Deta and DF are
+// not stored.
+template <typename T, int NMECH, int N, bool NT, typename S, bool RD, bool WR>
 __global__ __launch_bounds__(256) void k_memory_update_n(const StepArgs<T, S> a) {
   const unsigned wg = xcd_logical(blockIdx.x, gridDim.x, a.mem_xcd);
   const unsigned id = (unsigned)a.node0 + (wg * 256u + threadIdx.x) * (unsigned)N;
@@ -590,7 +592,7 @@ __global__ __launch_bounds__(256) void k_memory_update_n(const StepArgs<T, S> a)
 #pragma unroll
   for (int n = 0; n < N; ++n)
     if (hk[n]) hooke_rates(lam[n], l2m[n], mu[n], e[n], DU[n]);
-  memory_update_planes_n<T, NMECH, N, NT, RD>(a.eta, a.Deta, a.Qs, a.Qp, P, a.mech_skip, a.wgt,
+  memory_update_planes_n<T, NMECH, N, NT, RD, WR>(a.eta, a.Deta, a.Qs, a.Qp, P, a.mech_skip, a.wgt,
                                       a.tau_inv, a.A, a.Bdt, id, mu, l2m, e, DU);
 #pragma unroll
   for (int c = 3; c < 9; ++c) {
@@ -607,7 +609,7 @@ __global__ __launch_bounds__(256) void k_memory_update_n(const StepArgs<T, S> a)
         df[n] = scale_rate(a.A, dfo[n]) + (double)DU[n][c];
         fo[n] = fin[n] + a.Bdt * df[n];
       }
-      stn<N, NT>(a.DF + c * P, id, df);
+      if constexpr (WR) stn<N, NT>(a.DF + c * P, id, df);
       stn<N, NT>(a.Fout + c * P, id, fo);
       if constexpr (kOpCopy<T, S>) {
         T fo32[N];
@@ -626,7 +628,7 @@ __global__ __launch_bounds__(256) void k_memory_update_n(const StepArgs<T, S> a)
         cn[n] = (t - fin[n]) - y;
         fo[n] = t;
       }
-      stn<N, NT>(a.DF + c * P, id, d);
+      if constexpr (WR) stn<N, NT>(a.DF + c * P, id, d);
       stn<N, NT>(a.Fc + c * P, id, cn);
       stn<N, NT>(a.Fout + c * P, id, fo);
     }
@@ -1181,8 +1183,11 @@ static hipError_t launch_node(const StepArgs<T, S>& a, int mesh, hipStream_t s) 
 
 template <typename T, int NM, int NP, bool NT, typename S>
 static void launch_memory_update_n(const StepArgs<T, S>& a, dim3 g, hipStream_t s) {
-  if (a.A != 0.0) hipLaunchKernelGGL((k_memory_update_n<T, NM, NP, NT, S, true>), g, dim3(256), 0, s, a);
-  else hipLaunchKernelGGL((k_memory_update_n<T, NM, NP, NT, S, false>), g, dim3(256), 0, s, a);
+  const bool rd = a.A != 0.0, wr = !a.dead_rates;
+  if (rd && wr) hipLaunchKernelGGL((k_memory_update_n<T, NM, NP, NT, S, true, true>), g, dim3(256), 0, s, a);
+  else if (rd) hipLaunchKernelGGL((k_memory_update_n<T, NM, NP, NT, S, true, false>), g, dim3(256), 0, s, a);
+  else if (wr) hipLaunchKernelGGL((k_memory_update_n<T, NM, NP, NT, S, false, true>), g, dim3(256), 0, s, a);
+  else hipLaunchKernelGGL((k_memory_update_n<T, NM, NP, NT, S, false, false>), g, dim3(256), 0, s, a);
 }
 
 template <typename T, bool NT, typename S>
