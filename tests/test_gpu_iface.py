@@ -489,3 +489,33 @@ def test_linear_friction_reference_alpha_is_the_locked_interface(oracle):
     errs = field_errors(outs[0][0], outs[1][0]) + field_errors(outs[0][1], outs[1][1])
     print("linear(1e14) vs locked:", max(errs), flush=True)
     assert max(errs) <= TOL64
+
+
+@pytest.mark.parametrize("precision", [L.WQL_FP64, L.WQL_FP32_MIXED])
+def test_two_block_dead_rate_stores_change_nothing(oracle, precision):
+    """wql_iface_time_step skips the blocks' last-stage rate stores of every
+    step but the last of a call, as wql_time_step does: one call of 6 steps
+    leaves both blocks' state (F, DF, eta, Deta, PML Q and DQ), the slip
+    state and the fault output bit-identical to 6 calls of one step (curved
+    interface, Q4, PML, linear friction)."""
+    tp = PB.linear_friction_curvilinear(24, alpha=5.0, nsteps=6)
+    runs = []
+    for per_call in (6, 1):
+        bl = gpu_block(tp.left, oracle.metrics(tp.left.X), precision=precision)
+        br = gpu_block(tp.right, oracle.metrics(tp.right.X), precision=precision)
+        itf = Interface(bl, br, tp.yf, tp.zf, coupling=tp.coupling, alpha=5.0)
+        itf.set_frames(6)
+        t = 0.0
+        for _ in range(6 // per_call):
+            t = itf.time_step(tp.dt, per_call, t)
+        out = [np.asarray(v).copy() for v in itf.state()] + list(itf.frames().values())
+        for b in (bl, br):
+            b.download(L.WQL_F | L.WQL_DF | L.WQL_MEMORY | L.WQL_PML)
+            out += [b.F.copy(), b.DF.copy(), b.eta.copy(), b.Deta.copy()]
+            out += [q.copy() for q in b.pml_Q if q is not None]
+            out += [q.copy() for q in b.pml_DQ if q is not None]
+        itf.close()
+        runs.append(out)
+    assert np.abs(runs[0][0]).max() > 0   # slip
+    for a, b in zip(*runs):
+        assert np.array_equal(a, b)

@@ -2043,7 +2043,8 @@ extern "C" void wql_iface_destroy(wql_iface* I) {
 // previous stage), block L's fused stage behind it on L's stream, block R's on
 // R's stream after k_iface
 template <typename T, typename S = double>
-static int iface_stage_t(wql_iface* I, double A, double Bdt, double t, int stage1) {
+static int iface_stage_t(wql_iface* I, double A, double Bdt, double t, int stage1,
+                         bool dead_rates = false) {
   wql_block* L = I->L;
   wql_block* R = I->R;
   HIP_TRY(hipStreamWaitEvent(L->stream, I->ev_right, 0));
@@ -2078,10 +2079,10 @@ static int iface_stage_t(wql_iface* I, double A, double Bdt, double t, int stage
   HIP_TRY((launch_iface<T, S>(a, L->stream)));
   if (a.frame) ++I->slot;
   HIP_TRY(hipEventRecord(I->ev_iface, L->stream));
-  int rc = stage_fused_t<T, S>(L, A, Bdt, t);
+  int rc = stage_fused_t<T, S>(L, A, Bdt, t, false, dead_rates);
   if (rc) return rc;
   HIP_TRY(hipStreamWaitEvent(R->stream, I->ev_iface, 0));
-  rc = stage_fused_t<T, S>(R, A, Bdt, t);
+  rc = stage_fused_t<T, S>(R, A, Bdt, t, false, dead_rates);
   if (rc) return rc;
   HIP_TRY(hipEventRecord(I->ev_right, R->stream));
   // block L's stream then orders everything of the stage (callers time and
@@ -2092,13 +2093,17 @@ static int iface_stage_t(wql_iface* I, double A, double Bdt, double t, int stage
   return WQL_OK;
 }
 
+static int iface_stage(wql_iface* I, double A, double Bdt, double t, int stage1, bool dead_rates) {
+  switch (I->L->d.precision) {
+    case WQL_FP64: return iface_stage_t<double>(I, A, Bdt, t, stage1, dead_rates);
+    case WQL_FP32_MIXED: return iface_stage_t<float>(I, A, Bdt, t, stage1, dead_rates);
+    default: return iface_stage_t<float, float>(I, A, Bdt, t, stage1, dead_rates);
+  }
+}
+
 extern "C" int wql_iface_stage(wql_iface* I, double A, double Bdt, double t, int32_t stage1) {
   if (int rc = iface_ok(I)) return rc;
-  switch (I->L->d.precision) {
-    case WQL_FP64: return iface_stage_t<double>(I, A, Bdt, t, stage1);
-    case WQL_FP32_MIXED: return iface_stage_t<float>(I, A, Bdt, t, stage1);
-    default: return iface_stage_t<float, float>(I, A, Bdt, t, stage1);
-  }
+  return iface_stage(I, A, Bdt, t, stage1, false);
 }
 
 extern "C" int wql_iface_get_friction_state(wql_iface* I, double* W) {
@@ -2120,7 +2125,10 @@ extern "C" int wql_iface_time_step(wql_iface* I, double dt, int32_t nsteps, doub
         if (!rc) rc = record(I->R);
         if (rc) return rc;
       }
-      int rc = wql_iface_stage(I, kRkA[s], kRkB[s] * dt, *t, s == 0);
+      // the blocks' last-stage rates are dead when another step follows
+      // (as in wql_time_step)
+      const bool dead = s == 4 && n + 1 < nsteps && kRkA[0] == 0.0;
+      int rc = iface_stage(I, kRkA[s], kRkB[s] * dt, *t, s == 0, dead);
       if (rc) return rc;
       *t = t0 + kRkC[s + 1] * dt;
     }
