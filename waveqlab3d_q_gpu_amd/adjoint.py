@@ -69,11 +69,19 @@ class AdjointRunner:
         blk.set_receivers(self.receivers, self.N)
         t = self.t0
         self.ts = []
-        for n in range(self.N):
-            if n % self.C == 0:
-                check(lib().wql_state_save(blk._h, n // self.C))
-            self.ts.append(t)
-            t = blk.time_step(self.dt, 1, t)
+        for seg in range(self.nck):
+            n0, n1 = seg * self.C, min(self.N, (seg + 1) * self.C)
+            check(lib().wql_state_save(blk._h, seg))
+            # one call per segment (its steps' last-stage rates are not stored
+            # but the segment's last); the step start times are those the call
+            # forms, t0 + C(5) dt per step
+            tn = t
+            for _ in range(n0, n1):
+                self.ts.append(tn)
+                tn = tn + RK_C[5] * self.dt
+            t = blk.time_step(self.dt, n1 - n0, t)
+            if t != tn:
+                raise RuntimeError("step times of the segment differ from the library's")
         return torch.from_numpy(blk.seismograms()).to(blk.M.device)
 
     # -- reverse sweep -------------------------------------------------------
