@@ -110,7 +110,10 @@ typedef struct wql_block_desc {
 /* moment tensor point source (moment_tensor.f90: moment_tensor type) */
 typedef struct wql_source {
   int32_t type;              /* WQL_SRC_* ; for RICKER duration = peak frequency */
-  int32_t order;             /* 2 or 4 (singular_source) */
+  int32_t order;             /* 2 or 4 (singular_source); 0 = mollified:
+                                set_moment_tensor_smooth (moment_tensor.f90:
+                                1255-1330, the reference's mollify_source),
+                                needs wql_block_set_coordinates first */
   int32_t alpha;             /* 0, 25, 50, 100 (order 4 stencil) */
   int32_t near[3];           /* nearest node, 1-based */
   double duration, t_init;
@@ -148,6 +151,10 @@ void wql_block_destroy(wql_block *b);
 /* ---- sources and receivers ---------------------------------------------- */
 /* init_moment_tensor result (moment_tensor.f90:8-242); n <= 8 */
 int wql_block_set_sources(wql_block *b, int32_t n, const wql_source *src);
+/* node coordinates B%G%X (datatypes.f90 block_grid_t), read by mollified
+ * sources: X[c*nx*ny*nz + idx], c = x, y, z, idx = x + nx*(y + ny*z) (0-based),
+ * a host array, or a device array when on_device != 0 */
+int wql_block_set_coordinates(wql_block *b, const double *X, int32_t on_device);
 /* receivers at nodes ijk[3*n] (1-based), sampled at stage 1 of each step
  * (time_step.f90:137) into a device buffer of nslots steps */
 int wql_block_set_receivers(wql_block *b, int32_t n, const int32_t *ijk, int32_t nslots);

@@ -57,7 +57,7 @@ class _Block(C.Structure):
                 ("Qs_inv", _P), ("Qp_inv", _P),
                 ("pQ", _P * 6), ("pDQ", _P * 6),
                 ("nl", _P * 6), ("nm", _P * 6), ("nn", _P * 6),
-                ("src", C.POINTER(_Source))]
+                ("src", C.POINTER(_Source)), ("X", _P)]
 
 
 class _Iface(C.Structure):
@@ -263,6 +263,7 @@ class OracleBlock:
         self.nm = [np.zeros(s) for s in fshape]
         self.nn = [np.zeros(s) for s in fshape]
         self.sources = []
+        self.X = None            # (3, nz, ny, nx) coordinates, for mollified sources
         self._struct = None
 
     # -- setup helpers --------------------------------------------------
@@ -320,6 +321,11 @@ class OracleBlock:
         if self.sources:
             self._src_arr = (_Source * len(self.sources))(*self.sources)
             b.src = C.cast(self._src_arr, C.POINTER(_Source))
+        if any(src.order == 0 for src in self.sources) and self.X is None:
+            raise ValueError("a mollified source (order 0) needs the coordinates X")
+        if self.X is not None:
+            self.X = np.ascontiguousarray(self.X, dtype=np.float64)
+        b.X = _ptr(self.X)
         self._struct = b
         return b
 

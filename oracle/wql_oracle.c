@@ -703,12 +703,50 @@ static double singular_source(int i, int i0, double h, double a, int order, int 
   return W[k] / h;
 }
 
+/* set_moment_tensor_smooth (moment_tensor.f90:1255-1330): a Gaussian of
+ * standard deviation 2h per axis over every node, h the coordinate's
+ * backward difference along its own index (forward at the first node), the
+ * reference's truncated pi */
+static void mollified_source(orc_block *b, const orc_source *S, double A) {
+  const double pi = 3.14159265359;
+  long N = (long)b->nx * b->ny * b->nz;
+  const double *X = b->X;
+  for (int n = 1; n <= b->nz; ++n)
+    for (int m = 1; m <= b->ny; ++m)
+      for (int l = 1; l <= b->nx; ++l) {
+        long id = IDX(b, l, m, n);
+        double hx = l == 1 ? X[IDX(b, 2, m, n)] - X[id] : X[id] - X[IDX(b, l - 1, m, n)];
+        double hy = m == 1 ? X[N + IDX(b, l, 2, n)] - X[N + id]
+                           : X[N + id] - X[N + IDX(b, l, m - 1, n)];
+        double hz = n == 1 ? X[2 * N + IDX(b, l, m, 2)] - X[2 * N + id]
+                           : X[2 * N + id] - X[2 * N + IDX(b, l, m, n - 1)];
+        double sx = 2.0 * hx, sy = 2.0 * hy, sz = 2.0 * hz;
+        double ux = (X[id] - S->loc[0]) / sx;
+        double uy = (X[N + id] - S->loc[1]) / sy;
+        double uz = (X[2 * N + id] - S->loc[2]) / sz;
+        double dx = exp(-0.5 * (ux * ux)) / (sx * sqrt(2.0 * pi));
+        double dy = exp(-0.5 * (uy * uy)) / (sy * sqrt(2.0 * pi));
+        double dz = exp(-0.5 * (uz * uz)) / (sz * sqrt(2.0 * pi));
+        double delta = dx * dy * dz;
+        b->DF[3 * N + id] = b->DF[3 * N + id] - A * S->mXX * delta;
+        b->DF[4 * N + id] = b->DF[4 * N + id] - A * S->mYY * delta;
+        b->DF[5 * N + id] = b->DF[5 * N + id] - A * S->mZZ * delta;
+        b->DF[6 * N + id] = b->DF[6 * N + id] - A * S->mXY * delta;
+        b->DF[7 * N + id] = b->DF[7 * N + id] - A * S->mXZ * delta;
+        b->DF[8 * N + id] = b->DF[8 * N + id] - A * S->mYZ * delta;
+      }
+}
+
 void orc_set_moment_tensor(orc_block *b, double t) {
   long N = (long)b->nx * b->ny * b->nz;
   for (int s = 0; s < b->nsrc; ++s) {
     const orc_source *S = &b->src[s];
-    int r = S->order == 2 ? 2 : 4;
     double A = orc_source_time(S->type, t, S->t_init, S->duration);
+    if (S->order == 0) {
+      mollified_source(b, S, A);
+      continue;
+    }
+    int r = S->order == 2 ? 2 : 4;
     for (int n = S->near[2] - r; n <= S->near[2] + r; ++n)
       for (int m = S->near[1] - r; m <= S->near[1] + r; ++m)
         for (int l = S->near[0] - r; l <= S->near[0] + r; ++l) {

@@ -27,7 +27,7 @@ enum { ORC_SRC_GAUSSIAN = 0, ORC_SRC_COSBELL = 1, ORC_SRC_RICKER = 2 };
 
 typedef struct orc_source {
   int type;
-  int order;          /* 2 or 4 (singular_source) */
+  int order;          /* 2 or 4 (singular_source); 0 = mollified (set_moment_tensor_smooth) */
   int alpha;          /* 0, 25, 50, 100 for order 4 */
   int near[3];        /* nearest node, 1-based */
   double duration, t_init;
@@ -61,6 +61,7 @@ typedef struct orc_block {
   double *pQ[6], *pDQ[6];
   double *nl[6], *nm[6], *nn[6];
   orc_source *src;
+  double *X;          /* node coordinates [3][N] (x, y, z); read by mollified sources */
 } orc_block;
 
 /* KC(5,4) low-storage coefficients (time_step.f90:53-62): A[5], B[5], C[6] */
@@ -89,7 +90,7 @@ int orc_q_modulus_correction(orc_block *b, double fref);
 /* stage pieces, in reference order */
 void orc_scale_rates(orc_block *b, double A);               /* fields.f90:51-155 */
 void orc_set_rates(orc_block *b);                           /* elastic.f90:13-29 */
-void orc_set_moment_tensor(orc_block *b, double t);         /* moment_tensor.f90:875-935 */
+void orc_set_moment_tensor(orc_block *b, double t);         /* moment_tensor.f90:875-935, 1255-1330 */
 void orc_impose_boundary_condition(orc_block *b);           /* RHS_Interior.f90:1584-1768 */
 void orc_update_fields(orc_block *b, double inc);           /* fields.f90:158-263 */
 

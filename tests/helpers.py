@@ -23,6 +23,8 @@ def oracle_block(O, prob, metrics):
         b.add_source(kind=s.kind, t_init=s.t_init, duration=s.duration, mxx=s.mxx, myy=s.myy,
                      mzz=s.mzz, mxy=s.mxy, mxz=s.mxz, myz=s.myz, loc=s.location, near=s.near,
                      near_phys=s.near_phys, order=s.order, alpha=s.alpha)
+    if any(s.order == 0 for s in prob.sources):
+        b.X = np.ascontiguousarray(prob.X, dtype=np.float64)
     if prob.F0 is not None:
         b.F[...] = prob.F0
     return b

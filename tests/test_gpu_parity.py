@@ -99,6 +99,53 @@ def test_curvilinear_q_pml(oracle, nmech, fused):
         assert rel_l2(seis_g, seis_o) <= TOL64
 
 
+@pytest.mark.parametrize("precision", [L.WQL_FP64, L.WQL_FP32_MIXED])
+@pytest.mark.parametrize("fused", [True, False])
+def test_mollified_source_matches_oracle(oracle, fused, precision):
+    """The mollified moment-tensor source (set_moment_tensor_smooth, the
+    reference's mollify_source: a Gaussian of standard deviation 2h per axis
+    from the node coordinates) on the curvilinear Q8 + PML block, fused
+    stage (source_at) and reference-order seams (k_source): fields, memory
+    variables and seismograms against the oracle's whole-block loop (FP64
+    1e-12; FP32-mixed to its budget, 1e-5)."""
+    if precision != L.WQL_FP64 and not fused:
+        pytest.skip("the one-kernel-per-seam path is FP64")
+    prob = small_curvilinear(nmech=8)
+    for src in prob.sources:
+        src.order = 0
+    met = oracle.metrics(prob.X)
+    orc = oracle_block(oracle, prob, met)
+    _, seis_o = orc.time_step(prob.dt, prob.nsteps, receivers=receiver_flat(prob))
+    blk = gpu_block(prob, met, precision=precision)
+    blk.set_receivers(prob.receivers, prob.nsteps)
+    time_step_RK(blk, prob.dt, prob.nsteps, fused=fused)
+    _download_all(blk)
+    errs = field_errors(blk.F, orc.F)
+    tol = TOL64 if precision == L.WQL_FP64 else 1e-5
+    print(f"mollified fused={fused} precision={precision} field rel L2:",
+          ["%.2e" % e for e in errs], flush=True)
+    assert np.abs(orc.F[3:]).max() > 0
+    assert max(errs) <= tol
+    assert rel_l2(blk.eta, orc.eta) <= tol
+    if fused:
+        assert rel_l2(blk.seismograms(), seis_o) <= tol
+
+
+def test_mollified_source_needs_coordinates(oracle):
+    """wql_block_set_sources refuses a mollified source before the node
+    coordinates are set (WQL_ESTATE) and takes it afterwards."""
+    prob = small_curvilinear(nmech=0, npml=0)
+    for src in prob.sources:
+        src.order = 0
+    met = oracle.metrics(prob.X)
+    blk = PB.make_block(prob, met, upload=False)
+    with pytest.raises(L.WqlError):
+        blk.set_sources(prob.sources)
+    blk.set_coordinates(prob.X)
+    blk.set_sources(prob.sources)
+    blk.close()
+
+
 @pytest.mark.parametrize("fused", [True, False])
 def test_clamped_faces_match_oracle(oracle, fused):
     """Clamped faces (bc 3, BoundaryConditions.f90:151-161) on a curvilinear

@@ -58,3 +58,58 @@ def test_ricker_zero_mean_and_peak_frequency(oracle):
     spec = np.abs(np.fft.rfft(r, n))
     freq = np.fft.rfftfreq(n, dt)
     assert abs(freq[np.argmax(spec)] - f0) < 0.01
+
+
+def _mollified_weights(O, X, loc, near):
+    """delta of the oracle's mollified source (set_moment_tensor_smooth) at
+    every node of coordinates X (3, nz, ny, nx): DF(sxx) after one call at the
+    Gaussian source-time function's peak with mXX = -1/A(peak)."""
+    nz, ny, nx = X.shape[1:]
+    b = O.OracleBlock(nx, ny, nz, curvilinear=1)
+    b.X = np.ascontiguousarray(X)
+    b.DF[...] = 0.0
+    t0 = 0.5
+    A = O.source_time(0, t0, t0, 10.0)            # Gaussian source-time at its peak
+    b.add_source(kind=0, t_init=t0, duration=10.0, mxx=-1.0 / A, loc=loc, near=near,
+                 near_phys=loc, order=0)
+    b.set_moment_tensor(t0)
+    return b.DF[3].copy()
+
+
+def test_mollified_source_is_a_unit_gaussian_of_width_2h(oracle):
+    """set_moment_tensor_smooth (moment_tensor.f90:1255-1330) on a uniform
+    grid: the weights are a sampled 3-D Gaussian of standard deviation 2h, so
+    their quadrature sum is 1 (to the reference's truncated pi, 1e-13), their
+    first moments vanish about the source location and their second
+    moments are (2h)^2 per axis; the other moment-tensor components are
+    untouched."""
+    n, h = 45, 50.0
+    ax = np.arange(n) * h
+    Z, Y, Xc = np.meshgrid(ax, ax, ax, indexing="ij")
+    X = np.stack([Xc, Y, Z])
+    loc = (22.3 * h, 21.6 * h, 22.9 * h)
+    d = _mollified_weights(oracle, X, loc, (23, 23, 24))
+    w = h ** 3
+    assert abs(d.sum() * w - 1.0) < 1e-11
+    for k, c in enumerate(loc):
+        r = X[k] - c
+        assert abs((d * r).sum() * w) < 1e-9 * h
+        assert abs((d * r * r).sum() * w / (2 * h) ** 2 - 1.0) < 1e-9
+    assert d.max() > 0
+
+
+def test_mollified_source_takes_the_local_spacing(oracle):
+    """On a grid whose x spacing doubles from one half to the other, the
+    weight at each node is the uniform-grid weight of that node's own backward
+    spacing (the forward one at the first node): a source in the fine half
+    and one in the coarse half each integrate to 1 against their local cell."""
+    n = 61
+    hx = np.where(np.arange(n - 1) < 30, 40.0, 80.0)
+    x = np.concatenate([[0.0], np.cumsum(hx)])
+    ay = np.arange(n) * 40.0
+    Z, Y, Xc = np.meshgrid(ay, ay, x, indexing="ij")
+    X = np.stack([Xc, Y, Z])
+    for i0, hloc in ((12, 40.0), (48, 80.0)):
+        loc = (x[i0] + 0.3 * hloc, 30.2 * 40.0, 29.7 * 40.0)
+        d = _mollified_weights(oracle, X, loc, (i0 + 1, 31, 30))
+        assert abs(d.sum() * hloc * 40.0 * 40.0 - 1.0) < 1e-6

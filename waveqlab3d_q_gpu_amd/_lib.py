@@ -78,6 +78,7 @@ SIGNATURES = {
     "wql_block_init_state": (C.c_int, [_P]),
     "wql_block_destroy": (None, [_P]),
     "wql_block_set_sources": (C.c_int, [_P, C.c_int32, C.POINTER(Source)]),
+    "wql_block_set_coordinates": (C.c_int, [_P, _P, C.c_int32]),
     "wql_block_set_receivers": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32), C.c_int32]),
     "wql_block_get_seismograms": (C.c_int, [_P, C.c_int32, _D]),
     "wql_rk_scale": (C.c_int, [_P, C.c_double]),

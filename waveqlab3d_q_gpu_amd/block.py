@@ -64,7 +64,7 @@ class SourceSpec:
     mxy: float = 0.0
     mxz: float = 0.0
     myz: float = 0.0
-    order: int = 4
+    order: int = 4       # 2 or 4 (singular_source); 0: mollified (Gaussian, sigma = 2h)
     alpha: int = 0
 
 
@@ -213,6 +213,18 @@ class Block:
             o.location[:] = list(s.location)
             o.near_phys[:] = list(s.near_phys)
         check(lib().wql_block_set_sources(self._h, len(sources), arr))
+
+    def set_coordinates(self, X):
+        """Node coordinates (3, nz, ny, nx) float64, a numpy array or a CUDA
+        tensor, for mollified sources (order 0)."""
+        if hasattr(X, "data_ptr"):
+            if X.dtype != __import__("torch").float64 or not X.is_contiguous():
+                raise ValueError("coordinates: contiguous float64 tensor expected")
+            check(lib().wql_block_set_coordinates(self._h, C.c_void_p(X.data_ptr()),
+                                                  1 if X.is_cuda else 0))
+        else:
+            X = np.ascontiguousarray(X, dtype=np.float64)
+            check(lib().wql_block_set_coordinates(self._h, X.ctypes.data_as(C.c_void_p), 0))
 
     def set_receivers(self, ijk, nslots):
         ijk = np.ascontiguousarray(np.asarray(ijk, dtype=np.int32).reshape(-1, 3))

@@ -93,6 +93,7 @@ struct wql_block {
   int nslots;
   long long* rec_idx;
   double* seis;
+  double* X;      // node coordinates, 3 planes (wql_block_set_coordinates)
   int slot;
   long long bytes;
   int tune_nt, tune_xcd, tune_band, tune_split;  // node-kernel launch shape
@@ -458,8 +459,8 @@ extern "C" void wql_block_destroy(wql_block* b) {
     if (b->iface->L == b) b->iface->L = nullptr;
     if (b->iface->R == b) b->iface->R = nullptr;
   }
-  void* ptrs[] = {b->arena, b->rec_idx, b->seis, b->adj_Fb, b->adj_Eb, b->adj_DFb, b->adj_DEb,
-                  b->adj_g, b->adj_Z, b->slot_F, b->slot_E};
+  void* ptrs[] = {b->arena, b->rec_idx, b->seis, b->X, b->adj_Fb, b->adj_Eb, b->adj_DFb,
+                  b->adj_DEb, b->adj_g, b->adj_Z, b->slot_F, b->slot_E};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (int k = 0; k < 2; ++k) {
@@ -1070,11 +1071,28 @@ extern "C" int wql_block_set_sources(wql_block* b, int32_t n, const wql_source* 
   if (n < 0 || n > kMaxSources) return fail(WQL_EINVAL, "at most %d sources", kMaxSources);
   for (int i = 0; i < n; ++i) {
     const wql_source& s = src[i];
-    if (s.order != 2 && s.order != 4) return fail(WQL_EINVAL, "source order must be 2 or 4");
+    if (s.order != 0 && s.order != 2 && s.order != 4)
+      return fail(WQL_EINVAL, "source order must be 2 or 4, or 0 (mollified)");
+    if (s.order == 0 && !b->X)
+      return fail(WQL_ESTATE, "a mollified source needs the node coordinates (wql_block_set_coordinates)");
     if (s.type < 0 || s.type > 2) return fail(WQL_EINVAL, "invalid source type %d", s.type);
     b->src[i] = s;
   }
   b->nsrc = n;
+  return WQL_OK;
+}
+
+extern "C" int wql_block_set_coordinates(wql_block* b, const double* X, int32_t on_device) {
+  if (!b || !X) return fail(WQL_EINVAL, "null argument");
+  const long long n = (long long)b->nx * b->ny * b->nz;
+  if (!b->X) {
+    int rc = dalloc(b, (void**)&b->X, 3 * (size_t)b->P * sizeof(double));
+    if (rc) return rc;
+  }
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  for (int c = 0; c < 3; ++c)
+    HIP_TRY(hipMemcpy(b->X + c * b->P, X + c * n, (size_t)n * sizeof(double),
+                      on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice));
   return WQL_OK;
 }
 
@@ -1163,11 +1181,16 @@ static StepArgs<T, S> make_args(wql_block* b, double t) {
     a.iface[f] = d.bc[f] == 0 ? b->iface_sat[f] : nullptr;
   }
   a.sat_form = d.sat_form;
+  a.X = b->X;
   a.nsrc = b->nsrc;
   for (int s = 0; s < b->nsrc; ++s) {
     const wql_source& Sr = b->src[s];
     DevSource& D = a.src[s];
-    for (int k = 0; k < 3; ++k) { D.near[k] = Sr.near[k] - 1; D.off[k] = Sr.location[k] - Sr.near_phys[k]; }
+    for (int k = 0; k < 3; ++k) {
+      D.near[k] = Sr.near[k] - 1;
+      D.off[k] = Sr.location[k] - Sr.near_phys[k];
+      D.loc[k] = Sr.location[k];
+    }
     D.order = Sr.order;
     D.alpha = Sr.alpha;
     const double A = source_time(Sr.type, t, Sr.t_init, Sr.duration);

@@ -424,25 +424,58 @@ __device__ __forceinline__ double singular_source(int i, int i0, double h, doubl
 }
 
 // source contribution (to subtract from DF(4..9)) at node (x,y,z)
+// support half-width of source Sr in nodes (the box source_at visits)
+__host__ __device__ __forceinline__ int source_radius(const DevSource& Sr) {
+  return Sr.order == 0 ? kMollifyRadius : (Sr.order == 2 ? 2 : 4);
+}
+
+// set_moment_tensor_smooth (moment_tensor.f90:1255-1330) at node (x, y, z):
+// a Gaussian of standard deviation 2h per axis, h the coordinate's backward
+// difference along its own index (forward at the first node), with the
+// reference's truncated pi
+template <typename T, typename S>
+__device__ __forceinline__ double mollified_delta(const StepArgs<T, S>& a, int x, int y, int z,
+                                                  long long id, const DevSource& Sr) {
+  const double pi = 3.14159265359;
+  const long long P = a.P, sy = a.nx, sz = (long long)a.nx * a.ny;
+  const double* X = a.X;
+  const double hx = x == 0 ? X[id + 1] - X[id] : X[id] - X[id - 1];
+  const double hy = y == 0 ? X[P + id + sy] - X[P + id] : X[P + id] - X[P + id - sy];
+  const double hz = z == 0 ? X[2 * P + id + sz] - X[2 * P + id] : X[2 * P + id] - X[2 * P + id - sz];
+  const double sgx = 2.0 * hx, sgy = 2.0 * hy, sgz = 2.0 * hz;
+  const double ux = (X[id] - Sr.loc[0]) / sgx;
+  const double uy = (X[P + id] - Sr.loc[1]) / sgy;
+  const double uz = (X[2 * P + id] - Sr.loc[2]) / sgz;
+  const double dx = exp(-0.5 * (ux * ux)) / (sgx * sqrt(2.0 * pi));
+  const double dy = exp(-0.5 * (uy * uy)) / (sgy * sqrt(2.0 * pi));
+  const double dz = exp(-0.5 * (uz * uz)) / (sgz * sqrt(2.0 * pi));
+  return dx * dy * dz;
+}
+
 template <typename T, int MESH, typename S>
 __device__ __forceinline__ bool source_at(const StepArgs<T, S>& a, int x, int y, int z, long long id,
                                           double out[6]) {
   bool any = false;
   for (int s = 0; s < a.nsrc; ++s) {
     const DevSource& Sr = a.src[s];
-    const int r = Sr.order == 2 ? 2 : 4;
+    const int r = source_radius(Sr);
     if (x < Sr.near[0] - r || x > Sr.near[0] + r || y < Sr.near[1] - r || y > Sr.near[1] + r ||
         z < Sr.near[2] - r || z > Sr.near[2] + r)
       continue;
-    const long long P = a.P;
-    const double qx = (double)a.G[0 * P + id];
-    const double ry = (double)a.G[(MESH ? 4 : 1) * P + id];
-    const double sz = (double)a.G[(MESH ? 8 : 2) * P + id];
-    const double hx = a.h[0] / qx, hy = a.h[1] / ry, hz = a.h[2] / sz;
-    const double dx = singular_source(x, Sr.near[0], hx, Sr.off[0] / hx, Sr.order, Sr.alpha);
-    const double dy = singular_source(y, Sr.near[1], hy, Sr.off[1] / hy, Sr.order, Sr.alpha);
-    const double dz = singular_source(z, Sr.near[2], hz, Sr.off[2] / hz, Sr.order, Sr.alpha);
-    const double delta = dx * dy * dz;
+    double delta;
+    if (Sr.order == 0) {
+      delta = mollified_delta(a, x, y, z, id, Sr);
+    } else {
+      const long long P = a.P;
+      const double qx = (double)a.G[0 * P + id];
+      const double ry = (double)a.G[(MESH ? 4 : 1) * P + id];
+      const double sz = (double)a.G[(MESH ? 8 : 2) * P + id];
+      const double hx = a.h[0] / qx, hy = a.h[1] / ry, hz = a.h[2] / sz;
+      const double dx = singular_source(x, Sr.near[0], hx, Sr.off[0] / hx, Sr.order, Sr.alpha);
+      const double dy = singular_source(y, Sr.near[1], hy, Sr.off[1] / hy, Sr.order, Sr.alpha);
+      const double dz = singular_source(z, Sr.near[2], hz, Sr.off[2] / hz, Sr.order, Sr.alpha);
+      delta = dx * dy * dz;
+    }
     if (!any) {
 #pragma unroll
       for (int c = 0; c < 6; ++c) out[c] = 0.0;

@@ -31,6 +31,10 @@
 namespace wql {
 
 constexpr int kMaxSources = 8;
+// support of a mollified (Gaussian, sigma = 2h) source in nodes per axis: at
+// 40 nodes the weight is below exp(-50) of its peak even where the spacing
+// grows to twice the source's
+constexpr int kMollifyRadius = 40;
 // z-chunk pipeline of the anelastic fused stage: chunks per stage
 constexpr int kMaxChunks = 16;
 constexpr int kDefaultChunks = 1;
@@ -45,8 +49,9 @@ using OpType = typename std::conditional<kOpCopy<T, S>, T, S>::type;
 
 struct DevSource {
   int near[3];      // 0-based nearest node
-  int order, alpha;
+  int order, alpha; // order 2 or 4: singular_source; 0: mollified (Gaussian)
   double off[3];    // location - near_phys, per axis
+  double loc[3];    // location
   double am[6];     // A(t) * (mXX, mYY, mZZ, mXY, mXZ, mYZ) for the stage
 };
 
@@ -72,6 +77,7 @@ struct StepArgs {
   const T* Fo_in;
   T* Fo_out;
   const T* G;
+  const double* X;  // node coordinates (3 planes), for mollified sources; else null
   const T* M;
   T* eta;
   T* Deta;

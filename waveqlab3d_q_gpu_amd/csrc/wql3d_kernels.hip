@@ -407,7 +407,7 @@ __device__ __forceinline__ bool hooke_node(const StepArgs<T, S>& a, unsigned lon
   if (x < a.ix0 || x > a.ix1 || y < a.iy0 || y > a.iy1 || z < a.iz0 || z > a.iz1) return false;
   for (int k = 0; k < a.nsrc; ++k) {
     const DevSource& Sr = a.src[k];
-    const int rr = Sr.order == 2 ? 2 : 4;   // the box of source_at
+    const int rr = source_radius(Sr);   // the box of source_at
     if (x >= Sr.near[0] - rr && x <= Sr.near[0] + rr && y >= Sr.near[1] - rr &&
         y <= Sr.near[1] + rr && z >= Sr.near[2] - rr && z <= Sr.near[2] + rr)
       return false;
@@ -1021,7 +1021,7 @@ __global__ __launch_bounds__(256) void k_sat_face(const StepArgs<double> a, int 
 // set_moment_tensor over the +-4 (order 4) or +-2 (order 2) node box
 __global__ __launch_bounds__(256) void k_source(const StepArgs<double> a, int s, int mesh) {
   const DevSource& S = a.src[s];
-  const int r = S.order == 2 ? 2 : 4;
+  const int r = source_radius(S);
   const int w = 2 * r + 1;
   const int t = blockIdx.x * 256 + threadIdx.x;
   if (t >= w * w * w) return;
@@ -1031,14 +1031,19 @@ __global__ __launch_bounds__(256) void k_source(const StepArgs<double> a, int s,
   if (x < 0 || x >= a.nx || y < 0 || y >= a.ny || z < 0 || z >= a.nz) return;
   const long long P = a.P;
   const long long id = x + (long long)a.nx * (y + (long long)a.ny * z);
-  const double qx = a.G[id];
-  const double ry = a.G[(mesh ? 4 : 1) * P + id];
-  const double sz = a.G[(mesh ? 8 : 2) * P + id];
-  const double hx = a.h[0] / qx, hy = a.h[1] / ry, hz = a.h[2] / sz;
-  const double dx = singular_source(x, S.near[0], hx, S.off[0] / hx, S.order, S.alpha);
-  const double dy = singular_source(y, S.near[1], hy, S.off[1] / hy, S.order, S.alpha);
-  const double dz = singular_source(z, S.near[2], hz, S.off[2] / hz, S.order, S.alpha);
-  const double delta = dx * dy * dz;
+  double delta;
+  if (S.order == 0) {
+    delta = mollified_delta(a, x, y, z, id, S);
+  } else {
+    const double qx = a.G[id];
+    const double ry = a.G[(mesh ? 4 : 1) * P + id];
+    const double sz = a.G[(mesh ? 8 : 2) * P + id];
+    const double hx = a.h[0] / qx, hy = a.h[1] / ry, hz = a.h[2] / sz;
+    const double dx = singular_source(x, S.near[0], hx, S.off[0] / hx, S.order, S.alpha);
+    const double dy = singular_source(y, S.near[1], hy, S.off[1] / hy, S.order, S.alpha);
+    const double dz = singular_source(z, S.near[2], hz, S.off[2] / hz, S.order, S.alpha);
+    delta = dx * dy * dz;
+  }
   for (int c = 0; c < 6; ++c) a.DF[(3 + c) * P + id] = a.DF[(3 + c) * P + id] - S.am[c] * delta;
 }
 
@@ -1402,7 +1407,7 @@ hipError_t launch_sat(const StepArgs<double>& a, hipStream_t s) {
 
 hipError_t launch_source(const StepArgs<double>& a, int mesh, hipStream_t s) {
   for (int k = 0; k < a.nsrc; ++k) {
-    const int r = a.src[k].order == 2 ? 2 : 4;
+    const int r = source_radius(a.src[k]);
     const int w = 2 * r + 1;
     hipLaunchKernelGGL(k_source, dim3((w * w * w + 255) / 256), dim3(256), 0, s, a, k, mesh);
     const hipError_t e = hipGetLastError();

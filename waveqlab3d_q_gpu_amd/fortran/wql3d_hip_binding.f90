@@ -93,6 +93,12 @@ module wql3d_hip_binding
        import; type(c_ptr), value :: blk; integer(c_int32_t), value :: n
        type(wql_source), intent(in) :: src(*)
      end function
+     ! node coordinates (B%G%X without ghosts, x fastest), for mollified
+     ! sources (wql_source%order = 0, the reference's mollify_source)
+     integer(c_int) function wql_block_set_coordinates(blk, X, on_device) bind(C)
+       import; type(c_ptr), value :: blk; real(c_double), intent(in) :: X(*)
+       integer(c_int32_t), value :: on_device
+     end function
      integer(c_int) function wql_block_set_receivers(blk, n, ijk, nslots) bind(C)
        import; type(c_ptr), value :: blk; integer(c_int32_t), value :: n, nslots
        integer(c_int32_t), intent(in) :: ijk(*)

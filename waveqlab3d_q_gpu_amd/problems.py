@@ -854,6 +854,8 @@ def make_device_block(prob: Problem, precision=L.WQL_FP64, device="cuda", state_
                 M=torch.from_numpy(np.ascontiguousarray(prob.M)).to(dev),
                 n_l=nl, n_m=nm_, n_n=nn_, on_device=True, sat_form=prob.sat_form, **kw)
     blk.upload(L.WQL_GEOMETRY | L.WQL_MATERIAL | L.WQL_QINV | L.WQL_FACES)
+    if any(src.order == 0 for src in prob.sources):
+        blk.set_coordinates(prob.X)
     blk.set_sources(prob.sources)
     return blk
 
@@ -915,5 +917,7 @@ def make_block(prob: Problem, metrics, precision=L.WQL_FP64, frames=None, upload
                 pml_Q=pq, pml_DQ=pdq, n_l=nl, n_m=nm, n_n=nn, sat_form=prob.sat_form, **kw)
     if upload:
         blk.upload(L.WQL_GEOMETRY | L.WQL_MATERIAL | L.WQL_QINV | L.WQL_FACES | L.WQL_F)
+        if any(src.order == 0 for src in prob.sources):
+            blk.set_coordinates(prob.X)
         blk.set_sources(prob.sources)
     return blk
