@@ -227,6 +227,27 @@ def test_f_only_slots():
     assert same and float(torch.abs(first).max()) > 0
 
 
+def test_adjoint_refuses_pml_slabs():
+    """A block with PML slabs has no adjoint: wql_adjoint_create and
+    wql_adjoint_create_f return WQL_EINVAL naming the face, allocate nothing,
+    and the block keeps stepping."""
+    prob = PB.curvilinear_q(n=24, h=100.0, nsteps=2, nmech=4, npml=5, seed=31, nrec=4)
+    blk = PB.make_device_block(prob, precision=L.WQL_FP64, state_mirror=True)
+    lib, h = L.lib(), blk._h
+    try:
+        assert lib.wql_adjoint_create(h, 2) == L.WQL_EINVAL
+        assert b"PML" in lib.wql_last_error()
+        assert lib.wql_adjoint_create_f(h, 1, 5) == L.WQL_EINVAL
+        assert lib.wql_state_save(h, 0) == L.WQL_ESTATE
+        blk.time_step(prob.dt, 2, 0.0)
+        blk.download(L.WQL_F)
+        import torch
+        torch.cuda.synchronize()
+        assert bool(torch.isfinite(blk.F).all())
+    finally:
+        blk.close()
+
+
 def test_config4_adjoint_gradient_129():
     """The 129^3 Q4, 400-step workload with checkpoints every 20 steps: the
     loss is the L2 velocity misfit at 64 surface receivers against synthetic
