@@ -369,7 +369,9 @@ int wql_iface_get_friction_state(wql_iface *I, double *W);
  * PML slabs, interfaces and the alpha/theta SAT form are not differentiated
  * (wql_adjoint_create fails with any of them); all three precisions are.
  * Dense arrays below are DEVICE pointers, node order x fastest, no ghosts. */
-/* adjoint state, rate adjoints, gradients (zeroed) and nslots state slots */
+/* adjoint state, rate adjoints, gradients (zeroed) and nslots state slots;
+ * WQL_EINVAL for a block with PML slabs, an interface or the alpha/theta
+ * SAT form, which the adjoint does not differentiate */
 int wql_adjoint_create(wql_block *b, int32_t nslots);
 /* the same with nslots_f more slots, numbered nslots .. nslots+nslots_f-1,
  * that hold F only (the memory a slot of F needs, without eta): for the
