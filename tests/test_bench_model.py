@@ -88,3 +88,16 @@ def test_stencil_and_end_to_end_bytes():
     assert np.isclose(e2e / NODES, 961.4, atol=0.1)
     assert np.isclose(m["end_to_end"] / NODES, 871.3, atol=0.1)
     assert m["end_to_end_all_mechanisms"] / NODES > e2e / NODES
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus 2 exits with status 2 and says so,
+    before it touches a device."""
+    import subprocess
+    import sys
+    from pathlib import Path
+    root = Path(__file__).resolve().parents[1]
+    r = subprocess.run([sys.executable, str(root / "bench.py"), "--gpus", "2"], cwd=root,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one MI355X" in r.stderr and r.stdout == ""
