@@ -77,3 +77,21 @@ def test_capacity_check_env_override(L, monkeypatch):
     assert L.lib().wql_capacity_check(10) == L.WQL_OK
     assert L.lib().wql_capacity_check(5000) == L.WQL_ENOMEM
     assert b"RUN-GPU-MEM-002" in L.lib().wql_last_error()
+
+
+def test_missing_library_fails_loudly(tmp_path):
+    """Without the built library the package raises on first use: there is
+    no CPU fallback for any entry point (fresh interpreter, library path
+    pointed at an empty folder)."""
+    code = ("import waveqlab3d_q_gpu_amd._lib as L\n"
+            f"L.LIB_PATH = {str(tmp_path / 'libwql3d_hip.so')!r}\n"
+            "try:\n"
+            "    L.lib()\n"
+            "except RuntimeError as e:\n"
+            "    print('raised:', e)\n"
+            "else:\n"
+            "    print('loaded')\n")
+    r = subprocess.run(["python", "-c", code], cwd=ROOT, capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.startswith("raised:") and "is missing" in r.stdout
